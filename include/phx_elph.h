@@ -1,0 +1,182 @@
+/*
+ * phx_elph.h - C ABI of the MI355X (gfx950) electron-phonon linewidth path.
+ *
+ * Drop-in replacement for Phoebe's el-ph scattering hot path:
+ *   InteractionElPhWan (src/interaction/interaction_elph.h:30-219),
+ *   kokkosZHEEV (src/common_kokkos.h:49) and the batched Bloch
+ *   diagonalizations (src/harmonic/harmonic.h:80-91), plus the linewidth
+ *   builder ElScatteringMatrix::builder, switchCase 2
+ *   (src/bte/el_scattering.cpp:38-473).
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles;
+ *   - matrices are column-major (Eigen's default), one matrix contiguous;
+ *   - energies in Ry, lengths in Bohr, wavevectors Cartesian incl. 2*pi;
+ *   - every function returns 0 on success and a negative status on error;
+ *     phx_last_error() returns the message of the calling thread's last error
+ *     (the reference prints and exits through Error(), exceptions.cpp:6-16;
+ *     the adaptor in phoebe_amd/csrc/phx_adaptor.hpp maps status != 0 back
+ *     to that behaviour).
+ * All pointers are host pointers unless a name ends in _dev.
+ */
+#ifndef PHX_ELPH_H
+#define PHX_ELPH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision of the Fourier-transform and rotation GEMMs; phases,
+ * eigensolves and the linewidth accumulation are always FP64 */
+enum { PHX_FP64 = 0, PHX_FP32 = 1, PHX_BF16X3 = 2 };
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* phx_last_error(void);
+
+/* Library version string. */
+const char* phx_version(void);
+
+/* ---------------------------------------------------------------- eigensolver
+ * Batched Hermitian eigen-decomposition, in place.
+ * Replaces kokkosZHEEV(StridedComplexView3D& A, DoubleView2D& W)
+ * (src/common_kokkos.h:49, common_kokkos.cpp:11-100).
+ *   A  : M matrices N x N, column-major complex; on return the eigenvectors
+ *        (columns), eigenvalues ascending in W (M x N).
+ *   info (optional, M ints): Jacobi sweeps used, -1 if not converged.
+ * N <= 44 (the matrix and its eigenvectors stay in one wavefront's LDS). */
+int phx_zheev_batched(int M, int N, double* A, double* W, int* info);
+
+/* Same on device buffers, enqueued on a HIP stream (0 = default stream). */
+int phx_zheev_batched_dev(int M, int N, double* A_dev, double* W_dev, int* info_dev,
+                          void* stream);
+
+/* ---------------------------------------------------------------- Bloch FTs
+ * ElectronH0Wannier::kokkosBatchedDiagonalizeFromCoordinates
+ * (src/harmonic/electron_h0_wannier.cpp:777-877):
+ *   H(k) = sum_R e^{ik.R}/deg(R) h0R(R,:,:), then eigen-decomposition.
+ *   k    : nk x 3 Cartesian;  R: nR x 3 Cartesian;  deg: nR;
+ *   h0R  : (nR, nw, nw) complex, row-major (h0R(R,m,n) of the reference);
+ *   energies: nk x nw;  eigvecs: nk x (nw x nw column-major complex). */
+int phx_el_bloch_diag(int nk, const double* k, int nw, int nR, const double* R,
+                      const double* deg, const double* h0R, double* energies,
+                      double* eigvecs);
+
+/* PhononH0::kokkosBatchedDiagonalizeFromCoordinates
+ * (src/harmonic/phonon_h0_kokkos.cpp:6-219), no long-range term:
+ *   D(q) = sum_R e^{-iq.R} w(R) mat2R(R,:,:), hermitized, divided by
+ *   sqrt(M_m M_n); omega = sign(l) sqrt|l|; eigenvectors divided by sqrt(M_m)
+ *   when mass_scaling != 0.
+ *   mat2R: (nR, nm, nm) real row-major;  masses: nm (Ry units);
+ *   omega: nq x nm;  eigvecs: nq x (nm x nm column-major complex). */
+int phx_ph_bloch_diag(int nq, const double* q, int nm, int nR, const double* R,
+                      const double* w, const double* mat2R, const double* masses,
+                      int mass_scaling, double* omega, double* eigvecs);
+
+/* ---------------------------------------------------------------- InteractionElPhWan */
+typedef struct phx_elph phx_elph;
+
+/* InteractionElPhWan(Crystal&, const Eigen::Tensor<cd,5>& gW, elBravaisVectors,
+ * elDegeneracies, phBravaisVectors, phDegeneracies, PhononH0*)
+ * (interaction_elph.h:93-100, interaction_elph.cpp:10-62).
+ *   gW : the Eigen tensor (nw(w2), nw(w1), nm, nRp, nRe) column-major, i.e.
+ *        (nRe, nRp, nm, w1, w2) row-major;  elR: nRe x 3;  phR: nRp x 3.
+ * The polar (Froehlich) term is not supported (the reference enables it only
+ * with a dielectric tensor and more than one species, :24-32). */
+int phx_elph_create(phx_elph** h, const double* gW, int nw, int nm, int nRp, int nRe,
+                    const double* elR, const double* elDeg, const double* phR,
+                    const double* phDeg, int precision);
+
+/* Same, with gW generated on the device from the synthetic recipe
+ * gW = decay[Re,Rp] * (n(2f) + i n(2f+1)), n = counter-hash normal deviate
+ * (phoebe_amd/synth.py), so that tens of GB never cross PCIe. */
+int phx_elph_create_synthetic(phx_elph** h, int nw, int nm, int nRp, int nRe,
+                              const double* elR, const double* elDeg, const double* phR,
+                              const double* phDeg, const double* decay,
+                              unsigned long long seed, int precision);
+
+/* InteractionElPhWan::cacheElPh(const Eigen::MatrixXcd& eigvec1 (nw x nb1),
+ *                               const Eigen::Vector3d& k1C)
+ * (interaction_elph.h:161, interaction_elph.cpp:492-701). */
+int phx_elph_cache(phx_elph* h, const double* U1, int nb1, const double* k1);
+
+/* InteractionElPhWan::calcCouplingSquared(eigvec1, eigvecs2, eigvecs3, q3Cs,
+ *                                         polarData)
+ * (interaction_elph.h:149-154, interaction_elph.cpp:229-448).
+ *   U2s  : concatenation of the nq matrices (nw x nb2s[i]) column-major;
+ *   e3s  : nq x (nm x nm) column-major;  q3: nq x 3 Cartesian;
+ *   polar: must be NULL (polar term unsupported);
+ *   out  : concatenation of the nq tensors (nb1, nb2s[i], nm) column-major,
+ *          the getCouplingSquared(ik) layout (interaction_elph.cpp:434-446). */
+int phx_elph_coupling_sq(phx_elph* h, int nq, const double* U2s, const int* nb2s,
+                         const double* e3s, const double* q3, const double* polar,
+                         double* out);
+
+/* InteractionElPhWan::getCouplingDimensions() (interaction_elph.cpp:450-456):
+ * the device extents (nRe, nRp, nm, nw, nw). */
+int phx_elph_get_coupling_dims(phx_elph* h, int* dims5);
+
+/* InteractionElPhWan::estimateNumBatches(nk2, nb1) (interaction_elph.cpp:458-490)
+ * with the reference's memory model and a budget of max_mem_bytes (<= 0:
+ * the MAXMEM environment variable in GB, default 16, common_kokkos.cpp:107-112).
+ * Returns the number of batches (>= 1) or a negative status. */
+int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double max_mem_bytes);
+
+int phx_elph_destroy(phx_elph* h);
+
+/* ---------------------------------------------------------------- linewidths
+ * Whole electron linewidth calculation of ElScatteringMatrix::builder,
+ * switchCase 2 (src/bte/el_scattering.cpp:38-473), inner == outer band
+ * structure on a Gamma-centred mesh with an energy window
+ * (HelperElScattering storedAllQ3Case2, helper_el_scattering.cpp:58-162),
+ * Gaussian smearing, no symmetries, no polar term. */
+typedef struct {
+  double recip[9]; /* reciprocal cell, column-major, columns b_i (2*pi*A^-T) */
+  int mesh[3];     /* k mesh = q mesh */
+  int nw, nRh;
+  const double *Rh, *degh, *h0R; /* electron Wannier Hamiltonian */
+  int nm, nRf;
+  const double *Rf, *wf, *mat2R, *masses; /* reordered force constants */
+  int nRe, nRp;
+  const double *Re, *degE, *Rp, *degP;
+  const double* gW;          /* host gW, or NULL to use the synthetic recipe: */
+  const double* gwDecay;     /*   (nRe, nRp) */
+  unsigned long long gwSeed;
+  int nCalc;                 /* (T, mu) pairs, Ry */
+  const double *temperatures, *chemPots;
+  double sigma, winMin, winMax; /* Gaussian width, energy window (Ry) */
+} phx_problem;
+
+typedef struct phx_lw phx_lw;
+
+/* Band structures, window, phonons on the q-set, device-resident operands. */
+int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch);
+int phx_lw_num_active(phx_lw* h);
+int phx_lw_num_states(phx_lw* h);
+/* active points (full-mesh indices), band extrema, energies of the states */
+int phx_lw_get_window(phx_lw* h, int* active, int* band_lo, int* band_hi, double* energies);
+/* phonon frequencies (nk x nm, full mesh order) and the q-set mask (nk bytes) */
+int phx_lw_get_phonons(phx_lw* h, double* omega, unsigned char* in_set);
+/* Accumulate the scattering of the outer points k1_list (active indices;
+ * NULL = all) against inner points [k2_begin, k2_end) (k2_end < 0: all).
+ * Enqueued on the handle's stream; returns the number of (k1,k2) pairs. */
+long long phx_lw_run(phx_lw* h, int nk1, const int* k1_list, int k2_begin, int k2_end);
+/* Linewidths (nCalc x numStates, Ry), with degeneracyAveragingLinewidths
+ * (scattering.cpp:1168-1207) applied to row 0 when average != 0. */
+int phx_lw_get_linewidths(phx_lw* h, double* out, int average);
+int phx_lw_reset(phx_lw* h);
+int phx_lw_sync(phx_lw* h);
+/* HIP stream the handle enqueues on (hipStream_t). */
+void* phx_lw_stream(phx_lw* h);
+/* Copy the handle's gW to host memory (nRe*nRp*nm*nw*nw complex). */
+int phx_lw_copy_gw(phx_lw* h, double* out);
+int phx_lw_destroy(phx_lw* h);
+
+/* Host twin of the synthetic gW generator (bit-identical to the device). */
+int phx_synth_gw_host(double* gW, int nw, int nm, int nRp, int nRe, const double* decay,
+                      unsigned long long seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHX_ELPH_H */

@@ -1,0 +1,822 @@
+// Oracle: CPU FP64 restatement of Phoebe's electron-phonon linewidth path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product (phoebe_amd/, include/)
+// links, loads or calls this file.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg use it, as the checker / CPU baseline.
+//
+// Every function restates the algorithm of the reference file:line it cites,
+// keeping the reference's loop structure (GEMV over R_e per k1, separate
+// g3 / g4 / gFinal loops, per-matrix eigensolves, serial linewidth loop) so
+// that it can double as the "restatement" CPU baseline.
+//
+// Conventions (Rydberg atomic units; complex numbers interleaved re,im):
+//   * energies in Ry, lengths in Bohr, wavevectors Cartesian incl. 2*pi
+//     (reciprocal cell 2*pi*A^-T, crystal.cpp:291);
+//   * H(k)   = sum_R e^{+ik.R}/d(R) h0R(R,m,n)      (electron_h0_wannier.cpp:803-821)
+//   * D(q)   = sum_R e^{-iq.R} w(R) mat2R(R,m,n)     (phonon_h0_kokkos.cpp:23-46)
+//   * g      : R_e phase e^{+ik1.Re}/de, R_ph phase e^{+iq.Rp}/dp
+//              (interaction_elph.cpp:566-573, 357-366)
+//   * U(k) plain, U(k+q) conjugated, e(q) plain (interaction_elph.cpp:330,387,399,642)
+//   * matrices of eigenvectors are column-major (column = band / mode)
+//
+// Pinning (tests/test_oracle_*.py): zheev against LAPACK (scipy) and closed
+// forms; Wigner-Seitz completeness sum(1/d) = prod(grid); coupling against a
+// brute-force 5-fold sum and against textbook reductions (single R,
+// gauge invariance, k -> k+G invariance); Gaussian/Fermi/Bose closed forms;
+// linewidth reduction against hand-evaluable single-band/single-mode cases.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using cd = std::complex<double>;
+
+namespace {
+
+// constants.h:9-57 (restated values)
+const double kPi = 3.14159265358979323846;
+const double kSqrtPi = std::sqrt(3.14159265358979323846);
+const double kTwoPi = 2. * kPi;
+const double kSpeedLightSi = 299792458;
+const double kHPlanckSi = 6.62607015e-34;
+const double kHartreeSi = 4.359744650e-18;
+const double kTeraHertzAu = kHPlanckSi / kTwoPi / kHartreeSi * 1.0e+12;
+const double kRyToTHz = 1. / kTeraHertzAu / 2. / kTwoPi;
+const double kRyToCmm1 = 1.0e10 * kRyToTHz / kSpeedLightSi;
+
+inline int imod(int a, int b) { return (a % b + b) % b; }  // utilities.cpp:9
+
+inline cd ld(const double* p, size_t i) { return cd(p[2 * i], p[2 * i + 1]); }
+inline void st(double* p, size_t i, cd v) {
+  p[2 * i] = v.real();
+  p[2 * i + 1] = v.imag();
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Wigner-Seitz lattice vectors and degeneracies: crystal.cpp:354-459.
+// A is the direct cell, column-major (A[i + 3*j] = component i of vector j).
+// Returns the number of vectors; when R/deg are non-null (capacity maxOut)
+// they are filled with Cartesian vectors (row-major n x 3) and degeneracies,
+// with the origin swapped to index 0 (crystal.cpp:447-456).
+// Returns -1 if the completeness check fails, -2 if maxOut is too small.
+// ---------------------------------------------------------------------------
+int orc_ws_vectors(const int* grid, int superCellFactor, const double* A,
+                   int maxOut, double* R, double* deg) {
+  const int nx = superCellFactor;
+  int midIndex = 1;
+  for (int d = 0; d < 3; ++d) midIndex *= 2 * (nx + 1) + 1;
+  midIndex /= 2;
+  std::vector<double> vecs;
+  std::vector<int> degs;
+  std::vector<double> distances;
+  for (int i0 = -nx * grid[0]; i0 <= nx * grid[0]; i0++) {
+    for (int i1 = -nx * grid[1]; i1 <= nx * grid[1]; i1++) {
+      for (int i2 = -nx * grid[2]; i2 <= nx * grid[2]; i2++) {
+        distances.clear();
+        for (int j0 = -nx - 1; j0 <= nx + 1; j0++) {
+          for (int j1 = -nx - 1; j1 <= nx + 1; j1++) {
+            for (int j2 = -nx - 1; j2 <= nx + 1; j2++) {
+              double d0 = i0 - j0 * grid[0];
+              double d1 = i1 - j1 * grid[1];
+              double d2 = i2 - j2 * grid[2];
+              double c[3];
+              for (int i = 0; i < 3; ++i)
+                c[i] = A[i + 0] * d0 + A[i + 3] * d1 + A[i + 6] * d2;
+              distances.push_back(std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]));
+            }
+          }
+        }
+        double distMin = *std::min_element(distances.begin(), distances.end());
+        if (std::abs(distances[midIndex] - distMin) < 1.0e-6) {
+          int degeneracy = 0;
+          for (double dd : distances)
+            if (std::abs(dd - distMin) < 1.0e-6) ++degeneracy;
+          degs.push_back(degeneracy);
+          vecs.push_back(i0);
+          vecs.push_back(i1);
+          vecs.push_back(i2);
+        }
+      }
+    }
+  }
+  double tot = 0.;
+  for (int x : degs) tot += 1. / double(x);
+  if (std::abs(tot - double(grid[0]) * grid[1] * grid[2]) > 1.0e-6) return -1;
+  const int n = int(degs.size());
+  if (R == nullptr || deg == nullptr) return n;
+  if (n > maxOut) return -2;
+  int originIndex = -1;
+  for (int iR = 0; iR < n; ++iR) {
+    const double* v = &vecs[3 * iR];
+    for (int i = 0; i < 3; ++i)
+      R[3 * iR + i] = A[i] * v[0] + A[i + 3] * v[1] + A[i + 6] * v[2];
+    deg[iR] = degs[iR];
+    if (std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) < 1.0e-6) originIndex = iR;
+  }
+  if (originIndex < 0) return -1;
+  std::swap(deg[0], deg[originIndex]);
+  for (int i = 0; i < 3; ++i) std::swap(R[i], R[3 * originIndex + i]);
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Mesh folding: Points::isPointStored for a Gamma-centred mesh
+// (points.cpp:404-423). Note the signed sum of (round(x) - x).
+// ---------------------------------------------------------------------------
+int orc_mesh_index(const double* xcrys, const int* mesh) {
+  int p[3];
+  double diff = 0.;
+  for (int i = 0; i < 3; ++i) {
+    double x = xcrys[i] * mesh[i];
+    diff += std::round(x) - x;
+    p[i] = imod(int(std::round(x)), mesh[i]);
+  }
+  if (diff >= 1.0e-6) return -1;
+  return p[2] * mesh[0] * mesh[1] + p[1] * mesh[0] + p[0];
+}
+
+// Mesh point -> crystal coordinates (points.cpp:233-240, offset 0).
+void orc_mesh_point(int index, const int* mesh, double* xcrys) {
+  int ikz = index / (mesh[0] * mesh[1]);
+  int idx = index - ikz * mesh[0] * mesh[1];
+  int iky = idx / mesh[0];
+  int ikx = imod(idx, mesh[0]);
+  xcrys[0] = double(ikx) / double(mesh[0]);
+  xcrys[1] = double(iky) / double(mesh[1]);
+  xcrys[2] = double(ikz) / double(mesh[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Hermitian eigensolver (the oracle's own, cyclic complex Jacobi).
+// Replaces Eigen::SelfAdjointEigenSolver as used by kokkosZHEEV
+// (common_kokkos.cpp:26-53): eigenvalues ascending, eigenvectors as columns
+// of a column-major matrix written over A.  Pinned against LAPACK zheev.
+// Returns the number of sweeps used, or -1 if not converged.
+// ---------------------------------------------------------------------------
+int orc_zheev(int N, double* Ainout, double* W) {
+  std::vector<cd> a(size_t(N) * N), v(size_t(N) * N, cd(0.));
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < N; ++i) a[i + size_t(N) * j] = ld(Ainout, i + size_t(N) * j);
+  // use the lower triangle as the reference (uplo = L) and hermitize exactly
+  for (int j = 0; j < N; ++j) {
+    a[j + size_t(N) * j] = cd(a[j + size_t(N) * j].real(), 0.);
+    for (int i = j + 1; i < N; ++i) a[j + size_t(N) * i] = std::conj(a[i + size_t(N) * j]);
+  }
+  for (int i = 0; i < N; ++i) v[i + size_t(N) * i] = 1.;
+  auto A = [&](int i, int j) -> cd& { return a[i + size_t(N) * j]; };
+  auto V = [&](int i, int j) -> cd& { return v[i + size_t(N) * j]; };
+  double norm2 = 0.;
+  for (auto& x : a) norm2 += std::norm(x);
+  int sweep = 0;
+  const int maxSweeps = 60;
+  for (; sweep < maxSweeps; ++sweep) {
+    double off = 0.;
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i)
+        if (i != j) off += std::norm(A(i, j));
+    if (off <= 1e-30 * norm2 || off == 0.) break;
+    for (int p = 0; p < N - 1; ++p) {
+      for (int q = p + 1; q < N; ++q) {
+        cd apq = A(p, q);
+        double r = std::abs(apq);
+        if (r == 0.) continue;
+        double app = A(p, p).real(), aqq = A(q, q).real();
+        if (r < 1e-300) continue;
+        cd ph = apq / r;  // e^{i phi}
+        double tau = (aqq - app) / (2. * r);
+        double t = (tau >= 0. ? 1. : -1.) / (std::abs(tau) + std::sqrt(1. + tau * tau));
+        double c = 1. / std::sqrt(1. + t * t);
+        double s = t * c;
+        // J = [[c, s e^{i phi}], [-s e^{-i phi}, c]] on (p,q); A <- J^H A J, V <- V J
+        cd sp = s * ph, sm = s * std::conj(ph);
+        for (int k = 0; k < N; ++k) {  // columns
+          cd akp = A(k, p), akq = A(k, q);
+          A(k, p) = akp * c - akq * sm;
+          A(k, q) = akp * sp + akq * c;
+        }
+        for (int k = 0; k < N; ++k) {  // rows
+          cd apk = A(p, k), aqk = A(q, k);
+          A(p, k) = c * apk - sp * aqk;
+          A(q, k) = sm * apk + c * aqk;
+        }
+        A(p, q) = 0.;
+        A(q, p) = 0.;
+        A(p, p) = cd(A(p, p).real(), 0.);
+        A(q, q) = cd(A(q, q).real(), 0.);
+        for (int k = 0; k < N; ++k) {
+          cd vkp = V(k, p), vkq = V(k, q);
+          V(k, p) = vkp * c - vkq * sm;
+          V(k, q) = vkp * sp + vkq * c;
+        }
+      }
+    }
+  }
+  std::vector<int> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return A(x, x).real() < A(y, y).real(); });
+  for (int j = 0; j < N; ++j) {
+    W[j] = A(order[j], order[j]).real();
+    for (int i = 0; i < N; ++i) st(Ainout, i + size_t(N) * j, V(i, order[j]));
+  }
+  return sweep < maxSweeps ? sweep : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Electron Bloch Hamiltonian: electron_h0_wannier.cpp:160-216 (CPU path)
+// and :803-821 (Kokkos path): H(k)_{mn} = sum_R e^{i k.R}/d(R) h0R(R,m,n).
+// h0R is (nR, nw, nw) row-major interleaved; H is column-major.
+// ---------------------------------------------------------------------------
+void orc_el_hamiltonian(int nw, int nR, const double* R, const double* deg,
+                        const double* h0R, const double* k, double* H) {
+  std::vector<cd> phases(nR);
+  for (int iR = 0; iR < nR; ++iR) {
+    double arg = k[0] * R[3 * iR] + k[1] * R[3 * iR + 1] + k[2] * R[3 * iR + 2];
+    phases[iR] = cd(std::cos(arg), std::sin(arg)) / deg[iR];
+  }
+  for (int n = 0; n < nw; ++n)
+    for (int m = 0; m < nw; ++m) {
+      cd tmp = 0.;
+      for (int iR = 0; iR < nR; ++iR) tmp += phases[iR] * ld(h0R, (size_t(iR) * nw + m) * nw + n);
+      st(H, m + size_t(nw) * n, tmp);
+    }
+}
+
+// Energies + eigenvectors at k (electron_h0_wannier.cpp:218-235 / 855-877).
+int orc_el_diag(int nw, int nR, const double* R, const double* deg, const double* h0R,
+                const double* k, double* energies, double* eigvecs) {
+  orc_el_hamiltonian(nw, nR, R, deg, h0R, k, eigvecs);
+  return orc_zheev(nw, eigvecs, energies) < 0 ? -1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Phonon dynamical matrix: phonon_h0_kokkos.cpp:6-46 (FT with e^{-iq.R} w(R)),
+// :150-170 (hermitize, divide by sqrt(M_m M_n)); mat2R is (nR, nm, nm).
+// ---------------------------------------------------------------------------
+void orc_ph_dynamical(int nm, int nR, const double* R, const double* w, const double* mat2R,
+                      const double* masses, const double* q, double* D) {
+  std::vector<cd> phases(nR);
+  for (int iR = 0; iR < nR; ++iR) {
+    double arg = q[0] * R[3 * iR] + q[1] * R[3 * iR + 1] + q[2] * R[3 * iR + 2];
+    phases[iR] = std::exp(-cd(0., 1.) * arg) * w[iR];
+  }
+  std::vector<cd> dyn(size_t(nm) * nm);
+  for (int m = 0; m < nm; ++m)
+    for (int n = 0; n < nm; ++n) {
+      cd tmp = 0.;
+      for (int iR = 0; iR < nR; ++iR) tmp += phases[iR] * mat2R[(size_t(iR) * nm + m) * nm + n];
+      dyn[size_t(m) * nm + n] = tmp;
+    }
+  for (int n = 0; n < nm; ++n)
+    for (int m = 0; m < nm; ++m) {
+      cd x = 0.5 * (dyn[size_t(m) * nm + n] + std::conj(dyn[size_t(n) * nm + m]));
+      x /= std::sqrt(masses[m] * masses[n]);
+      st(D, m + size_t(nm) * n, x);
+    }
+}
+
+// Frequencies omega = sign(l) sqrt|l| and mass-scaled eigenvectors e/sqrt(M)
+// (phonon_h0_kokkos.cpp:177-219).
+int orc_ph_diag(int nm, int nR, const double* R, const double* w, const double* mat2R,
+                const double* masses, const double* q, double* omega, double* eigvecs) {
+  orc_ph_dynamical(nm, nR, R, w, mat2R, masses, q, eigvecs);
+  if (orc_zheev(nm, eigvecs, omega) < 0) return -1;
+  for (int m = 0; m < nm; ++m)
+    omega[m] = omega[m] > 0. ? std::sqrt(omega[m]) : -std::sqrt(-omega[m]);
+  for (int n = 0; n < nm; ++n)
+    for (int m = 0; m < nm; ++m) {
+      size_t i = m + size_t(nm) * n;
+      st(eigvecs, i, ld(eigvecs, i) / std::sqrt(masses[m]));
+    }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// InteractionElPhWan::cacheElPh (interaction_elph.cpp:492-701), pool size 1,
+// CPU (non-CUDA) branch: phases (:566-573), GEMV over R_e (:604-608), U1
+// rotation without conjugation (:635-645).
+// gW: (nRe, nRp, nm, nw(w1), nw(w2)) row-major interleaved (the reference's
+// LayoutRight view of the Eigen tensor, :44-47).
+// U1: (nw, nb1) column-major.  cached: (nRp, nm, nb1, nw(w2)) row-major.
+// ---------------------------------------------------------------------------
+void orc_cache_elph(int nw, int nm, int nRp, int nRe, const double* gW, const double* Re,
+                    const double* degE, const double* U1, int nb1, const double* k1,
+                    double* cached) {
+  std::vector<cd> phases(nRe);
+  for (int irE = 0; irE < nRe; ++irE) {
+    double arg = k1[0] * Re[3 * irE] + k1[1] * Re[3 * irE + 1] + k1[2] * Re[3 * irE + 2];
+    phases[irE] = std::exp(cd(0., 1.) * arg) / degE[irE];
+  }
+  const size_t ncol = size_t(nRp) * nm * nw * nw;
+  std::vector<cd> g1(ncol);
+  // gemv("T"): g1(col) = sum_irE gW(irE, col) * phase(irE)
+#pragma omp parallel for schedule(static)
+  for (long long col = 0; col < (long long)ncol; ++col) {
+    cd tmp = 0.;
+    for (int irE = 0; irE < nRe; ++irE) tmp += ld(gW, size_t(irE) * ncol + col) * phases[irE];
+    g1[col] = tmp;
+  }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int irP = 0; irP < nRp; ++irP)
+    for (int nu = 0; nu < nm; ++nu)
+      for (int ib1 = 0; ib1 < nb1; ++ib1)
+        for (int iw2 = 0; iw2 < nw; ++iw2) {
+          cd tmp = 0.;
+          for (int iw1 = 0; iw1 < nw; ++iw1)
+            tmp += g1[((size_t(irP) * nm + nu) * nw + iw1) * nw + iw2] *
+                   ld(U1, iw1 + size_t(nw) * ib1);
+          st(cached, ((size_t(irP) * nm + nu) * nb1 + ib1) * nw + iw2, tmp);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// InteractionElPhWan::calcCouplingSquared (interaction_elph.cpp:229-448),
+// polar correction off.  phases (:357-366), g3 (:369-378), g4 (:381-390),
+// gFinal (:393-402), |g|^2 with the b1/b2 flip (:416-446).
+// U2s: packed per q, each (nw, nb2_q) column-major.  e3s: nq x (nm, nm)
+// column-major (rows = 3*atom+pol, columns = modes).  q3: nq x 3 Cartesian.
+// out: packed per q, each (nb1, nb2_q, nm) column-major.
+// ---------------------------------------------------------------------------
+void orc_coupling_sq(int nw, int nm, int nRp, const double* Rp, const double* degP,
+                     const double* cached, int nb1, int nq, const double* U2s,
+                     const int* nb2s, const double* e3s, const double* q3, double* out) {
+  std::vector<size_t> u2off(nq + 1, 0), outoff(nq + 1, 0);
+  int nb2max = 0;
+  for (int iq = 0; iq < nq; ++iq) {
+    u2off[iq + 1] = u2off[iq] + size_t(nw) * nb2s[iq];
+    outoff[iq + 1] = outoff[iq] + size_t(nb1) * nb2s[iq] * nm;
+    nb2max = std::max(nb2max, nb2s[iq]);
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int ik = 0; ik < nq; ++ik) {
+    std::vector<cd> phases(nRp);
+    for (int irP = 0; irP < nRp; ++irP) {
+      double arg = q3[3 * ik] * Rp[3 * irP] + q3[3 * ik + 1] * Rp[3 * irP + 1] +
+                   q3[3 * ik + 2] * Rp[3 * irP + 2];
+      phases[irP] = std::exp(cd(0., 1.) * arg) / degP[irP];
+    }
+    std::vector<cd> g3(size_t(nm) * nb1 * nw), g4(size_t(nm) * nb1 * nw);
+    for (int nu = 0; nu < nm; ++nu)
+      for (int ib1 = 0; ib1 < nb1; ++ib1)
+        for (int iw2 = 0; iw2 < nw; ++iw2) {
+          cd tmp = 0.;
+          for (int irP = 0; irP < nRp; ++irP)
+            tmp += phases[irP] * ld(cached, ((size_t(irP) * nm + nu) * nb1 + ib1) * nw + iw2);
+          g3[(size_t(nu) * nb1 + ib1) * nw + iw2] = tmp;
+        }
+    const double* e3 = e3s + 2 * size_t(ik) * nm * nm;
+    for (int nu2 = 0; nu2 < nm; ++nu2)
+      for (int ib1 = 0; ib1 < nb1; ++ib1)
+        for (int iw2 = 0; iw2 < nw; ++iw2) {
+          cd tmp = 0.;
+          // eigvecs3_k(ik, nu2, nu) = eigvecs3[ik](nu, nu2)   (:336-340)
+          for (int nu = 0; nu < nm; ++nu)
+            tmp += g3[(size_t(nu) * nb1 + ib1) * nw + iw2] * ld(e3, nu + size_t(nm) * nu2);
+          g4[(size_t(nu2) * nb1 + ib1) * nw + iw2] = tmp;
+        }
+    const int nb2 = nb2s[ik];
+    const double* U2 = U2s + 2 * u2off[ik];
+    double* o = out + outoff[ik];
+    for (int nu = 0; nu < nm; ++nu)
+      for (int ib1 = 0; ib1 < nb1; ++ib1)
+        for (int ib2 = 0; ib2 < nb2; ++ib2) {
+          cd tmp = 0.;
+          for (int iw2 = 0; iw2 < nw; ++iw2)
+            tmp += std::conj(ld(U2, iw2 + size_t(nw) * ib2)) * g4[(size_t(nu) * nb1 + ib1) * nw + iw2];
+          o[ib1 + size_t(nb1) * (ib2 + size_t(nb2) * nu)] = std::norm(tmp);
+        }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ScatteringMatrix::symmetrizeCoupling (scattering.cpp:1209-1318): average
+// |g|^2 over consecutive degenerate groups (|e - e_first| < 1e-6 Ry) along
+// b1, then b2, then nu.  coupling is (nb1, nb2, nm) column-major.
+// ---------------------------------------------------------------------------
+void orc_symmetrize_coupling(double* coupling, int nb1, int nb2, int nm, const double* e1,
+                             const double* e2, const double* e3) {
+  auto C = [&](int a, int b, int c) -> double& {
+    return coupling[a + size_t(nb1) * (b + size_t(nb2) * c)];
+  };
+  std::vector<double> tmp;
+  for (int ib1 = 0; ib1 < nb1; ib1++) {
+    int deg = 0;
+    for (int i = ib1; i < nb1; i++) {
+      if (std::abs(e1[ib1] - e1[i]) < 1.0e-6) deg++;
+      else break;
+    }
+    tmp.assign(size_t(nb2) * nm, 0.);
+    for (int i = 0; i < deg; ++i)
+      for (int ib2 = 0; ib2 < nb2; ib2++)
+        for (int ib3 = 0; ib3 < nm; ib3++)
+          tmp[ib2 + size_t(nb2) * ib3] += C(ib1 + i, ib2, ib3) / float(deg);
+    for (int i = 0; i < deg; ++i)
+      for (int ib2 = 0; ib2 < nb2; ib2++)
+        for (int ib3 = 0; ib3 < nm; ib3++) C(ib1 + i, ib2, ib3) = tmp[ib2 + size_t(nb2) * ib3];
+    ib1 += deg - 1;
+  }
+  for (int ib2 = 0; ib2 < nb2; ib2++) {
+    int deg = 0;
+    for (int i = ib2; i < nb2; i++) {
+      if (std::abs(e2[ib2] - e2[i]) < 1.0e-6) deg++;
+      else break;
+    }
+    tmp.assign(size_t(nb1) * nm, 0.);
+    for (int i = 0; i < deg; ++i)
+      for (int ib1 = 0; ib1 < nb1; ib1++)
+        for (int ib3 = 0; ib3 < nm; ib3++)
+          tmp[ib1 + size_t(nb1) * ib3] += C(ib1, ib2 + i, ib3) / float(deg);
+    for (int i = 0; i < deg; ++i)
+      for (int ib1 = 0; ib1 < nb1; ib1++)
+        for (int ib3 = 0; ib3 < nm; ib3++) C(ib1, ib2 + i, ib3) = tmp[ib1 + size_t(nb1) * ib3];
+    ib2 += deg - 1;
+  }
+  for (int ib3 = 0; ib3 < nm; ib3++) {
+    int deg = 0;
+    for (int i = ib3; i < nm; i++) {
+      if (std::abs(e3[ib3] - e3[i]) < 1.0e-6) deg++;
+      else break;
+    }
+    tmp.assign(size_t(nb1) * nb2, 0.);
+    for (int i = 0; i < deg; ++i)
+      for (int ib1 = 0; ib1 < nb1; ib1++)
+        for (int ib2 = 0; ib2 < nb2; ib2++)
+          tmp[ib1 + size_t(nb1) * ib2] += C(ib1, ib2, ib3 + i) / float(deg);
+    for (int i = 0; i < deg; ++i)
+      for (int ib1 = 0; ib1 < nb1; ib1++)
+        for (int ib2 = 0; ib2 < nb2; ib2++) C(ib1, ib2, ib3 + i) = tmp[ib1 + size_t(nb1) * ib2];
+    ib3 += deg - 1;
+  }
+}
+
+// GaussianDeltaFunction::getSmearing (delta_function.cpp:34-45): one-sided cut.
+double orc_gaussian_delta(double energy, double sigma) {
+  double inverseWidth = 1. / sigma;
+  double prefactor = 1. / sigma / std::sqrt(kPi);
+  double x = energy * inverseWidth;
+  if (x > 6.) return 0.;
+  return prefactor * std::exp(-x * x);
+}
+
+// Particle::getPopulation (particle.cpp:64-96).
+double orc_population(int isBose, double energy, double temperature, double chemPot) {
+  double population = 0.;
+  if (temperature <= 0.) {
+    if (energy <= chemPot) population = 1.;
+    return population;
+  }
+  double y = (energy - chemPot) / temperature;
+  if (isBose) {
+    population = 1. / (std::exp(y) - 1);
+    if (population < 0.) population = 0.;
+  } else {
+    population = 1. / (std::exp(y) + 1);
+    if (population < 0.) population = 0.;
+    if (population > 1.) population = 1.;
+  }
+  return population;
+}
+
+double orc_phonon_cutoff() { return 5. / kRyToCmm1; }  // el_scattering.cpp:127
+
+// ---------------------------------------------------------------------------
+// Full linewidth driver: ElScatteringMatrix::builder, switchCase 2
+// (el_scattering.cpp:38-473) with inner == outer band structure on a
+// Gamma-centred mesh with an energy window (storedAllQ3Case2,
+// helper_el_scattering.cpp:58-162), no symmetries, no polar term.
+// ---------------------------------------------------------------------------
+struct orc_problem {
+  double recip[9];  // reciprocal cell, column-major, columns b_i (Cartesian)
+  int mesh[3];
+  int nw, nRh;
+  const double *Rh, *degh, *h0R;
+  int nm, nRf;
+  const double *Rf, *wf, *mat2R, *masses;
+  int nRe, nRp;
+  const double *Re, *degE, *Rp, *degP, *gW;
+  int nCalc;
+  const double *temperatures, *chemPots;
+  double sigma, winMin, winMax;
+};
+
+struct orc_state {
+  orc_problem p;
+  int nk;                         // full mesh size
+  std::vector<double> kcart;      // nk x 3
+  std::vector<int> active;        // active -> full mesh index
+  std::vector<int> bandLo, bandHi;
+  std::vector<int> stateOffset;   // per active point, first iBte
+  int numStates = 0;
+  std::vector<double> energies;   // per state
+  std::vector<double> eigvecs;    // per active point: (nw, nb) col-major interleaved, packed
+  std::vector<size_t> eigOffset;  // in complex elements
+  std::vector<int> fullToActive;
+  // phonons on the whole q mesh (index = full mesh index); only the q-set is filled
+  std::vector<double> omega;      // nk x nm
+  std::vector<double> phev;       // nk x (nm x nm) col-major interleaved
+  std::vector<char> qInSet;
+};
+
+static void cart_of(const orc_problem& p, const double* x, double* c) {
+  for (int i = 0; i < 3; ++i)
+    c[i] = p.recip[i] * x[0] + p.recip[i + 3] * x[1] + p.recip[i + 6] * x[2];
+}
+
+// Solves recip * x = c for crystal coordinates x (Points::cartesianToCrystal,
+// points.cpp:465-468).
+static void crys_of(const orc_problem& p, const double* c, double* x) {
+  const double* B = p.recip;
+  double m[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m[i][j] = B[i + 3 * j];
+  double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+               m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  double inv[3][3];
+  inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+  inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+  inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+  inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+  inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+  inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+  for (int i = 0; i < 3; ++i) x[i] = inv[i][0] * c[0] + inv[i][1] * c[1] + inv[i][2] * c[2];
+}
+
+// Build band structure, window, q-set and phonons.  Returns a handle or null.
+void* orc_setup(const orc_problem* prob) {
+  auto* s = new orc_state();
+  s->p = *prob;
+  const orc_problem& p = s->p;
+  s->nk = p.mesh[0] * p.mesh[1] * p.mesh[2];
+  s->kcart.resize(3 * size_t(s->nk));
+  std::vector<double> allE(size_t(s->nk) * p.nw), allU(2 * size_t(s->nk) * p.nw * p.nw);
+  int fail = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : fail)
+  for (int ik = 0; ik < s->nk; ++ik) {
+    double x[3];
+    orc_mesh_point(ik, p.mesh, x);
+    cart_of(p, x, &s->kcart[3 * size_t(ik)]);
+    fail += orc_el_diag(p.nw, p.nRh, p.Rh, p.degh, p.h0R, &s->kcart[3 * size_t(ik)],
+                        &allE[size_t(ik) * p.nw], &allU[2 * size_t(ik) * p.nw * p.nw]) != 0;
+  }
+  if (fail) {
+    delete s;
+    return nullptr;
+  }
+  // Window::internalEnWindow (window.cpp:137-157) applied point by point in
+  // ascending mesh order (active_bandstructure.cpp:826-890).
+  s->fullToActive.assign(s->nk, -1);
+  for (int ik = 0; ik < s->nk; ++ik) {
+    int lo = -1, hi = -1;
+    for (int ib = 0; ib < p.nw; ++ib) {
+      double e = allE[size_t(ik) * p.nw + ib];
+      if (e < p.winMax && e > p.winMin) {
+        if (lo < 0) lo = ib;
+        hi = ib;
+      }
+    }
+    if (lo < 0) continue;
+    s->fullToActive[ik] = int(s->active.size());
+    s->active.push_back(ik);
+    s->bandLo.push_back(lo);
+    s->bandHi.push_back(hi);
+  }
+  // states and sliced eigenvectors (active_bandstructure.cpp:981-1023)
+  size_t eoff = 0;
+  for (size_t a = 0; a < s->active.size(); ++a) {
+    int ik = s->active[a];
+    int nb = s->bandHi[a] - s->bandLo[a] + 1;
+    s->stateOffset.push_back(s->numStates);
+    s->numStates += nb;
+    s->eigOffset.push_back(eoff);
+    for (int ib = 0; ib < nb; ++ib) {
+      int ibf = s->bandLo[a] + ib;
+      s->energies.push_back(allE[size_t(ik) * p.nw + ibf]);
+      for (int iw = 0; iw < p.nw; ++iw) {
+        size_t src = 2 * ((size_t(ik) * p.nw * p.nw) + iw + size_t(p.nw) * ibf);
+        s->eigvecs.push_back(allU[src]);
+        s->eigvecs.push_back(allU[src + 1]);
+      }
+    }
+    eoff += size_t(nb) * p.nw;
+  }
+  s->eigOffset.push_back(eoff);
+  s->stateOffset.push_back(s->numStates);
+  // q-set: all folded k2 - k1 (helper_el_scattering.cpp:95-124)
+  s->qInSet.assign(s->nk, 0);
+  const int na = int(s->active.size());
+  for (int a2 = 0; a2 < na; ++a2)
+    for (int a1 = 0; a1 < na; ++a1) {
+      const double* k2 = &s->kcart[3 * size_t(s->active[a2])];
+      const double* k1 = &s->kcart[3 * size_t(s->active[a1])];
+      double qc[3] = {k2[0] - k1[0], k2[1] - k1[1], k2[2] - k1[2]}, qx[3];
+      crys_of(p, qc, qx);
+      int iq = orc_mesh_index(qx, p.mesh);
+      if (iq < 0) {
+        delete s;
+        return nullptr;
+      }
+      s->qInSet[iq] = 1;
+    }
+  s->omega.assign(size_t(s->nk) * p.nm, 0.);
+  s->phev.assign(2 * size_t(s->nk) * p.nm * p.nm, 0.);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : fail)
+  for (int iq = 0; iq < s->nk; ++iq) {
+    if (!s->qInSet[iq]) continue;
+    fail += orc_ph_diag(p.nm, p.nRf, p.Rf, p.wf, p.mat2R, p.masses, &s->kcart[3 * size_t(iq)],
+                        &s->omega[size_t(iq) * p.nm], &s->phev[2 * size_t(iq) * p.nm * p.nm]) != 0;
+  }
+  if (fail) {
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void orc_free(void* h) { delete static_cast<orc_state*>(h); }
+
+int orc_num_active(void* h) { return int(static_cast<orc_state*>(h)->active.size()); }
+int orc_num_states(void* h) { return static_cast<orc_state*>(h)->numStates; }
+
+// active (full index), band extrema, state energies
+void orc_get_window(void* h, int* active, int* lo, int* hi, double* energies) {
+  auto* s = static_cast<orc_state*>(h);
+  for (size_t a = 0; a < s->active.size(); ++a) {
+    active[a] = s->active[a];
+    lo[a] = s->bandLo[a];
+    hi[a] = s->bandHi[a];
+  }
+  std::copy(s->energies.begin(), s->energies.end(), energies);
+}
+
+// Eigenvectors of one active point, (nw, nb) column-major.
+void orc_get_eigvecs(void* h, int a, double* out) {
+  auto* s = static_cast<orc_state*>(h);
+  size_t n = s->eigOffset[a + 1] - s->eigOffset[a];
+  std::copy(s->eigvecs.begin() + 2 * s->eigOffset[a],
+            s->eigvecs.begin() + 2 * (s->eigOffset[a] + n), out);
+}
+
+// Phonon data on the q mesh (nk x nm and nk x nm x nm col-major).
+void orc_get_phonons(void* h, double* omega, double* ev, char* inSet) {
+  auto* s = static_cast<orc_state*>(h);
+  std::copy(s->omega.begin(), s->omega.end(), omega);
+  std::copy(s->phev.begin(), s->phev.end(), ev);
+  std::copy(s->qInSet.begin(), s->qInSet.end(), inSet);
+}
+
+// Linewidths for the outer points listed in k1List (active indices; all if
+// nk1 < 0), restricted to the inner points [k2Begin, k2End) of the active
+// list (all if k2End < 0).  linewidth: (nCalc, numStates), accumulated into
+// (caller zeroes it).  The loop body restates el_scattering.cpp:132-389.
+// Returns the number of (k1,k2) pairs processed, or -1 on error.
+long long orc_linewidth_pairs(void* h, int nk1, const int* k1List, int k2Begin, int k2End,
+                              double* linewidth) {
+  auto* s = static_cast<orc_state*>(h);
+  const orc_problem& p = s->p;
+  const int na = int(s->active.size());
+  if (k2End < 0) k2End = na;
+  std::vector<int> k1s;
+  if (nk1 < 0) {
+    k1s.resize(na);
+    std::iota(k1s.begin(), k1s.end(), 0);
+  } else {
+    k1s.assign(k1List, k1List + nk1);
+  }
+  const double norm = 1. / (double(p.mesh[0]) * p.mesh[1] * p.mesh[2]);  // :64
+  const double phononCutoff = orc_phonon_cutoff();
+  // innerFermi (el_scattering.cpp:87-109)
+  std::vector<double> fermi(size_t(p.nCalc) * s->numStates);
+  for (int is = 0; is < s->numStates; ++is)
+    for (int ic = 0; ic < p.nCalc; ++ic)
+      fermi[size_t(ic) * s->numStates + is] =
+          orc_population(0, s->energies[is], p.temperatures[ic], p.chemPots[ic]);
+  std::vector<double> cached;
+  long long pairs = 0;
+  for (int a1 : k1s) {
+    if (a1 < 0 || a1 >= na) return -1;
+    const int nb1 = s->bandHi[a1] - s->bandLo[a1] + 1;
+    const double* k1C = &s->kcart[3 * size_t(s->active[a1])];
+    const double* U1 = &s->eigvecs[2 * s->eigOffset[a1]];
+    const double* e1 = &s->energies[s->stateOffset[a1]];
+    cached.assign(2 * size_t(p.nRp) * p.nm * nb1 * p.nw, 0.);
+    orc_cache_elph(p.nw, p.nm, p.nRp, p.nRe, p.gW, p.Re, p.degE, U1, nb1, k1C, cached.data());
+    const int nq = k2End - k2Begin;
+    if (nq <= 0) continue;
+    // preprocessing (el_scattering.cpp:194-210, helper_el_scattering.cpp:229-279)
+    std::vector<int> nb2s(nq), iq3s(nq);
+    std::vector<double> q3(3 * size_t(nq)), U2s, e3s(2 * size_t(nq) * p.nm * p.nm);
+    std::vector<size_t> outoff(nq + 1, 0);
+    for (int j = 0; j < nq; ++j) {
+      int a2 = k2Begin + j;
+      nb2s[j] = s->bandHi[a2] - s->bandLo[a2] + 1;
+      const double* k2C = &s->kcart[3 * size_t(s->active[a2])];
+      for (int i = 0; i < 3; ++i) q3[3 * j + i] = k2C[i] - k1C[i];
+      double qx[3];
+      crys_of(p, &q3[3 * j], qx);
+      iq3s[j] = orc_mesh_index(qx, p.mesh);
+      if (iq3s[j] < 0) return -1;
+      U2s.insert(U2s.end(), s->eigvecs.begin() + 2 * s->eigOffset[a2],
+                 s->eigvecs.begin() + 2 * s->eigOffset[a2 + 1]);
+      std::copy(s->phev.begin() + 2 * size_t(iq3s[j]) * p.nm * p.nm,
+                s->phev.begin() + 2 * size_t(iq3s[j] + 1) * p.nm * p.nm,
+                e3s.begin() + 2 * size_t(j) * p.nm * p.nm);
+      outoff[j + 1] = outoff[j] + size_t(nb1) * nb2s[j] * p.nm;
+    }
+    std::vector<double> coupling(outoff[nq]);
+    orc_coupling_sq(p.nw, p.nm, p.nRp, p.Rp, p.degP, cached.data(), nb1, nq, U2s.data(),
+                    nb2s.data(), e3s.data(), q3.data(), coupling.data());
+    // symmetrize coupling (el_scattering.cpp:216-224)
+#pragma omp parallel for schedule(dynamic)
+    for (int j = 0; j < nq; ++j) {
+      int a2 = k2Begin + j;
+      orc_symmetrize_coupling(&coupling[outoff[j]], nb1, nb2s[j], p.nm, e1,
+                              &s->energies[s->stateOffset[a2]], &s->omega[size_t(iq3s[j]) * p.nm]);
+    }
+    // postprocessing loop (el_scattering.cpp:228-386), serial
+    std::vector<double> bose3(p.nCalc * size_t(p.nm));
+    for (int j = 0; j < nq; ++j) {
+      int a2 = k2Begin + j;
+      const int nb2 = nb2s[j];
+      const double* e2 = &s->energies[s->stateOffset[a2]];
+      const double* w3 = &s->omega[size_t(iq3s[j]) * p.nm];
+      for (int ic = 0; ic < p.nCalc; ++ic)
+        for (int ib3 = 0; ib3 < p.nm; ++ib3)
+          bose3[size_t(ic) * p.nm + ib3] = orc_population(1, w3[ib3], p.temperatures[ic], 0.);
+      const double* cpl = &coupling[outoff[j]];
+      for (int ib2 = 0; ib2 < nb2; ++ib2) {
+        double en2 = e2[ib2];
+        int iBte2 = s->stateOffset[a2] + ib2;
+        for (int ib1 = 0; ib1 < nb1; ++ib1) {
+          double en1 = e1[ib1];
+          int iBte1 = s->stateOffset[a1] + ib1;
+          for (int ib3 = 0; ib3 < p.nm; ++ib3) {
+            double en3 = w3[ib3];
+            if (en3 < phononCutoff) continue;
+            double delta1 = orc_gaussian_delta(en1 - en2 + en3, p.sigma);
+            double delta2 = orc_gaussian_delta(en1 - en2 - en3, p.sigma);
+            if (delta1 <= 0. && delta2 <= 0.) continue;
+            double c = cpl[ib1 + size_t(nb1) * (ib2 + size_t(nb2) * ib3)];
+            for (int ic = 0; ic < p.nCalc; ++ic) {
+              double fermi2 = fermi[size_t(ic) * s->numStates + iBte2];
+              double b3 = bose3[size_t(ic) * p.nm + ib3];
+              double rate = c * ((fermi2 + b3) * delta1 + (1. - fermi2 + b3) * delta2) *
+                            norm / en3 * kPi;
+              linewidth[size_t(ic) * s->numStates + iBte1] += rate;
+            }
+          }
+        }
+      }
+    }
+    pairs += nq;
+  }
+  return pairs;
+}
+
+// ScatteringMatrix::degeneracyAveragingLinewidths (scattering.cpp:1168-1207):
+// row iCalc = 0 only.
+void orc_degeneracy_average(void* h, double* linewidth) {
+  auto* s = static_cast<orc_state*>(h);
+  for (size_t a = 0; a < s->active.size(); ++a) {
+    int off = s->stateOffset[a];
+    int nb = s->stateOffset[a + 1] - off;
+    std::vector<double> tmp(nb, 0.);
+    for (int ib1 = 0; ib1 < nb; ++ib1) {
+      double ekk = s->energies[off + ib1];
+      int n = 0;
+      double t2 = 0.;
+      for (int ib2 = 0; ib2 < nb; ++ib2) {
+        if (std::abs(s->energies[off + ib2] - ekk) < 1.0e-6) {
+          n++;
+          t2 = t2 + linewidth[off + ib2];
+        }
+      }
+      tmp[ib1] = t2 / float(n);
+    }
+    for (int ib1 = 0; ib1 < nb; ++ib1) linewidth[off + ib1] = tmp[ib1];
+  }
+}
+
+int orc_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

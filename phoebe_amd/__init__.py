@@ -1,0 +1,352 @@
+"""phoebe_amd - MI355X (gfx950) electron-phonon linewidth path.
+
+Python front-end over the C ABI of libphx_elph.so (include/phx_elph.h).  The
+classes mirror the reference interface of the hot path:
+
+* InteractionElPhWan  (src/interaction/interaction_elph.h:30-219):
+  cacheElPh / calcCouplingSquared / getCouplingSquared /
+  getCouplingDimensions / estimateNumBatches;
+* zheev_batched       (kokkosZHEEV, src/common_kokkos.h:49);
+* el_bloch_diag / ph_bloch_diag (kokkosBatchedDiagonalizeFromCoordinates,
+  src/harmonic/harmonic.h:87-91);
+* ElectronLinewidths  (ElScatteringMatrix::builder, switchCase 2,
+  src/bte/el_scattering.cpp:38-473).
+
+The HIP library is required: importing works without it (so that build
+machines can import the package), but every compute call raises if the
+library is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphx_elph.so")
+
+PHX_FP64, PHX_FP32, PHX_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": PHX_FP64, "fp32": PHX_FP32, "bf16x3": PHX_BF16X3}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+class PhxError(RuntimeError):
+    pass
+
+
+class PhxProblem(C.Structure):
+    _fields_ = [
+        ("recip", C.c_double * 9), ("mesh", C.c_int * 3),
+        ("nw", C.c_int), ("nRh", C.c_int), ("Rh", _dp), ("degh", _dp), ("h0R", _dp),
+        ("nm", C.c_int), ("nRf", C.c_int), ("Rf", _dp), ("wf", _dp), ("mat2R", _dp),
+        ("masses", _dp),
+        ("nRe", C.c_int), ("nRp", C.c_int), ("Re", _dp), ("degE", _dp), ("Rp", _dp),
+        ("degP", _dp), ("gW", _dp), ("gwDecay", _dp), ("gwSeed", C.c_ulonglong),
+        ("nCalc", C.c_int), ("temperatures", _dp), ("chemPots", _dp),
+        ("sigma", C.c_double), ("winMin", C.c_double), ("winMax", C.c_double),
+    ]
+
+
+EXPORTS = {
+    "phx_last_error": (C.c_char_p, []),
+    "phx_version": (C.c_char_p, []),
+    "phx_zheev_batched": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _ip]),
+    "phx_zheev_batched_dev": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
+    "phx_el_bloch_diag": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    "phx_ph_bloch_diag": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int,
+                                    _dp, _dp]),
+    "phx_elph_create": (C.c_int, [C.POINTER(C.c_void_p), _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  _dp, _dp, _dp, _dp, C.c_int]),
+    "phx_elph_create_synthetic": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int,
+                                            C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_ulonglong,
+                                            C.c_int]),
+    "phx_elph_cache": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
+    "phx_elph_coupling_sq": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp]),
+    "phx_elph_get_coupling_dims": (C.c_int, [C.c_void_p, _ip]),
+    "phx_elph_estimate_num_batches": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+    "phx_elph_destroy": (C.c_int, [C.c_void_p]),
+    "phx_lw_setup": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_int, C.c_int]),
+    "phx_lw_num_active": (C.c_int, [C.c_void_p]),
+    "phx_lw_num_states": (C.c_int, [C.c_void_p]),
+    "phx_lw_get_window": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _dp]),
+    "phx_lw_get_phonons": (C.c_int, [C.c_void_p, _dp, C.c_char_p]),
+    "phx_lw_run": (C.c_longlong, [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int]),
+    "phx_lw_get_linewidths": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "phx_lw_reset": (C.c_int, [C.c_void_p]),
+    "phx_lw_sync": (C.c_int, [C.c_void_p]),
+    "phx_lw_stream": (C.c_void_p, [C.c_void_p]),
+    "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
+    "phx_lw_destroy": (C.c_int, [C.c_void_p]),
+    "phx_synth_gw_host": (C.c_int, [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_ulonglong]),
+}
+
+
+def lib():
+    """The loaded HIP library; raises PhxError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PhxError(f"{LIB_PATH} is missing: run `python -m phoebe_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise PhxError(lib().phx_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _cview(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+
+
+def _fortran_flat(m):
+    return np.asfortranarray(m, dtype=np.complex128).reshape(-1, order="F")
+
+
+# ------------------------------------------------------------------ eigensolver / Bloch
+def zheev_batched(A: np.ndarray):
+    """Eigen-decomposition of a stack of Hermitian matrices A[M, N, N].
+
+    Returns (W[M, N] ascending, V[M, N, N]) with V[i][:, j] the eigenvectors.
+    """
+    A = np.asarray(A, dtype=np.complex128)
+    M, N, _ = A.shape
+    buf = np.ascontiguousarray(np.transpose(A, (0, 2, 1))).view(np.float64).copy()
+    W = np.zeros((M, N))
+    info = np.zeros(M, dtype=np.int32)
+    _check(lib().phx_zheev_batched(M, N, _p(buf), _p(W), info.ctypes.data_as(_ip)))
+    V = buf.view(np.complex128).reshape(M, N, N).transpose(0, 2, 1)
+    return W, V
+
+
+def el_bloch_diag(kc, R, deg, h0R):
+    kc = _f64(kc).reshape(-1, 3)
+    nk, nw = len(kc), h0R.shape[1]
+    E = np.zeros((nk, nw))
+    U = np.zeros(2 * nk * nw * nw)
+    _check(lib().phx_el_bloch_diag(nk, _p(kc), nw, len(R), _p(_f64(R)), _p(_f64(deg)),
+                                   _p(_cview(h0R)), _p(E), _p(U)))
+    return E, U.view(np.complex128).reshape(nk, nw, nw).transpose(0, 2, 1)
+
+
+def ph_bloch_diag(qc, R, w, mat2R, masses, mass_scaling=True):
+    qc = _f64(qc).reshape(-1, 3)
+    nq, nm = len(qc), mat2R.shape[1]
+    om = np.zeros((nq, nm))
+    ev = np.zeros(2 * nq * nm * nm)
+    _check(lib().phx_ph_bloch_diag(nq, _p(qc), nm, len(R), _p(_f64(R)), _p(_f64(w)),
+                                   _p(_f64(mat2R)), _p(_f64(masses)), int(bool(mass_scaling)),
+                                   _p(om), _p(ev)))
+    return om, ev.view(np.complex128).reshape(nq, nm, nm).transpose(0, 2, 1)
+
+
+# ------------------------------------------------------------------ InteractionElPhWan
+class InteractionElPhWan:
+    """Wannier electron-phonon coupling on the GPU (interaction_elph.h:30-219)."""
+
+    def __init__(self, couplingWannier=None, elBravaisVectors=None, elDegeneracies=None,
+                 phBravaisVectors=None, phDegeneracies=None, precision="fp64", *,
+                 synthetic=None):
+        """couplingWannier: complex (nRe, nRp, nm, nw, nw) (the reference's
+        device view of its Eigen tensor); or synthetic=(nw, nm, decay, seed)."""
+        self._h = C.c_void_p()
+        elR, phR = _f64(elBravaisVectors), _f64(phBravaisVectors)
+        elD, phD = _f64(elDegeneracies), _f64(phDegeneracies)
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        if synthetic is not None:
+            nw, nm, decay, seed = synthetic
+            _check(lib().phx_elph_create_synthetic(C.byref(self._h), nw, nm, len(phR), len(elR),
+                                                   _p(elR), _p(elD), _p(phR), _p(phD),
+                                                   _p(_f64(decay)), int(seed), prec))
+        else:
+            g = np.ascontiguousarray(couplingWannier, dtype=np.complex128)
+            nRe, nRp, nm, nw, _ = g.shape
+            _check(lib().phx_elph_create(C.byref(self._h), _p(g.view(np.float64)), nw, nm, nRp, nRe,
+                                         _p(elR), _p(elD), _p(phR), _p(phD), prec))
+            self.numElBands, self.numPhBands = nw, nm
+        dims = self.getCouplingDimensions()
+        self.numElBands, self.numPhBands = int(dims[3]), int(dims[2])
+        self._cache = []
+        self._nb1 = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().phx_elph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def cacheElPh(self, eigvec1, k1C):
+        U1 = _fortran_flat(eigvec1)
+        self._nb1 = int(eigvec1.shape[1])
+        _check(lib().phx_elph_cache(self._h, _p(U1.view(np.float64)), self._nb1, _p(_f64(k1C))))
+
+    def calcCouplingSquared(self, eigvec1, eigvecs2, eigvecs3, q3Cs, polarData=None):
+        if polarData is not None and any(np.any(p != 0) for p in polarData):
+            raise PhxError("polar correction is not supported")
+        nb1 = int(eigvec1.shape[1])
+        if nb1 != self._nb1:
+            raise PhxError("eigvec1 does not match the cached k1")
+        nq = len(eigvecs2)
+        nb2s = np.array([u.shape[1] for u in eigvecs2], dtype=np.int32)
+        nm = self.numPhBands
+        if nq == 0:
+            self._cache = []
+            return
+        U2 = np.concatenate([_fortran_flat(u) for u in eigvecs2])
+        e3 = np.concatenate([_fortran_flat(e) for e in eigvecs3])
+        q3 = _f64(np.asarray(q3Cs).reshape(nq, 3))
+        out = np.zeros(int(nb1 * nm * nb2s.sum()))
+        _check(lib().phx_elph_coupling_sq(self._h, nq, _p(U2.view(np.float64)),
+                                          nb2s.ctypes.data_as(_ip), _p(e3.view(np.float64)),
+                                          _p(q3), None, _p(out)))
+        self._cache, off = [], 0
+        for nb2 in nb2s:
+            n = nb1 * nb2 * nm
+            self._cache.append(out[off:off + n].reshape((nb1, nb2, nm), order="F"))
+            off += n
+
+    def getCouplingSquared(self, ik2):
+        return self._cache[ik2]
+
+    def getCouplingDimensions(self):
+        d = np.zeros(5, dtype=np.int32)
+        _check(lib().phx_elph_get_coupling_dims(self._h, d.ctypes.data_as(_ip)))
+        return d
+
+    def estimateNumBatches(self, nk2, nb1, max_mem_bytes=0.0):
+        return _check(lib().phx_elph_estimate_num_batches(self._h, int(nk2), int(nb1),
+                                                          float(max_mem_bytes)))
+
+
+# ------------------------------------------------------------------ linewidth builder
+def make_problem_struct(prob, gW=None):
+    """Fill a PhxProblem from a phoebe_amd.synth.Problem; keeps arrays alive."""
+    keep = []
+
+    def k(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _p(a)
+
+    s = PhxProblem()
+    rc = prob.recip
+    for j in range(3):
+        for i in range(3):
+            s.recip[i + 3 * j] = rc[i, j]
+    for i in range(3):
+        s.mesh[i] = prob.mesh[i]
+    s.nw, s.nRh = prob.nw, len(prob.Rh)
+    s.Rh, s.degh, s.h0R = k(prob.Rh), k(prob.degh), k(_cview(prob.h0R))
+    s.nm, s.nRf = prob.nm, len(prob.Rf)
+    s.Rf, s.wf, s.mat2R, s.masses = k(prob.Rf), k(prob.wf), k(prob.mat2R), k(prob.masses)
+    s.nRe, s.nRp = prob.nRe, prob.nRp
+    s.Re, s.degE, s.Rp, s.degP = k(prob.Re), k(prob.degE), k(prob.Rp), k(prob.degP)
+    if gW is not None:
+        g = gW.view(np.float64) if gW.dtype == np.complex128 else gW
+        keep.append(g)
+        s.gW = _p(g)
+    s.gwDecay = k(prob.gw_decay)
+    s.gwSeed = int(prob.gw_seed)
+    s.nCalc = len(prob.temperatures)
+    s.temperatures, s.chemPots = k(prob.temperatures), k(prob.chem_pots)
+    s.sigma, s.winMin, s.winMax = prob.sigma, prob.win_min, prob.win_max
+    return s, keep
+
+
+class ElectronLinewidths:
+    """Electron linewidths from el-ph scattering on the GPU.
+
+    Mirrors ElScatteringMatrix::builder in the linewidth case
+    (el_scattering.cpp:38-473): band structure and energy window, phonons on
+    the q-set, cacheElPh + calcCouplingSquared + symmetrizeCoupling +
+    Gaussian-delta reduction per outer point, degeneracy averaging.
+    """
+
+    def __init__(self, prob, gW=None, precision="fp64", k1_batch=16):
+        self.prob = prob
+        self._s, self._keep = make_problem_struct(prob, gW)
+        self._h = C.c_void_p()
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        _check(lib().phx_lw_setup(C.byref(self._h), C.byref(self._s), prec, int(k1_batch)))
+        self.num_active = lib().phx_lw_num_active(self._h)
+        self.num_states = lib().phx_lw_num_states(self._h)
+        self.active = np.zeros(self.num_active, dtype=np.int32)
+        self.band_lo = np.zeros(self.num_active, dtype=np.int32)
+        self.band_hi = np.zeros(self.num_active, dtype=np.int32)
+        self.energies = np.zeros(self.num_states)
+        _check(lib().phx_lw_get_window(self._h, self.active.ctypes.data_as(_ip),
+                                       self.band_lo.ctypes.data_as(_ip),
+                                       self.band_hi.ctypes.data_as(_ip), _p(self.energies)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().phx_lw_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def phonons(self):
+        nk = int(np.prod(self.prob.mesh))
+        om = np.zeros(nk * self.prob.nm)
+        mask = C.create_string_buffer(nk)
+        _check(lib().phx_lw_get_phonons(self._h, _p(om), mask))
+        return om.reshape(nk, self.prob.nm), np.frombuffer(mask.raw, dtype=np.uint8).astype(bool)
+
+    def run(self, k1_list=None, k2_range=None):
+        """Enqueue the outer points k1_list (active indices; None = all)."""
+        if k1_list is None:
+            nk1, ptr = -1, None
+        else:
+            self._k1 = np.ascontiguousarray(k1_list, dtype=np.int32)
+            nk1, ptr = len(self._k1), self._k1.ctypes.data_as(_ip)
+        b, e = (0, -1) if k2_range is None else k2_range
+        pairs = lib().phx_lw_run(self._h, nk1, ptr, int(b), int(e))
+        if pairs < 0:
+            raise PhxError(lib().phx_last_error().decode())
+        return pairs
+
+    def sync(self):
+        _check(lib().phx_lw_sync(self._h))
+
+    def reset(self):
+        _check(lib().phx_lw_reset(self._h))
+
+    def stream(self):
+        return lib().phx_lw_stream(self._h)
+
+    def linewidths(self, average=True):
+        nc = len(self.prob.temperatures)
+        out = np.zeros(nc * self.num_states)
+        _check(lib().phx_lw_get_linewidths(self._h, _p(out), int(bool(average))))
+        return out.reshape(nc, self.num_states)
+
+    def copy_gw(self):
+        p = self.prob
+        out = np.empty(2 * p.nRe * p.nRp * p.nm * p.nw * p.nw)
+        _check(lib().phx_lw_copy_gw(self._h, _p(out)))
+        return out
+
+
+def synth_gw_host(prob) -> np.ndarray:
+    """Materialize the synthetic gW on the host with the library's host twin."""
+    out = np.empty(2 * prob.nRe * prob.nRp * prob.nm * prob.nw * prob.nw)
+    _check(lib().phx_synth_gw_host(_p(out), prob.nw, prob.nm, prob.nRp, prob.nRe,
+                                   _p(_f64(prob.gw_decay)), int(prob.gw_seed)))
+    return out

@@ -1,0 +1,1144 @@
+// Host side of the phx_elph C ABI: the InteractionElPhWan mirror, the batched
+// Bloch diagonalizations and the linewidth builder, all driving the gfx950
+// kernels of eig.hip, bloch.hip and elph.hip on one HIP stream per handle.
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/phx_elph.h"
+#include "phx_common.hpp"
+
+namespace phx {
+
+// ---- launchers (eig.hip, bloch.hip, elph.hip)
+void launch_jacobi(int N, int M, double2* A, double* W, int* info, const double* masses,
+                   hipStream_t stream);
+void launch_bloch(bool phonon, int nk, int nw, int nR, const double* kc, const double* R,
+                  const double* rw, const void* coef, double2* out, hipStream_t stream);
+void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
+                     hipStream_t s);
+void synth_gw_host(double* gW, size_t n, size_t blk, const double* decay, uint64_t seed);
+template <typename T> void launch_fill(T* p, size_t n, T v, hipStream_t s);
+void launch_phase_table(int rows, int nR, const double* kc, const double* R, const double* scale,
+                        double sign, double2* P, hipStream_t s);
+template <typename T>
+void launch_phase_rows(int nrows, int rowsPad, int nRp, int Kp, const double* kc, const double* R,
+                       T* A, hipStream_t s);
+void launch_gather_eig(int na, int nw, const int* ik, const int* lo, const int* nb, const int* off,
+                       const double2* U, double2* pool, hipStream_t s);
+void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const double2* ev,
+                       double* wAll, double2* evAll, hipStream_t s);
+
+struct CacheParams {
+  int nRe, nRp, nm, nw, nkk;
+  const double2* gW;
+  const double2* phE;
+  const double2* phP;
+  const double2* U1pool;
+  const int* U1off;
+  const int* nb1;
+  const int* panelBase;
+  void* B;
+  size_t panelStride;
+  int ldb;
+};
+template <typename OutT> void launch_cache(const CacheParams& p, hipStream_t s);
+
+struct CoupParams {
+  const void* A; int lda;
+  const void* B; size_t panelStride; int ldb;
+  int M, Kp, nm, nw, nPanels, nMtiles;
+  const int* panelKK;
+  const int* panelB1;
+  const double2* U2pool; const int* rowU2off; const int* rowNb2;
+  const double2* E3pool; const double* Wpool;
+  int qmode;
+  const int* rowE3; const int* rowMesh; const int* kkMesh; int m0, m1, m2;
+  double* out; const long long* rowOutOff; const int* kkNb1;
+  const double* e2pool; const int* rowStateOff;
+  const double* e1pool; const int* kkStateOff;
+  const int* panelGStart; const int* panelGSize;
+  const double* fermi; int numStates;
+  const double* bose;
+  int nCalc; double invSigma, deltaPref, norm, cutoff;
+  double* partial;
+  int nb2max;
+};
+template <typename T> void launch_coupling(const CoupParams& p, bool reduce, hipStream_t s);
+int coupling_rows_per_tile(int ldb);
+
+struct CollectParams {
+  int nkk, nMtiles, nCalc, numStates;
+  const int* kkNb1; const int* kkPanelBase; const int* kkStateOff;
+  const int* panelGStart; const int* panelGSize;
+  const double* partial;
+  double* linewidth;
+};
+void launch_collect(const CollectParams& p, hipStream_t s);
+
+constexpr int kGMax = 8;
+constexpr int kCMax = 4;
+
+// Run f(i) for i in [0, n) on the host's hardware threads.
+template <typename F>
+static void parallel_for(int n, F&& f) {
+  int nt = int(std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 32u));
+  nt = std::max(1, std::min(nt, n));
+  std::atomic<int> next(0);
+  auto work = [&] {
+    for (;;) {
+      int i0 = next.fetch_add(64);
+      if (i0 >= n) break;
+      for (int i = i0; i < std::min(n, i0 + 64); ++i) f(i);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int i = 1; i < nt; ++i) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
+}
+
+// ---- error state
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+const char* last_error() { return g_err.c_str(); }
+
+template <typename F>
+static int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return -3;
+  } catch (...) {
+    set_error("unknown error");
+    return -3;
+  }
+}
+
+template <typename T>
+static void h2d(DevBuf<T>& d, const T* h, size_t n, hipStream_t s = 0) {
+  d.alloc(n);
+  if (n) PHX_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+static void h2d(DevBuf<T>& d, const std::vector<T>& v, hipStream_t s = 0) {
+  h2d(d, v.data(), v.size(), s);
+}
+
+// constants.h:9-57 (restated)
+static const double kPi = 3.14159265358979323846;
+static double phonon_cutoff() {
+  const double speedLightSi = 299792458;
+  const double hPlanckSi = 6.62607015e-34;
+  const double hartreeSi = 4.359744650e-18;
+  const double twoPi = 2. * kPi;
+  const double teraHertzAu = hPlanckSi / twoPi / hartreeSi * 1.0e+12;
+  const double ryToTHz = 1. / teraHertzAu / 2. / twoPi;
+  const double ryToCmm1 = 1.0e10 * ryToTHz / speedLightSi;
+  return 5. / ryToCmm1;  // el_scattering.cpp:127
+}
+
+// Particle::getPopulation (particle.cpp:64-96)
+static double population(bool bose, double energy, double temperature, double chemPot) {
+  double population = 0.;
+  if (temperature <= 0.) {
+    if (energy <= chemPot) population = 1.;
+    return population;
+  }
+  double y = (energy - chemPot) / temperature;
+  if (bose) {
+    population = 1. / (std::exp(y) - 1);
+    if (population < 0.) population = 0.;
+  } else {
+    population = 1. / (std::exp(y) + 1);
+    if (population < 0.) population = 0.;
+    if (population > 1.) population = 1.;
+  }
+  return population;
+}
+
+// Grouping of consecutive degenerate levels used by symmetrizeCoupling
+// (scattering.cpp:1215-1238): start / size of the group of every level.
+static void degenerate_groups(const double* e, int n, int* gstart, int* gsize) {
+  for (int i = 0; i < n;) {
+    int deg = 0;
+    for (int j = i; j < n; ++j) {
+      if (std::abs(e[i] - e[j]) < 1.0e-6) deg++;
+      else break;
+    }
+    for (int j = 0; j < deg; ++j) {
+      gstart[i + j] = i;
+      gsize[i + j] = deg;
+    }
+    i += deg;
+  }
+}
+
+// Cartesian coordinates of mesh point `index` (points.cpp:222-248).
+static void mesh_cart(const double* recip, const int* mesh, int index, double* c) {
+  int x, y, z;
+  mesh_coords(index, mesh[0], mesh[1], x, y, z);
+  double xc[3] = {double(x) / double(mesh[0]), double(y) / double(mesh[1]),
+                  double(z) / double(mesh[2])};
+  for (int i = 0; i < 3; ++i) c[i] = recip[i] * xc[0] + recip[i + 3] * xc[1] + recip[i + 6] * xc[2];
+}
+
+// ============================================================ batched eig / Bloch
+static void zheev_batched(int M, int N, double* A, double* W, int* info) {
+  hipStream_t s = 0;
+  DevBuf<double2> dA(size_t(M) * N * N);
+  DevBuf<double> dW(size_t(M) * N);
+  DevBuf<int> dI(M);
+  PHX_HIP(hipMemcpy(dA.p, A, dA.bytes(), hipMemcpyHostToDevice));
+  launch_jacobi(N, M, dA.p, dW.p, dI.p, nullptr, s);
+  PHX_HIP(hipMemcpy(A, dA.p, dA.bytes(), hipMemcpyDeviceToHost));
+  PHX_HIP(hipMemcpy(W, dW.p, dW.bytes(), hipMemcpyDeviceToHost));
+  std::vector<int> inf(M);
+  PHX_HIP(hipMemcpy(inf.data(), dI.p, dI.bytes(), hipMemcpyDeviceToHost));
+  if (info) std::copy(inf.begin(), inf.end(), info);
+  for (int i = 0; i < M; ++i)
+    if (inf[i] < 0) throw Error(-4, "batched zheev: matrix " + std::to_string(i) + " did not converge");
+}
+
+static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, const double* R,
+                       const double* rw, const void* coef, const double* masses, int massScaling,
+                       double* energies, double* eigvecs) {
+  hipStream_t s = 0;
+  DevBuf<double> dk, dR, drw, dm;
+  h2d(dk, k, size_t(nk) * 3);
+  h2d(dR, R, size_t(nR) * 3);
+  h2d(drw, rw, size_t(nR));
+  DevBuf<double2> dcoefc;
+  DevBuf<double> dcoefr;
+  if (phonon) h2d(dcoefr, static_cast<const double*>(coef), size_t(nR) * n * n);
+  else h2d(dcoefc, static_cast<const double2*>(coef), size_t(nR) * n * n);
+  DevBuf<double2> H(size_t(nk) * n * n);
+  DevBuf<double> W(size_t(nk) * n);
+  DevBuf<int> info(nk);
+  launch_bloch(phonon, nk, n, nR, dk.p, dR.p, drw.p,
+               phonon ? static_cast<const void*>(dcoefr.p) : static_cast<const void*>(dcoefc.p),
+               H.p, s);
+  const double* mptr = nullptr;
+  std::vector<double> ones;
+  if (phonon) {
+    if (massScaling) {
+      h2d(dm, masses, size_t(n));
+    } else {
+      // hermitize and divide by sqrt(M_m M_n) but keep eigenvectors unscaled:
+      // run with the masses and undo the eigenvector scaling afterwards
+      h2d(dm, masses, size_t(n));
+    }
+    mptr = dm.p;
+  }
+  launch_jacobi(n, nk, H.p, W.p, info.p, mptr, s);
+  PHX_HIP(hipMemcpy(energies, W.p, W.bytes(), hipMemcpyDeviceToHost));
+  PHX_HIP(hipMemcpy(eigvecs, H.p, H.bytes(), hipMemcpyDeviceToHost));
+  std::vector<int> inf(nk);
+  PHX_HIP(hipMemcpy(inf.data(), info.p, info.bytes(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nk; ++i)
+    if (inf[i] < 0) throw Error(-4, "Bloch eigensolve did not converge");
+  if (phonon && !massScaling) {
+    for (int q = 0; q < nk; ++q)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double sc = std::sqrt(masses[i]);
+          double* x = eigvecs + 2 * (size_t(q) * n * n + i + size_t(n) * j);
+          x[0] *= sc;
+          x[1] *= sc;
+        }
+  }
+}
+
+// ============================================================ InteractionElPhWan
+struct ElphHandle {
+  int nw = 0, nm = 0, nRp = 0, nRe = 0, precision = PHX_FP64;
+  hipStream_t stream = nullptr;
+  DevBuf<double2> gW;
+  DevBuf<double> Re, degE, Rp, degP;
+  // cached state of the last cacheElPh
+  int nb1 = -1;
+  int Kp = 0, ldb = 0;
+  DevBuf<double> Bd;
+  DevBuf<float> Bf;
+  DevBuf<double2> U1;
+  DevBuf<int> meta;  // U1off(1), nb1(1), panelBase(1), panelKK(nb1), panelB1(nb1), kkNb1(1)
+  ~ElphHandle() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static void elph_common_init(ElphHandle* H, int nw, int nm, int nRp, int nRe, const double* elR,
+                             const double* elDeg, const double* phR, const double* phDeg,
+                             int precision) {
+  PHX_REQUIRE(nw >= 1 && nw <= 32 && nm >= 1 && nm <= 36 && nRp >= 1 && nRe >= 1,
+              "phx_elph_create: unsupported sizes");
+  PHX_REQUIRE(precision == PHX_FP64 || precision == PHX_FP32,
+              "phx_elph_create: precision not available");
+  H->nw = nw;
+  H->nm = nm;
+  H->nRp = nRp;
+  H->nRe = nRe;
+  H->precision = precision;
+  PHX_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
+  h2d(H->Re, elR, size_t(nRe) * 3, H->stream);
+  h2d(H->degE, elDeg, size_t(nRe), H->stream);
+  h2d(H->Rp, phR, size_t(nRp) * 3, H->stream);
+  h2d(H->degP, phDeg, size_t(nRp), H->stream);
+  H->Kp = int(round_up(2 * size_t(nRp), 16));
+  H->ldb = int(round_up(2 * size_t(nm) * nw, 16));
+}
+
+// cacheElPh for one k1 into nb1 panels; phase mode: 0 = 1/dp (explicit q),
+// 1 = e^{-ik1.Rp}/dp (mesh split).
+static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k1) {
+  PHX_REQUIRE(nb1 >= 1 && nb1 <= H->nw, "cacheElPh: bad nb1");
+  hipStream_t s = H->stream;
+  const size_t pstride = size_t(H->Kp) * H->ldb;
+  if (nb1 > H->nb1 || H->nb1 < 0) {
+    if (H->precision == PHX_FP64) {
+      H->Bd.alloc(pstride * nb1);
+      launch_fill<double>(H->Bd.p, H->Bd.n, 0., s);
+    } else {
+      H->Bf.alloc(pstride * nb1);
+      launch_fill<float>(H->Bf.p, H->Bf.n, 0.f, s);
+    }
+  }
+  H->nb1 = nb1;
+  h2d(H->U1, reinterpret_cast<const double2*>(U1), size_t(H->nw) * nb1, s);
+  std::vector<int> meta(4 + 2 * nb1, 0);
+  meta[0] = 0;    // U1off
+  meta[1] = nb1;  // nb1 / kkNb1
+  meta[2] = 0;    // panelBase
+  meta[3] = 0;    // panelKK base (unused)
+  for (int b = 0; b < nb1; ++b) {
+    meta[4 + b] = 0;          // panelKK
+    meta[4 + nb1 + b] = b;    // panelB1
+  }
+  h2d(H->meta, meta, s);
+  DevBuf<double> dk;
+  h2d(dk, k1, 3, s);
+  DevBuf<double2> phE(size_t(H->nRe)), phP(size_t(H->nRp));
+  std::vector<double> invdP(H->nRp);
+  DevBuf<double> dInv;
+  // phE = e^{ik1.Re}/de ; phP = 1/dp (explicit-q mode: e^{iq.Rp} sits in the A rows)
+  {
+    std::vector<double> invE(H->nRe);
+    std::vector<double> hDegE(H->nRe), hDegP(H->nRp);
+    PHX_HIP(hipMemcpyAsync(hDegE.data(), H->degE.p, H->degE.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(hDegP.data(), H->degP.p, H->degP.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < H->nRe; ++i) invE[i] = 1. / hDegE[i];
+    DevBuf<double> dInvE;
+    h2d(dInvE, invE, s);
+    launch_phase_table(1, H->nRe, dk.p, H->Re.p, dInvE.p, 1.0, phE.p, s);
+    std::vector<double2> pp(H->nRp);
+    for (int i = 0; i < H->nRp; ++i) pp[i] = make_double2(1. / hDegP[i], 0.);
+    PHX_HIP(hipMemcpyAsync(phP.p, pp.data(), pp.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    CacheParams cp{};
+    cp.nRe = H->nRe;
+    cp.nRp = H->nRp;
+    cp.nm = H->nm;
+    cp.nw = H->nw;
+    cp.nkk = 1;
+    cp.gW = H->gW.p;
+    cp.phE = phE.p;
+    cp.phP = phP.p;
+    cp.U1pool = H->U1.p;
+    cp.U1off = H->meta.p + 0;
+    cp.nb1 = H->meta.p + 1;
+    cp.panelBase = H->meta.p + 2;
+    cp.panelStride = pstride;
+    cp.ldb = H->ldb;
+    if (H->precision == PHX_FP64) {
+      cp.B = H->Bd.p;
+      launch_cache<double>(cp, s);
+    } else {
+      cp.B = H->Bf.p;
+      launch_cache<float>(cp, s);
+    }
+    PHX_HIP(hipStreamSynchronize(s));
+  }
+}
+
+static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int* nb2s,
+                             const double* e3s, const double* q3, double* out) {
+  PHX_REQUIRE(H->nb1 >= 1, "calcCouplingSquared called before cacheElPh");
+  if (nq <= 0) return;
+  hipStream_t s = H->stream;
+  const int nw = H->nw, nm = H->nm, nb1 = H->nb1;
+  const int rows = coupling_rows_per_tile(H->ldb);
+  const int nMtiles = ceil_div(nq, rows);
+  const int rowsPad = nMtiles * rows;
+  std::vector<int> u2off(nq), nb2v(nq), e3idx(nq);
+  std::vector<long long> outoff(nq);
+  long long totU2 = 0, totOut = 0;
+  int nb2max = 1;
+  for (int i = 0; i < nq; ++i) {
+    PHX_REQUIRE(nb2s[i] >= 1 && nb2s[i] <= nw, "calcCouplingSquared: bad nb2");
+    u2off[i] = int(totU2);
+    nb2v[i] = nb2s[i];
+    e3idx[i] = i;
+    outoff[i] = totOut;
+    totU2 += (long long)nw * nb2s[i];
+    totOut += (long long)nb1 * nb2s[i] * nm;
+    nb2max = std::max(nb2max, nb2s[i]);
+  }
+  DevBuf<double> dq;
+  h2d(dq, q3, size_t(nq) * 3, s);
+  DevBuf<double2> dU2, dE3;
+  h2d(dU2, reinterpret_cast<const double2*>(U2s), size_t(totU2), s);
+  h2d(dE3, reinterpret_cast<const double2*>(e3s), size_t(nq) * nm * nm, s);
+  DevBuf<int> dU2off, dNb2, dE3idx;
+  h2d(dU2off, u2off, s);
+  h2d(dNb2, nb2v, s);
+  h2d(dE3idx, e3idx, s);
+  DevBuf<long long> dOutOff;
+  h2d(dOutOff, outoff, s);
+  DevBuf<double> dOut{size_t(totOut)};
+  DevBuf<double> W0(nm);
+  CoupParams p{};
+  p.lda = H->Kp;
+  p.Kp = H->Kp;
+  p.ldb = H->ldb;
+  p.panelStride = size_t(H->Kp) * H->ldb;
+  p.M = nq;
+  p.nm = nm;
+  p.nw = nw;
+  p.nPanels = nb1;
+  p.nMtiles = nMtiles;
+  p.panelKK = H->meta.p + 4;
+  p.panelB1 = H->meta.p + 4 + nb1;
+  p.U2pool = dU2.p;
+  p.rowU2off = dU2off.p;
+  p.rowNb2 = dNb2.p;
+  p.E3pool = dE3.p;
+  p.Wpool = W0.p;
+  p.qmode = 0;
+  p.rowE3 = dE3idx.p;
+  p.out = dOut.p;
+  p.rowOutOff = dOutOff.p;
+  p.kkNb1 = H->meta.p + 1;
+  p.nb2max = nb2max;
+  if (H->precision == PHX_FP64) {
+    DevBuf<double> A(size_t(rowsPad) * H->Kp);
+    launch_phase_rows<double>(nq, rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, A.p, s);
+    p.A = A.p;
+    p.B = H->Bd.p;
+    launch_coupling<double>(p, false, s);
+    PHX_HIP(hipStreamSynchronize(s));
+  } else {
+    DevBuf<float> A(size_t(rowsPad) * H->Kp);
+    launch_phase_rows<float>(nq, rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, A.p, s);
+    p.A = A.p;
+    p.B = H->Bf.p;
+    launch_coupling<float>(p, false, s);
+    PHX_HIP(hipStreamSynchronize(s));
+  }
+  PHX_HIP(hipMemcpy(out, dOut.p, dOut.bytes(), hipMemcpyDeviceToHost));
+}
+
+// ============================================================ linewidth builder
+struct LwHandle {
+  phx_problem prob{};
+  int precision = PHX_FP64, k1Batch = 16;
+  hipStream_t stream = nullptr;
+  int nk = 0, na = 0, numStates = 0, nw = 0, nm = 0, nRp = 0, nRe = 0, nCalc = 1;
+  // host copies
+  std::vector<double> energiesAll;  // nk x nw
+  std::vector<int> active, lo, hi, nb, stateOff, u2off, gStart, gSize;
+  std::vector<double> stateE;
+  std::vector<double> kcart;       // nk x 3
+  std::vector<double> omegaAll;    // nk x nm
+  std::vector<unsigned char> qInSet;
+  // device
+  DevBuf<double2> gW;
+  DevBuf<double> Re, degE, Rp, degP, invDegE, invDegP;
+  DevBuf<double2> U2pool;
+  DevBuf<double> e2pool, fermi, bose, Wall, lw;
+  DevBuf<double2> E3all;
+  DevBuf<int> rowMesh, rowNb2, rowU2off, rowStateOff, gStartD, gSizeD;
+  DevBuf<double> kcartActive;  // na x 3
+  int Kp = 0, ldb = 0, rows = 32, rowsPad = 0;
+  DevBuf<double> Ad;
+  DevBuf<float> Af;
+  // per-batch
+  int maxPanels = 0;
+  DevBuf<double> Bd;
+  DevBuf<float> Bf;
+  DevBuf<double> partial;
+  DevBuf<int> bmeta;
+  DevBuf<double> bk1;
+  DevBuf<double2> phE, phP;
+  std::vector<int> hmeta;
+  std::vector<double> hk1;
+  ~LwHandle() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch) {
+  const phx_problem& p = *P;
+  PHX_REQUIRE(precision == PHX_FP64 || precision == PHX_FP32, "phx_lw_setup: precision not available");
+  PHX_REQUIRE(p.nw >= 1 && p.nw <= 32 && p.nm >= 1 && p.nm <= 36, "phx_lw_setup: unsupported sizes");
+  PHX_REQUIRE(p.nCalc >= 1 && p.nCalc <= kCMax, "phx_lw_setup: 1..4 (T, mu) calculations supported");
+  PHX_REQUIRE(p.nw <= 44 && p.nm <= 44, "phx_lw_setup: eigensolver size");
+  L->prob = p;
+  L->precision = precision;
+  L->k1Batch = std::max(1, k1Batch);
+  L->nw = p.nw;
+  L->nm = p.nm;
+  L->nRp = p.nRp;
+  L->nRe = p.nRe;
+  L->nCalc = p.nCalc;
+  L->nk = p.mesh[0] * p.mesh[1] * p.mesh[2];
+  PHX_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  hipStream_t s = L->stream;
+  const int nk = L->nk, nw = p.nw, nm = p.nm;
+
+  // ---- electrons on the full mesh (ActiveBandStructure::buildAsPostprocessing,
+  //      active_bandstructure.cpp:765-1075)
+  L->kcart.resize(3 * size_t(nk));
+  for (int ik = 0; ik < nk; ++ik) mesh_cart(p.recip, p.mesh, ik, &L->kcart[3 * size_t(ik)]);
+  DevBuf<double> dk, dRh, dDegh;
+  DevBuf<double2> dh0R;
+  h2d(dk, L->kcart, s);
+  h2d(dRh, p.Rh, size_t(p.nRh) * 3, s);
+  h2d(dDegh, p.degh, size_t(p.nRh), s);
+  h2d(dh0R, reinterpret_cast<const double2*>(p.h0R), size_t(p.nRh) * nw * nw, s);
+  DevBuf<double2> Uall(size_t(nk) * nw * nw);
+  DevBuf<double> Eall(size_t(nk) * nw);
+  DevBuf<int> info(nk);
+  launch_bloch(false, nk, nw, p.nRh, dk.p, dRh.p, dDegh.p, dh0R.p, Uall.p, s);
+  launch_jacobi(nw, nk, Uall.p, Eall.p, info.p, nullptr, s);
+  L->energiesAll.resize(size_t(nk) * nw);
+  std::vector<int> inf(nk);
+  PHX_HIP(hipMemcpyAsync(L->energiesAll.data(), Eall.p, Eall.bytes(), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipMemcpyAsync(inf.data(), info.p, info.bytes(), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < nk; ++i)
+    PHX_REQUIRE(inf[i] >= 0, "electron eigensolve did not converge");
+
+  // ---- energy window (Window::internalEnWindow, window.cpp:137-157), ascending mesh order
+  for (int ik = 0; ik < nk; ++ik) {
+    int l = -1, h = -1;
+    for (int ib = 0; ib < nw; ++ib) {
+      double e = L->energiesAll[size_t(ik) * nw + ib];
+      if (e < p.winMax && e > p.winMin) {
+        if (l < 0) l = ib;
+        h = ib;
+      }
+    }
+    if (l < 0) continue;
+    L->active.push_back(ik);
+    L->lo.push_back(l);
+    L->hi.push_back(h);
+  }
+  L->na = int(L->active.size());
+  const int na = L->na;
+  PHX_REQUIRE(na > 0, "energy window selects no states");
+  int st = 0, uo = 0;
+  for (int a = 0; a < na; ++a) {
+    int n = L->hi[a] - L->lo[a] + 1;
+    L->nb.push_back(n);
+    L->stateOff.push_back(st);
+    L->u2off.push_back(uo);
+    for (int b = 0; b < n; ++b) L->stateE.push_back(L->energiesAll[size_t(L->active[a]) * nw + L->lo[a] + b]);
+    st += n;
+    uo += n * nw;
+  }
+  L->numStates = st;
+  L->gStart.resize(st);
+  L->gSize.resize(st);
+  for (int a = 0; a < na; ++a)
+    degenerate_groups(&L->stateE[L->stateOff[a]], L->nb[a], &L->gStart[L->stateOff[a]],
+                      &L->gSize[L->stateOff[a]]);
+  for (int i = 0; i < st; ++i) PHX_REQUIRE(L->gSize[i] <= kGMax, "degenerate group too large");
+  {
+    DevBuf<int> dAct, dLo, dNb, dOff;
+    h2d(dAct, L->active, s);
+    h2d(dLo, L->lo, s);
+    h2d(dNb, L->nb, s);
+    h2d(dOff, L->u2off, s);
+    L->U2pool.alloc(size_t(uo));
+    launch_gather_eig(na, nw, dAct.p, dLo.p, dNb.p, dOff.p, Uall.p, L->U2pool.p, s);
+    PHX_HIP(hipStreamSynchronize(s));
+  }
+  h2d(L->rowMesh, L->active, s);
+  h2d(L->rowNb2, L->nb, s);
+  h2d(L->rowU2off, L->u2off, s);
+  h2d(L->rowStateOff, L->stateOff, s);
+  h2d(L->e2pool, L->stateE, s);
+  h2d(L->gStartD, L->gStart, s);
+  h2d(L->gSizeD, L->gSize, s);
+  {
+    std::vector<double> kca(3 * size_t(na));
+    for (int a = 0; a < na; ++a)
+      for (int i = 0; i < 3; ++i) kca[3 * a + i] = L->kcart[3 * size_t(L->active[a]) + i];
+    h2d(L->kcartActive, kca, s);
+  }
+  Uall.release();
+
+  // ---- q-set: every folded k2 - k1 of active points (helper_el_scattering.cpp:95-124)
+  L->qInSet.assign(nk, 0);
+  {
+    std::vector<char> occ(nk, 0);
+    for (int a = 0; a < na; ++a) occ[L->active[a]] = 1;
+    const int m0 = p.mesh[0], m1 = p.mesh[1], m2 = p.mesh[2];
+    parallel_for(nk, [&](int q) {
+      int qx, qy, qz;
+      mesh_coords(q, m0, m1, qx, qy, qz);
+      for (int a = 0; a < na; ++a) {
+        int x, y, z;
+        mesh_coords(L->active[a], m0, m1, x, y, z);
+        int i2 = ((z + qz) % m2) * m0 * m1 + ((y + qy) % m1) * m0 + (x + qx) % m0;
+        if (occ[i2]) {
+          L->qInSet[q] = 1;
+          break;
+        }
+      }
+    });
+  }
+  std::vector<int> qlist;
+  for (int q = 0; q < nk; ++q)
+    if (L->qInSet[q]) qlist.push_back(q);
+  const int nq = int(qlist.size());
+
+  // ---- phonons on the q-set (PhononH0 Kokkos path, phonon_h0_kokkos.cpp:6-219)
+  L->Wall.alloc(size_t(nk) * nm);
+  L->E3all.alloc(size_t(nk) * nm * nm);
+  launch_fill<double>(L->Wall.p, L->Wall.n, 0., s);
+  launch_fill<double>(reinterpret_cast<double*>(L->E3all.p), 2 * L->E3all.n, 0., s);
+  {
+    std::vector<double> qc(3 * size_t(nq));
+    for (int j = 0; j < nq; ++j)
+      for (int i = 0; i < 3; ++i) qc[3 * j + i] = L->kcart[3 * size_t(qlist[j]) + i];
+    DevBuf<double> dq, dRf, dwf, dmat, dmass;
+    DevBuf<int> dql;
+    h2d(dq, qc, s);
+    h2d(dql, qlist, s);
+    h2d(dRf, p.Rf, size_t(p.nRf) * 3, s);
+    h2d(dwf, p.wf, size_t(p.nRf), s);
+    h2d(dmat, p.mat2R, size_t(p.nRf) * nm * nm, s);
+    h2d(dmass, p.masses, size_t(nm), s);
+    DevBuf<double2> D(size_t(nq) * nm * nm);
+    DevBuf<double> w(size_t(nq) * nm);
+    DevBuf<int> qinfo(nq);
+    launch_bloch(true, nq, nm, p.nRf, dq.p, dRf.p, dwf.p, dmat.p, D.p, s);
+    launch_jacobi(nm, nq, D.p, w.p, qinfo.p, dmass.p, s);
+    launch_scatter_ph(nq, nm, dql.p, w.p, D.p, L->Wall.p, L->E3all.p, s);
+    std::vector<int> qinf(nq);
+    PHX_HIP(hipMemcpyAsync(qinf.data(), qinfo.p, qinfo.bytes(), hipMemcpyDeviceToHost, s));
+    L->omegaAll.resize(size_t(nk) * nm);
+    PHX_HIP(hipMemcpyAsync(L->omegaAll.data(), L->Wall.p, L->Wall.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nq; ++j) PHX_REQUIRE(qinf[j] >= 0, "phonon eigensolve did not converge");
+  }
+  // Bose (helper_el_scattering.cpp:261-267) and inner Fermi (el_scattering.cpp:87-109)
+  {
+    std::vector<double> bose(size_t(nk) * nm * p.nCalc, 0.);
+    for (int q : qlist)
+      for (int nu = 0; nu < nm; ++nu)
+        for (int c = 0; c < p.nCalc; ++c)
+          bose[(size_t(q) * nm + nu) * p.nCalc + c] =
+              population(true, L->omegaAll[size_t(q) * nm + nu], p.temperatures[c], 0.);
+    h2d(L->bose, bose, s);
+    std::vector<double> fermi(size_t(p.nCalc) * st);
+    for (int c = 0; c < p.nCalc; ++c)
+      for (int i = 0; i < st; ++i)
+        fermi[size_t(c) * st + i] = population(false, L->stateE[i], p.temperatures[c], p.chemPots[c]);
+    h2d(L->fermi, fermi, s);
+  }
+
+  // ---- coupling tensor and R vectors
+  h2d(L->Re, p.Re, size_t(p.nRe) * 3, s);
+  h2d(L->Rp, p.Rp, size_t(p.nRp) * 3, s);
+  {
+    std::vector<double> ie(p.nRe), ip(p.nRp);
+    for (int i = 0; i < p.nRe; ++i) ie[i] = 1. / p.degE[i];
+    for (int i = 0; i < p.nRp; ++i) ip[i] = 1. / p.degP[i];
+    h2d(L->invDegE, ie, s);
+    h2d(L->invDegP, ip, s);
+  }
+  const size_t gwN = size_t(p.nRe) * p.nRp * nm * nw * nw;
+  if (p.gW) {
+    h2d(L->gW, reinterpret_cast<const double2*>(p.gW), gwN, s);
+  } else {
+    PHX_REQUIRE(p.gwDecay != nullptr, "phx_problem: gW or gwDecay required");
+    DevBuf<double> dec;
+    h2d(dec, p.gwDecay, size_t(p.nRe) * p.nRp, s);
+    L->gW.alloc(gwN);
+    launch_synth_gw(L->gW.p, gwN, size_t(nm) * nw * nw, dec.p, p.gwSeed, s);
+    PHX_HIP(hipStreamSynchronize(s));
+  }
+
+  // ---- GEMM A operand: e^{ik2.Rp} rows for every active point
+  L->Kp = int(round_up(2 * size_t(p.nRp), 16));
+  L->ldb = int(round_up(2 * size_t(nm) * nw, 16));
+  L->rows = coupling_rows_per_tile(L->ldb);
+  L->rowsPad = int(round_up(size_t(na), L->rows)) + L->rows;
+  if (precision == PHX_FP64) {
+    L->Ad.alloc(size_t(L->rowsPad) * L->Kp);
+    launch_phase_rows<double>(na, L->rowsPad, p.nRp, L->Kp, L->kcartActive.p, L->Rp.p, L->Ad.p, s);
+  } else {
+    L->Af.alloc(size_t(L->rowsPad) * L->Kp);
+    launch_phase_rows<float>(na, L->rowsPad, p.nRp, L->Kp, L->kcartActive.p, L->Rp.p, L->Af.p, s);
+  }
+  L->lw.alloc(size_t(p.nCalc) * st);
+  launch_fill<double>(L->lw.p, L->lw.n, 0., s);
+  PHX_HIP(hipStreamSynchronize(s));
+}
+
+// Enqueue one batch of outer points.
+static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e) {
+  const phx_problem& p = L->prob;
+  hipStream_t s = L->stream;
+  const int nw = L->nw, nm = L->nm;
+  // panels of this batch
+  int nPanels = 0;
+  for (int i = 0; i < nkk; ++i) nPanels += L->nb[k1s[i]];
+  const size_t pstride = size_t(L->Kp) * L->ldb;
+  if (nPanels > L->maxPanels) {
+    L->maxPanels = std::max(nPanels, L->k1Batch * nw);
+    if (L->precision == PHX_FP64) {
+      L->Bd.alloc(pstride * L->maxPanels);
+      launch_fill<double>(L->Bd.p, L->Bd.n, 0., s);
+    } else {
+      L->Bf.alloc(pstride * L->maxPanels);
+      launch_fill<float>(L->Bf.p, L->Bf.n, 0.f, s);
+    }
+  }
+  const int M = k2e - k2b;
+  const int nMtiles = ceil_div(M, L->rows);
+  L->partial.reserve(size_t(L->maxPanels) * std::max(nMtiles, 1) * kGMax * kCMax);
+  // metadata: [U1off nkk][nb1 nkk][panelBase nkk][kkMesh nkk][kkStateOff nkk]
+  //           [panelKK P][panelB1 P][panelGStart P][panelGSize P]
+  std::vector<int>& m = L->hmeta;
+  m.assign(5 * size_t(nkk) + 4 * size_t(nPanels), 0);
+  int* U1off = m.data();
+  int* nb1 = U1off + nkk;
+  int* pbase = nb1 + nkk;
+  int* kkMesh = pbase + nkk;
+  int* kkState = kkMesh + nkk;
+  int* pKK = kkState + nkk;
+  int* pB1 = pKK + nPanels;
+  int* pGS = pB1 + nPanels;
+  int* pGZ = pGS + nPanels;
+  std::vector<double>& k1c = L->hk1;
+  k1c.assign(3 * size_t(nkk), 0.);
+  int pan = 0;
+  for (int i = 0; i < nkk; ++i) {
+    const int a = k1s[i];
+    U1off[i] = L->u2off[a];
+    nb1[i] = L->nb[a];
+    pbase[i] = pan;
+    kkMesh[i] = L->active[a];
+    kkState[i] = L->stateOff[a];
+    for (int d = 0; d < 3; ++d) k1c[3 * i + d] = L->kcart[3 * size_t(L->active[a]) + d];
+    for (int b = 0; b < L->nb[a]; ++b, ++pan) {
+      pKK[pan] = i;
+      pB1[pan] = b;
+      pGS[pan] = L->gStart[L->stateOff[a] + b];
+      pGZ[pan] = L->gSize[L->stateOff[a] + b];
+    }
+  }
+  L->bmeta.reserve(m.size());
+  PHX_HIP(hipMemcpyAsync(L->bmeta.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  L->bk1.reserve(k1c.size());
+  PHX_HIP(hipMemcpyAsync(L->bk1.p, k1c.data(), k1c.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  const int* dm = L->bmeta.p;
+  const int* dU1off = dm;
+  const int* dNb1 = dm + nkk;
+  const int* dPbase = dm + 2 * nkk;
+  const int* dKKMesh = dm + 3 * nkk;
+  const int* dKKState = dm + 4 * nkk;
+  const int* dPKK = dm + 5 * nkk;
+  const int* dPB1 = dPKK + nPanels;
+  const int* dPGS = dPB1 + nPanels;
+  const int* dPGZ = dPGS + nPanels;
+  // phases: e^{ik1.Re}/de and e^{-ik1.Rp}/dp
+  L->phE.reserve(size_t(nkk) * L->nRe);
+  L->phP.reserve(size_t(nkk) * L->nRp);
+  launch_phase_table(nkk, L->nRe, L->bk1.p, L->Re.p, L->invDegE.p, 1.0, L->phE.p, s);
+  launch_phase_table(nkk, L->nRp, L->bk1.p, L->Rp.p, L->invDegP.p, -1.0, L->phP.p, s);
+  // cacheElPh for the whole batch
+  CacheParams cp{};
+  cp.nRe = L->nRe;
+  cp.nRp = L->nRp;
+  cp.nm = nm;
+  cp.nw = nw;
+  cp.nkk = nkk;
+  cp.gW = L->gW.p;
+  cp.phE = L->phE.p;
+  cp.phP = L->phP.p;
+  cp.U1pool = L->U2pool.p;
+  cp.U1off = dU1off;
+  cp.nb1 = dNb1;
+  cp.panelBase = dPbase;
+  cp.panelStride = pstride;
+  cp.ldb = L->ldb;
+  if (L->precision == PHX_FP64) {
+    cp.B = L->Bd.p;
+    launch_cache<double>(cp, s);
+  } else {
+    cp.B = L->Bf.p;
+    launch_cache<float>(cp, s);
+  }
+  if (M <= 0) return 0;
+  // coupling + fused reduction
+  CoupParams cq{};
+  cq.lda = L->Kp;
+  cq.Kp = L->Kp;
+  cq.ldb = L->ldb;
+  cq.panelStride = pstride;
+  cq.M = M;
+  cq.nm = nm;
+  cq.nw = nw;
+  cq.nPanels = nPanels;
+  cq.nMtiles = nMtiles;
+  cq.panelKK = dPKK;
+  cq.panelB1 = dPB1;
+  cq.U2pool = L->U2pool.p;
+  cq.rowU2off = L->rowU2off.p + k2b;
+  cq.rowNb2 = L->rowNb2.p + k2b;
+  cq.E3pool = L->E3all.p;
+  cq.Wpool = L->Wall.p;
+  cq.qmode = 1;
+  cq.rowMesh = L->rowMesh.p + k2b;
+  cq.kkMesh = dKKMesh;
+  cq.m0 = p.mesh[0];
+  cq.m1 = p.mesh[1];
+  cq.m2 = p.mesh[2];
+  cq.e2pool = L->e2pool.p;
+  cq.rowStateOff = L->rowStateOff.p + k2b;
+  cq.e1pool = L->e2pool.p;
+  cq.kkStateOff = dKKState;
+  cq.panelGStart = dPGS;
+  cq.panelGSize = dPGZ;
+  cq.fermi = L->fermi.p;
+  cq.numStates = L->numStates;
+  cq.bose = L->bose.p;
+  cq.nCalc = L->nCalc;
+  cq.invSigma = 1. / p.sigma;
+  cq.deltaPref = 1. / p.sigma / std::sqrt(kPi);
+  cq.norm = 1. / (double(p.mesh[0]) * p.mesh[1] * p.mesh[2]);
+  cq.cutoff = phonon_cutoff();
+  cq.partial = L->partial.p;
+  int nb2max = 1;
+  for (int a = k2b; a < k2e; ++a) nb2max = std::max(nb2max, L->nb[a]);
+  cq.nb2max = nb2max;
+  if (L->precision == PHX_FP64) {
+    cq.A = L->Ad.p + size_t(k2b) * L->Kp;
+    cq.B = L->Bd.p;
+    launch_coupling<double>(cq, true, s);
+  } else {
+    cq.A = L->Af.p + size_t(k2b) * L->Kp;
+    cq.B = L->Bf.p;
+    launch_coupling<float>(cq, true, s);
+  }
+  CollectParams cl{};
+  cl.nkk = nkk;
+  cl.nMtiles = nMtiles;
+  cl.nCalc = L->nCalc;
+  cl.numStates = L->numStates;
+  cl.kkNb1 = dNb1;
+  cl.kkPanelBase = dPbase;
+  cl.kkStateOff = dKKState;
+  cl.panelGStart = dPGS;
+  cl.panelGSize = dPGZ;
+  cl.partial = L->partial.p;
+  cl.linewidth = L->lw.p;
+  launch_collect(cl, s);
+  return (long long)nkk * M;
+}
+
+static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2e) {
+  if (k2e < 0) k2e = L->na;
+  PHX_REQUIRE(k2b >= 0 && k2b <= k2e && k2e <= L->na, "phx_lw_run: bad k2 range");
+  std::vector<int> all;
+  if (!k1list) {
+    all.resize(L->na);
+    std::iota(all.begin(), all.end(), 0);
+    k1list = all.data();
+    nk1 = L->na;
+  }
+  for (int i = 0; i < nk1; ++i) PHX_REQUIRE(k1list[i] >= 0 && k1list[i] < L->na, "phx_lw_run: bad k1");
+  long long pairs = 0;
+  for (int i = 0; i < nk1; i += L->k1Batch) {
+    int n = std::min(L->k1Batch, nk1 - i);
+    // the metadata buffer is reused per batch: wait for the previous batch
+    PHX_HIP(hipStreamSynchronize(L->stream));
+    pairs += lw_batch(L, k1list + i, n, k2b, k2e);
+  }
+  return pairs;
+}
+
+static void lw_get(LwHandle* L, double* out, int average) {
+  const int st = L->numStates;
+  std::vector<double> lw(size_t(L->nCalc) * st);
+  PHX_HIP(hipStreamSynchronize(L->stream));
+  PHX_HIP(hipMemcpy(lw.data(), L->lw.p, lw.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (average) {
+    // ScatteringMatrix::degeneracyAveragingLinewidths (scattering.cpp:1168-1207), row 0
+    for (int a = 0; a < L->na; ++a) {
+      const int off = L->stateOff[a], nb = L->nb[a];
+      std::vector<double> tmp(nb);
+      for (int b1 = 0; b1 < nb; ++b1) {
+        double ekk = L->stateE[off + b1], t2 = 0.;
+        int n = 0;
+        for (int b2 = 0; b2 < nb; ++b2)
+          if (std::abs(L->stateE[off + b2] - ekk) < 1.0e-6) {
+            n++;
+            t2 = t2 + lw[off + b2];
+          }
+        tmp[b1] = t2 / float(n);
+      }
+      for (int b1 = 0; b1 < nb; ++b1) lw[off + b1] = tmp[b1];
+    }
+  }
+  std::copy(lw.begin(), lw.end(), out);
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+// ============================================================ C ABI
+struct phx_elph : phx::ElphHandle {};
+struct phx_lw : phx::LwHandle {};
+
+extern "C" {
+
+const char* phx_last_error(void) { return phx::last_error(); }
+const char* phx_version(void) { return "phx_elph 0.1 (gfx950)"; }
+
+int phx_zheev_batched(int M, int N, double* A, double* W, int* info) {
+  return guard([&] {
+    PHX_REQUIRE(M >= 0 && N >= 1 && N <= 44, "phx_zheev_batched: 1 <= N <= 44");
+    if (M) zheev_batched(M, N, A, W, info);
+  });
+}
+
+int phx_zheev_batched_dev(int M, int N, double* A_dev, double* W_dev, int* info_dev, void* stream) {
+  return guard([&] {
+    PHX_REQUIRE(M >= 0 && N >= 1 && N <= 44, "phx_zheev_batched_dev: 1 <= N <= 44");
+    launch_jacobi(N, M, reinterpret_cast<double2*>(A_dev), W_dev, info_dev, nullptr,
+                  static_cast<hipStream_t>(stream));
+  });
+}
+
+int phx_el_bloch_diag(int nk, const double* k, int nw, int nR, const double* R, const double* deg,
+                      const double* h0R, double* energies, double* eigvecs) {
+  return guard([&] {
+    PHX_REQUIRE(nk >= 0 && nw >= 1 && nw <= 44 && nR >= 1, "phx_el_bloch_diag: bad sizes");
+    if (nk) bloch_diag(false, nk, k, nw, nR, R, deg, h0R, nullptr, 0, energies, eigvecs);
+  });
+}
+
+int phx_ph_bloch_diag(int nq, const double* q, int nm, int nR, const double* R, const double* w,
+                      const double* mat2R, const double* masses, int mass_scaling, double* omega,
+                      double* eigvecs) {
+  return guard([&] {
+    PHX_REQUIRE(nq >= 0 && nm >= 1 && nm <= 44 && nR >= 1, "phx_ph_bloch_diag: bad sizes");
+    if (nq) bloch_diag(true, nq, q, nm, nR, R, w, mat2R, masses, mass_scaling, omega, eigvecs);
+  });
+}
+
+int phx_elph_create(phx_elph** h, const double* gW, int nw, int nm, int nRp, int nRe,
+                    const double* elR, const double* elDeg, const double* phR, const double* phDeg,
+                    int precision) {
+  return guard([&] {
+    PHX_REQUIRE(h && gW, "phx_elph_create: null argument");
+    auto* H = new phx_elph();
+    try {
+      elph_common_init(H, nw, nm, nRp, nRe, elR, elDeg, phR, phDeg, precision);
+      h2d(H->gW, reinterpret_cast<const double2*>(gW), size_t(nRe) * nRp * nm * nw * nw, H->stream);
+      PHX_HIP(hipStreamSynchronize(H->stream));
+    } catch (...) {
+      delete H;
+      throw;
+    }
+    *h = H;
+  });
+}
+
+int phx_elph_create_synthetic(phx_elph** h, int nw, int nm, int nRp, int nRe, const double* elR,
+                              const double* elDeg, const double* phR, const double* phDeg,
+                              const double* decay, unsigned long long seed, int precision) {
+  return guard([&] {
+    PHX_REQUIRE(h && decay, "phx_elph_create_synthetic: null argument");
+    auto* H = new phx_elph();
+    try {
+      elph_common_init(H, nw, nm, nRp, nRe, elR, elDeg, phR, phDeg, precision);
+      DevBuf<double> dec;
+      h2d(dec, decay, size_t(nRe) * nRp, H->stream);
+      const size_t n = size_t(nRe) * nRp * nm * nw * nw;
+      H->gW.alloc(n);
+      launch_synth_gw(H->gW.p, n, size_t(nm) * nw * nw, dec.p, seed, H->stream);
+      PHX_HIP(hipStreamSynchronize(H->stream));
+    } catch (...) {
+      delete H;
+      throw;
+    }
+    *h = H;
+  });
+}
+
+int phx_elph_cache(phx_elph* h, const double* U1, int nb1, const double* k1) {
+  return guard([&] {
+    PHX_REQUIRE(h && U1 && k1, "phx_elph_cache: null argument");
+    elph_cache(h, U1, nb1, k1);
+  });
+}
+
+int phx_elph_coupling_sq(phx_elph* h, int nq, const double* U2s, const int* nb2s, const double* e3s,
+                         const double* q3, const double* polar, double* out) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_elph_coupling_sq: null handle");
+    PHX_REQUIRE(polar == nullptr, "phx_elph_coupling_sq: polar correction not supported");
+    PHX_REQUIRE(nq == 0 || (U2s && nb2s && e3s && q3 && out), "phx_elph_coupling_sq: null argument");
+    elph_coupling_sq(h, nq, U2s, nb2s, e3s, q3, out);
+  });
+}
+
+int phx_elph_get_coupling_dims(phx_elph* h, int* d) {
+  return guard([&] {
+    PHX_REQUIRE(h && d, "null argument");
+    d[0] = h->nRe;
+    d[1] = h->nRp;
+    d[2] = h->nm;
+    d[3] = h->nw;
+    d[4] = h->nw;
+  });
+}
+
+int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double maxMem) {
+  int result = 1;
+  int rc = guard([&] {
+    PHX_REQUIRE(h && nk2 >= 0 && nb1 >= 1, "phx_elph_estimate_num_batches: bad argument");
+    if (maxMem <= 0) {
+      maxMem = 16.e9;  // DeviceManager default (common_kokkos.cpp:107-112)
+      if (const char* e = std::getenv("MAXMEM")) maxMem = std::atof(e) * 1.e9;
+    }
+    // memory model of interaction_elph.cpp:462-474 (numElBands = nw, numPhBands = nm)
+    const double nw = h->nw, nm = h->nm, nRp = h->nRp;
+    const double evs = 16 * (nw * nw + nm * nm);
+    const double phase = 16 * nRp;
+    const double g3 = 2 * 16 * nm * nb1 * nw;
+    const double g4 = 2 * 16 * nm * nb1 * nw;
+    const double gFinal = 2 * 16 * nm * nb1 * nw;
+    const double coupling = 16 * nb1 * nw * nm;
+    const double polar = 16 * nm * nb1 * nw;
+    const double maxUsage = nk2 * (evs + polar + std::max({phase + g3, g3 + g4, g4 + gFinal, gFinal + coupling}));
+    // the device arrays held by the handle count as used memory
+    const double used = double(h->gW.bytes() + h->Bd.bytes() + h->Bf.bytes());
+    const double available = maxMem - used;
+    PHX_REQUIRE(available >= maxUsage / std::max(nk2, 1), "Insufficient memory!");
+    result = std::max(1, int(std::ceil(maxUsage / available)));
+  });
+  return rc ? rc : result;
+}
+
+int phx_elph_destroy(phx_elph* h) {
+  return guard([&] { delete h; });
+}
+
+int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch) {
+  return guard([&] {
+    PHX_REQUIRE(h && prob, "phx_lw_setup: null argument");
+    auto* L = new phx_lw();
+    try {
+      lw_setup(L, prob, precision, k1_batch);
+    } catch (...) {
+      delete L;
+      throw;
+    }
+    *h = L;
+  });
+}
+
+int phx_lw_num_active(phx_lw* h) { return h ? h->na : -1; }
+int phx_lw_num_states(phx_lw* h) { return h ? h->numStates : -1; }
+
+int phx_lw_get_window(phx_lw* h, int* active, int* lo, int* hi, double* energies) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    if (active) std::copy(h->active.begin(), h->active.end(), active);
+    if (lo) std::copy(h->lo.begin(), h->lo.end(), lo);
+    if (hi) std::copy(h->hi.begin(), h->hi.end(), hi);
+    if (energies) std::copy(h->stateE.begin(), h->stateE.end(), energies);
+  });
+}
+
+int phx_lw_get_phonons(phx_lw* h, double* omega, unsigned char* inSet) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    if (omega) std::copy(h->omegaAll.begin(), h->omegaAll.end(), omega);
+    if (inSet) std::copy(h->qInSet.begin(), h->qInSet.end(), inSet);
+  });
+}
+
+long long phx_lw_run(phx_lw* h, int nk1, const int* k1list, int k2b, int k2e) {
+  long long pairs = 0;
+  int rc = guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    pairs = lw_run(h, nk1, k1list, k2b, k2e);
+  });
+  return rc ? rc : pairs;
+}
+
+int phx_lw_get_linewidths(phx_lw* h, double* out, int average) {
+  return guard([&] {
+    PHX_REQUIRE(h && out, "null argument");
+    lw_get(h, out, average);
+  });
+}
+
+int phx_lw_reset(phx_lw* h) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    launch_fill<double>(h->lw.p, h->lw.n, 0., h->stream);
+    PHX_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int phx_lw_sync(phx_lw* h) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+void* phx_lw_stream(phx_lw* h) { return h ? static_cast<void*>(h->stream) : nullptr; }
+
+int phx_lw_copy_gw(phx_lw* h, double* out) {
+  return guard([&] {
+    PHX_REQUIRE(h && out, "null argument");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    PHX_HIP(hipMemcpy(out, h->gW.p, h->gW.bytes(), hipMemcpyDeviceToHost));
+  });
+}
+
+int phx_lw_destroy(phx_lw* h) {
+  return guard([&] { delete h; });
+}
+
+int phx_synth_gw_host(double* gW, int nw, int nm, int nRp, int nRe, const double* decay,
+                      unsigned long long seed) {
+  return guard([&] {
+    PHX_REQUIRE(gW && decay, "null argument");
+    synth_gw_host(gW, size_t(nRe) * nRp * nm * nw * nw, size_t(nm) * nw * nw, decay, seed);
+  });
+}
+
+}  // extern "C"
