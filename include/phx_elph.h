@@ -164,6 +164,12 @@ long long phx_lw_run(phx_lw* h, int nk1, const int* k1_list, int k2_begin, int k
  * (scattering.cpp:1168-1207) applied to row 0 when average != 0. */
 int phx_lw_get_linewidths(phx_lw* h, double* out, int average);
 int phx_lw_reset(phx_lw* h);
+/* Per-kernel timing with HIP events recorded on the handle's stream around
+ * every batch; turning it on (or off) clears the accumulators. */
+int phx_lw_set_timing(phx_lw* h, int on);
+/* out5 = {cacheElPh ms, coupling ms, collect ms, batches timed,
+ *         algorithmic FLOPs of the timed coupling launches}. */
+int phx_lw_kernel_times(phx_lw* h, double* out5);
 int phx_lw_sync(phx_lw* h);
 /* HIP stream the handle enqueues on (hipStream_t). */
 void* phx_lw_stream(phx_lw* h);

@@ -619,6 +619,8 @@ void* orc_setup(const orc_problem* prob) {
   // q-set: all folded k2 - k1 (helper_el_scattering.cpp:95-124)
   s->qInSet.assign(s->nk, 0);
   const int na = int(s->active.size());
+  int offMesh = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : offMesh)
   for (int a2 = 0; a2 < na; ++a2)
     for (int a1 = 0; a1 < na; ++a1) {
       const double* k2 = &s->kcart[3 * size_t(s->active[a2])];
@@ -627,11 +629,16 @@ void* orc_setup(const orc_problem* prob) {
       crys_of(p, qc, qx);
       int iq = orc_mesh_index(qx, p.mesh);
       if (iq < 0) {
-        delete s;
-        return nullptr;
+        offMesh++;
+        continue;
       }
+#pragma omp atomic write
       s->qInSet[iq] = 1;
     }
+  if (offMesh) {
+    delete s;
+    return nullptr;
+  }
   s->omega.assign(size_t(s->nk) * p.nm, 0.);
   s->phev.assign(2 * size_t(s->nk) * p.nm * p.nm, 0.);
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : fail)

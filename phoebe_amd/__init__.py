@@ -79,6 +79,8 @@ EXPORTS = {
     "phx_lw_get_linewidths": (C.c_int, [C.c_void_p, _dp, C.c_int]),
     "phx_lw_reset": (C.c_int, [C.c_void_p]),
     "phx_lw_sync": (C.c_int, [C.c_void_p]),
+    "phx_lw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_stream": (C.c_void_p, [C.c_void_p]),
     "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_destroy": (C.c_int, [C.c_void_p]),
@@ -327,6 +329,16 @@ class ElectronLinewidths:
 
     def reset(self):
         _check(lib().phx_lw_reset(self._h))
+
+    def set_timing(self, on=True):
+        _check(lib().phx_lw_set_timing(self._h, int(bool(on))))
+
+    def kernel_times(self):
+        """dict: cache_ms, coupling_ms, collect_ms, batches, coupling_flops."""
+        out = np.zeros(5)
+        _check(lib().phx_lw_kernel_times(self._h, _p(out)))
+        return dict(cache_ms=out[0], coupling_ms=out[1], collect_ms=out[2],
+                    batches=int(out[3]), coupling_flops=out[4])
 
     def stream(self):
         return lib().phx_lw_stream(self._h)

@@ -482,10 +482,33 @@ struct LwHandle {
   DevBuf<double2> phE, phP;
   std::vector<int> hmeta;
   std::vector<double> hk1;
+  // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool evPending = false;
+  double kernelMs[3] = {0., 0., 0.};
+  long long kernelLaunches = 0;
+  double couplingFlops = 0.;  // algorithmic FLOPs of the coupling launches timed
   ~LwHandle() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+// Fold the events of the last timed batch into kernelMs (stream must be idle).
+static void lw_harvest_timing(LwHandle* L) {
+  if (!L->evPending) return;
+  float a = 0.f, b = 0.f, c = 0.f;
+  PHX_HIP(hipEventElapsedTime(&a, L->ev[0], L->ev[1]));
+  PHX_HIP(hipEventElapsedTime(&b, L->ev[1], L->ev[2]));
+  PHX_HIP(hipEventElapsedTime(&c, L->ev[2], L->ev[3]));
+  L->kernelMs[0] += a;
+  L->kernelMs[1] += b;
+  L->kernelMs[2] += c;
+  L->kernelLaunches += 1;
+  L->evPending = false;
+}
 
 static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch) {
   const phx_problem& p = *P;
@@ -771,6 +794,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   L->phP.reserve(size_t(nkk) * L->nRp);
   launch_phase_table(nkk, L->nRe, L->bk1.p, L->Re.p, L->invDegE.p, 1.0, L->phE.p, s);
   launch_phase_table(nkk, L->nRp, L->bk1.p, L->Rp.p, L->invDegP.p, -1.0, L->phP.p, s);
+  if (L->timing) PHX_HIP(hipEventRecord(L->ev[0], s));
   // cacheElPh for the whole batch
   CacheParams cp{};
   cp.nRe = L->nRe;
@@ -794,7 +818,16 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cp.B = L->Bf.p;
     launch_cache<float>(cp, s);
   }
-  if (M <= 0) return 0;
+  if (M <= 0) {
+    if (L->timing) {
+      PHX_HIP(hipEventRecord(L->ev[1], s));
+      PHX_HIP(hipEventRecord(L->ev[2], s));
+      PHX_HIP(hipEventRecord(L->ev[3], s));
+      L->evPending = true;
+    }
+    return 0;
+  }
+  if (L->timing) PHX_HIP(hipEventRecord(L->ev[1], s));
   // coupling + fused reduction
   CoupParams cq{};
   cq.lda = L->Kp;
@@ -846,6 +879,16 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.B = L->Bf.p;
     launch_coupling<float>(cq, true, s);
   }
+  if (L->timing) {
+    PHX_HIP(hipEventRecord(L->ev[2], s));
+    // algorithmic FLOPs per (pair, b1): R_ph transform, phonon and electron rotations, |g|^2
+    double f = 0.;
+    for (int a = k2b; a < k2e; ++a) {
+      const double nb2 = L->nb[a];
+      f += 8. * L->nRp * nm * nw + 8. * nm * nm * nw + 8. * nm * nw * nb2 + 3. * nm * nb2;
+    }
+    L->couplingFlops += f * nPanels;
+  }
   CollectParams cl{};
   cl.nkk = nkk;
   cl.nMtiles = nMtiles;
@@ -859,6 +902,10 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cl.partial = L->partial.p;
   cl.linewidth = L->lw.p;
   launch_collect(cl, s);
+  if (L->timing) {
+    PHX_HIP(hipEventRecord(L->ev[3], s));
+    L->evPending = true;
+  }
   return (long long)nkk * M;
 }
 
@@ -878,6 +925,7 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
     int n = std::min(L->k1Batch, nk1 - i);
     // the metadata buffer is reused per batch: wait for the previous batch
     PHX_HIP(hipStreamSynchronize(L->stream));
+    lw_harvest_timing(L);
     pairs += lw_batch(L, k1list + i, n, k2b, k2e);
   }
   return pairs;
@@ -1109,6 +1157,33 @@ int phx_lw_reset(phx_lw* h) {
     PHX_REQUIRE(h, "null handle");
     launch_fill<double>(h->lw.p, h->lw.n, 0., h->stream);
     PHX_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int phx_lw_set_timing(phx_lw* h, int on) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    lw_harvest_timing(h);
+    if (on && !h->ev[0])
+      for (auto& e : h->ev) PHX_HIP(hipEventCreate(&e));
+    h->timing = on != 0;
+    h->kernelMs[0] = h->kernelMs[1] = h->kernelMs[2] = 0.;
+    h->kernelLaunches = 0;
+    h->couplingFlops = 0.;
+  });
+}
+
+int phx_lw_kernel_times(phx_lw* h, double* out5) {
+  return guard([&] {
+    PHX_REQUIRE(h && out5, "null argument");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    lw_harvest_timing(h);
+    out5[0] = h->kernelMs[0];
+    out5[1] = h->kernelMs[1];
+    out5[2] = h->kernelMs[2];
+    out5[3] = double(h->kernelLaunches);
+    out5[4] = h->couplingFlops;
   });
 }
 
