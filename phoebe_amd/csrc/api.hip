@@ -71,7 +71,7 @@ struct CoupParams {
   int nb2max;
 };
 template <typename T> void launch_coupling(const CoupParams& p, bool reduce, hipStream_t s);
-int coupling_rows_per_tile(int ldb);
+int coupling_rows_per_tile_f64(int ldb);
 
 struct CollectParams {
   int nkk, nMtiles, nCalc, numStates;
@@ -377,7 +377,7 @@ static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int
   if (nq <= 0) return;
   hipStream_t s = H->stream;
   const int nw = H->nw, nm = H->nm, nb1 = H->nb1;
-  const int rows = coupling_rows_per_tile(H->ldb);
+  const int rows = coupling_rows_per_tile_f64(H->ldb);
   const int nMtiles = ceil_div(nq, rows);
   const int rowsPad = nMtiles * rows;
   std::vector<int> u2off(nq), nb2v(nq), e3idx(nq);
@@ -708,7 +708,7 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   // ---- GEMM A operand: e^{ik2.Rp} rows for every active point
   L->Kp = int(round_up(2 * size_t(p.nRp), 16));
   L->ldb = int(round_up(2 * size_t(nm) * nw, 16));
-  L->rows = coupling_rows_per_tile(L->ldb);
+  L->rows = coupling_rows_per_tile_f64(L->ldb);
   L->rowsPad = int(round_up(size_t(na), L->rows)) + L->rows;
   if (precision == PHX_FP64) {
     L->Ad.alloc(size_t(L->rowsPad) * L->Kp);

@@ -1,0 +1,7 @@
+// F32 instantiation of the coupling kernel (coupling_kernel.hpp).
+#include "coupling_kernel.hpp"
+
+namespace phx {
+template void launch_coupling<float>(const CoupParams&, bool, hipStream_t);
+int coupling_rows_per_tile_f32(int ldb) { return coupling_rows_per_tile(ldb); }
+}  // namespace phx

@@ -1,0 +1,7 @@
+// F64 instantiation of the coupling kernel (coupling_kernel.hpp).
+#include "coupling_kernel.hpp"
+
+namespace phx {
+template void launch_coupling<double>(const CoupParams&, bool, hipStream_t);
+int coupling_rows_per_tile_f64(int ldb) { return coupling_rows_per_tile(ldb); }
+}  // namespace phx
