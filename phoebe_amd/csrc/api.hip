@@ -23,6 +23,9 @@ void launch_bloch(bool phonon, int nk, int nw, int nR, const double* kc, const d
                   const double* rw, const void* coef, double2* out, hipStream_t stream);
 void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
                      hipStream_t s);
+void launch_synth_gw_f32(float2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
+                         hipStream_t s);
+void launch_to_float(const double* a, float* b, size_t n, hipStream_t s);
 void synth_gw_host(double* gW, size_t n, size_t blk, const double* decay, uint64_t seed);
 template <typename T> void launch_fill(T* p, size_t n, T v, hipStream_t s);
 void launch_phase_table(int rows, int nR, const double* kc, const double* R, const double* scale,
@@ -37,7 +40,7 @@ void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const dou
 
 struct CacheParams {
   int nRe, nRp, nm, nw, nkk;
-  const double2* gW;
+  const void* gW;
   const double2* phE;
   const double2* phP;
   const double2* U1pool;
@@ -48,7 +51,8 @@ struct CacheParams {
   size_t panelStride;
   int ldb;
 };
-template <typename OutT> void launch_cache(const CacheParams& p, hipStream_t s);
+template <typename T, typename OutT> void launch_cache(const CacheParams& p, hipStream_t s);
+constexpr int kCacheNKK = 32;  // outer points per cacheElPh launch
 
 struct CoupParams {
   const void* A; int lda;
@@ -264,8 +268,12 @@ static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, cons
 struct ElphHandle {
   int nw = 0, nm = 0, nRp = 0, nRe = 0, precision = PHX_FP64;
   hipStream_t stream = nullptr;
-  DevBuf<double2> gW;
+  DevBuf<double2> gW;   // FP64 coupling tensor (FP64 handles)
+  DevBuf<float2> gWf;   // FP32 copy read by the reduced-precision GEMMs
   DevBuf<double> Re, degE, Rp, degP;
+  const void* gw_ptr() const {
+    return precision == PHX_FP64 ? static_cast<const void*>(gW.p) : static_cast<const void*>(gWf.p);
+  }
   // cached state of the last cacheElPh
   int nb1 = -1;
   int Kp = 0, ldb = 0;
@@ -351,7 +359,7 @@ static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k
     cp.nm = H->nm;
     cp.nw = H->nw;
     cp.nkk = 1;
-    cp.gW = H->gW.p;
+    cp.gW = H->gw_ptr();
     cp.phE = phE.p;
     cp.phP = phP.p;
     cp.U1pool = H->U1.p;
@@ -362,10 +370,10 @@ static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k
     cp.ldb = H->ldb;
     if (H->precision == PHX_FP64) {
       cp.B = H->Bd.p;
-      launch_cache<double>(cp, s);
+      launch_cache<double, double>(cp, s);
     } else {
       cp.B = H->Bf.p;
-      launch_cache<float>(cp, s);
+      launch_cache<float, float>(cp, s);
     }
     PHX_HIP(hipStreamSynchronize(s));
   }
@@ -462,8 +470,12 @@ struct LwHandle {
   std::vector<double> omegaAll;    // nk x nm
   std::vector<unsigned char> qInSet;
   // device
-  DevBuf<double2> gW;
+  DevBuf<double2> gW;   // FP64 coupling tensor (FP64 handles)
+  DevBuf<float2> gWf;   // FP32 copy (reduced-precision handles)
   DevBuf<double> Re, degE, Rp, degP, invDegE, invDegP;
+  const void* gw_ptr() const {
+    return precision == PHX_FP64 ? static_cast<const void*>(gW.p) : static_cast<const void*>(gWf.p);
+  }
   DevBuf<double2> U2pool;
   DevBuf<double> e2pool, fermi, bose, Wall, lw;
   DevBuf<double2> E3all;
@@ -518,7 +530,7 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   PHX_REQUIRE(p.nw <= 44 && p.nm <= 44, "phx_lw_setup: eigensolver size");
   L->prob = p;
   L->precision = precision;
-  L->k1Batch = std::max(1, k1Batch);
+  L->k1Batch = std::min(kCacheNKK, std::max(1, k1Batch));
   L->nw = p.nw;
   L->nm = p.nm;
   L->nRp = p.nRp;
@@ -696,12 +708,24 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   const size_t gwN = size_t(p.nRe) * p.nRp * nm * nw * nw;
   if (p.gW) {
     h2d(L->gW, reinterpret_cast<const double2*>(p.gW), gwN, s);
+    if (precision != PHX_FP64) {
+      L->gWf.alloc(gwN);
+      launch_to_float(reinterpret_cast<const double*>(L->gW.p), reinterpret_cast<float*>(L->gWf.p),
+                      2 * gwN, s);
+      PHX_HIP(hipStreamSynchronize(s));
+      L->gW.release();
+    }
   } else {
     PHX_REQUIRE(p.gwDecay != nullptr, "phx_problem: gW or gwDecay required");
     DevBuf<double> dec;
     h2d(dec, p.gwDecay, size_t(p.nRe) * p.nRp, s);
-    L->gW.alloc(gwN);
-    launch_synth_gw(L->gW.p, gwN, size_t(nm) * nw * nw, dec.p, p.gwSeed, s);
+    if (precision == PHX_FP64) {
+      L->gW.alloc(gwN);
+      launch_synth_gw(L->gW.p, gwN, size_t(nm) * nw * nw, dec.p, p.gwSeed, s);
+    } else {
+      L->gWf.alloc(gwN);
+      launch_synth_gw_f32(L->gWf.p, gwN, size_t(nm) * nw * nw, dec.p, p.gwSeed, s);
+    }
     PHX_HIP(hipStreamSynchronize(s));
   }
 
@@ -802,7 +826,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cp.nm = nm;
   cp.nw = nw;
   cp.nkk = nkk;
-  cp.gW = L->gW.p;
+  cp.gW = L->gw_ptr();
   cp.phE = L->phE.p;
   cp.phP = L->phP.p;
   cp.U1pool = L->U2pool.p;
@@ -813,10 +837,10 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cp.ldb = L->ldb;
   if (L->precision == PHX_FP64) {
     cp.B = L->Bd.p;
-    launch_cache<double>(cp, s);
+    launch_cache<double, double>(cp, s);
   } else {
     cp.B = L->Bf.p;
-    launch_cache<float>(cp, s);
+    launch_cache<float, float>(cp, s);
   }
   if (M <= 0) {
     if (L->timing) {
@@ -1010,7 +1034,15 @@ int phx_elph_create(phx_elph** h, const double* gW, int nw, int nm, int nRp, int
     auto* H = new phx_elph();
     try {
       elph_common_init(H, nw, nm, nRp, nRe, elR, elDeg, phR, phDeg, precision);
-      h2d(H->gW, reinterpret_cast<const double2*>(gW), size_t(nRe) * nRp * nm * nw * nw, H->stream);
+      const size_t n = size_t(nRe) * nRp * nm * nw * nw;
+      h2d(H->gW, reinterpret_cast<const double2*>(gW), n, H->stream);
+      if (precision != PHX_FP64) {
+        H->gWf.alloc(n);
+        launch_to_float(reinterpret_cast<const double*>(H->gW.p), reinterpret_cast<float*>(H->gWf.p),
+                        2 * n, H->stream);
+        PHX_HIP(hipStreamSynchronize(H->stream));
+        H->gW.release();
+      }
       PHX_HIP(hipStreamSynchronize(H->stream));
     } catch (...) {
       delete H;
@@ -1031,8 +1063,13 @@ int phx_elph_create_synthetic(phx_elph** h, int nw, int nm, int nRp, int nRe, co
       DevBuf<double> dec;
       h2d(dec, decay, size_t(nRe) * nRp, H->stream);
       const size_t n = size_t(nRe) * nRp * nm * nw * nw;
-      H->gW.alloc(n);
-      launch_synth_gw(H->gW.p, n, size_t(nm) * nw * nw, dec.p, seed, H->stream);
+      if (precision == PHX_FP64) {
+        H->gW.alloc(n);
+        launch_synth_gw(H->gW.p, n, size_t(nm) * nw * nw, dec.p, seed, H->stream);
+      } else {
+        H->gWf.alloc(n);
+        launch_synth_gw_f32(H->gWf.p, n, size_t(nm) * nw * nw, dec.p, seed, H->stream);
+      }
       PHX_HIP(hipStreamSynchronize(H->stream));
     } catch (...) {
       delete H;
@@ -1089,7 +1126,7 @@ int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double maxMem) 
     const double polar = 16 * nm * nb1 * nw;
     const double maxUsage = nk2 * (evs + polar + std::max({phase + g3, g3 + g4, g4 + gFinal, gFinal + coupling}));
     // the device arrays held by the handle count as used memory
-    const double used = double(h->gW.bytes() + h->Bd.bytes() + h->Bf.bytes());
+    const double used = double(h->gW.bytes() + h->gWf.bytes() + h->Bd.bytes() + h->Bf.bytes());
     const double available = maxMem - used;
     PHX_REQUIRE(available >= maxUsage / std::max(nk2, 1), "Insufficient memory!");
     result = std::max(1, int(std::ceil(maxUsage / available)));
@@ -1199,6 +1236,7 @@ void* phx_lw_stream(phx_lw* h) { return h ? static_cast<void*>(h->stream) : null
 int phx_lw_copy_gw(phx_lw* h, double* out) {
   return guard([&] {
     PHX_REQUIRE(h && out, "null argument");
+    PHX_REQUIRE(h->precision == PHX_FP64, "phx_lw_copy_gw: only FP64 handles keep the FP64 tensor");
     PHX_HIP(hipStreamSynchronize(h->stream));
     PHX_HIP(hipMemcpy(out, h->gW.p, h->gW.bytes(), hipMemcpyDeviceToHost));
   });

@@ -26,6 +26,9 @@
 
 #include "phx_common.hpp"
 
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
 namespace phx {
 
 // ------------------------------------------------------------------ synthetic gW
@@ -49,12 +52,16 @@ __host__ __device__ inline double hash_n(uint64_t seed, uint64_t idx) {
   return (s - 2.0) * 1.7320508075688772;
 }
 
-__global__ void k_synth_gw(double2* __restrict__ gW, size_t n, size_t blk,
+template <typename T2>
+__global__ void k_synth_gw(T2* __restrict__ gW, size_t n, size_t blk,
                            const double* __restrict__ decay, uint64_t seed) {
   for (size_t f = size_t(blockIdx.x) * blockDim.x + threadIdx.x; f < n;
        f += size_t(gridDim.x) * blockDim.x) {
     double d = decay[f / blk];
-    gW[f] = make_double2(d * hash_n(seed, 2 * f), d * hash_n(seed, 2 * f + 1));
+    T2 v;
+    v.x = d * hash_n(seed, 2 * f);
+    v.y = d * hash_n(seed, 2 * f + 1);
+    gW[f] = v;
   }
 }
 
@@ -74,7 +81,14 @@ void synth_gw_host(double* gW, size_t n, size_t blk, const double* decay, uint64
 
 void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_synth_gw, dim3(8192), dim3(256), 0, s, gW, n, blk, decay, seed);
+  hipLaunchKernelGGL(k_synth_gw<double2>, dim3(8192), dim3(256), 0, s, gW, n, blk, decay, seed);
+  PHX_HIP(hipGetLastError());
+}
+
+// FP32 copy of the same tensor, each element rounded once from its FP64 value.
+void launch_synth_gw_f32(float2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_synth_gw<float2>, dim3(8192), dim3(256), 0, s, gW, n, blk, decay, seed);
   PHX_HIP(hipGetLastError());
 }
 
@@ -188,11 +202,17 @@ void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const dou
 }
 
 // ------------------------------------------------------------------ cacheElPh
-constexpr int kCacheRC = 32;  // R_e chunk staged in LDS
-
+// g1(k1; Rp,nu,w1,w2) = sum_Re P(k1,Re) gW(Re,Rp,nu,w1,w2) as a real MFMA GEMM.
+// Output rows m < NKK hold Re(g1) of outer point m, rows NKK+m hold Im(g1);
+// the K dimension walks R_e in chunks of 8 vectors, ordered so that lane
+// group g of k-step ks reads (Re, Im) of gW at R_e = re0 + g + 4*(ks/2) with
+// one 2-element load serving k-steps 2q (Re part) and 2q+1 (Im part):
+//   A'(m, k) = [ Pr | -Pi ] for m < NKK,  [ Pi | Pr ] for m >= NKK.
+// One workgroup owns one (Rp, nu) block of nw^2 columns (256 per pass), so the
+// U1 rotation and the e^{-ik1.Rp}/dp factor run as its epilogue.
 struct CacheParams {
   int nRe, nRp, nm, nw, nkk;
-  const double2* gW;      // (nRe, nRp, nm, nw, nw)
+  const void* gW;         // (nRe, nRp, nm, nw, nw) complex, element type T
   const double2* phE;     // (nkk, nRe): e^{ik1.Re}/de
   const double2* phP;     // (nkk, nRp): e^{-ik1.Rp}/dp or 1/dp
   const double2* U1pool;  // per kk: (nw, nb1) column-major
@@ -204,98 +224,175 @@ struct CacheParams {
   int ldb;                // elements per panel row
 };
 
-// One workgroup per (Rp, nu): the (w1, w2) block of gW is streamed once over
-// all R_e and contracted with the phases of KK outer points at a time (EPT
-// entries of the block per thread).
-template <typename OutT, int EPT, int KK>
-__global__ void __launch_bounds__(256) k_cache(CacheParams p) {
-  __shared__ double2 ph[KK][kCacheRC];
-  __shared__ double2 buf[256 * EPT];
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+template <typename T> struct MfmaC;
+template <> struct MfmaC<double> {
+  typedef d4_t acc_t;
+  __device__ static acc_t mac(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lk, int i) { return lk + 4 * i; }
+};
+template <> struct MfmaC<float> {
+  typedef f4_t acc_t;
+  __device__ static acc_t mac(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lk, int i) { return 4 * lk + i; }
+};
+
+constexpr int kCacheNKK = 32;  // outer points per launch
+
+template <typename T, typename OutT>
+__global__ void __launch_bounds__(256, 2) k_cache(CacheParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NKK = kCacheNKK, MR = 2 * NKK, MTc = MR / 16, LDA = 17;
+  typedef typename Vec2<T>::type T2;
+  typedef typename MfmaC<T>::acc_t acc_t;
   const int rp = blockIdx.x / p.nm, nu = blockIdx.x % p.nm;
   const int E = p.nw * p.nw;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
   const size_t strideRe = size_t(p.nRp) * p.nm * E;
-  const double2* g = p.gW + (size_t(rp) * p.nm + nu) * E;
-  OutT* B = static_cast<OutT*>(p.B);
-  for (int kk0 = 0; kk0 < p.nkk; kk0 += KK) {
-    const int nkk = min(KK, p.nkk - kk0);
-    double2 acc[EPT][KK];
+  const T2* G = static_cast<const T2*>(p.gW) + (size_t(rp) * p.nm + nu) * E;
+  T* As = reinterpret_cast<T*>(smem);               // MR x LDA
+  T* Cb = reinterpret_cast<T*>(smem);               // epilogue: 2 x 16 x 256
+  OutT* Bout = static_cast<OutT*>(p.B);
+  const int nchunk = (p.nRe + 7) / 8;
+
+  for (int c0 = 0; c0 < E; c0 += 256) {
+    acc_t acc[MTc][4];
 #pragma unroll
-    for (int u = 0; u < EPT; ++u)
+    for (int m = 0; m < MTc; ++m)
 #pragma unroll
-      for (int k = 0; k < KK; ++k) acc[u][k] = make_double2(0., 0.);
-    for (int r0 = 0; r0 < p.nRe; r0 += kCacheRC) {
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[m][j][i] = 0;
+    T2 bq[2][4], bn[2][4];
+    auto load_b = [&](T2(&b)[2][4], int re0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int col = c0 + (wave * 4 + j) * 16 + lr;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int re = re0 + lg + 4 * h;
+          b[h][j] = (re < p.nRe && col < E) ? G[size_t(re) * strideRe + col] : T2{T(0), T(0)};
+        }
+      }
+    };
+    load_b(bq, 0);
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const int re0 = ch * 8;
       __syncthreads();
-      for (int i = t; i < KK * kCacheRC; i += 256) {
-        int k = i / kCacheRC, r = i % kCacheRC;
-        ph[k][r] = (k < nkk && r0 + r < p.nRe) ? p.phE[size_t(kk0 + k) * p.nRe + r0 + r]
-                                               : make_double2(0., 0.);
+      for (int i = t; i < MR * 16; i += 256) {
+        const int m = i / 16, k = i % 16, ks = k / 4, g = k % 4;
+        const int re = re0 + g + 4 * (ks >> 1), s = ks & 1, kk = m % NKK;
+        const double2 P = (kk < p.nkk && re < p.nRe) ? p.phE[size_t(kk) * p.nRe + re] : make_double2(0., 0.);
+        const double v = (m < NKK) ? (s ? -P.y : P.x) : (s ? P.x : P.y);
+        As[m * LDA + k] = T(v);
       }
       __syncthreads();
-      const int rmax = min(kCacheRC, p.nRe - r0);
+      if (ch + 1 < nchunk) load_b(bn, re0 + 8);
 #pragma unroll
-      for (int u = 0; u < EPT; ++u) {
-        const int e = t + 256 * u;
-        if (e < E) {
-          const double2* gp = g + size_t(r0) * strideRe + e;
-          for (int r = 0; r < rmax; ++r) {
-            double2 x = gp[size_t(r) * strideRe];
+      for (int ks = 0; ks < 4; ++ks) {
+        T a[MTc];
 #pragma unroll
-            for (int k = 0; k < KK; ++k) {
-              double2 q = ph[k][r];
-              acc[u][k].x += q.x * x.x - q.y * x.y;
-              acc[u][k].y += q.x * x.y + q.y * x.x;
-            }
+        for (int m = 0; m < MTc; ++m) a[m] = As[(16 * m + lr) * LDA + 4 * ks + lg];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const T b = (ks & 1) ? bq[ks >> 1][j].y : bq[ks >> 1][j].x;
+#pragma unroll
+          for (int m = 0; m < MTc; ++m) acc[m][j] = MfmaC<T>::mac(a[m], b, acc[m][j]);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bq[h][j] = bn[h][j];
+    }
+    // epilogue in passes of 16 outer points: real tile q and imaginary tile MTc/2 + q
+#pragma unroll
+    for (int q = 0; q < MTc / 2; ++q) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int cl = (wave * 4 + j) * 16 + lr;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = MfmaC<T>::row(lg, i);
+          Cb[r * 256 + cl] = acc[q][j][i];
+          Cb[(16 + r) * 256 + cl] = acc[MTc / 2 + q][j][i];
+        }
+      }
+      __syncthreads();
+      for (int kl = 0; kl < 16; ++kl) {
+        const int kk = 16 * q + kl;
+        if (kk >= p.nkk) break;
+        const int nb1 = p.nb1[kk];
+        const double2* U1 = p.U1pool + p.U1off[kk];
+        const double2 s = p.phP[size_t(kk) * p.nRp + rp];
+        // columns of this pass: w1 in [c0/nw, (c0+256)/nw), all w2
+        const int w1lo = c0 / p.nw, w1hi = min(p.nw, (c0 + 256) / p.nw);
+        for (int o = t; o < nb1 * p.nw; o += 256) {
+          const int b1 = o / p.nw, w2 = o % p.nw;
+          double2 v = make_double2(0., 0.);
+          for (int w1 = w1lo; w1 < w1hi; ++w1) {
+            const int cl = w1 * p.nw + w2 - c0;
+            const double ar = double(Cb[kl * 256 + cl]), ai = double(Cb[(16 + kl) * 256 + cl]);
+            const double2 u = U1[w1 + p.nw * b1];
+            v.x += ar * u.x - ai * u.y;
+            v.y += ar * u.y + ai * u.x;
+          }
+          const double2 c = make_double2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
+          OutT* panel = Bout + size_t(p.panelBase[kk] + b1) * p.panelStride;
+          const size_t col = 2 * (size_t(nu) * p.nw + w2);
+          if (c0 == 0) {
+            panel[size_t(2 * rp) * p.ldb + col] = OutT(c.x);
+            panel[size_t(2 * rp) * p.ldb + col + 1] = OutT(c.y);
+            panel[size_t(2 * rp + 1) * p.ldb + col] = OutT(-c.y);
+            panel[size_t(2 * rp + 1) * p.ldb + col + 1] = OutT(c.x);
+          } else {  // later column passes add their w1 range
+            panel[size_t(2 * rp) * p.ldb + col] += OutT(c.x);
+            panel[size_t(2 * rp) * p.ldb + col + 1] += OutT(c.y);
+            panel[size_t(2 * rp + 1) * p.ldb + col] += OutT(-c.y);
+            panel[size_t(2 * rp + 1) * p.ldb + col + 1] += OutT(c.x);
           }
         }
-      }
-    }
-    // epilogue: rotate by U1 and scatter into the panels
-    for (int k = 0; k < nkk; ++k) {
-      const int kk = kk0 + k;
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < EPT; ++u) {
-        const int e = t + 256 * u;
-        if (e < E) buf[e] = acc[u][k];
-      }
-      __syncthreads();
-      const int nb1 = p.nb1[kk];
-      const double2* U1 = p.U1pool + p.U1off[kk];
-      const double2 s = p.phP[size_t(kk) * p.nRp + rp];
-      for (int o = t; o < nb1 * p.nw; o += 256) {
-        const int b1 = o / p.nw, w2 = o % p.nw;
-        double2 v = make_double2(0., 0.);
-        for (int w1 = 0; w1 < p.nw; ++w1) {
-          double2 a = buf[w1 * p.nw + w2], u = U1[w1 + p.nw * b1];
-          v.x += a.x * u.x - a.y * u.y;
-          v.y += a.x * u.y + a.y * u.x;
-        }
-        double2 c = make_double2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
-        OutT* panel = B + size_t(p.panelBase[kk] + b1) * p.panelStride;
-        const size_t col = 2 * (size_t(nu) * p.nw + w2);
-        panel[size_t(2 * rp) * p.ldb + col] = OutT(c.x);
-        panel[size_t(2 * rp) * p.ldb + col + 1] = OutT(c.y);
-        panel[size_t(2 * rp + 1) * p.ldb + col] = OutT(-c.y);
-        panel[size_t(2 * rp + 1) * p.ldb + col + 1] = OutT(c.x);
       }
     }
   }
 }
 
-template <typename OutT>
+inline size_t cache_lds_bytes(int elem) {
+  return std::max(size_t(2 * kCacheNKK) * 17 * elem, size_t(2) * 16 * 256 * elem);
+}
+
+template <typename T, typename OutT>
 void launch_cache(const CacheParams& p, hipStream_t s) {
   PHX_REQUIRE(p.nw <= 32, "cacheElPh kernel supports nw <= 32");
+  PHX_REQUIRE(p.nkk <= kCacheNKK, "cacheElPh kernel: too many outer points per launch");
   if (p.nkk <= 0) return;
-  const int E = p.nw * p.nw;
-  if (E <= 256)
-    hipLaunchKernelGGL((k_cache<OutT, 1, 16>), dim3(p.nRp * p.nm), dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL((k_cache<OutT, 4, 4>), dim3(p.nRp * p.nm), dim3(256), 0, s, p);
+  const size_t lds = cache_lds_bytes(sizeof(T));
+  auto kern = k_cache<T, OutT>;
+  PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL(kern, dim3(p.nRp * p.nm), dim3(256), lds, s, p);
   PHX_HIP(hipGetLastError());
 }
-template void launch_cache<double>(const CacheParams&, hipStream_t);
-template void launch_cache<float>(const CacheParams&, hipStream_t);
+template void launch_cache<double, double>(const CacheParams&, hipStream_t);
+template void launch_cache<float, float>(const CacheParams&, hipStream_t);
+
+// FP64 -> FP32 copy of the coupling tensor (the reduced-precision GEMMs read it).
+__global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+void launch_to_float(const double* a, float* b, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_float, dim3(8192), dim3(256), 0, s, a, b, n);
+  PHX_HIP(hipGetLastError());
+}
 
 // ------------------------------------------------------------------ coupling
 // (coupling_kernel.hpp, instantiated in coupling_f64.hip and coupling_f32.hip)
