@@ -33,10 +33,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(P, prob, precision, make_batches, warmup, k1_batch):
+FORMS = {  # name: (precision, transform)
+    "fp64": ("fp64", "direct"),
+    "fp32": ("fp32", "direct"),
+    "sep64": ("fp64", "separable"),
+    "sep32": ("fp32", "separable"),
+}
+
+
+def run_form(P, prob, form, make_batches, warmup, k1_batch):
     t0 = time.perf_counter()
-    lw = P.ElectronLinewidths(prob, gW=None, precision=precision, k1_batch=k1_batch)
-    log(f"[{precision}] setup {time.perf_counter() - t0:.1f}s: active {lw.num_active}, "
+    precision, transform = FORMS[form]
+    lw = P.ElectronLinewidths(prob, gW=None, precision=precision, k1_batch=k1_batch,
+                              transform=transform)
+    log(f"[{form}] setup {time.perf_counter() - t0:.1f}s: active {lw.num_active}, "
         f"states {lw.num_states}")
     k1_batches = make_batches(lw.num_active)
     for w in range(warmup):
@@ -53,15 +63,18 @@ def run_form(P, prob, precision, make_batches, warmup, k1_batch):
     elapsed = time.perf_counter() - t0
     kt = lw.kernel_times()
     gam = lw.linewidths(average=False)
-    log(f"[{precision}] {len(k1_batches)} steps in {elapsed:.3f}s, {pairs} pairs, "
+    log(f"[{form}] {len(k1_batches)} steps in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
     return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, linewidths=gam)
 
 
-def roofline(kt, precision):
+def roofline(kt, form):
+    precision, transform = FORMS[form]
     ach = kt["coupling_flops"] / (kt["coupling_ms"] * 1e-3) / 1e12 if kt["coupling_ms"] > 0 else 0.0
     peak = PEAK_TFLOPS[precision]
-    return {"bound": "mfma", "kernel": "k_coupling", "achieved": round(ach, 3), "peak": peak,
+    kern = "k_coupling" if transform == "direct" else "k_sep_z + k_sep_y + k_sep_x"
+    return {"bound": "mfma" if transform == "direct" else "valu", "kernel": kern,
+            "achieved": round(ach, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
             "avg_launch_ms": round(kt["coupling_ms"] / max(kt["batches"], 1), 4)}
 
@@ -104,8 +117,8 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--k1-batch", type=int, default=16)
-    ap.add_argument("--chosen", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--k1-batch", type=int, default=32)
+    ap.add_argument("--chosen", default="sep64", choices=["fp32", "sep64", "sep32"])
     ap.add_argument("--cpu-k1", type=int, default=1, help="outer points in the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -161,13 +174,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.chosen,
+        "dtype": FORMS[args.chosen][0],
         "data": "synthetic",
         "config": {"workload": f"{args.config}: Nw={prob.nw}, {prob.nm} modes, "
                                f"R_e WS(10^3)={prob.nRe}, R_ph WS(9^3)={prob.nRp}, "
                                f"{prob.mesh[0]}^3 k/q mesh, 0.5 eV window",
                    "k1_per_step": B, "k2_per_k1": int(r64["pairs"] // (B * args.steps)),
-                   "active_k": int(na), "precision_chosen": args.chosen},
+                   "active_k": int(na), "chosen_form": args.chosen,
+                   "chosen_precision": FORMS[args.chosen][0],
+                   "chosen_transform": FORMS[args.chosen][1]},
         "plain_fp64": {"value": v64, "ms_per_step": r64["seconds"] * 1e3 / args.steps,
                        "roofline": rf64},
         "speedup_vs_fp64": vc / v64,

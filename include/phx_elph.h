@@ -30,6 +30,14 @@ extern "C" {
  * eigensolves and the linewidth accumulation are always FP64 */
 enum { PHX_FP64 = 0, PHX_FP32 = 1, PHX_BF16X3 = 2 };
 
+/* how the R_ph sum of calcCouplingSquared is evaluated in the linewidth builder:
+ * DIRECT    - one GEMM over the R_ph vectors for every (k1, k2) pair (the
+ *             reference's algorithm, interaction_elph.cpp:357-378);
+ * SEPARABLE - k2 on the Gamma-centred mesh: e^{ik2.R} = prod_d w_d^{j_d n_d},
+ *             three short lattice-direction transforms over the whole mesh
+ *             (exact, same FP64 accuracy, far fewer operations). */
+enum { PHX_TRANSFORM_DIRECT = 0, PHX_TRANSFORM_SEPARABLE = 1 };
+
 /* Message of the last error on the calling thread ("" if none). */
 const char* phx_last_error(void);
 
@@ -149,7 +157,8 @@ typedef struct {
 typedef struct phx_lw phx_lw;
 
 /* Band structures, window, phonons on the q-set, device-resident operands. */
-int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch);
+int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch,
+                 int transform);
 int phx_lw_num_active(phx_lw* h);
 int phx_lw_num_states(phx_lw* h);
 /* active points (full-mesh indices), band extrema, energies of the states */

@@ -28,6 +28,8 @@ LIB_PATH = os.path.join(_HERE, "libphx_elph.so")
 
 PHX_FP64, PHX_FP32, PHX_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": PHX_FP64, "fp32": PHX_FP32, "bf16x3": PHX_BF16X3}
+TRANSFORM_DIRECT, TRANSFORM_SEPARABLE = 0, 1
+TRANSFORMS = {"direct": TRANSFORM_DIRECT, "separable": TRANSFORM_SEPARABLE}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -70,7 +72,8 @@ EXPORTS = {
     "phx_elph_get_coupling_dims": (C.c_int, [C.c_void_p, _ip]),
     "phx_elph_estimate_num_batches": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
     "phx_elph_destroy": (C.c_int, [C.c_void_p]),
-    "phx_lw_setup": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_int, C.c_int]),
+    "phx_lw_setup": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_int, C.c_int,
+                               C.c_int]),
     "phx_lw_num_active": (C.c_int, [C.c_void_p]),
     "phx_lw_num_states": (C.c_int, [C.c_void_p]),
     "phx_lw_get_window": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _dp]),
@@ -283,12 +286,13 @@ class ElectronLinewidths:
     Gaussian-delta reduction per outer point, degeneracy averaging.
     """
 
-    def __init__(self, prob, gW=None, precision="fp64", k1_batch=16):
+    def __init__(self, prob, gW=None, precision="fp64", k1_batch=32, transform="direct"):
         self.prob = prob
         self._s, self._keep = make_problem_struct(prob, gW)
         self._h = C.c_void_p()
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        _check(lib().phx_lw_setup(C.byref(self._h), C.byref(self._s), prec, int(k1_batch)))
+        tr = TRANSFORMS[transform] if isinstance(transform, str) else int(transform)
+        _check(lib().phx_lw_setup(C.byref(self._h), C.byref(self._s), prec, int(k1_batch), tr))
         self.num_active = lib().phx_lw_num_active(self._h)
         self.num_states = lib().phx_lw_num_states(self._h)
         self.active = np.zeros(self.num_active, dtype=np.int32)

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/phx_elph.h"
+#include "coupling_kernel.hpp"
 #include "phx_common.hpp"
 
 namespace phx {
@@ -54,27 +55,14 @@ struct CacheParams {
 template <typename T, typename OutT> void launch_cache(const CacheParams& p, hipStream_t s);
 constexpr int kCacheNKK = 32;  // outer points per cacheElPh launch
 
-struct CoupParams {
-  const void* A; int lda;
-  const void* B; size_t panelStride; int ldb;
-  int M, Kp, nm, nw, nPanels, nMtiles;
-  const int* panelKK;
-  const int* panelB1;
-  const double2* U2pool; const int* rowU2off; const int* rowNb2;
-  const double2* E3pool; const double* Wpool;
-  int qmode;
-  const int* rowE3; const int* rowMesh; const int* kkMesh; int m0, m1, m2;
-  double* out; const long long* rowOutOff; const int* kkNb1;
-  const double* e2pool; const int* rowStateOff;
-  const double* e1pool; const int* kkStateOff;
-  const int* panelGStart; const int* panelGSize;
-  const double* fermi; int numStates;
-  const double* bose;
-  int nCalc; double invSigma, deltaPref, norm, cutoff;
-  double* partial;
-  int nb2max;
-};
-template <typename T> void launch_coupling(const CoupParams& p, bool reduce, hipStream_t s);
+// coupling kernels: CoupParams / SepParams and their launchers (instantiated in
+// coupling_f64.hip and coupling_f32.hip)
+extern template void launch_coupling<double>(const CoupParams&, bool, hipStream_t);
+extern template void launch_coupling<float>(const CoupParams&, bool, hipStream_t);
+extern template void launch_separable<double>(const SepParams&, const CoupParams&, bool, int, int,
+                                              hipStream_t);
+extern template void launch_separable<float>(const SepParams&, const CoupParams&, bool, int, int,
+                                             hipStream_t);
 int coupling_rows_per_tile_f64(int ldb);
 
 struct CollectParams {
@@ -86,8 +74,6 @@ struct CollectParams {
 };
 void launch_collect(const CollectParams& p, hipStream_t s);
 
-constexpr int kGMax = 8;
-constexpr int kCMax = 4;
 
 // Run f(i) for i in [0, n) on the host's hardware threads.
 template <typename F>
@@ -494,6 +480,13 @@ struct LwHandle {
   DevBuf<double2> phE, phP;
   std::vector<int> hmeta;
   std::vector<double> hk1;
+  // separable mesh transform of the R_ph sum (transform == PHX_TRANSFORM_SEPARABLE)
+  int transform = 0;
+  int nLines = 0, nG0 = 0, maxLineE = 0, maxGrpE = 0;
+  DevBuf<int> lineStart, lineF2, lineRp, grpStart, grpF1, grpLine, g0val, meshToRow;
+  DevBuf<double2> twx, twy, twz;
+  DevBuf<double> T1d, T2d;
+  DevBuf<float> T1f, T2f;
   // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -522,7 +515,8 @@ static void lw_harvest_timing(LwHandle* L) {
   L->evPending = false;
 }
 
-static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch) {
+static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch,
+                     int transform) {
   const phx_problem& p = *P;
   PHX_REQUIRE(precision == PHX_FP64 || precision == PHX_FP32, "phx_lw_setup: precision not available");
   PHX_REQUIRE(p.nw >= 1 && p.nw <= 32 && p.nm >= 1 && p.nm <= 36, "phx_lw_setup: unsupported sizes");
@@ -531,6 +525,9 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   L->prob = p;
   L->precision = precision;
   L->k1Batch = std::min(kCacheNKK, std::max(1, k1Batch));
+  PHX_REQUIRE(transform == PHX_TRANSFORM_DIRECT || transform == PHX_TRANSFORM_SEPARABLE,
+              "phx_lw_setup: unknown transform");
+  L->transform = transform;
   L->nw = p.nw;
   L->nm = p.nm;
   L->nRp = p.nRp;
@@ -734,12 +731,89 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   L->ldb = int(round_up(2 * size_t(nm) * nw, 16));
   L->rows = coupling_rows_per_tile_f64(L->ldb);
   L->rowsPad = int(round_up(size_t(na), L->rows)) + L->rows;
-  if (precision == PHX_FP64) {
-    L->Ad.alloc(size_t(L->rowsPad) * L->Kp);
-    launch_phase_rows<double>(na, L->rowsPad, p.nRp, L->Kp, L->kcartActive.p, L->Rp.p, L->Ad.p, s);
+  if (L->transform == PHX_TRANSFORM_DIRECT) {
+    if (precision == PHX_FP64) {
+      L->Ad.alloc(size_t(L->rowsPad) * L->Kp);
+      launch_phase_rows<double>(na, L->rowsPad, p.nRp, L->Kp, L->kcartActive.p, L->Rp.p, L->Ad.p, s);
+    } else {
+      L->Af.alloc(size_t(L->rowsPad) * L->Kp);
+      launch_phase_rows<float>(na, L->rowsPad, p.nRp, L->Kp, L->kcartActive.p, L->Rp.p, L->Af.p, s);
+    }
   } else {
-    L->Af.alloc(size_t(L->rowsPad) * L->Kp);
-    launch_phase_rows<float>(na, L->rowsPad, p.nRp, L->Kp, L->kcartActive.p, L->Rp.p, L->Af.p, s);
+    // ---- separable transform: integer coordinates n_d = b_d.R / 2pi of every R_ph,
+    //      folded mod m_d; z-lines = distinct (f0, f1), x-groups = distinct f0.
+    const int m[3] = {p.mesh[0], p.mesh[1], p.mesh[2]};
+    std::vector<int> f(3 * size_t(p.nRp));
+    for (int r = 0; r < p.nRp; ++r)
+      for (int d = 0; d < 3; ++d) {
+        double x = 0.;
+        for (int i = 0; i < 3; ++i) x += p.recip[i + 3 * d] * p.Rp[3 * r + i];
+        x /= 2. * kPi;
+        const double n = std::round(x);
+        PHX_REQUIRE(std::abs(n - x) < 1e-6, "separable transform: R_ph is not a lattice vector");
+        f[3 * r + d] = ((int(n) % m[d]) + m[d]) % m[d];
+      }
+    std::vector<int> lineOf(size_t(m[0]) * m[1], -1), grpOf(m[0], -1);
+    std::vector<std::vector<int>> lineE;  // R_ph indices of each line
+    std::vector<int> lineKey;             // f0 + m0 * f1
+    for (int r = 0; r < p.nRp; ++r) {
+      const int key = f[3 * r] + m[0] * f[3 * r + 1];
+      if (lineOf[key] < 0) {
+        lineOf[key] = int(lineE.size());
+        lineE.emplace_back();
+        lineKey.push_back(key);
+      }
+      lineE[lineOf[key]].push_back(r);
+    }
+    std::vector<std::vector<int>> grpE;   // lines of each group
+    std::vector<int> g0v;
+    for (size_t l = 0; l < lineE.size(); ++l) {
+      const int f0 = lineKey[l] % m[0];
+      if (grpOf[f0] < 0) {
+        grpOf[f0] = int(grpE.size());
+        grpE.emplace_back();
+        g0v.push_back(f0);
+      }
+      grpE[grpOf[f0]].push_back(int(l));
+    }
+    std::vector<int> ls{0}, lf2, lrp, gs{0}, gf1, gl;
+    for (auto& e : lineE) {
+      for (int r : e) {
+        lf2.push_back(f[3 * r + 2]);
+        lrp.push_back(r);
+      }
+      ls.push_back(int(lrp.size()));
+      L->maxLineE = std::max(L->maxLineE, int(e.size()));
+    }
+    for (auto& e : grpE) {
+      for (int l : e) {
+        gf1.push_back(lineKey[l] / m[0]);
+        gl.push_back(l);
+      }
+      gs.push_back(int(gl.size()));
+      L->maxGrpE = std::max(L->maxGrpE, int(e.size()));
+    }
+    L->nLines = int(lineE.size());
+    L->nG0 = int(grpE.size());
+    h2d(L->lineStart, ls, s);
+    h2d(L->lineF2, lf2, s);
+    h2d(L->lineRp, lrp, s);
+    h2d(L->grpStart, gs, s);
+    h2d(L->grpF1, gf1, s);
+    h2d(L->grpLine, gl, s);
+    h2d(L->g0val, g0v, s);
+    auto tw = [&](DevBuf<double2>& d, int n) {
+      std::vector<double2> v(n);
+      for (int t = 0; t < n; ++t) v[t] = make_double2(std::cos(2. * kPi * t / n), std::sin(2. * kPi * t / n));
+      h2d(d, v, s);
+      PHX_HIP(hipStreamSynchronize(s));
+    };
+    tw(L->twx, m[0]);
+    tw(L->twy, m[1]);
+    tw(L->twz, m[2]);
+    std::vector<int> m2r(nk, -1);
+    for (int a = 0; a < na; ++a) m2r[L->active[a]] = a;
+    h2d(L->meshToRow, m2r, s);
   }
   L->lw.alloc(size_t(p.nCalc) * st);
   launch_fill<double>(L->lw.p, L->lw.n, 0., s);
@@ -766,7 +840,8 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     }
   }
   const int M = k2e - k2b;
-  const int nMtiles = ceil_div(M, L->rows);
+  const int nMtiles = L->transform == PHX_TRANSFORM_DIRECT ? ceil_div(M, L->rows)
+                                                         : p.mesh[1] * p.mesh[2];
   L->partial.reserve(size_t(L->maxPanels) * std::max(nMtiles, 1) * kGMax * kCMax);
   // metadata: [U1off nkk][nb1 nkk][panelBase nkk][kkMesh nkk][kkStateOff nkk]
   //           [panelKK P][panelB1 P][panelGStart P][panelGSize P]
@@ -894,22 +969,84 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   int nb2max = 1;
   for (int a = k2b; a < k2e; ++a) nb2max = std::max(nb2max, L->nb[a]);
   cq.nb2max = nb2max;
-  if (L->precision == PHX_FP64) {
-    cq.A = L->Ad.p + size_t(k2b) * L->Kp;
-    cq.B = L->Bd.p;
-    launch_coupling<double>(cq, true, s);
+  if (L->transform == PHX_TRANSFORM_DIRECT) {
+    if (L->precision == PHX_FP64) {
+      cq.A = L->Ad.p + size_t(k2b) * L->Kp;
+      cq.B = L->Bd.p;
+      launch_coupling<double>(cq, true, s);
+    } else {
+      cq.A = L->Af.p + size_t(k2b) * L->Kp;
+      cq.B = L->Bf.p;
+      launch_coupling<float>(cq, true, s);
+    }
   } else {
-    cq.A = L->Af.p + size_t(k2b) * L->Kp;
-    cq.B = L->Bf.p;
-    launch_coupling<float>(cq, true, s);
+    const int ncol = nm * nw, mx = p.mesh[0], my = p.mesh[1], mz = p.mesh[2];
+    SepParams sp{};
+    sp.panelStride = pstride;
+    sp.ldb = L->ldb;
+    sp.ncol = ncol;
+    sp.nPanels = nPanels;
+    sp.mx = mx;
+    sp.my = my;
+    sp.mz = mz;
+    sp.nLines = L->nLines;
+    sp.lineStart = L->lineStart.p;
+    sp.lineF2 = L->lineF2.p;
+    sp.lineRp = L->lineRp.p;
+    sp.twz = L->twz.p;
+    sp.t1PanelStride = size_t(L->nLines) * mz * ncol * 2;
+    sp.nG0 = L->nG0;
+    sp.grpStart = L->grpStart.p;
+    sp.grpF1 = L->grpF1.p;
+    sp.grpLine = L->grpLine.p;
+    sp.twy = L->twy.p;
+    sp.t2PanelStride = size_t(L->nG0) * my * mz * ncol * 2;
+    // rows are addressed by active index: no k2 offset, range through rowLo/rowHi
+    cq.rowU2off = L->rowU2off.p;
+    cq.rowNb2 = L->rowNb2.p;
+    cq.rowStateOff = L->rowStateOff.p;
+    cq.rowLo = k2b;
+    cq.rowHi = k2e;
+    cq.meshToRow = L->meshToRow.p;
+    cq.t2PanelStride = sp.t2PanelStride;
+    cq.nG0 = L->nG0;
+    cq.g0val = L->g0val.p;
+    cq.tw0 = L->twx.p;
+    if (L->precision == PHX_FP64) {
+      L->T1d.reserve(sp.t1PanelStride * nPanels);
+      L->T2d.reserve(sp.t2PanelStride * nPanels);
+      sp.B = L->Bd.p;
+      sp.T1 = L->T1d.p;
+      sp.T2 = L->T2d.p;
+      cq.T2 = L->T2d.p;
+      launch_separable<double>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
+    } else {
+      L->T1f.reserve(sp.t1PanelStride * nPanels);
+      L->T2f.reserve(sp.t2PanelStride * nPanels);
+      sp.B = L->Bf.p;
+      sp.T1 = L->T1f.p;
+      sp.T2 = L->T2f.p;
+      cq.T2 = L->T2f.p;
+      launch_separable<float>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
+    }
   }
   if (L->timing) {
     PHX_HIP(hipEventRecord(L->ev[2], s));
-    // algorithmic FLOPs per (pair, b1): R_ph transform, phonon and electron rotations, |g|^2
     double f = 0.;
-    for (int a = k2b; a < k2e; ++a) {
-      const double nb2 = L->nb[a];
-      f += 8. * L->nRp * nm * nw + 8. * nm * nm * nw + 8. * nm * nw * nb2 + 3. * nm * nb2;
+    if (L->transform == PHX_TRANSFORM_DIRECT) {
+      // algorithmic FLOPs per (pair, b1): R_ph transform, electron and phonon rotations, |g|^2
+      for (int a = k2b; a < k2e; ++a) {
+        const double nb2 = L->nb[a];
+        f += 8. * L->nRp * nm * nw + 8. * nm * nw * nb2 + 8. * nm * nm * nb2 + 3. * nm * nb2;
+      }
+    } else {
+      // three lattice-direction transforms over the whole mesh, then the rotations per pair
+      const double ncol = double(nm) * nw, mx = p.mesh[0], my = p.mesh[1], mz = p.mesh[2];
+      f = 8. * ncol * (double(L->nRp) * mz + double(L->nLines) * my * mz + double(L->nG0) * mx * my * mz);
+      for (int a = k2b; a < k2e; ++a) {
+        const double nb2 = L->nb[a];
+        f += 8. * nm * nw * nb2 + 8. * nm * nm * nb2 + 3. * nm * nb2;
+      }
     }
     L->couplingFlops += f * nPanels;
   }
@@ -1138,12 +1275,12 @@ int phx_elph_destroy(phx_elph* h) {
   return guard([&] { delete h; });
 }
 
-int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch) {
+int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch, int transform) {
   return guard([&] {
     PHX_REQUIRE(h && prob, "phx_lw_setup: null argument");
     auto* L = new phx_lw();
     try {
-      lw_setup(L, prob, precision, k1_batch);
+      lw_setup(L, prob, precision, k1_batch, transform);
     } catch (...) {
       delete L;
       throw;

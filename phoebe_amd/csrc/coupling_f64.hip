@@ -1,7 +1,8 @@
-// F64 instantiation of the coupling kernel (coupling_kernel.hpp).
+// F64 instantiation of the coupling kernels (coupling_kernel.hpp).
 #include "coupling_kernel.hpp"
 
 namespace phx {
 template void launch_coupling<double>(const CoupParams&, bool, hipStream_t);
+template void launch_separable<double>(const SepParams&, const CoupParams&, bool, int, int, hipStream_t);
 int coupling_rows_per_tile_f64(int ldb) { return coupling_rows_per_tile(ldb); }
 }  // namespace phx

@@ -1,6 +1,22 @@
-// R_ph Fourier transform + rotations + |g|^2 (+ fused symmetrization and
-// Gaussian-delta reduction) as one MFMA kernel template; instantiated per
-// GEMM precision in coupling_f64.hip / coupling_f32.hip.
+// Coupling kernels for one batch of outer points k1 (panels = (k1, b1) pairs):
+//
+//  * k_coupling  : direct R_ph transform, g3 = A B' with A(row, .) = e^{ik2.Rp}
+//                  (or e^{iq.Rp} for explicit q) on v_mfma_{f64,f32}_16x16x4,
+//                  followed by the shared epilogue;
+//  * k_sep_*     : the same g3 for k2 on the Gamma-centred mesh as three short
+//                  lattice-direction transforms e^{ik2.Rp} = prod_d w_d^{j_d n_d}
+//                  (w_d = e^{2 pi i / m_d}, n = integer coordinates of Rp folded
+//                  mod m), the last one fused with the shared epilogue.
+//
+// Shared epilogue (per 16 rows, g3(nu, w2) of each row in LDS):
+//   g'(nu, b2)  = sum_w2 conj(U2(w2, b2)) g3(nu, w2)    (interaction_elph.cpp:393-402)
+//   g(nu2, b2)  = sum_nu e3(nu, nu2) g'(nu, b2)         (interaction_elph.cpp:381-390)
+// i.e. the reference's two rotations applied in the cheaper order (the electron
+// rotation first shrinks w2 -> b2), then |g|^2 and either a store in the
+// getCouplingSquared layout or symmetrizeCoupling along b2 and nu plus the
+// Gaussian-delta weights of el_scattering.cpp:277-380 into per-workgroup partials.
+//
+// Instantiated per precision of the transforms in coupling_f64.hip / coupling_f32.hip.
 #pragma once
 
 #include <algorithm>
@@ -12,13 +28,13 @@ namespace phx {
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
-// ------------------------------------------------------------------ coupling
 constexpr int kGMax = 8;  // largest degenerate b1 group handled in one panel
 constexpr int kCMax = 4;  // largest number of (T, mu) calculations
 constexpr int kNmMax = 36;
 constexpr int kKC = 16;   // K chunk staged in LDS
 
 struct CoupParams {
+  // direct transform operands
   const void* A; int lda;                 // rows (padded to the row tile) x lda
   const void* B; size_t panelStride; int ldb;
   int M, Kp, nm, nw, nPanels, nMtiles;
@@ -41,6 +57,12 @@ struct CoupParams {
   int nCalc; double invSigma, deltaPref, norm, cutoff;
   double* partial;                        // (L = panel*nMtiles + mtile, kGMax, kCMax)
   int nb2max;
+  // separable mesh transform
+  const int* meshToRow;                   // full mesh index -> row (active point) or -1
+  const void* T2; size_t t2PanelStride;   // (panel, f0 group, jy, jz, col) complex
+  int nG0; const int* g0val;              // folded x coordinate of each group
+  const double2* tw0;                     // e^{2 pi i t / m0}, t < m0
+  int rowLo, rowHi;                       // rows (active indices) taken by the mesh transform
 };
 
 __device__ inline double gaussian_delta(double energy, double invW, double pref) {
@@ -69,7 +91,149 @@ template <> struct Mfma4<float> {
   __device__ static int row(int lk, int i) { return 4 * lk + i; }
 };
 
-template <typename T, int MT, int NTW, bool REDUCE, int NMX>
+// LDS bytes of the shared epilogue (g3 tile, g', |g|^2 per row).
+inline size_t epilogue_lds_bytes(int elem, int ldb, int nm, int nb2max) {
+  return size_t(16) * ldb * elem + size_t(16) * nm * nb2max * (sizeof(double2) + sizeof(double));
+}
+
+// Epilogue for the 16 rows whose g3 blocks are in LDS (row r at g3 + r*PWp,
+// (re, im) of column (nu, w2) at 2(nu*nw + w2)).  Thread t serves row t/16;
+// myRow < 0 marks an empty slot.  Must be called by all 256 threads.
+template <typename T, bool REDUCE>
+__device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned char* scratch,
+                           int panel, int kk, int b1p, int myRow, int myIq,
+                           double (&part)[kGMax][kCMax]) {
+  const int t = threadIdx.x, r = t >> 4, sub = t & 15;
+  const int nm = p.nm, nw = p.nw;
+  const int stride = nm * p.nb2max;
+  double2* gp = reinterpret_cast<double2*>(scratch) + size_t(r) * stride;
+  double* gs = reinterpret_cast<double*>(reinterpret_cast<double2*>(scratch) + size_t(16) * stride) +
+               size_t(r) * stride;
+  const bool valid = myRow >= 0;
+  const int nb2 = valid ? p.rowNb2[myRow] : 0;
+  const T* g = g3 + size_t(r) * PWp;
+  const double2* U2 = p.U2pool + (valid ? p.rowU2off[myRow] : 0);
+  for (int o = sub; o < nm * nb2; o += 16) {
+    const int nu = o / nb2, b2 = o % nb2;
+    double2 s = make_double2(0., 0.);
+    for (int w2 = 0; w2 < nw; ++w2) {
+      const double2 u = U2[w2 + nw * b2];
+      const double xr = double(g[2 * (nu * nw + w2)]), xi = double(g[2 * (nu * nw + w2) + 1]);
+      s.x += u.x * xr + u.y * xi;
+      s.y += u.x * xi - u.y * xr;
+    }
+    gp[nu * nb2 + b2] = s;
+  }
+  __syncthreads();
+  const double2* e3 = p.E3pool + size_t(myIq) * nm * nm;
+  for (int o = sub; o < nm * nb2; o += 16) {
+    const int nu2 = o / nb2, b2 = o % nb2;
+    double2 s = make_double2(0., 0.);
+    for (int nu = 0; nu < nm; ++nu) {
+      const double2 e = e3[nu + nm * nu2], x = gp[nu * nb2 + b2];
+      s.x += x.x * e.x - x.y * e.y;
+      s.y += x.x * e.y + x.y * e.x;
+    }
+    const double v = s.x * s.x + s.y * s.y;
+    if (REDUCE) {
+      gs[nu2 * nb2 + b2] = v;
+    } else {
+      const int nb1 = p.kkNb1[kk];
+      p.out[p.rowOutOff[myRow] + b1p + size_t(nb1) * (b2 + size_t(nb2) * nu2)] = v;
+    }
+  }
+  if (!REDUCE) return;
+  __syncthreads();
+
+  const double* w3 = p.Wpool + size_t(myIq) * nm;
+  const double* e2 = p.e2pool + (valid ? p.rowStateOff[myRow] : 0);
+  // symmetrizeCoupling (scattering.cpp:1209-1318) along b2, then nu
+  for (int nu = sub; nu < nm && valid; nu += 16) {
+    for (int b = 0; b < nb2; ++b) {
+      int deg = 0;
+      for (int i = b; i < nb2; ++i) {
+        if (fabs(e2[b] - e2[i]) < 1.0e-6) deg++;
+        else break;
+      }
+      if (deg > 1) {
+        double s = 0.;
+        for (int i = 0; i < deg; ++i) s += gs[nu * nb2 + b + i] / double(float(deg));
+        for (int i = 0; i < deg; ++i) gs[nu * nb2 + b + i] = s;
+      }
+      b += deg - 1;
+    }
+  }
+  __syncthreads();
+  for (int b2 = sub; b2 < nb2; b2 += 16) {
+    for (int nu = 0; nu < nm; ++nu) {
+      int deg = 0;
+      for (int i = nu; i < nm; ++i) {
+        if (fabs(w3[nu] - w3[i]) < 1.0e-6) deg++;
+        else break;
+      }
+      if (deg > 1) {
+        double s = 0.;
+        for (int i = 0; i < deg; ++i) s += gs[(nu + i) * nb2 + b2] / double(float(deg));
+        for (int i = 0; i < deg; ++i) gs[(nu + i) * nb2 + b2] = s;
+      }
+      nu += deg - 1;
+    }
+  }
+  __syncthreads();
+
+  // Gaussian-delta weights (el_scattering.cpp:277-380, switchCase 2)
+  const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
+  const double* e1 = p.e1pool + p.kkStateOff[kk];
+  const int nCalc = p.nCalc;
+  const double inv_g = 1. / double(float(gsize));
+  for (int o = sub; o < nm * nb2; o += 16) {
+    const int nu = o / nb2, b2 = o % nb2;
+    const double en3 = w3[nu];
+    if (en3 < p.cutoff) continue;
+    const double cpl = gs[nu * nb2 + b2] * inv_g;
+    const double en2 = e2[b2];
+    const int is2 = p.rowStateOff[myRow] + b2;
+#pragma unroll
+    for (int gi = 0; gi < kGMax; ++gi) {
+      if (gi >= gsize) break;
+      const double en1 = e1[gstart + gi];
+      const double d1 = gaussian_delta(en1 - en2 + en3, p.invSigma, p.deltaPref);
+      const double d2 = gaussian_delta(en1 - en2 - en3, p.invSigma, p.deltaPref);
+      if (d1 <= 0. && d2 <= 0.) continue;
+#pragma unroll
+      for (int c = 0; c < kCMax; ++c) {
+        if (c >= nCalc) break;
+        const double f2 = p.fermi[size_t(c) * p.numStates + is2];
+        const double n3 = p.bose[(size_t(myIq) * nm + nu) * nCalc + c];
+        part[gi][c] += cpl * ((f2 + n3) * d1 + (1. - f2 + n3) * d2) * p.norm / en3 * 3.14159265358979323846;
+      }
+    }
+  }
+}
+
+// Block reduction of the per-thread partial linewidths into partial[L].
+__device__ inline void reduce_partials(const CoupParams& p, int L, unsigned char* smem,
+                                       double (&part)[kGMax][kCMax]) {
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  double* red = reinterpret_cast<double*>(smem);
+  __syncthreads();
+#pragma unroll
+  for (int gi = 0; gi < kGMax; ++gi)
+#pragma unroll
+    for (int c = 0; c < kCMax; ++c) {
+      double v = part[gi][c];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red[(gi * kCMax + c) * 4 + wave] = v;
+    }
+  __syncthreads();
+  if (t < kGMax * kCMax) {
+    double v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
+    p.partial[size_t(L) * kGMax * kCMax + t] = v;
+  }
+}
+
+// ------------------------------------------------------------------ direct transform
+template <typename T, int MT, int NTW, bool REDUCE>
 __global__ void __launch_bounds__(256, 2) k_coupling(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int ROWS = 16 * MT;
@@ -167,26 +331,15 @@ __global__ void __launch_bounds__(256, 2) k_coupling(CoupParams p) {
     __syncthreads();
   }
 
-  // ---------------- epilogue, 16 rows per pass: g3 rows into LDS (16 x PWp, type T),
-  // rotations, |g|^2, then store (calcCouplingSquared) or symmetrize + Gaussian reduce.
-  const int nm = p.nm, nw = p.nw;
-  constexpr int TPR = 16;  // threads per row
-  const int r = t / TPR, sub = t % TPR;
   const int kk = p.panelKK[panel];
   const int b1p = p.panelB1[panel];
   T* g3 = reinterpret_cast<T*>(smem);
-  T* g = g3 + size_t(r) * PWp;
-  double* gs = reinterpret_cast<double*>(g3 + size_t(16) * PWp) + size_t(r) * nm * p.nb2max;
-  const int gstart = REDUCE ? p.panelGStart[panel] : 0;
-  const int gsize = REDUCE ? p.panelGSize[panel] : 1;
-  const double* e1 = REDUCE ? p.e1pool + p.kkStateOff[kk] : nullptr;
-  const int nCalc = p.nCalc;
+  unsigned char* scratch = smem + size_t(16) * PWp * sizeof(T);
   double part[kGMax][kCMax];
 #pragma unroll
   for (int gi = 0; gi < kGMax; ++gi)
 #pragma unroll
     for (int c = 0; c < kCMax; ++c) part[gi][c] = 0.;
-
 #pragma unroll
   for (int h = 0; h < MT; ++h) {
     __syncthreads();
@@ -199,150 +352,28 @@ __global__ void __launch_bounds__(256, 2) k_coupling(CoupParams p) {
       }
     }
     __syncthreads();
-    const int row = row0 + 16 * h + r;
-    const bool valid = row < p.M;
-    int iq = 0;
-    if (valid) iq = p.qmode ? mesh_diff_index(p.rowMesh[row], p.kkMesh[kk], p.m0, p.m1, p.m2) : p.rowE3[row];
-    const double2* e3 = p.E3pool + size_t(iq) * nm * nm;
-
-    // g4(nu2, w2) = sum_nu e3(nu, nu2) g3(nu, w2)  (interaction_elph.cpp:381-390), in place per w2
-    if (valid) {
-      for (int w2 = sub; w2 < nw; w2 += TPR) {
-        double2 x[NMX];
-#pragma unroll
-        for (int nu = 0; nu < NMX; ++nu)
-          if (nu < nm) x[nu] = make_double2(double(g[2 * (nu * nw + w2)]), double(g[2 * (nu * nw + w2) + 1]));
-        for (int nu2 = 0; nu2 < nm; ++nu2) {
-          double2 s = make_double2(0., 0.);
-#pragma unroll
-          for (int nu = 0; nu < NMX; ++nu)
-            if (nu < nm) {
-              double2 e = e3[nu + nm * nu2];
-              s.x += x[nu].x * e.x - x[nu].y * e.y;
-              s.y += x[nu].x * e.y + x[nu].y * e.x;
-            }
-          g[2 * (nu2 * nw + w2)] = T(s.x);
-          g[2 * (nu2 * nw + w2) + 1] = T(s.y);
-        }
-      }
-    }
-    __syncthreads();
-
-    // g(nu, b1, b2) = sum_w2 conj(U2(w2, b2)) g4(nu, w2)  (interaction_elph.cpp:393-402)
-    const int nb2 = valid ? p.rowNb2[row] : 0;
-    const double2* U2 = p.U2pool + (valid ? p.rowU2off[row] : 0);
-    for (int o = sub; o < nm * nb2; o += TPR) {
-      const int nu = o / nb2, b2 = o % nb2;
-      double2 s = make_double2(0., 0.);
-      for (int w2 = 0; w2 < nw; ++w2) {
-        double2 u = U2[w2 + nw * b2];
-        double2 x = make_double2(double(g[2 * (nu * nw + w2)]), double(g[2 * (nu * nw + w2) + 1]));
-        s.x += u.x * x.x + u.y * x.y;
-        s.y += u.x * x.y - u.y * x.x;
-      }
-      const double v = s.x * s.x + s.y * s.y;
-      if (REDUCE) {
-        gs[nu * nb2 + b2] = v;
-      } else {
-        const int nb1 = p.kkNb1[kk];
-        p.out[p.rowOutOff[row] + b1p + size_t(nb1) * (b2 + size_t(nb2) * nu)] = v;
-      }
-    }
-    if (!REDUCE) continue;
-    __syncthreads();
-
-    const double* w3 = p.Wpool + size_t(iq) * nm;
-    const double* e2 = p.e2pool + (valid ? p.rowStateOff[row] : 0);
-    // symmetrizeCoupling (scattering.cpp:1209-1318) along b2, then nu
-    for (int nu = sub; nu < nm && valid; nu += TPR) {
-      for (int b = 0; b < nb2; ++b) {
-        int deg = 0;
-        for (int i = b; i < nb2; ++i) {
-          if (fabs(e2[b] - e2[i]) < 1.0e-6) deg++;
-          else break;
-        }
-        if (deg > 1) {
-          double s = 0.;
-          for (int i = 0; i < deg; ++i) s += gs[nu * nb2 + b + i] / double(float(deg));
-          for (int i = 0; i < deg; ++i) gs[nu * nb2 + b + i] = s;
-        }
-        b += deg - 1;
-      }
-    }
-    __syncthreads();
-    for (int b2 = sub; b2 < nb2; b2 += TPR) {
-      for (int nu = 0; nu < nm; ++nu) {
-        int deg = 0;
-        for (int i = nu; i < nm; ++i) {
-          if (fabs(w3[nu] - w3[i]) < 1.0e-6) deg++;
-          else break;
-        }
-        if (deg > 1) {
-          double s = 0.;
-          for (int i = 0; i < deg; ++i) s += gs[(nu + i) * nb2 + b2] / double(float(deg));
-          for (int i = 0; i < deg; ++i) gs[(nu + i) * nb2 + b2] = s;
-        }
-        nu += deg - 1;
-      }
-    }
-    __syncthreads();
-
-    // Gaussian-delta weights (el_scattering.cpp:277-380, switchCase 2)
-    const double inv_g = 1. / double(float(gsize));
-    for (int o = sub; o < nm * nb2; o += TPR) {
-      const int nu = o / nb2, b2 = o % nb2;
-      const double en3 = w3[nu];
-      if (en3 < p.cutoff) continue;
-      const double cpl = gs[nu * nb2 + b2] * inv_g;
-      const double en2 = e2[b2];
-      const int is2 = p.rowStateOff[row] + b2;
-#pragma unroll
-      for (int gi = 0; gi < kGMax; ++gi) {
-        if (gi >= gsize) break;
-        const double en1 = e1[gstart + gi];
-        const double d1 = gaussian_delta(en1 - en2 + en3, p.invSigma, p.deltaPref);
-        const double d2 = gaussian_delta(en1 - en2 - en3, p.invSigma, p.deltaPref);
-        if (d1 <= 0. && d2 <= 0.) continue;
-#pragma unroll
-        for (int c = 0; c < kCMax; ++c) {
-          if (c >= nCalc) break;
-          const double f2 = p.fermi[size_t(c) * p.numStates + is2];
-          const double n3 = p.bose[(size_t(iq) * nm + nu) * nCalc + c];
-          part[gi][c] += cpl * ((f2 + n3) * d1 + (1. - f2 + n3) * d2) * p.norm / en3 * 3.14159265358979323846;
-        }
-      }
-    }
+    const int row = row0 + 16 * h + (t >> 4);
+    const int myRow = row < p.M ? row : -1;
+    int myIq = 0;
+    if (myRow >= 0)
+      myIq = p.qmode ? mesh_diff_index(p.rowMesh[myRow], p.kkMesh[kk], p.m0, p.m1, p.m2) : p.rowE3[myRow];
+    epilogue16<T, REDUCE>(p, g3, PWp, scratch, panel, kk, b1p, myRow, myIq, part);
   }
-  if (!REDUCE) return;
-  // block reduction of part -> partial[L]
-  double* red = reinterpret_cast<double*>(smem);  // reuse (after the barrier)
-  __syncthreads();
-#pragma unroll
-  for (int gi = 0; gi < kGMax; ++gi)
-#pragma unroll
-    for (int c = 0; c < kCMax; ++c) {
-      double v = part[gi][c];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0) red[(gi * kCMax + c) * 4 + wave] = v;
-    }
-  __syncthreads();
-  if (t < kGMax * kCMax) {
-    double v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
-    p.partial[size_t(L) * kGMax * kCMax + t] = v;
-  }
+  if (REDUCE) reduce_partials(p, L, smem, part);
 }
 
 inline size_t coupling_lds_bytes(int elem, int MT, int ldb, int nm, int nb2max, bool reduce) {
+  (void)reduce;
   const int ROWS = 16 * MT;
   size_t a = 2 * size_t(ROWS) * (kKC + 1) * elem;
-  size_t e = size_t(16) * ldb * elem + (reduce ? size_t(16) * nm * nb2max * sizeof(double) : 0);
+  size_t e = epilogue_lds_bytes(elem, ldb, nm, nb2max);
   size_t r = size_t(kGMax) * kCMax * 4 * sizeof(double);
   return std::max(std::max(a, e), r);
 }
 
-template <typename T, int MT, int NTW, bool REDUCE, int NMX>
+template <typename T, int MT, int NTW, bool REDUCE>
 static void launch_coupling_t(const CoupParams& p, int grid, size_t lds, hipStream_t s) {
-  auto kern = k_coupling<T, MT, NTW, REDUCE, NMX>;
+  auto kern = k_coupling<T, MT, NTW, REDUCE>;
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, p);
@@ -351,10 +382,10 @@ static void launch_coupling_t(const CoupParams& p, int grid, size_t lds, hipStre
 
 // Panels up to 384 real columns (at most 6 column tiles per wave) use 32-row
 // tiles, wider panels 16-row tiles, so that accumulators stay in registers.
-template <typename T, bool REDUCE, int NMX>
+template <typename T, bool REDUCE>
 static void dispatch_ntw(int ntw, const CoupParams& p, int grid, size_t lds, hipStream_t s) {
   switch (ntw) {
-#define PHX_NTW(mt, n) case n: launch_coupling_t<T, mt, n, REDUCE, NMX>(p, grid, lds, s); break;
+#define PHX_NTW(mt, n) case n: launch_coupling_t<T, mt, n, REDUCE>(p, grid, lds, s); break;
     PHX_NTW(2, 1) PHX_NTW(2, 2) PHX_NTW(2, 3) PHX_NTW(2, 4) PHX_NTW(2, 5) PHX_NTW(2, 6)
     PHX_NTW(1, 7) PHX_NTW(1, 8) PHX_NTW(1, 9)
 #undef PHX_NTW
@@ -365,8 +396,7 @@ static void dispatch_ntw(int ntw, const CoupParams& p, int grid, size_t lds, hip
 inline int coupling_rows_per_tile(int ldb) { return ldb <= 384 ? 32 : 16; }
 
 template <typename T>
-void launch_coupling(const CoupParams& p0, bool reduce, hipStream_t s) {
-  CoupParams p = p0;
+void launch_coupling(const CoupParams& p, bool reduce, hipStream_t s) {
   PHX_REQUIRE(p.nm <= kNmMax, "coupling kernel supports nm <= 36");
   PHX_REQUIRE(p.ldb % 16 == 0 && p.Kp % kKC == 0, "coupling kernel: bad padding");
   const int MT = coupling_rows_per_tile(p.ldb) / 16;
@@ -376,17 +406,178 @@ void launch_coupling(const CoupParams& p0, bool reduce, hipStream_t s) {
   const int grid = ceil_div(tiles, 8) * 8;
   const size_t lds = coupling_lds_bytes(sizeof(T), MT, p.ldb, p.nm, p.nb2max, reduce);
   PHX_REQUIRE(lds <= 160 * 1024, "coupling kernel: LDS budget exceeded");
-  if (p.nm <= 12) {
-    if (reduce) dispatch_ntw<T, true, 12>(ntw, p, grid, lds, s);
-    else dispatch_ntw<T, false, 12>(ntw, p, grid, lds, s);
-  } else if (p.nm <= 18) {
-    if (reduce) dispatch_ntw<T, true, 18>(ntw, p, grid, lds, s);
-    else dispatch_ntw<T, false, 18>(ntw, p, grid, lds, s);
-  } else {
-    if (reduce) dispatch_ntw<T, true, 36>(ntw, p, grid, lds, s);
-    else dispatch_ntw<T, false, 36>(ntw, p, grid, lds, s);
+  if (reduce) dispatch_ntw<T, true>(ntw, p, grid, lds, s);
+  else dispatch_ntw<T, false>(ntw, p, grid, lds, s);
+}
+
+// ------------------------------------------------------------------ separable transform
+// Integer layout of the R_ph set folded on the mesh:
+//   lines  = distinct (f0, f1), entries (f2, Rp)          (stage z)
+//   groups = distinct f0,       entries (f1, line)        (stage y)
+struct SepParams {
+  const void* B; size_t panelStride; int ldb;   // panels: C(Rp, col) in row 2 Rp
+  int ncol, nPanels, mx, my, mz;
+  int nLines; const int* lineStart; const int* lineF2; const int* lineRp;
+  const double2* twz;                           // e^{2 pi i t / mz}
+  void* T1; size_t t1PanelStride;               // (panel, line, jz, col)
+  int nG0; const int* grpStart; const int* grpF1; const int* grpLine;
+  const double2* twy;                           // e^{2 pi i t / my}
+  void* T2; size_t t2PanelStride;               // (panel, group, jy, jz, col)
+};
+
+// T1(line, jz, c) = sum_{(f2, Rp) in line} w_z^{jz f2} C(Rp, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_sep_z(SepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
+  const int line = blockIdx.x, panel = blockIdx.y;
+  const int e0 = p.lineStart[line], ne = p.lineStart[line + 1] - e0;
+  const T* B = static_cast<const T*>(p.B) + size_t(panel) * p.panelStride;
+  for (int i = threadIdx.x; i < ne * p.ncol; i += 256) {
+    const int e = i / p.ncol, c = i % p.ncol;
+    const T* row = B + size_t(2 * p.lineRp[e0 + e]) * p.ldb;
+    X[i] = make_double2(double(row[2 * c]), double(row[2 * c + 1]));
+  }
+  __syncthreads();
+  T* out = static_cast<T*>(p.T1) + size_t(panel) * p.t1PanelStride + size_t(line) * p.mz * p.ncol * 2;
+  for (int i = threadIdx.x; i < p.mz * p.ncol; i += 256) {
+    const int jz = i / p.ncol, c = i % p.ncol;
+    double2 s = make_double2(0., 0.);
+    for (int e = 0; e < ne; ++e) {
+      const double2 w = p.twz[(jz * p.lineF2[e0 + e]) % p.mz], x = X[e * p.ncol + c];
+      s.x += w.x * x.x - w.y * x.y;
+      s.y += w.x * x.y + w.y * x.x;
+    }
+    out[2 * size_t(i)] = T(s.x);
+    out[2 * size_t(i) + 1] = T(s.y);
   }
 }
 
+// T2(group, jy, jz, c) = sum_{(f1, line) in group} w_y^{jy f1} T1(line, jz, c)
+template <typename T>
+__global__ void __launch_bounds__(256) k_sep_y(SepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
+  const int grp = blockIdx.x, jz = blockIdx.y, panel = blockIdx.z;
+  const int e0 = p.grpStart[grp], ne = p.grpStart[grp + 1] - e0;
+  const T* T1 = static_cast<const T*>(p.T1) + size_t(panel) * p.t1PanelStride;
+  for (int i = threadIdx.x; i < ne * p.ncol; i += 256) {
+    const int e = i / p.ncol, c = i % p.ncol;
+    const T* src = T1 + (size_t(p.grpLine[e0 + e]) * p.mz + jz) * p.ncol * 2;
+    X[i] = make_double2(double(src[2 * c]), double(src[2 * c + 1]));
+  }
+  __syncthreads();
+  T* out = static_cast<T*>(p.T2) + size_t(panel) * p.t2PanelStride;
+  for (int i = threadIdx.x; i < p.my * p.ncol; i += 256) {
+    const int jy = i / p.ncol, c = i % p.ncol;
+    double2 s = make_double2(0., 0.);
+    for (int e = 0; e < ne; ++e) {
+      const double2 w = p.twy[(jy * p.grpF1[e0 + e]) % p.my], x = X[e * p.ncol + c];
+      s.x += w.x * x.x - w.y * x.y;
+      s.y += w.x * x.y + w.y * x.x;
+    }
+    const size_t o = ((size_t(grp) * p.my + jy) * p.mz + jz) * p.ncol + c;
+    out[2 * o] = T(s.x);
+    out[2 * o + 1] = T(s.y);
+  }
+}
+
+// g3(jx; c) = sum_groups w_x^{jx f0} T2(group, jy, jz, c) for one (jy, jz) line
+// of mesh points, 16 jx at a time, each pass followed by the shared epilogue.
+template <typename T, bool REDUCE>
+__global__ void __launch_bounds__(256) k_sep_x(CoupParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nLinesYZ = p.m1 * p.m2;
+  const int nTilesTot = p.nPanels * nLinesYZ;
+  const int bid = blockIdx.x;
+  const int perXcd = gridDim.x / 8;
+  const int L = (bid % 8) * perXcd + bid / 8;
+  if (L >= nTilesTot) return;
+  const int panel = L / nLinesYZ, yz = L % nLinesYZ;
+  const int jy = yz % p.m1, jz = yz / p.m1;
+  const int t = threadIdx.x;
+  const int ncol = p.nm * p.nw, PWp = p.ldb;
+  double2* X = reinterpret_cast<double2*>(smem);                 // (groups, ncol)
+  T* g3 = reinterpret_cast<T*>(X + size_t(p.nG0) * ncol);        // 16 x PWp
+  unsigned char* scratch = reinterpret_cast<unsigned char*>(g3 + size_t(16) * PWp);
+  const T* T2 = static_cast<const T*>(p.T2) + size_t(panel) * p.t2PanelStride;
+  for (int i = t; i < p.nG0 * ncol; i += 256) {
+    const int g = i / ncol, c = i % ncol;
+    const size_t o = ((size_t(g) * p.m1 + jy) * p.m2 + jz) * ncol + c;
+    X[i] = make_double2(double(T2[2 * o]), double(T2[2 * o + 1]));
+  }
+  const int kk = p.panelKK[panel];
+  const int b1p = p.panelB1[panel];
+  double part[kGMax][kCMax];
+#pragma unroll
+  for (int gi = 0; gi < kGMax; ++gi)
+#pragma unroll
+    for (int c = 0; c < kCMax; ++c) part[gi][c] = 0.;
+  for (int jx0 = 0; jx0 < p.m0; jx0 += 16) {
+    __syncthreads();
+    for (int i = t; i < 16 * ncol; i += 256) {
+      const int r = i / ncol, c = i % ncol, jx = jx0 + r;
+      double2 s = make_double2(0., 0.);
+      if (jx < p.m0) {
+        for (int g = 0; g < p.nG0; ++g) {
+          const double2 w = p.tw0[(jx * p.g0val[g]) % p.m0], x = X[g * ncol + c];
+          s.x += w.x * x.x - w.y * x.y;
+          s.y += w.x * x.y + w.y * x.x;
+        }
+      }
+      g3[r * PWp + 2 * c] = T(s.x);
+      g3[r * PWp + 2 * c + 1] = T(s.y);
+    }
+    __syncthreads();
+    const int jx = jx0 + (t >> 4);
+    int myRow = -1, myIq = 0;
+    if (jx < p.m0) {
+      const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
+      myRow = p.meshToRow[mesh];
+      if (myRow < p.rowLo || myRow >= p.rowHi) myRow = -1;
+      if (myRow >= 0) myIq = mesh_diff_index(mesh, p.kkMesh[kk], p.m0, p.m1, p.m2);
+    }
+    epilogue16<T, REDUCE>(p, g3, PWp, scratch, panel, kk, b1p, myRow, myIq, part);
+  }
+  if (REDUCE) reduce_partials(p, L, smem, part);
+}
+
+inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int ldb, int nm, int nb2max) {
+  return size_t(nG0) * ncol * sizeof(double2) + epilogue_lds_bytes(elem, ldb, nm, nb2max);
+}
+
+// maxLineE / maxGrpE: largest number of entries of a line / group (LDS sizing).
+template <typename T>
+void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int maxLineE,
+                      int maxGrpE, hipStream_t s) {
+  PHX_REQUIRE(p.nm <= kNmMax, "coupling kernel supports nm <= 36");
+  if (sp.nPanels <= 0) return;
+  const size_t ldsZ = size_t(maxLineE) * sp.ncol * sizeof(double2);
+  const size_t ldsY = size_t(maxGrpE) * sp.ncol * sizeof(double2);
+  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.ldb, p.nm, p.nb2max),
+                               size_t(kGMax) * kCMax * 4 * sizeof(double));
+  PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024 && ldsX <= 160 * 1024,
+              "separable transform: LDS budget exceeded");
+  PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_z<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(ldsZ, 16))));
+  PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_y<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(ldsY, 16))));
+  hipLaunchKernelGGL(k_sep_z<T>, dim3(sp.nLines, sp.nPanels), dim3(256), ldsZ, s, sp);
+  PHX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sep_y<T>, dim3(sp.nG0, sp.mz, sp.nPanels), dim3(256), ldsY, s, sp);
+  PHX_HIP(hipGetLastError());
+  const int tiles = sp.nPanels * sp.my * sp.mz;
+  const int grid = ceil_div(tiles, 8) * 8;
+  if (reduce) {
+    PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_x<T, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsX)));
+    hipLaunchKernelGGL((k_sep_x<T, true>), dim3(grid), dim3(256), ldsX, s, p);
+  } else {
+    PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_x<T, false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsX)));
+    hipLaunchKernelGGL((k_sep_x<T, false>), dim3(grid), dim3(256), ldsX, s, p);
+  }
+  PHX_HIP(hipGetLastError());
+}
 
 }  // namespace phx

@@ -106,14 +106,15 @@ def test_coupling_squared_vs_oracle(precision, tol):
     assert worst < tol
 
 
+@pytest.mark.parametrize("transform", ["direct", "separable"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
-def test_linewidths_vs_oracle(precision, tol):
-    say("linewidths", precision)
+def test_linewidths_vs_oracle(precision, tol, transform):
+    say("linewidths", precision, transform)
     p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
     gW = p.gw_host()
     orc = O.Oracle(p, gW.reshape(-1))
     ref = orc.linewidths()
-    lw = P.ElectronLinewidths(p, gW=gW, precision=precision, k1_batch=7)
+    lw = P.ElectronLinewidths(p, gW=gW, precision=precision, k1_batch=7, transform=transform)
     # window / state selection and q-set: bit-exact
     assert np.array_equal(lw.active, orc.active)
     assert np.array_equal(lw.band_lo, orc.band_lo)
@@ -140,3 +141,33 @@ def test_linewidths_synthetic_gw_on_device_matches_host_gw():
     a.run()
     b.run()
     assert np.array_equal(a.linewidths(), b.linewidths())
+
+
+@pytest.mark.parametrize("mesh", [(4, 5, 6), (6, 6, 6)])
+def test_separable_matches_direct_on_uneven_mesh(mesh):
+    say("separable vs direct", mesh)
+    p = synth.make_problem(2, 5, 3, 3, mesh, seed=44, name="uneven")
+    gW = p.gw_host()
+    d = P.ElectronLinewidths(p, gW=gW, precision="fp64", transform="direct")
+    s = P.ElectronLinewidths(p, gW=gW, precision="fp64", transform="separable")
+    d.run()
+    s.run()
+    a, b = d.linewidths(), s.linewidths()
+    err = rel_err(b, a)
+    say("  max rel err", err.max())
+    assert err.max() < 1e-11
+
+
+def test_separable_k2_range_and_subsets():
+    say("separable k2 range")
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    gW = p.gw_host()
+    orc = O.Oracle(p, gW.reshape(-1))
+    k1 = [3, 17, 40]
+    ref = orc.linewidths(k1_list=k1, k2_range=(10, 60), average=False)
+    s = P.ElectronLinewidths(p, gW=gW, precision="fp64", transform="separable", k1_batch=2)
+    s.run(k1_list=k1, k2_range=(10, 60))
+    got = s.linewidths(average=False)
+    mask = ref != 0
+    assert np.array_equal(mask, got != 0)
+    assert rel_err(got[mask], ref[mask]).max() < 1e-10
