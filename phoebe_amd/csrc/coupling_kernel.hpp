@@ -498,13 +498,18 @@ __global__ void __launch_bounds__(256) k_sep_x(CoupParams p) {
   const int t = threadIdx.x;
   const int ncol = p.nm * p.nw, PWp = p.ldb;
   double2* X = reinterpret_cast<double2*>(smem);                 // (groups, ncol)
-  T* g3 = reinterpret_cast<T*>(X + size_t(p.nG0) * ncol);        // 16 x PWp
+  double2* W = X + size_t(p.nG0) * ncol;                          // (m0, groups) twiddles
+  T* g3 = reinterpret_cast<T*>(W + size_t(p.m0) * p.nG0);        // 16 x PWp
   unsigned char* scratch = reinterpret_cast<unsigned char*>(g3 + size_t(16) * PWp);
   const T* T2 = static_cast<const T*>(p.T2) + size_t(panel) * p.t2PanelStride;
   for (int i = t; i < p.nG0 * ncol; i += 256) {
     const int g = i / ncol, c = i % ncol;
     const size_t o = ((size_t(g) * p.m1 + jy) * p.m2 + jz) * ncol + c;
     X[i] = make_double2(double(T2[2 * o]), double(T2[2 * o + 1]));
+  }
+  for (int i = t; i < p.m0 * p.nG0; i += 256) {
+    const int jx = i / p.nG0, g = i % p.nG0;
+    W[i] = p.tw0[(jx * p.g0val[g]) % p.m0];
   }
   const int kk = p.panelKK[panel];
   const int b1p = p.panelB1[panel];
@@ -515,18 +520,26 @@ __global__ void __launch_bounds__(256) k_sep_x(CoupParams p) {
     for (int c = 0; c < kCMax; ++c) part[gi][c] = 0.;
   for (int jx0 = 0; jx0 < p.m0; jx0 += 16) {
     __syncthreads();
-    for (int i = t; i < 16 * ncol; i += 256) {
-      const int r = i / ncol, c = i % ncol, jx = jx0 + r;
-      double2 s = make_double2(0., 0.);
-      if (jx < p.m0) {
-        for (int g = 0; g < p.nG0; ++g) {
-          const double2 w = p.tw0[(jx * p.g0val[g]) % p.m0], x = X[g * ncol + c];
-          s.x += w.x * x.x - w.y * x.y;
-          s.y += w.x * x.y + w.y * x.x;
+    // 16 rows per column held in registers: every X element is read once per pass
+    for (int c = t; c < ncol; c += 256) {
+      double2 acc[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = make_double2(0., 0.);
+      for (int g = 0; g < p.nG0; ++g) {
+        const double2 x = X[g * ncol + c];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int jx = min(jx0 + r, p.m0 - 1);
+          const double2 w = W[jx * p.nG0 + g];
+          acc[r].x += w.x * x.x - w.y * x.y;
+          acc[r].y += w.x * x.y + w.y * x.x;
         }
       }
-      g3[r * PWp + 2 * c] = T(s.x);
-      g3[r * PWp + 2 * c + 1] = T(s.y);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        g3[r * PWp + 2 * c] = T(acc[r].x);
+        g3[r * PWp + 2 * c + 1] = T(acc[r].y);
+      }
     }
     __syncthreads();
     const int jx = jx0 + (t >> 4);
@@ -542,8 +555,8 @@ __global__ void __launch_bounds__(256) k_sep_x(CoupParams p) {
   if (REDUCE) reduce_partials(p, L, smem, part);
 }
 
-inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int ldb, int nm, int nb2max) {
-  return size_t(nG0) * ncol * sizeof(double2) + epilogue_lds_bytes(elem, ldb, nm, nb2max);
+inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int m0, int ldb, int nm, int nb2max) {
+  return size_t(nG0) * (ncol + m0) * sizeof(double2) + epilogue_lds_bytes(elem, ldb, nm, nb2max);
 }
 
 // maxLineE / maxGrpE: largest number of entries of a line / group (LDS sizing).
@@ -554,7 +567,7 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
   if (sp.nPanels <= 0) return;
   const size_t ldsZ = size_t(maxLineE) * sp.ncol * sizeof(double2);
   const size_t ldsY = size_t(maxGrpE) * sp.ncol * sizeof(double2);
-  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.ldb, p.nm, p.nb2max),
+  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.ldb, p.nm, p.nb2max),
                                size_t(kGMax) * kCMax * 4 * sizeof(double));
   PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024 && ldsX <= 160 * 1024,
               "separable transform: LDS budget exceeded");
