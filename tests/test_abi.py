@@ -43,3 +43,11 @@ def test_synthetic_gw_host_twin_matches_numpy():
     a = phoebe_amd.synth_gw_host(p)
     b = p.gw_host().reshape(-1).view(np.float64)
     assert np.array_equal(a, b)
+
+
+def test_cpp_adaptor_compiles():
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "adaptor_check.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

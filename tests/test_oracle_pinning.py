@@ -1,0 +1,209 @@
+"""Pins the oracle functions against closed forms, invariants and brute force.
+
+Each check is chosen so that a plausible slip (dropped conjugation, wrong
+phase sign, transposed operand, wrong degeneracy weight, one-sided cutoff on
+the wrong side) fails it.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from phoebe_amd import synth
+from tests.helpers import tiny_problem
+
+A_FCC = synth._fcc(10.26)
+
+
+# ------------------------------------------------------------------ WS vectors
+@pytest.mark.parametrize("grid", [(2, 2, 2), (3, 3, 3), (4, 4, 4), (2, 3, 4), (6, 6, 6)])
+def test_ws_completeness_origin_and_inversion(grid):
+    R, deg = O.ws_vectors(grid, A_FCC)
+    assert abs((1.0 / deg).sum() - np.prod(grid)) < 1e-9          # crystal.cpp:420-428
+    assert np.allclose(R[0], 0) and deg[0] == 1                   # origin swapped to 0
+    keys = {tuple(np.round(r, 6)) for r in R}
+    assert all(tuple(np.round(-r, 6) + 0.0) in keys for r in R)  # inversion symmetric
+    # every vector is a lattice vector and its images in the supercell are not shorter
+    n = np.linalg.solve(A_FCC, R.T).T
+    assert np.allclose(n, np.round(n), atol=1e-9)
+    sup = A_FCC * np.array(grid)
+    for r, d in zip(R[:50], deg[:50]):
+        imgs = [np.linalg.norm(r - sup @ np.array(s)) for s in itertools.product(range(-2, 3), repeat=3)]
+        assert np.linalg.norm(r) <= min(imgs) + 1e-6
+        assert d == sum(abs(x - min(imgs)) < 1e-6 for x in imgs)
+
+
+def test_ws_generator_matches_oracle_bit_for_bit():
+    for grid in [(3, 3, 3), (7, 7, 7)]:
+        R1, d1 = synth.ws_vectors(grid, A_FCC)
+        R2, d2 = O.ws_vectors(grid, A_FCC)
+        assert np.array_equal(R1, R2) and np.array_equal(d1, d2)
+
+
+# ------------------------------------------------------------------ mesh indexing
+def test_mesh_index_matches_integer_folding():
+    rng = np.random.default_rng(0)
+    for mesh in [(4, 4, 4), (5, 6, 7), (12, 12, 12)]:
+        for _ in range(300):
+            i1 = rng.integers(0, mesh, 3)
+            i2 = rng.integers(0, mesh, 3)
+            x = (i2 - i1) / np.array(mesh) + rng.integers(-2, 3, 3)   # unfolded difference
+            iq = O.mesh_index(x, mesh)
+            d = (i2 - i1) % np.array(mesh)
+            assert iq == d[2] * mesh[0] * mesh[1] + d[1] * mesh[0] + d[0]
+        assert O.mesh_index(np.array([0.5 / mesh[0], 0, 0]), mesh) == -1   # off-mesh point
+
+
+# ------------------------------------------------------------------ coupling
+def _brute_force_g(p, gW, U1, k1, U2, e3, q):
+    """Direct 5-fold sum of the coupling restated in SURVEY/Appendix A form."""
+    pe = np.exp(1j * p.Re @ k1) / p.degE
+    pp = np.exp(1j * p.Rp @ q) / p.degP
+    # g(b1, b2, nu2) = sum e3(nu,nu2) conj(U2(w2,b2)) pp(Rp) U1(w1,b1) pe(Re) gW(Re,Rp,nu,w1,w2)
+    return np.einsum("e,p,epnij,ia,jb,nm->abm", pe, pp, gW, U1, U2.conj(), e3, optimize=True)
+
+
+def test_coupling_against_brute_force():
+    p = tiny_problem(natoms=1, nw=3, rg_e=2, rg_p=2, mesh=4, seed=5)
+    gW = p.gw_host()
+    rng = np.random.default_rng(3)
+    k1 = p.recip @ rng.uniform(0, 1, 3)
+    _, U1 = O.el_diag(p, k1)
+    U1 = U1[:, :2]
+    cached = O.cache_elph(p.nw, p.nm, p.Rp, p.Re, p.degE, gW, U1, k1)
+    qs = [p.recip @ rng.uniform(-1, 1, 3) for _ in range(5)]
+    U2s = [O.el_diag(p, k1 + q)[1][:, :2] for q in qs]
+    e3s = [O.ph_diag(p, q)[1] for q in qs]
+    got = O.coupling_sq(p.nw, p.nm, p.Rp, p.degP, cached, U2s, e3s, qs)
+    for q, U2, e3, g2 in zip(qs, U2s, e3s, got):
+        ref = np.abs(_brute_force_g(p, gW, U1, k1, U2, e3, q)) ** 2
+        assert np.allclose(g2, ref, rtol=1e-11, atol=1e-14 * ref.max())
+        # a transposed gW (w1 <-> w2) or a missing conjugation of U2 is detected
+        bad = np.abs(_brute_force_g(p, np.swapaxes(gW, 3, 4), U1, k1, U2, e3, q)) ** 2
+        assert not np.allclose(g2, bad, rtol=1e-6, atol=0)
+        bad = np.abs(_brute_force_g(p, gW, U1, k1, U2.conj(), e3, q)) ** 2
+        assert not np.allclose(g2, bad, rtol=1e-6, atol=0)
+
+
+def test_coupling_single_R_textbook_product():
+    # with one R_e = R_ph = 0 vector the coupling is |U2^H gW(0) e3 U1|^2 per mode
+    rng = np.random.default_rng(4)
+    nw, nm = 3, 3
+    gW = (rng.standard_normal((1, 1, nm, nw, nw)) + 1j * rng.standard_normal((1, 1, nm, nw, nw)))
+    R0, d0 = np.zeros((1, 3)), np.ones(1)
+    U1 = np.linalg.qr(rng.standard_normal((nw, nw)) + 1j * rng.standard_normal((nw, nw)))[0][:, :2]
+    U2 = np.linalg.qr(rng.standard_normal((nw, nw)) + 1j * rng.standard_normal((nw, nw)))[0]
+    e3 = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+    k1, q = rng.standard_normal(3), rng.standard_normal(3)
+    cached = O.cache_elph(nw, nm, R0, R0, d0, gW, U1, k1)
+    got = O.coupling_sq(nw, nm, R0, d0, cached, [U2], [e3], [q])[0]
+    # g(b1,b2,nu2) = sum_nu e3(nu,nu2) sum_{w1,w2} conj(U2(w2,b2)) gW(nu,w1,w2) U1(w1,b1)
+    G = gW[0, 0]  # (nu, w1, w2)
+    ref = np.zeros((2, nw, nm))
+    for nu2 in range(nm):
+        M = sum(e3[nu, nu2] * (U2.conj().T @ G[nu].T @ U1) for nu in range(nm))  # (b2, b1)
+        ref[:, :, nu2] = np.abs(M.T) ** 2
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_coupling_gauge_and_reciprocal_lattice_invariance():
+    p = tiny_problem(natoms=1, nw=3, rg_e=2, rg_p=2, mesh=4, seed=6)
+    gW = p.gw_host()
+    rng = np.random.default_rng(9)
+    k1 = p.recip @ rng.uniform(0, 1, 3)
+    q = p.recip @ rng.uniform(0, 1, 3)
+    _, U1 = O.el_diag(p, k1)
+    _, U2 = O.el_diag(p, k1 + q)
+    _, e3 = O.ph_diag(p, q)
+
+    def g2(U1_, U2_, e3_, k1_, q_):
+        c = O.cache_elph(p.nw, p.nm, p.Rp, p.Re, p.degE, gW, U1_, k1_)
+        return O.coupling_sq(p.nw, p.nm, p.Rp, p.degP, c, [U2_], [e3_], [q_])[0]
+
+    base = g2(U1, U2, e3, k1, q)
+    ph = lambda n: np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    assert np.allclose(g2(U1 * ph(p.nw), U2 * ph(p.nw), e3 * ph(p.nm), k1, q), base, rtol=1e-11)
+    G = p.recip @ np.array([1, -2, 1])
+    assert np.allclose(g2(U1, U2, e3, k1 + G, q - G), base, rtol=1e-9)
+
+
+# ------------------------------------------------------------------ delta, populations, reduction
+def test_gaussian_delta_closed_form_and_one_sided_cutoff():
+    s = 0.01
+    for e in [0.0, 0.003, -0.02, 0.059, -0.061, -0.1]:
+        x = e / s
+        ref = 0.0 if x > 6 else np.exp(-x * x) / (s * np.sqrt(np.pi))
+        assert O.gaussian_delta(e, s) == pytest.approx(ref, rel=1e-15, abs=0)
+    assert O.gaussian_delta(0.0601, s) == 0.0           # x > 6: cut
+    assert O.gaussian_delta(-0.0601, s) > 0.0           # the cut is one-sided
+    assert O.gaussian_delta(-0.3, s) == 0.0             # underflow, not the cut
+
+
+def test_populations():
+    T = 300.0 / synth.TEMPERATURE_AU_TO_SI
+    assert O.population(0, 0.0, T, 0.0) == pytest.approx(0.5)
+    assert O.population(0, 0.01, T, 0.0) + O.population(0, -0.01, T, 0.0) == pytest.approx(1.0)
+    w = 0.003
+    assert O.population(1, w, T) == pytest.approx(1.0 / np.expm1(w / T), rel=1e-12)
+    assert O.population(1, -w, T) == 0.0                # clamped, particle.cpp:85-87
+    assert O.population(0, 1.0, 0.0, 0.5) == 0.0 and O.population(0, 0.2, 0.0, 0.5) == 1.0
+
+
+def test_symmetrize_coupling_groups():
+    rng = np.random.default_rng(1)
+    c = rng.uniform(0, 1, (3, 2, 4))
+    e1 = np.array([0.0, 5e-7, 1.0])        # first two degenerate (< 1e-6)
+    e2 = np.array([0.2, 0.3])
+    e3 = np.array([1.0, 1.0 + 2e-7, 1.0 + 9e-7, 2.0])   # grouping relative to the first member
+    out = O.symmetrize_coupling(c, e1, e2, e3)
+    ref = c.copy()
+    ref[0:2] = ref[0:2].mean(axis=0)
+    ref[:, :, 0:3] = ref[:, :, 0:3].mean(axis=2, keepdims=True)
+    assert np.allclose(out, ref, rtol=1e-14)
+
+
+def test_single_band_linewidth_closed_form():
+    """One Wannier function and R = 0 only: |g(nu2)|^2 = |sum_nu e3(nu, nu2) gW(nu)|^2,
+    and the linewidth is a hand-evaluable sum of Gaussians over the whole mesh
+    (el_scattering.cpp:277-380), with the nu-degeneracy average of
+    symmetrizeCoupling (scattering.cpp:1288-1316)."""
+    import dataclasses
+    p = synth.make_problem(1, 1, 1, 1, 3, seed=2, name="one")
+    p = dataclasses.replace(p, win_min=-10.0, win_max=10.0)
+    assert p.nRe == 1 and p.nRp == 1
+    gW = p.gw_host()
+    o = O.Oracle(p, gW.reshape(-1))
+    lw = o.linewidths(average=False)[0]
+    om, ev, _ = o.phonons()
+    E = o.energies
+    cut = O.phonon_cutoff()
+    T, mu, s = p.temperatures[0], p.chem_pots[0], p.sigma
+    mesh = np.array(p.mesh)
+    nk = int(mesh.prod())
+    coords = np.array([[i % mesh[0], (i // mesh[0]) % mesh[1], i // (mesh[0] * mesh[1])] for i in range(nk)])
+    gw = gW[0, 0, :, 0, 0]
+    ref = np.zeros(nk)
+    for a1 in range(nk):
+        for a2 in range(nk):
+            d = (coords[a2] - coords[a1]) % mesh
+            iq = d[2] * mesh[0] * mesh[1] + d[1] * mesh[0] + d[0]
+            g2 = np.abs(gw @ ev[iq]) ** 2              # sum_nu gW(nu) e3(nu, nu2)
+            w = om[iq]
+            i = 0
+            while i < p.nm:                              # degenerate-mode average
+                j = i + 1
+                while j < p.nm and abs(w[j] - w[i]) < 1e-6:
+                    j += 1
+                g2[i:j] = g2[i:j].mean()
+                i = j
+            f2 = 1.0 / (np.exp((E[a2] - mu) / T) + 1.0)
+            for nu in range(p.nm):
+                if w[nu] < cut:
+                    continue
+                n3 = max(0.0, 1.0 / np.expm1(w[nu] / T))
+                d1 = O.gaussian_delta(E[a1] - E[a2] + w[nu], s)
+                d2 = O.gaussian_delta(E[a1] - E[a2] - w[nu], s)
+                ref[a1] += g2[nu] * ((f2 + n3) * d1 + (1 - f2 + n3) * d2) / nk / w[nu] * np.pi
+    assert np.all(ref > 0)
+    assert np.allclose(lw, ref, rtol=1e-11, atol=0)
