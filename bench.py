@@ -33,19 +33,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-FORMS = {  # name: (precision, transform)
-    "fp64": ("fp64", "direct"),
-    "fp32": ("fp32", "direct"),
-    "sep64": ("fp64", "separable"),
-    "sep32": ("fp32", "separable"),
+FORMS = {  # name: (precision, transform, energy screen in sigma)
+    "fp64": ("fp64", "direct", 0.0),
+    "fp32": ("fp32", "direct", 0.0),
+    "sep64": ("fp64", "separable", 0.0),
+    "sep32": ("fp32", "separable", 0.0),
+    "sep32s": ("fp32", "separable", 7.0),
+    "sep64s": ("fp64", "separable", 7.0),
 }
 
 
 def run_form(P, prob, form, make_batches, warmup, k1_batch):
     t0 = time.perf_counter()
-    precision, transform = FORMS[form]
+    precision, transform, screen = FORMS[form]
     lw = P.ElectronLinewidths(prob, gW=None, precision=precision, k1_batch=k1_batch,
                               transform=transform)
+    if screen:
+        lw.set_screen(screen)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f}s: active {lw.num_active}, "
         f"states {lw.num_states}")
     k1_batches = make_batches(lw.num_active)
@@ -69,7 +73,7 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
 
 
 def roofline(kt, form):
-    precision, transform = FORMS[form]
+    precision, transform, _ = FORMS[form]
     ach = kt["coupling_flops"] / (kt["coupling_ms"] * 1e-3) / 1e12 if kt["coupling_ms"] > 0 else 0.0
     peak = PEAK_TFLOPS[precision]
     kern = "k_coupling" if transform == "direct" else "k_sep_z + k_sep_y + k_sep_x"
@@ -118,7 +122,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--k1-batch", type=int, default=32)
-    ap.add_argument("--chosen", default="sep64", choices=["fp32", "sep64", "sep32"])
+    ap.add_argument("--chosen", default="sep32s", choices=sorted(k for k in FORMS if k != "fp64"))
     ap.add_argument("--cpu-k1", type=int, default=1, help="outer points in the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -182,7 +186,8 @@ def main():
                    "k1_per_step": B, "k2_per_k1": int(r64["pairs"] // (B * args.steps)),
                    "active_k": int(na), "chosen_form": args.chosen,
                    "chosen_precision": FORMS[args.chosen][0],
-                   "chosen_transform": FORMS[args.chosen][1]},
+                   "chosen_transform": FORMS[args.chosen][1],
+                   "chosen_screen_sigma": FORMS[args.chosen][2]},
         "plain_fp64": {"value": v64, "ms_per_step": r64["seconds"] * 1e3 / args.steps,
                        "roofline": rf64},
         "speedup_vs_fp64": vc / v64,

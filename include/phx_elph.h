@@ -173,6 +173,13 @@ long long phx_lw_run(phx_lw* h, int nk1, const int* k1_list, int k2_begin, int k
  * (scattering.cpp:1168-1207) applied to row 0 when average != 0. */
 int phx_lw_get_linewidths(phx_lw* h, double* out, int average);
 int phx_lw_reset(phx_lw* h);
+/* Energy-conservation screen of the separable transform: an inner point k2 is
+ * skipped for a panel (k1, b1) when every band e2 of k2 and every mode omega
+ * >= 5 cm^-1 at q = k2 - k1 give |e1 - e2 +- omega| >= nsigma * sigma, i.e.
+ * when all its Gaussian weights (delta_function.cpp:40-45) are below
+ * exp(-nsigma^2) of their peak.  0 turns it off (the default); otherwise
+ * nsigma >= 6.  No counterpart in the reference, which evaluates every pair. */
+int phx_lw_set_screen(phx_lw* h, double nsigma);
 /* Per-kernel timing with HIP events recorded on the handle's stream around
  * every batch; turning it on (or off) clears the accumulators. */
 int phx_lw_set_timing(phx_lw* h, int on);

@@ -83,6 +83,7 @@ EXPORTS = {
     "phx_lw_reset": (C.c_int, [C.c_void_p]),
     "phx_lw_sync": (C.c_int, [C.c_void_p]),
     "phx_lw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_set_screen": (C.c_int, [C.c_void_p, C.c_double]),
     "phx_lw_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_stream": (C.c_void_p, [C.c_void_p]),
     "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
@@ -333,6 +334,10 @@ class ElectronLinewidths:
 
     def reset(self):
         _check(lib().phx_lw_reset(self._h))
+
+    def set_screen(self, nsigma):
+        """Energy-conservation screen (separable transform), in units of sigma; 0 = off."""
+        _check(lib().phx_lw_set_screen(self._h, float(nsigma)))
 
     def set_timing(self, on=True):
         _check(lib().phx_lw_set_timing(self._h, int(bool(on))))

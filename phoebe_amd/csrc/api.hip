@@ -487,6 +487,9 @@ struct LwHandle {
   DevBuf<double2> twx, twy, twz;
   DevBuf<double> T1d, T2d;
   DevBuf<float> T1f, T2f;
+  // energy-conservation screen of the mesh transform (units of sigma, 0 = off)
+  double screen = 0.;
+  DevBuf<double> qWlo, qWhi;
   // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -675,6 +678,22 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     PHX_HIP(hipMemcpyAsync(L->omegaAll.data(), L->Wall.p, L->Wall.bytes(), hipMemcpyDeviceToHost, s));
     PHX_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < nq; ++j) PHX_REQUIRE(qinf[j] >= 0, "phonon eigensolve did not converge");
+  }
+  // lowest / highest phonon energy above the cutoff at every q (screen tables)
+  {
+    const double cut = phonon_cutoff();
+    std::vector<double> lo(nk, 1e300), hi(nk, -1e300);
+    for (int q : qlist)
+      for (int nu = 0; nu < nm; ++nu) {
+        const double w = L->omegaAll[size_t(q) * nm + nu];
+        if (w >= cut) {
+          lo[q] = std::min(lo[q], w);
+          hi[q] = std::max(hi[q], w);
+        }
+      }
+    h2d(L->qWlo, lo, s);
+    h2d(L->qWhi, hi, s);
+    PHX_HIP(hipStreamSynchronize(s));
   }
   // Bose (helper_el_scattering.cpp:261-267) and inner Fermi (el_scattering.cpp:87-109)
   {
@@ -1012,6 +1031,9 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.nG0 = L->nG0;
     cq.g0val = L->g0val.p;
     cq.tw0 = L->twx.p;
+    cq.screen = L->screen;
+    cq.qWlo = L->qWlo.p;
+    cq.qWhi = L->qWhi.p;
     if (L->precision == PHX_FP64) {
       L->T1d.reserve(sp.t1PanelStride * nPanels);
       L->T2d.reserve(sp.t2PanelStride * nPanels);
@@ -1331,6 +1353,16 @@ int phx_lw_reset(phx_lw* h) {
     PHX_REQUIRE(h, "null handle");
     launch_fill<double>(h->lw.p, h->lw.n, 0., h->stream);
     PHX_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int phx_lw_set_screen(phx_lw* h, double nsigma) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_REQUIRE(nsigma == 0. || nsigma >= 6., "phx_lw_set_screen: use 0 (off) or >= 6 sigma");
+    PHX_REQUIRE(nsigma == 0. || h->transform == PHX_TRANSFORM_SEPARABLE,
+                "phx_lw_set_screen: the screen runs with the separable transform");
+    h->screen = nsigma;
   });
 }
 

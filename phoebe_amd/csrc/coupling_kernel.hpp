@@ -63,7 +63,13 @@ struct CoupParams {
   int nG0; const int* g0val;              // folded x coordinate of each group
   const double2* tw0;                     // e^{2 pi i t / m0}, t < m0
   int rowLo, rowHi;                       // rows (active indices) taken by the mesh transform
+  double screen;                          // energy screen in units of sigma (0: off)
+  const double* qWlo; const double* qWhi; // per q: lowest / highest omega >= cutoff
 };
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
 
 __device__ inline double gaussian_delta(double energy, double invW, double pref) {
   // GaussianDeltaFunction::getSmearing, delta_function.cpp:40-45
@@ -211,23 +217,27 @@ __device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned c
   }
 }
 
-// Block reduction of the per-thread partial linewidths into partial[L].
+// Block reduction of the per-thread partial linewidths into partial[L]
+// (CM calculations held per thread; slots c >= CM are written as zero).
+template <int CM>
 __device__ inline void reduce_partials(const CoupParams& p, int L, unsigned char* smem,
-                                       double (&part)[kGMax][kCMax]) {
+                                       double (&part)[kGMax][CM]) {
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   double* red = reinterpret_cast<double*>(smem);
   __syncthreads();
 #pragma unroll
   for (int gi = 0; gi < kGMax; ++gi)
 #pragma unroll
-    for (int c = 0; c < kCMax; ++c) {
+    for (int c = 0; c < CM; ++c) {
       double v = part[gi][c];
       for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
       if (lane == 0) red[(gi * kCMax + c) * 4 + wave] = v;
     }
   __syncthreads();
   if (t < kGMax * kCMax) {
-    double v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
+    const int c = t % kCMax;
+    double v = 0.;
+    if (c < CM) v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
     p.partial[size_t(L) * kGMax * kCMax + t] = v;
   }
 }
@@ -359,7 +369,7 @@ __global__ void __launch_bounds__(256, 2) k_coupling(CoupParams p) {
       myIq = p.qmode ? mesh_diff_index(p.rowMesh[myRow], p.kkMesh[kk], p.m0, p.m1, p.m2) : p.rowE3[myRow];
     epilogue16<T, REDUCE>(p, g3, PWp, scratch, panel, kk, b1p, myRow, myIq, part);
   }
-  if (REDUCE) reduce_partials(p, L, smem, part);
+  if (REDUCE) reduce_partials<kCMax>(p, L, smem, part);
 }
 
 inline size_t coupling_lds_bytes(int elem, int MT, int ldb, int nm, int nb2max, bool reduce) {
@@ -482,11 +492,31 @@ __global__ void __launch_bounds__(256) k_sep_y(SepParams p) {
   }
 }
 
-// g3(jx; c) = sum_groups w_x^{jx f0} T2(group, jy, jz, c) for one (jy, jz) line
-// of mesh points, 16 jx at a time, each pass followed by the shared epilogue.
-template <typename T, bool REDUCE>
-__global__ void __launch_bounds__(256) k_sep_x(CoupParams p) {
+// Last stage + epilogue for one (jy, jz) line of mesh points and one panel.
+//
+//  1. rows of the line: active point, q index, and (screen > 0) the energy
+//     screen: a row is kept only if some e1 of the panel's degenerate group,
+//     band e2 of the row and mode omega >= cutoff at q give
+//     |e1 - e2 +- omega| < screen * sigma (an interval test on the band and
+//     mode ranges, so no contributing row is dropped; every skipped term has
+//     all its Gaussians below exp(-screen^2) of their peak);
+//  2. surviving rows, 8 per pass: g3(jx; c) = sum_groups w_x^{jx f0} T2(group, jy, jz, c);
+//  3. epilogue with 32 threads per row: thread nu2 owns the column e3(., nu2) (prefetched),
+//     g'(nu, b2) via U2 staged in LDS, g(nu2, b2) and |g|^2 for all b2 in registers,
+//     symmetrization along b2 in registers and along nu through LDS (only when the
+//     modes at q are degenerate), Gaussian weights.
+constexpr int kSepRows = 8;   // rows per pass
+constexpr int kSepNb2 = 8;    // largest band count of an inner point on this path
+constexpr int kSepNm = 32;    // largest mode count on this path
+
+struct SepXShared {            // per-row data staged in LDS
+  int row, iq, nb2, u2off, soff;
+};
+
+template <typename T, bool REDUCE, int NMX, int CM>
+__global__ void __launch_bounds__(256, 2) k_sep_x(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Vec2<T>::type T2v;
   const int nLinesYZ = p.m1 * p.m2;
   const int nTilesTot = p.nPanels * nLinesYZ;
   const int bid = blockIdx.x;
@@ -496,78 +526,269 @@ __global__ void __launch_bounds__(256) k_sep_x(CoupParams p) {
   const int panel = L / nLinesYZ, yz = L % nLinesYZ;
   const int jy = yz % p.m1, jz = yz / p.m1;
   const int t = threadIdx.x;
-  const int ncol = p.nm * p.nw, PWp = p.ldb;
-  double2* X = reinterpret_cast<double2*>(smem);                 // (groups, ncol)
-  double2* W = X + size_t(p.nG0) * ncol;                          // (m0, groups) twiddles
-  T* g3 = reinterpret_cast<T*>(W + size_t(p.m0) * p.nG0);        // 16 x PWp
-  unsigned char* scratch = reinterpret_cast<unsigned char*>(g3 + size_t(16) * PWp);
-  const T* T2 = static_cast<const T*>(p.T2) + size_t(panel) * p.t2PanelStride;
+  const int nm = p.nm, nw = p.nw, ncol = nm * nw;
+  const int kk = p.panelKK[panel];
+  const int b1p = p.panelB1[panel];
+
+  // LDS carve-up
+  T2v* X = reinterpret_cast<T2v*>(smem);                        // (groups, ncol)
+  T2v* W = X + size_t(p.nG0) * ncol;                             // (m0, groups)
+  const size_t xwBytes = (size_t(p.nG0) * ncol + size_t(p.m0) * p.nG0) * sizeof(T2v);
+  double2* g3 = reinterpret_cast<double2*>(smem + (xwBytes + 15) / 16 * 16);  // 8 x ncol
+  double2* sU2 = g3 + size_t(kSepRows) * ncol;                   // 8 x (nw * nb2max)
+  double2* gp = sU2 + size_t(kSepRows) * nw * p.nb2max;          // 8 x (nm * nb2max)
+  double* gs = reinterpret_cast<double*>(gp + size_t(kSepRows) * nm * p.nb2max);  // 8 x nm*nb2max
+  double* se2 = gs + size_t(kSepRows) * nm * p.nb2max;           // 8 x nb2max
+  double* sw3 = se2 + size_t(kSepRows) * p.nb2max;               // 8 x nm
+  SepXShared* rinfo = reinterpret_cast<SepXShared*>(sw3 + size_t(kSepRows) * nm);  // m0
+  int* list = reinterpret_cast<int*>(rinfo + p.m0);              // m0 + 1 (count at [m0])
+
+  // ---- rows of the line and the screen
+  for (int jx = t; jx < p.m0; jx += 256) {
+    const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
+    int row = p.meshToRow[mesh];
+    if (row < p.rowLo || row >= p.rowHi) row = -1;
+    SepXShared ri{row, 0, 0, 0, 0};
+    if (row >= 0) {
+      ri.iq = mesh_diff_index(mesh, p.kkMesh[kk], p.m0, p.m1, p.m2);
+      ri.nb2 = p.rowNb2[row];
+      ri.u2off = p.rowU2off[row];
+      ri.soff = p.rowStateOff[row];
+      if (p.screen > 0.) {
+        const double wlo = p.qWlo[ri.iq], whi = p.qWhi[ri.iq];
+        const double e2lo = p.e2pool[ri.soff], e2hi = p.e2pool[ri.soff + ri.nb2 - 1];
+        const double lim = p.screen / p.invSigma;
+        const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
+        bool keep = false;
+        if (whi >= wlo) {  // at least one mode above the cutoff
+          for (int gi = 0; gi < p.panelGSize[panel]; ++gi) {
+            const double a_lo = e1[gi] - e2hi + wlo, a_hi = e1[gi] - e2lo + whi;
+            const double b_lo = e1[gi] - e2hi - whi, b_hi = e1[gi] - e2lo - wlo;
+            keep |= (a_hi > -lim && a_lo < lim) || (b_hi > -lim && b_lo < lim);
+          }
+        }
+        if (!keep) ri.row = -1;
+      }
+    }
+    rinfo[jx] = ri;
+  }
+  const T2v* T2g = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2);
   for (int i = t; i < p.nG0 * ncol; i += 256) {
     const int g = i / ncol, c = i % ncol;
-    const size_t o = ((size_t(g) * p.m1 + jy) * p.m2 + jz) * ncol + c;
-    X[i] = make_double2(double(T2[2 * o]), double(T2[2 * o + 1]));
+    X[i] = T2g[((size_t(g) * p.m1 + jy) * p.m2 + jz) * ncol + c];
   }
   for (int i = t; i < p.m0 * p.nG0; i += 256) {
     const int jx = i / p.nG0, g = i % p.nG0;
-    W[i] = p.tw0[(jx * p.g0val[g]) % p.m0];
+    const double2 w = p.tw0[(jx * p.g0val[g]) % p.m0];
+    W[i] = T2v{T(w.x), T(w.y)};
   }
-  const int kk = p.panelKK[panel];
-  const int b1p = p.panelB1[panel];
-  double part[kGMax][kCMax];
+  __syncthreads();
+  if (t == 0) {
+    int n = 0;
+    for (int jx = 0; jx < p.m0; ++jx)
+      if (rinfo[jx].row >= 0) list[n++] = jx;
+    list[p.m0] = n;
+  }
+  __syncthreads();
+  const int nkeep = list[p.m0];
+
+  double part[kGMax][CM];
 #pragma unroll
   for (int gi = 0; gi < kGMax; ++gi)
 #pragma unroll
-    for (int c = 0; c < kCMax; ++c) part[gi][c] = 0.;
-  for (int jx0 = 0; jx0 < p.m0; jx0 += 16) {
-    __syncthreads();
-    // 16 rows per column held in registers: every X element is read once per pass
+    for (int c = 0; c < CM; ++c) part[gi][c] = 0.;
+  const int r = t >> 5, sub = t & 31;   // 32 threads per row in the epilogue
+  const int nCalc = p.nCalc;
+  const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
+  const double* e1 = p.e1pool + p.kkStateOff[kk] + gstart;
+  const double inv_g = 1. / double(float(gsize));
+
+  for (int base = 0; base < nkeep; base += kSepRows) {
+    // ---- g3 of the pass's rows (16 B of X per 8 complex FMAs)
     for (int c = t; c < ncol; c += 256) {
-      double2 acc[16];
+      double2 acc[kSepRows];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = make_double2(0., 0.);
+      for (int q = 0; q < kSepRows; ++q) acc[q] = make_double2(0., 0.);
+      int jxs[kSepRows];
+#pragma unroll
+      for (int q = 0; q < kSepRows; ++q) jxs[q] = list[min(base + q, nkeep - 1)];
       for (int g = 0; g < p.nG0; ++g) {
-        const double2 x = X[g * ncol + c];
+        const T2v xv = X[g * ncol + c];
+        const double xr = double(xv.x), xi = double(xv.y);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int jx = min(jx0 + r, p.m0 - 1);
-          const double2 w = W[jx * p.nG0 + g];
-          acc[r].x += w.x * x.x - w.y * x.y;
-          acc[r].y += w.x * x.y + w.y * x.x;
+        for (int q = 0; q < kSepRows; ++q) {
+          const T2v w = W[jxs[q] * p.nG0 + g];
+          const double wr = double(w.x), wi = double(w.y);
+          acc[q].x += wr * xr - wi * xi;
+          acc[q].y += wr * xi + wi * xr;
         }
       }
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        g3[r * PWp + 2 * c] = T(acc[r].x);
-        g3[r * PWp + 2 * c + 1] = T(acc[r].y);
+      for (int q = 0; q < kSepRows; ++q) g3[q * ncol + c] = acc[q];
+    }
+    // ---- per-row prefetch: U2 and energies in LDS, e3 column / omega / Bose in registers
+    const bool rowOn = base + r < nkeep;
+    SepXShared ri{-1, 0, 0, 0, 0};
+    if (rowOn) ri = rinfo[list[base + r]];
+    const int nb2 = ri.row >= 0 ? ri.nb2 : 0;
+    for (int i = sub; i < nw * nb2; i += 32) sU2[r * nw * p.nb2max + i] = p.U2pool[ri.u2off + i];
+    if (sub < nb2) se2[r * p.nb2max + sub] = p.e2pool[ri.soff + sub];
+    double2 e3c[NMX];
+    double w3 = 0., n3[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) n3[c] = 0.;
+    const bool modeOn = nb2 > 0 && sub < nm;
+    if (modeOn) {
+      const double2* e3 = p.E3pool + size_t(ri.iq) * nm * nm + size_t(nm) * sub;
+#pragma unroll
+      for (int nu = 0; nu < NMX; ++nu)
+        if (nu < nm) e3c[nu] = e3[nu];
+      w3 = p.Wpool[size_t(ri.iq) * nm + sub];
+      sw3[r * nm + sub] = w3;
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (c < nCalc) n3[c] = p.bose[(size_t(ri.iq) * nm + sub) * nCalc + c];
+    }
+    __syncthreads();
+    // ---- g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2), thread nu = sub
+    if (modeOn) {
+      const double2* gr = g3 + r * ncol + sub * nw;
+      for (int b2 = 0; b2 < nb2; ++b2) {
+        const double2* u = sU2 + r * nw * p.nb2max + nw * b2;
+        double2 s = make_double2(0., 0.);
+        for (int w2 = 0; w2 < nw; ++w2) {
+          s.x += u[w2].x * gr[w2].x + u[w2].y * gr[w2].y;
+          s.y += u[w2].x * gr[w2].y - u[w2].y * gr[w2].x;
+        }
+        gp[(r * nm + sub) * p.nb2max + b2] = s;
       }
     }
     __syncthreads();
-    const int jx = jx0 + (t >> 4);
-    int myRow = -1, myIq = 0;
-    if (jx < p.m0) {
-      const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
-      myRow = p.meshToRow[mesh];
-      if (myRow < p.rowLo || myRow >= p.rowHi) myRow = -1;
-      if (myRow >= 0) myIq = mesh_diff_index(mesh, p.kkMesh[kk], p.m0, p.m1, p.m2);
+    // ---- g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub; |g|^2 in registers
+    double v[kSepNb2];
+#pragma unroll
+    for (int b = 0; b < kSepNb2; ++b) v[b] = 0.;
+    if (modeOn) {
+#pragma unroll
+      for (int b2 = 0; b2 < kSepNb2; ++b2) {
+        if (b2 < nb2) {
+          double2 s = make_double2(0., 0.);
+#pragma unroll
+          for (int nu = 0; nu < NMX; ++nu)
+            if (nu < nm) {
+              const double2 x = gp[(r * nm + nu) * p.nb2max + b2];
+              s.x += x.x * e3c[nu].x - x.y * e3c[nu].y;
+              s.y += x.x * e3c[nu].y + x.y * e3c[nu].x;
+            }
+          v[b2] = s.x * s.x + s.y * s.y;
+        }
+      }
+      if (!REDUCE) {
+        const int nb1 = p.kkNb1[kk];
+        for (int b2 = 0; b2 < nb2; ++b2)
+          p.out[p.rowOutOff[ri.row] + b1p + size_t(nb1) * (b2 + size_t(nb2) * sub)] = v[b2];
+      }
     }
-    epilogue16<T, REDUCE>(p, g3, PWp, scratch, panel, kk, b1p, myRow, myIq, part);
+    if (REDUCE) {
+      // symmetrizeCoupling along b2 (scattering.cpp:1250-1284), in registers
+      if (modeOn) {
+        const double* e2r = se2 + r * p.nb2max;
+        for (int b = 0; b < nb2;) {
+          int deg = 0;
+          for (int i = b; i < nb2; ++i) {
+            if (fabs(e2r[b] - e2r[i]) < 1.0e-6) deg++;
+            else break;
+          }
+          if (deg > 1) {
+            double s = 0.;
+#pragma unroll
+            for (int i = 0; i < kSepNb2; ++i)
+              if (i >= b && i < b + deg) s += v[i] / double(float(deg));
+#pragma unroll
+            for (int i = 0; i < kSepNb2; ++i)
+              if (i >= b && i < b + deg) v[i] = s;
+          }
+          b += deg;
+        }
+      }
+      // ... and along nu (scattering.cpp:1286-1316), through LDS when the modes are degenerate
+      int mydeg = 1, mystart = sub;
+      if (modeOn) {
+        const double* wr = sw3 + r * nm;
+        for (int s0 = 0; s0 < nm;) {
+          int deg = 0;
+          for (int i = s0; i < nm; ++i) {
+            if (fabs(wr[s0] - wr[i]) < 1.0e-6) deg++;
+            else break;
+          }
+          if (sub >= s0 && sub < s0 + deg) {
+            mystart = s0;
+            mydeg = deg;
+          }
+          s0 += deg;
+        }
+      }
+      if (__syncthreads_or(mydeg > 1)) {
+        if (modeOn)
+          for (int b2 = 0; b2 < nb2; ++b2) gs[(r * nm + sub) * p.nb2max + b2] = v[b2];
+        __syncthreads();
+        if (modeOn && mydeg > 1)
+          for (int b2 = 0; b2 < nb2; ++b2) {
+            double s = 0.;
+            for (int i = 0; i < mydeg; ++i) s += gs[(r * nm + mystart + i) * p.nb2max + b2] / double(float(mydeg));
+#pragma unroll
+            for (int b = 0; b < kSepNb2; ++b)
+              if (b == b2) v[b] = s;
+          }
+      }
+      // Gaussian-delta weights (el_scattering.cpp:277-380)
+      if (modeOn && w3 >= p.cutoff) {
+#pragma unroll
+        for (int b2 = 0; b2 < kSepNb2; ++b2) {
+          if (b2 >= nb2) break;
+          const double en2 = se2[r * p.nb2max + b2];
+          const double cpl = v[b2] * inv_g;
+          const int is2 = ri.soff + b2;
+#pragma unroll
+          for (int gi = 0; gi < kGMax; ++gi) {
+            if (gi >= gsize) break;
+            const double d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
+            const double d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+            if (d1 <= 0. && d2 <= 0.) continue;
+#pragma unroll
+            for (int c = 0; c < CM; ++c) {
+              if (c >= nCalc) break;
+              const double f2 = p.fermi[size_t(c) * p.numStates + is2];
+              part[gi][c] += cpl * ((f2 + n3[c]) * d1 + (1. - f2 + n3[c]) * d2) * p.norm / w3 *
+                             3.14159265358979323846;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
-  if (REDUCE) reduce_partials(p, L, smem, part);
+  if (REDUCE) reduce_partials<CM>(p, L, smem, part);
 }
 
-inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int m0, int ldb, int nm, int nb2max) {
-  return size_t(nG0) * (ncol + m0) * sizeof(double2) + epilogue_lds_bytes(elem, ldb, nm, nb2max);
+inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int m0, int nw, int nm, int nb2max) {
+  const size_t t2 = 2 * size_t(elem);
+  return (size_t(nG0) * ncol * t2 + size_t(m0) * nG0 * t2 + 15) / 16 * 16 +
+         size_t(kSepRows) * (ncol + nw * nb2max + nm * nb2max) * sizeof(double2) +
+         size_t(kSepRows) * (nm * nb2max + nb2max + nm) * sizeof(double) +
+         size_t(m0) * sizeof(SepXShared) + size_t(m0 + 1) * sizeof(int) + 64;
 }
 
 // maxLineE / maxGrpE: largest number of entries of a line / group (LDS sizing).
 template <typename T>
 void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int maxLineE,
                       int maxGrpE, hipStream_t s) {
-  PHX_REQUIRE(p.nm <= kNmMax, "coupling kernel supports nm <= 36");
+  PHX_REQUIRE(p.nm <= kSepNm && p.nb2max <= kSepNb2,
+              "separable transform supports nm <= 32 and at most 8 bands per inner point");
   if (sp.nPanels <= 0) return;
   const size_t ldsZ = size_t(maxLineE) * sp.ncol * sizeof(double2);
   const size_t ldsY = size_t(maxGrpE) * sp.ncol * sizeof(double2);
-  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.ldb, p.nm, p.nb2max),
+  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
                                size_t(kGMax) * kCMax * 4 * sizeof(double));
   PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024 && ldsX <= 160 * 1024,
               "separable transform: LDS budget exceeded");
@@ -581,14 +802,21 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
   PHX_HIP(hipGetLastError());
   const int tiles = sp.nPanels * sp.my * sp.mz;
   const int grid = ceil_div(tiles, 8) * 8;
+  auto go = [&](auto kern) {
+    PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsX)));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsX, s, p);
+  };
+  const int nmb = p.nm <= 12 ? 12 : p.nm <= 18 ? 18 : kSepNm;
+  const bool one = p.nCalc == 1;
   if (reduce) {
-    PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_x<T, true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsX)));
-    hipLaunchKernelGGL((k_sep_x<T, true>), dim3(grid), dim3(256), ldsX, s, p);
+    if (nmb == 12) one ? go(k_sep_x<T, true, 12, 1>) : go(k_sep_x<T, true, 12, kCMax>);
+    else if (nmb == 18) one ? go(k_sep_x<T, true, 18, 1>) : go(k_sep_x<T, true, 18, kCMax>);
+    else one ? go(k_sep_x<T, true, kSepNm, 1>) : go(k_sep_x<T, true, kSepNm, kCMax>);
   } else {
-    PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_x<T, false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsX)));
-    hipLaunchKernelGGL((k_sep_x<T, false>), dim3(grid), dim3(256), ldsX, s, p);
+    if (nmb == 12) go(k_sep_x<T, false, 12, 1>);
+    else if (nmb == 18) go(k_sep_x<T, false, 18, 1>);
+    else go(k_sep_x<T, false, kSepNm, 1>);
   }
   PHX_HIP(hipGetLastError());
 }

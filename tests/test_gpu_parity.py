@@ -171,3 +171,29 @@ def test_separable_k2_range_and_subsets():
     mask = ref != 0
     assert np.array_equal(mask, got != 0)
     assert rel_err(got[mask], ref[mask]).max() < 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+def test_energy_screen_keeps_parity(precision, tol):
+    """Rows skipped by the 7-sigma energy screen carry only Gaussian tails below
+    exp(-49) of their peak: the linewidths still match the oracle."""
+    say("energy screen", precision)
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    gW = p.gw_host()
+    ref = O.Oracle(p, gW.reshape(-1)).linewidths()
+    s = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="separable")
+    s.set_screen(7.0)
+    s.run()
+    err = rel_err(s.linewidths(), ref)
+    say("  max rel err", err.max())
+    assert err.max() < tol
+
+
+def test_screen_requires_separable_and_valid_width():
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    d = P.ElectronLinewidths(p, gW=p.gw_host(), precision="fp64", transform="direct")
+    with pytest.raises(P.PhxError):
+        d.set_screen(7.0)
+    s = P.ElectronLinewidths(p, gW=p.gw_host(), precision="fp64", transform="separable")
+    with pytest.raises(P.PhxError):
+        s.set_screen(3.0)
