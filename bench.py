@@ -1,10 +1,12 @@
 #!/usr/bin/env python
 """el-ph (k,q) pairs/s on one MI355X: chosen form vs plain FP64.
 
-A step is one batch of outer points k1 (k1_batch of them, drawn from a
-seeded permutation of the active points) scattered against every active
+A step is one mesh line of outer points k1 (the active points with equal j1,
+j2, lines drawn from a seeded permutation) scattered against every active
 inner point k2: cacheElPh + calcCouplingSquared + symmetrizeCoupling +
 Gaussian-delta linewidth reduction, with all inputs already resident in HBM.
+The plain FP64 form processes the same outer points in its own k1-batched
+GEMM batches of --k1-batch points.
 The workload is the Nw=16, 12-mode, 40^3 k/q mesh system (R_e on WS(10^3),
 R_ph on WS(9^3)); band structures, phonons and the coupling tensor are built
 on the device before the timed region.
@@ -33,26 +35,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-FORMS = {  # name: (precision, transform, energy screen in sigma)
-    "fp64": ("fp64", "direct", 0.0),
-    "fp32": ("fp32", "direct", 0.0),
-    "sep64": ("fp64", "separable", 0.0),
-    "sep32": ("fp32", "separable", 0.0),
-    "sep32s": ("fp32", "separable", 7.0),
-    "sep64s": ("fp64", "separable", 7.0),
+FORMS = {  # name: (precision, transform, energy screen in sigma, line batching min points)
+    "fp64": ("fp64", "direct", 0.0, 0),
+    "fp32": ("fp32", "direct", 0.0, 0),
+    "sep64": ("fp64", "separable", 0.0, 0),
+    "sep32": ("fp32", "separable", 0.0, 0),
+    "sep32s": ("fp32", "separable", 7.0, 0),
+    "sep64s": ("fp64", "separable", 7.0, 0),
+    "line32s": ("fp32", "separable", 7.0, 8),
+    "line64s": ("fp64", "separable", 7.0, 8),
 }
 
 
 def run_form(P, prob, form, make_batches, warmup, k1_batch):
     t0 = time.perf_counter()
-    precision, transform, screen = FORMS[form]
+    precision, transform, screen, line_min = FORMS[form]
     lw = P.ElectronLinewidths(prob, gW=None, precision=precision, k1_batch=k1_batch,
                               transform=transform)
     if screen:
         lw.set_screen(screen)
+    if line_min:
+        lw.set_line_batching(line_min)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f}s: active {lw.num_active}, "
         f"states {lw.num_states}")
-    k1_batches = make_batches(lw.num_active)
+    k1_batches = make_batches(lw.active, line_min > 0)
     for w in range(warmup):
         lw.run(k1_list=k1_batches[w % len(k1_batches)])
     lw.sync()
@@ -69,11 +75,12 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
     gam = lw.linewidths(average=False)
     log(f"[{form}] {len(k1_batches)} steps in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
-    return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, linewidths=gam)
+    return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, linewidths=gam,
+                    k1=int(sum(len(b) for b in k1_batches)))
 
 
 def roofline(kt, form):
-    precision, transform, _ = FORMS[form]
+    precision, transform = FORMS[form][:2]
     ach = kt["coupling_flops"] / (kt["coupling_ms"] * 1e-3) / 1e12 if kt["coupling_ms"] > 0 else 0.0
     peak = PEAK_TFLOPS[precision]
     kern = "k_coupling" if transform == "direct" else "k_sep_z + k_sep_y + k_sep_x"
@@ -122,7 +129,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--k1-batch", type=int, default=32)
-    ap.add_argument("--chosen", default="sep32s", choices=sorted(k for k in FORMS if k != "fp64"))
+    ap.add_argument("--chosen", default="line32s", choices=sorted(k for k in FORMS if k != "fp64"))
     ap.add_argument("--cpu-k1", type=int, default=1, help="outer points in the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -140,15 +147,24 @@ def main():
         f"mesh {prob.mesh} gW {prob.gw_bytes() / 1e9:.1f} GB ({time.perf_counter() - t0:.1f}s)")
 
     B = args.k1_batch
+    m0 = prob.mesh[0]
 
-    def make_batches(num_active):
-        perm = np.random.default_rng(12345).permutation(num_active).astype(np.int32)
-        return [perm[i * B:(i + 1) * B] for i in range(args.steps)]
+    def make_batches(active, by_line):
+        """The outer points of `steps` mesh lines: one list per line, or the
+        same points in k1-batched lists of B."""
+        line = np.asarray(active) // m0
+        ids = np.unique(line)
+        pick = np.random.default_rng(12345).permutation(ids)[:args.steps]
+        lines = [np.flatnonzero(line == l).astype(np.int32) for l in pick]
+        if by_line:
+            return lines
+        pts = np.concatenate(lines)
+        return [pts[i:i + B] for i in range(0, len(pts), B)]
 
     # plain FP64
     lw64, r64 = run_form(P, prob, "fp64", make_batches, args.warmup, B)
     na = lw64.num_active
-    batches = make_batches(na)
+    batches = make_batches(lw64.active, False)
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -183,11 +199,13 @@ def main():
         "config": {"workload": f"{args.config}: Nw={prob.nw}, {prob.nm} modes, "
                                f"R_e WS(10^3)={prob.nRe}, R_ph WS(9^3)={prob.nRp}, "
                                f"{prob.mesh[0]}^3 k/q mesh, 0.5 eV window",
-                   "k1_per_step": B, "k2_per_k1": int(r64["pairs"] // (B * args.steps)),
+                   "k1_per_step": int(rc["k1"] // args.steps) if FORMS[args.chosen][3] else B,
+                   "k2_per_k1": int(rc["pairs"] // rc["k1"]),
                    "active_k": int(na), "chosen_form": args.chosen,
                    "chosen_precision": FORMS[args.chosen][0],
                    "chosen_transform": FORMS[args.chosen][1],
-                   "chosen_screen_sigma": FORMS[args.chosen][2]},
+                   "chosen_screen_sigma": FORMS[args.chosen][2],
+                   "chosen_line_batching": FORMS[args.chosen][3]},
         "plain_fp64": {"value": v64, "ms_per_step": r64["seconds"] * 1e3 / args.steps,
                        "roofline": rf64},
         "speedup_vs_fp64": vc / v64,

@@ -180,6 +180,16 @@ int phx_lw_reset(phx_lw* h);
  * exp(-nsigma^2) of their peak.  0 turns it off (the default); otherwise
  * nsigma >= 6.  No counterpart in the reference, which evaluates every pair. */
 int phx_lw_set_screen(phx_lw* h, double nsigma);
+/* Mesh-line batching of the outer points: the points of a phx_lw_run call
+ * that share a mesh line (equal j1, j2 of k1 = sum_d j_d/m_d b_d) are
+ * processed together, up to 64 per batch, when the line holds at least
+ * min_points of them; cacheElPh then streams gW once per line and factors
+ * e^{ik1.Re} = e^{2 pi i j0 n0/m0} e^{2 pi i (j1 n1/m1 + j2 n2/m2)} over the
+ * lattice coordinates n_d of R_e.  The remaining points keep the k1-batched
+ * GEMM.  0 turns it off (the default).  Results agree with the unbatched run
+ * to rounding.  No counterpart in the reference, whose cacheElPh
+ * (interaction_elph.cpp:229-448) sums over R_e once per outer point. */
+int phx_lw_set_line_batching(phx_lw* h, int min_points);
 /* Per-kernel timing with HIP events recorded on the handle's stream around
  * every batch; turning it on (or off) clears the accumulators. */
 int phx_lw_set_timing(phx_lw* h, int on);

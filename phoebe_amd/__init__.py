@@ -84,6 +84,7 @@ EXPORTS = {
     "phx_lw_sync": (C.c_int, [C.c_void_p]),
     "phx_lw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_screen": (C.c_int, [C.c_void_p, C.c_double]),
+    "phx_lw_set_line_batching": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_stream": (C.c_void_p, [C.c_void_p]),
     "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
@@ -338,6 +339,11 @@ class ElectronLinewidths:
     def set_screen(self, nsigma):
         """Energy-conservation screen (separable transform), in units of sigma; 0 = off."""
         _check(lib().phx_lw_set_screen(self._h, float(nsigma)))
+
+    def set_line_batching(self, min_points):
+        """Batch outer points by mesh line (cacheElPh streams gW once per line) when
+        a line holds at least min_points of a run's points; 0 = off."""
+        _check(lib().phx_lw_set_line_batching(self._h, int(min_points)))
 
     def set_timing(self, on=True):
         _check(lib().phx_lw_set_timing(self._h, int(bool(on))))

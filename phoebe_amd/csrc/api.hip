@@ -39,21 +39,6 @@ void launch_gather_eig(int na, int nw, const int* ik, const int* lo, const int* 
 void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const double2* ev,
                        double* wAll, double2* evAll, hipStream_t s);
 
-struct CacheParams {
-  int nRe, nRp, nm, nw, nkk;
-  const void* gW;
-  const double2* phE;
-  const double2* phP;
-  const double2* U1pool;
-  const int* U1off;
-  const int* nb1;
-  const int* panelBase;
-  void* B;
-  size_t panelStride;
-  int ldb;
-};
-template <typename T, typename OutT> void launch_cache(const CacheParams& p, hipStream_t s);
-constexpr int kCacheNKK = 32;  // outer points per cacheElPh launch
 
 // coupling kernels: CoupParams / SepParams and their launchers (instantiated in
 // coupling_f64.hip and coupling_f32.hip)
@@ -65,14 +50,6 @@ extern template void launch_separable<float>(const SepParams&, const CoupParams&
                                              hipStream_t);
 int coupling_rows_per_tile_f64(int ldb);
 
-struct CollectParams {
-  int nkk, nMtiles, nCalc, numStates;
-  const int* kkNb1; const int* kkPanelBase; const int* kkStateOff;
-  const int* panelGStart; const int* panelGSize;
-  const double* partial;
-  double* linewidth;
-};
-void launch_collect(const CollectParams& p, hipStream_t s);
 
 
 // Run f(i) for i in [0, n) on the host's hardware threads.
@@ -182,6 +159,28 @@ static void mesh_cart(const double* recip, const int* mesh, int index, double* c
   double xc[3] = {double(x) / double(mesh[0]), double(y) / double(mesh[1]),
                   double(z) / double(mesh[2])};
   for (int i = 0; i < 3; ++i) c[i] = recip[i] * xc[0] + recip[i + 3] * xc[1] + recip[i + 6] * xc[2];
+}
+
+// Integer coordinates n_d = b_d.R / 2pi of lattice vectors R (3 x n, Cartesian),
+// folded into [0, m_d).  False when some R is not a lattice vector.
+static bool folded_lattice_coords(const double* recip, const int* mesh, const double* R, int n,
+                                  int* f) {
+  for (int r = 0; r < n; ++r)
+    for (int d = 0; d < 3; ++d) {
+      double x = 0.;
+      for (int i = 0; i < 3; ++i) x += recip[i + 3 * d] * R[3 * r + i];
+      x /= 2. * kPi;
+      const double v = std::round(x);
+      if (!(std::abs(v - x) < 1e-6)) return false;
+      f[3 * r + d] = ((int(v) % mesh[d]) + mesh[d]) % mesh[d];
+    }
+  return true;
+}
+
+// e^{2 pi i t / m}
+static double2 twiddle(long long t, int m) {
+  const double a = 2. * kPi * double(t % m) / double(m);
+  return make_double2(std::cos(a), std::sin(a));
 }
 
 // ============================================================ batched eig / Bloch
@@ -487,6 +486,18 @@ struct LwHandle {
   DevBuf<double2> twx, twy, twz;
   DevBuf<double> T1d, T2d;
   DevBuf<float> T1f, T2f;
+  // cacheElPh on mesh lines: outer points of a run that share (j1, j2) are
+  // batched together when the line holds at least lineMin of them (0 = off)
+  int lineMin = 0;
+  bool reLattice = false;  // every R_e is a lattice vector (required by the line form)
+  int nGe = 0;
+  std::vector<int> reF1, reF2, grpF0e;  // n1, n2 mod m of the grouped R_e; n0 mod m0 per group
+  std::vector<double> reInvDeg;         // 1/de of the grouped R_e
+  DevBuf<int> reOrderD, reGroupEndD;
+  DevBuf<double2> phLineD, wLineD, TgD;
+  DevBuf<float2> phLineF, TgF;
+  std::vector<double2> hphLine, hwLine;
+  std::vector<float2> hphLineF;
   // energy-conservation screen of the mesh transform (units of sigma, 0 = off)
   double screen = 0.;
   DevBuf<double> qWlo, qWhi;
@@ -763,15 +774,8 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     //      folded mod m_d; z-lines = distinct (f0, f1), x-groups = distinct f0.
     const int m[3] = {p.mesh[0], p.mesh[1], p.mesh[2]};
     std::vector<int> f(3 * size_t(p.nRp));
-    for (int r = 0; r < p.nRp; ++r)
-      for (int d = 0; d < 3; ++d) {
-        double x = 0.;
-        for (int i = 0; i < 3; ++i) x += p.recip[i + 3 * d] * p.Rp[3 * r + i];
-        x /= 2. * kPi;
-        const double n = std::round(x);
-        PHX_REQUIRE(std::abs(n - x) < 1e-6, "separable transform: R_ph is not a lattice vector");
-        f[3 * r + d] = ((int(n) % m[d]) + m[d]) % m[d];
-      }
+    PHX_REQUIRE(folded_lattice_coords(p.recip, p.mesh, p.Rp, p.nRp, f.data()),
+                "separable transform: R_ph is not a lattice vector");
     std::vector<int> lineOf(size_t(m[0]) * m[1], -1), grpOf(m[0], -1);
     std::vector<std::vector<int>> lineE;  // R_ph indices of each line
     std::vector<int> lineKey;             // f0 + m0 * f1
@@ -834,13 +838,44 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     for (int a = 0; a < na; ++a) m2r[L->active[a]] = a;
     h2d(L->meshToRow, m2r, s);
   }
+  // ---- R_e grouped by n0 mod m0 for the mesh-line form of cacheElPh
+  {
+    std::vector<int> f(3 * size_t(p.nRe));
+    L->reLattice = folded_lattice_coords(p.recip, p.mesh, p.Re, p.nRe, f.data());
+    if (L->reLattice) {
+      std::vector<int> order(p.nRe);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return f[3 * a] < f[3 * b]; });
+      std::vector<int> groupEnd(p.nRe, -1);
+      L->reF1.resize(p.nRe);
+      L->reF2.resize(p.nRe);
+      L->reInvDeg.resize(p.nRe);
+      L->grpF0e.clear();
+      for (int i = 0; i < p.nRe; ++i) {
+        const int r = order[i];
+        L->reF1[i] = f[3 * r + 1];
+        L->reF2[i] = f[3 * r + 2];
+        L->reInvDeg[i] = 1. / p.degE[r];
+        if (i + 1 == p.nRe || f[3 * order[i + 1]] != f[3 * r]) {
+          groupEnd[i] = int(L->grpF0e.size());
+          L->grpF0e.push_back(f[3 * r]);
+        }
+      }
+      L->nGe = int(L->grpF0e.size());
+      h2d(L->reOrderD, order, s);
+      h2d(L->reGroupEndD, groupEnd, s);
+      PHX_HIP(hipStreamSynchronize(s));
+    }
+  }
   L->lw.alloc(size_t(p.nCalc) * st);
   launch_fill<double>(L->lw.p, L->lw.n, 0., s);
   PHX_HIP(hipStreamSynchronize(s));
 }
 
-// Enqueue one batch of outer points.
-static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e) {
+// Enqueue one batch of outer points.  line: every k1 of the batch lies on one
+// mesh line (same j1, j2) and cacheElPh runs in its line form.
+static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, bool line) {
   const phx_problem& p = L->prob;
   hipStream_t s = L->stream;
   const int nw = L->nw, nm = L->nm;
@@ -907,10 +942,46 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   const int* dPB1 = dPKK + nPanels;
   const int* dPGS = dPB1 + nPanels;
   const int* dPGZ = dPGS + nPanels;
-  // phases: e^{ik1.Re}/de and e^{-ik1.Rp}/dp
-  L->phE.reserve(size_t(nkk) * L->nRe);
+  // phases: e^{ik1.Re}/de (GEMM form) or the line tables, and e^{-ik1.Rp}/dp
+  if (line) {
+    const int m0 = p.mesh[0], m1 = p.mesh[1], m2 = p.mesh[2];
+    int j0, j1, j2;
+    mesh_coords(L->active[k1s[0]], m0, m1, j0, j1, j2);
+    L->hphLine.resize(L->nRe);
+    for (int i = 0; i < L->nRe; ++i) {
+      const double2 a = twiddle((long long)j1 * L->reF1[i], m1);
+      const double2 b = twiddle((long long)j2 * L->reF2[i], m2);
+      const double d = L->reInvDeg[i];
+      L->hphLine[i] = make_double2((a.x * b.x - a.y * b.y) * d, (a.x * b.y + a.y * b.x) * d);
+    }
+    L->hwLine.resize(size_t(nkk) * L->nGe);
+    for (int i = 0; i < nkk; ++i) {
+      int x, y, z;
+      mesh_coords(L->active[k1s[i]], m0, m1, x, y, z);
+      PHX_REQUIRE(y == j1 && z == j2, "line batch: outer points on different mesh lines");
+      for (int g = 0; g < L->nGe; ++g)
+        L->hwLine[size_t(i) * L->nGe + g] = twiddle((long long)x * L->grpF0e[g], m0);
+    }
+    L->wLineD.reserve(L->hwLine.size());
+    PHX_HIP(hipMemcpyAsync(L->wLineD.p, L->hwLine.data(), L->hwLine.size() * sizeof(double2),
+                           hipMemcpyHostToDevice, s));
+    if (L->precision == PHX_FP64) {
+      L->phLineD.reserve(L->nRe);
+      PHX_HIP(hipMemcpyAsync(L->phLineD.p, L->hphLine.data(), L->nRe * sizeof(double2),
+                             hipMemcpyHostToDevice, s));
+    } else {
+      L->hphLineF.resize(L->nRe);
+      for (int i = 0; i < L->nRe; ++i)
+        L->hphLineF[i] = make_float2(float(L->hphLine[i].x), float(L->hphLine[i].y));
+      L->phLineF.reserve(L->nRe);
+      PHX_HIP(hipMemcpyAsync(L->phLineF.p, L->hphLineF.data(), L->nRe * sizeof(float2),
+                             hipMemcpyHostToDevice, s));
+    }
+  } else {
+    L->phE.reserve(size_t(nkk) * L->nRe);
+    launch_phase_table(nkk, L->nRe, L->bk1.p, L->Re.p, L->invDegE.p, 1.0, L->phE.p, s);
+  }
   L->phP.reserve(size_t(nkk) * L->nRp);
-  launch_phase_table(nkk, L->nRe, L->bk1.p, L->Re.p, L->invDegE.p, 1.0, L->phE.p, s);
   launch_phase_table(nkk, L->nRp, L->bk1.p, L->Rp.p, L->invDegP.p, -1.0, L->phP.p, s);
   if (L->timing) PHX_HIP(hipEventRecord(L->ev[0], s));
   // cacheElPh for the whole batch
@@ -929,12 +1000,33 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cp.panelBase = dPbase;
   cp.panelStride = pstride;
   cp.ldb = L->ldb;
+  if (line) {
+    cp.N = size_t(L->nRp) * nm * nw * nw;
+    cp.nGe = L->nGe;
+    cp.reOrder = L->reOrderD.p;
+    cp.reGroupEnd = L->reGroupEndD.p;
+    cp.wLine = L->wLineD.p;
+  }
   if (L->precision == PHX_FP64) {
     cp.B = L->Bd.p;
-    launch_cache<double, double>(cp, s);
+    if (line) {
+      L->TgD.reserve(size_t(L->nGe) * cp.N);
+      cp.phLine = L->phLineD.p;
+      cp.Tg = L->TgD.p;
+      launch_cache_line<double, double>(cp, s);
+    } else {
+      launch_cache<double, double>(cp, s);
+    }
   } else {
     cp.B = L->Bf.p;
-    launch_cache<float, float>(cp, s);
+    if (line) {
+      L->TgF.reserve(size_t(L->nGe) * cp.N);
+      cp.phLine = L->phLineF.p;
+      cp.Tg = L->TgF.p;
+      launch_cache_line<float, float>(cp, s);
+    } else {
+      launch_cache<float, float>(cp, s);
+    }
   }
   if (M <= 0) {
     if (L->timing) {
@@ -1104,13 +1196,34 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
   }
   for (int i = 0; i < nk1; ++i) PHX_REQUIRE(k1list[i] >= 0 && k1list[i] < L->na, "phx_lw_run: bad k1");
   long long pairs = 0;
-  for (int i = 0; i < nk1; i += L->k1Batch) {
-    int n = std::min(L->k1Batch, nk1 - i);
-    // the metadata buffer is reused per batch: wait for the previous batch
+  auto run = [&](const int* k1s, int n, bool line) {
+    // the metadata buffers are reused per batch: wait for the previous batch
     PHX_HIP(hipStreamSynchronize(L->stream));
     lw_harvest_timing(L);
-    pairs += lw_batch(L, k1list + i, n, k2b, k2e);
+    pairs += lw_batch(L, k1s, n, k2b, k2e, line);
+  };
+  // outer points on well-filled mesh lines (same j1, j2): one line batch each
+  std::vector<int> rest;
+  if (L->lineMin > 0) {
+    const int m0 = L->prob.mesh[0];
+    std::vector<int> order(k1list, k1list + nk1);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return L->active[a] / m0 < L->active[b] / m0;  // mesh index / m0 = line (j1, j2)
+    });
+    for (size_t i = 0; i < order.size();) {
+      size_t j = i;
+      while (j < order.size() && L->active[order[j]] / m0 == L->active[order[i]] / m0) ++j;
+      if (int(j - i) >= L->lineMin) {
+        for (size_t c = i; c < j; c += kLineMax) run(order.data() + c, int(std::min<size_t>(kLineMax, j - c)), true);
+      } else {
+        rest.insert(rest.end(), order.begin() + i, order.begin() + j);
+      }
+      i = j;
+    }
+    k1list = rest.data();
+    nk1 = int(rest.size());
   }
+  for (int i = 0; i < nk1; i += L->k1Batch) run(k1list + i, std::min(L->k1Batch, nk1 - i), false);
   return pairs;
 }
 
@@ -1363,6 +1476,16 @@ int phx_lw_set_screen(phx_lw* h, double nsigma) {
     PHX_REQUIRE(nsigma == 0. || h->transform == PHX_TRANSFORM_SEPARABLE,
                 "phx_lw_set_screen: the screen runs with the separable transform");
     h->screen = nsigma;
+  });
+}
+
+int phx_lw_set_line_batching(phx_lw* h, int min_points) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_REQUIRE(min_points >= 0, "phx_lw_set_line_batching: min_points must be >= 0");
+    PHX_REQUIRE(min_points == 0 || h->reLattice,
+                "phx_lw_set_line_batching: the electronic R vectors are not lattice vectors");
+    h->lineMin = min_points;
   });
 }
 

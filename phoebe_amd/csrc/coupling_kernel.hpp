@@ -21,6 +21,7 @@
 
 #include <algorithm>
 
+#include "elph_params.hpp"
 #include "phx_common.hpp"
 
 namespace phx {
@@ -28,8 +29,6 @@ namespace phx {
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kGMax = 8;  // largest degenerate b1 group handled in one panel
-constexpr int kCMax = 4;  // largest number of (T, mu) calculations
 constexpr int kNmMax = 36;
 constexpr int kKC = 16;   // K chunk staged in LDS
 

@@ -24,6 +24,7 @@
 #include <thread>
 #include <vector>
 
+#include "elph_params.hpp"
 #include "phx_common.hpp"
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
@@ -210,19 +211,6 @@ void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const dou
 //   A'(m, k) = [ Pr | -Pi ] for m < NKK,  [ Pi | Pr ] for m >= NKK.
 // One workgroup owns one (Rp, nu) block of nw^2 columns (256 per pass), so the
 // U1 rotation and the e^{-ik1.Rp}/dp factor run as its epilogue.
-struct CacheParams {
-  int nRe, nRp, nm, nw, nkk;
-  const void* gW;         // (nRe, nRp, nm, nw, nw) complex, element type T
-  const double2* phE;     // (nkk, nRe): e^{ik1.Re}/de
-  const double2* phP;     // (nkk, nRp): e^{-ik1.Rp}/dp or 1/dp
-  const double2* U1pool;  // per kk: (nw, nb1) column-major
-  const int* U1off;       // complex offset per kk
-  const int* nb1;         // per kk
-  const int* panelBase;   // first panel of kk
-  void* B;                // panels
-  size_t panelStride;     // elements
-  int ldb;                // elements per panel row
-};
 
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
@@ -244,7 +232,52 @@ template <> struct MfmaC<float> {
   __device__ static int row(int lk, int i) { return 4 * lk + i; }
 };
 
-constexpr int kCacheNKK = 32;  // outer points per launch
+// Panel epilogue shared by the cacheElPh kernels.  Cb holds g1 of 16 outer
+// points (kbase .. kbase+15) for the 256 columns c0 .. c0+255 of block (rp, nu):
+// real parts in rows 0..15, imaginary parts in rows 16..31.  Applies the U1
+// rotation over the w1 range of those columns and the factor s(k1, Rp), and
+// writes (c0 == 0) or adds (later column passes) the panel entries.  Each
+// output (b1, w2) is owned by the same thread in every pass.
+template <typename T, typename OutT>
+__device__ __forceinline__ void cache_panel_epilogue(const CacheParams& p, const T* Cb, int c0,
+                                                     int kbase, int rp, int nu) {
+  const int t = threadIdx.x;
+  OutT* Bout = static_cast<OutT*>(p.B);
+  for (int kl = 0; kl < 16; ++kl) {
+    const int kk = kbase + kl;
+    if (kk >= p.nkk) break;
+    const int nb1 = p.nb1[kk];
+    const double2* U1 = p.U1pool + p.U1off[kk];
+    const double2 s = p.phP[size_t(kk) * p.nRp + rp];
+    // columns of this pass: w1 in [c0/nw, (c0+256)/nw), all w2
+    const int w1lo = c0 / p.nw, w1hi = min(p.nw, (c0 + 256) / p.nw);
+    for (int o = t; o < nb1 * p.nw; o += 256) {
+      const int b1 = o / p.nw, w2 = o % p.nw;
+      double2 v = make_double2(0., 0.);
+      for (int w1 = w1lo; w1 < w1hi; ++w1) {
+        const int cl = w1 * p.nw + w2 - c0;
+        const double ar = double(Cb[kl * 256 + cl]), ai = double(Cb[(16 + kl) * 256 + cl]);
+        const double2 u = U1[w1 + p.nw * b1];
+        v.x += ar * u.x - ai * u.y;
+        v.y += ar * u.y + ai * u.x;
+      }
+      const double2 c = make_double2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
+      OutT* panel = Bout + size_t(p.panelBase[kk] + b1) * p.panelStride;
+      const size_t col = 2 * (size_t(nu) * p.nw + w2);
+      if (c0 == 0) {
+        panel[size_t(2 * rp) * p.ldb + col] = OutT(c.x);
+        panel[size_t(2 * rp) * p.ldb + col + 1] = OutT(c.y);
+        panel[size_t(2 * rp + 1) * p.ldb + col] = OutT(-c.y);
+        panel[size_t(2 * rp + 1) * p.ldb + col + 1] = OutT(c.x);
+      } else {  // later column passes add their w1 range
+        panel[size_t(2 * rp) * p.ldb + col] += OutT(c.x);
+        panel[size_t(2 * rp) * p.ldb + col + 1] += OutT(c.y);
+        panel[size_t(2 * rp + 1) * p.ldb + col] += OutT(-c.y);
+        panel[size_t(2 * rp + 1) * p.ldb + col + 1] += OutT(c.x);
+      }
+    }
+  }
+}
 
 template <typename T, typename OutT>
 __global__ void __launch_bounds__(256, 2) k_cache(CacheParams p) {
@@ -259,7 +292,6 @@ __global__ void __launch_bounds__(256, 2) k_cache(CacheParams p) {
   const T2* G = static_cast<const T2*>(p.gW) + (size_t(rp) * p.nm + nu) * E;
   T* As = reinterpret_cast<T*>(smem);               // MR x LDA
   T* Cb = reinterpret_cast<T*>(smem);               // epilogue: 2 x 16 x 256
-  OutT* Bout = static_cast<OutT*>(p.B);
   const int nchunk = (p.nRe + 7) / 8;
 
   for (int c0 = 0; c0 < E; c0 += 256) {
@@ -327,40 +359,7 @@ __global__ void __launch_bounds__(256, 2) k_cache(CacheParams p) {
         }
       }
       __syncthreads();
-      for (int kl = 0; kl < 16; ++kl) {
-        const int kk = 16 * q + kl;
-        if (kk >= p.nkk) break;
-        const int nb1 = p.nb1[kk];
-        const double2* U1 = p.U1pool + p.U1off[kk];
-        const double2 s = p.phP[size_t(kk) * p.nRp + rp];
-        // columns of this pass: w1 in [c0/nw, (c0+256)/nw), all w2
-        const int w1lo = c0 / p.nw, w1hi = min(p.nw, (c0 + 256) / p.nw);
-        for (int o = t; o < nb1 * p.nw; o += 256) {
-          const int b1 = o / p.nw, w2 = o % p.nw;
-          double2 v = make_double2(0., 0.);
-          for (int w1 = w1lo; w1 < w1hi; ++w1) {
-            const int cl = w1 * p.nw + w2 - c0;
-            const double ar = double(Cb[kl * 256 + cl]), ai = double(Cb[(16 + kl) * 256 + cl]);
-            const double2 u = U1[w1 + p.nw * b1];
-            v.x += ar * u.x - ai * u.y;
-            v.y += ar * u.y + ai * u.x;
-          }
-          const double2 c = make_double2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
-          OutT* panel = Bout + size_t(p.panelBase[kk] + b1) * p.panelStride;
-          const size_t col = 2 * (size_t(nu) * p.nw + w2);
-          if (c0 == 0) {
-            panel[size_t(2 * rp) * p.ldb + col] = OutT(c.x);
-            panel[size_t(2 * rp) * p.ldb + col + 1] = OutT(c.y);
-            panel[size_t(2 * rp + 1) * p.ldb + col] = OutT(-c.y);
-            panel[size_t(2 * rp + 1) * p.ldb + col + 1] = OutT(c.x);
-          } else {  // later column passes add their w1 range
-            panel[size_t(2 * rp) * p.ldb + col] += OutT(c.x);
-            panel[size_t(2 * rp) * p.ldb + col + 1] += OutT(c.y);
-            panel[size_t(2 * rp + 1) * p.ldb + col] += OutT(-c.y);
-            panel[size_t(2 * rp + 1) * p.ldb + col + 1] += OutT(c.x);
-          }
-        }
-      }
+      cache_panel_epilogue<T, OutT>(p, Cb, c0, 16 * q, rp, nu);
     }
   }
 }
@@ -384,6 +383,116 @@ void launch_cache(const CacheParams& p, hipStream_t s) {
 template void launch_cache<double, double>(const CacheParams&, hipStream_t);
 template void launch_cache<float, float>(const CacheParams&, hipStream_t);
 
+// ------------------------------------------------------------------ cacheElPh on a mesh line
+// Outer points that share the mesh line (j1, j2) and differ in j0 factor the
+// phase: with n_d = b_d.R_e / 2pi,
+//   e^{ik1.R_e} = e^{2 pi i j0 n0/m0} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}.
+// Stage a (HBM stream over gW, once per line):
+//   Tg(g, c) = sum_{R_e in group g} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}/de gW(R_e, c)
+// with groups of equal n0 mod m0 (about 2 * 10 + 1 of them for a 10^3 R_e grid).
+// Stage b (per outer point): g1(k1, c) = sum_g e^{2 pi i j0 n0(g)/m0} Tg(g, c),
+// then the same U1 / s(k1, Rp) panel epilogue as k_cache.
+constexpr int kLineDepth = 8;  // gW rows in flight per thread
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
+  typedef typename Vec2<T>::type T2;
+  constexpr int D = kLineDepth;
+  const size_t col = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const bool ok = col < p.N;
+  const T2* G = static_cast<const T2*>(p.gW);
+  const T2* ph = static_cast<const T2*>(p.phLine);
+  T2* out = static_cast<T2*>(p.Tg);
+  T2 cur[D], nxt[D];
+  auto load = [&](T2(&b)[D], int i0) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int i = i0 + u;
+      b[u] = (ok && i < p.nRe) ? G[size_t(p.reOrder[i]) * p.N + col] : T2{T(0), T(0)};
+    }
+  };
+  load(cur, 0);
+  T ar = T(0), ai = T(0);
+  for (int i0 = 0; i0 < p.nRe; i0 += D) {
+    load(nxt, i0 + D);
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int i = i0 + u;
+      if (i < p.nRe) {
+        const T2 f = ph[i];
+        ar += f.x * cur[u].x - f.y * cur[u].y;
+        ai += f.x * cur[u].y + f.y * cur[u].x;
+        const int g = p.reGroupEnd[i];
+        if (g >= 0) {
+          if (ok) out[size_t(g) * p.N + col] = T2{ar, ai};
+          ar = T(0);
+          ai = T(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) cur[u] = nxt[u];
+  }
+}
+
+// One workgroup per (Rp, nu) block of nw^2 columns, in passes of 256 columns
+// and 16 outer points.
+template <typename T, typename OutT>
+__global__ void __launch_bounds__(256) k_cache_line_b(CacheParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Vec2<T>::type T2;
+  const int rp = blockIdx.x / p.nm, nu = blockIdx.x % p.nm;
+  const int E = p.nw * p.nw, t = threadIdx.x;
+  T* Cb = reinterpret_cast<T*>(smem);  // 2 x 16 x 256
+  const T2* Tg = static_cast<const T2*>(p.Tg);
+  for (int c0 = 0; c0 < E; c0 += 256) {
+    const int cl = c0 + t;
+    const bool ok = cl < E;
+    const size_t colg = (size_t(rp) * p.nm + nu) * E + cl;
+    for (int kbase = 0; kbase < p.nkk; kbase += 16) {
+      T accr[16], acci[16];
+#pragma unroll
+      for (int kl = 0; kl < 16; ++kl) accr[kl] = acci[kl] = T(0);
+      for (int g = 0; g < p.nGe; ++g) {
+        const T2 v = ok ? Tg[size_t(g) * p.N + colg] : T2{T(0), T(0)};
+#pragma unroll
+        for (int kl = 0; kl < 16; ++kl) {
+          const int kk = min(kbase + kl, p.nkk - 1);
+          const double2 w = p.wLine[size_t(kk) * p.nGe + g];
+          const T wr = T(w.x), wi = T(w.y);
+          accr[kl] += wr * v.x - wi * v.y;
+          acci[kl] += wr * v.y + wi * v.x;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kl = 0; kl < 16; ++kl) {
+        Cb[kl * 256 + t] = accr[kl];
+        Cb[(16 + kl) * 256 + t] = acci[kl];
+      }
+      __syncthreads();
+      cache_panel_epilogue<T, OutT>(p, Cb, c0, kbase, rp, nu);
+    }
+  }
+}
+
+template <typename T, typename OutT>
+void launch_cache_line(const CacheParams& p, hipStream_t s) {
+  PHX_REQUIRE(p.nw <= 32, "cacheElPh kernel supports nw <= 32");
+  PHX_REQUIRE(p.nGe >= 1 && p.N == size_t(p.nRp) * p.nm * p.nw * p.nw, "cacheElPh line: bad sizes");
+  if (p.nkk <= 0) return;
+  hipLaunchKernelGGL(k_cache_line_a<T>, dim3(ceil_div((long long)p.N, 256)), dim3(256), 0, s, p);
+  PHX_HIP(hipGetLastError());
+  const size_t lds = size_t(2) * 16 * 256 * sizeof(T);
+  auto kb = k_cache_line_b<T, OutT>;
+  PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kb),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL(kb, dim3(p.nRp * p.nm), dim3(256), lds, s, p);
+  PHX_HIP(hipGetLastError());
+}
+template void launch_cache_line<double, double>(const CacheParams&, hipStream_t);
+template void launch_cache_line<float, float>(const CacheParams&, hipStream_t);
+
 // FP64 -> FP32 copy of the coupling tensor (the reduced-precision GEMMs read it).
 __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, size_t n) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
@@ -396,17 +505,8 @@ void launch_to_float(const double* a, float* b, size_t n, hipStream_t s) {
 
 // ------------------------------------------------------------------ coupling
 // (coupling_kernel.hpp, instantiated in coupling_f64.hip and coupling_f32.hip)
-constexpr int kGMax = 8;
-constexpr int kCMax = 4;
 
 // ------------------------------------------------------------------ collect
-struct CollectParams {
-  int nkk, nMtiles, nCalc, numStates;
-  const int* kkNb1; const int* kkPanelBase; const int* kkStateOff;
-  const int* panelGStart; const int* panelGSize;
-  const double* partial;
-  double* linewidth;  // (nCalc, numStates), accumulated into
-};
 
 __global__ void k_collect(CollectParams p) {
   const int kk = blockIdx.x;

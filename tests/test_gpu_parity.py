@@ -197,3 +197,53 @@ def test_screen_requires_separable_and_valid_width():
     s = P.ElectronLinewidths(p, gW=p.gw_host(), precision="fp64", transform="separable")
     with pytest.raises(P.PhxError):
         s.set_screen(3.0)
+
+
+@pytest.mark.parametrize("transform", ["direct", "separable"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+def test_line_batching_vs_oracle(precision, tol, transform):
+    """Outer points batched by mesh line (cacheElPh streams gW once per line)
+    give the oracle's linewidths; lines below the threshold take the GEMM path."""
+    say("line batching", precision, transform)
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    gW = p.gw_host()
+    orc = O.Oracle(p, gW.reshape(-1))
+    ref = orc.linewidths()
+    lw = P.ElectronLinewidths(p, gW=gW, precision=precision, k1_batch=7, transform=transform)
+    lw.set_line_batching(4)
+    lines = lw.active // p.mesh[0]
+    counts = np.bincount(lines)
+    say("  lines with >= 4 points:", int((counts >= 4).sum()), "of", int((counts > 0).sum()))
+    assert (counts >= 4).any() and ((counts > 0) & (counts < 4)).any()
+    assert lw.run() == orc.last_pairs
+    err = rel_err(lw.linewidths(), ref)
+    say("  max rel err", err.max())
+    assert err.max() < tol
+
+
+@pytest.mark.parametrize("mesh", [(4, 5, 6), (7, 3, 2)])
+def test_line_batching_matches_gemm_cache_on_uneven_mesh(mesh):
+    say("line batching uneven", mesh)
+    p = synth.make_problem(2, 5, 3, 3, mesh, seed=44, name="uneven")
+    gW = p.gw_host()
+    a = P.ElectronLinewidths(p, gW=gW, precision="fp64", transform="separable")
+    b = P.ElectronLinewidths(p, gW=gW, precision="fp64", transform="separable")
+    b.set_line_batching(1)
+    # a shuffled subset: line grouping must not depend on the order of the list
+    k1 = np.random.default_rng(3).permutation(a.num_active)[: max(2, a.num_active * 2 // 3)]
+    a.run(k1_list=k1)
+    b.run(k1_list=k1)
+    x, y = a.linewidths(average=False), b.linewidths(average=False)
+    assert np.array_equal(x != 0, y != 0)
+    m = x != 0
+    err = rel_err(y[m], x[m])
+    say("  max rel err", err.max())
+    assert err.max() < 1e-11
+
+
+def test_line_batching_argument_validation():
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    s = P.ElectronLinewidths(p, gW=p.gw_host(), precision="fp64")
+    with pytest.raises(P.PhxError):
+        s.set_line_batching(-1)
+    s.set_line_batching(0)
