@@ -58,9 +58,9 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
         lw.set_line_batching(line_min)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f}s: active {lw.num_active}, "
         f"states {lw.num_states}")
-    k1_batches = make_batches(lw.active, line_min > 0)
-    for w in range(warmup):
-        lw.run(k1_list=k1_batches[w % len(k1_batches)])
+    k1_batches, warm_batches = make_batches(lw.active, line_min > 0, warmup)
+    for b in warm_batches:
+        lw.run(k1_list=b)
     lw.sync()
     lw.reset()
     lw.set_timing(True)
@@ -73,7 +73,7 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
     elapsed = time.perf_counter() - t0
     kt = lw.kernel_times()
     gam = lw.linewidths(average=False)
-    log(f"[{form}] {len(k1_batches)} steps in {elapsed:.3f}s, {pairs} pairs, "
+    log(f"[{form}] {len(k1_batches)} run calls in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
     return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, linewidths=gam,
                     k1=int(sum(len(b) for b in k1_batches)))
@@ -149,22 +149,23 @@ def main():
     B = args.k1_batch
     m0 = prob.mesh[0]
 
-    def make_batches(active, by_line):
-        """The outer points of `steps` mesh lines: one list per line, or the
-        same points in k1-batched lists of B."""
+    def make_batches(active, by_line, warmup):
+        """(timed, warmup) lists of outer points: the points of `steps` mesh
+        lines (then of `warmup` further lines), as one list handed to the
+        library's line batching, or as k1-batched lists of B points."""
         line = np.asarray(active) // m0
-        ids = np.unique(line)
-        pick = np.random.default_rng(12345).permutation(ids)[:args.steps]
-        lines = [np.flatnonzero(line == l).astype(np.int32) for l in pick]
+        pick = np.random.default_rng(12345).permutation(np.unique(line))
+        def pts(ls):
+            return np.concatenate([np.flatnonzero(line == l) for l in ls]).astype(np.int32)
+        timed, warm = pts(pick[:args.steps]), pts(pick[args.steps:args.steps + max(warmup, 1)])
         if by_line:
-            return lines
-        pts = np.concatenate(lines)
-        return [pts[i:i + B] for i in range(0, len(pts), B)]
+            return [timed], ([warm] if warmup else [])
+        return [timed[i:i + B] for i in range(0, len(timed), B)], [warm[i * B:(i + 1) * B] for i in range(warmup)]
 
     # plain FP64
     lw64, r64 = run_form(P, prob, "fp64", make_batches, args.warmup, B)
     na = lw64.num_active
-    batches = make_batches(lw64.active, False)
+    batches, _ = make_batches(lw64.active, False, 0)
     cpu = None
     if not args.no_cpu_baseline:
         try:

@@ -190,6 +190,10 @@ int phx_lw_set_screen(phx_lw* h, double nsigma);
  * to rounding.  No counterpart in the reference, whose cacheElPh
  * (interaction_elph.cpp:229-448) sums over R_e once per outer point. */
 int phx_lw_set_line_batching(phx_lw* h, int min_points);
+/* Panels ((k1, b1) pairs) per launch of the separable transform (default 48).
+ * Its two stage buffers hold one chunk: (lines * mz + groups * my * mz) *
+ * nm * nw complex values per panel.  No counterpart in the reference. */
+int phx_lw_set_panel_chunk(phx_lw* h, int panels);
 /* Per-kernel timing with HIP events recorded on the handle's stream around
  * every batch; turning it on (or off) clears the accumulators. */
 int phx_lw_set_timing(phx_lw* h, int on);

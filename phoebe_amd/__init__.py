@@ -85,6 +85,7 @@ EXPORTS = {
     "phx_lw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_screen": (C.c_int, [C.c_void_p, C.c_double]),
     "phx_lw_set_line_batching": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_set_panel_chunk": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_stream": (C.c_void_p, [C.c_void_p]),
     "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
@@ -344,6 +345,10 @@ class ElectronLinewidths:
         """Batch outer points by mesh line (cacheElPh streams gW once per line) when
         a line holds at least min_points of a run's points; 0 = off."""
         _check(lib().phx_lw_set_line_batching(self._h, int(min_points)))
+
+    def set_panel_chunk(self, panels):
+        """Panels per separable-transform launch (bounds its stage buffers)."""
+        _check(lib().phx_lw_set_panel_chunk(self._h, int(panels)))
 
     def set_timing(self, on=True):
         _check(lib().phx_lw_set_timing(self._h, int(bool(on))))

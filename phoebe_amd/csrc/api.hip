@@ -489,6 +489,7 @@ struct LwHandle {
   // cacheElPh on mesh lines: outer points of a run that share (j1, j2) are
   // batched together when the line holds at least lineMin of them (0 = off)
   int lineMin = 0;
+  int panelChunk = 48;     // panels per separable-transform launch (bounds T1 / T2)
   bool reLattice = false;  // every R_e is a lattice vector (required by the line form)
   int nGe = 0;
   std::vector<int> reF1, reF2, grpF0e;  // n1, n2 mod m of the grouped R_e; n0 mod m0 per group
@@ -497,6 +498,8 @@ struct LwHandle {
   DevBuf<double2> phLineD, wLineD, TgD;
   DevBuf<float2> phLineF, TgF;
   std::vector<double2> hphLine, hwLine;
+  std::vector<int> hLineKK;
+  DevBuf<int> lineKKD;
   std::vector<float2> hphLineF;
   // energy-conservation screen of the mesh transform (units of sigma, 0 = off)
   double screen = 0.;
@@ -943,38 +946,52 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   const int* dPGS = dPB1 + nPanels;
   const int* dPGZ = dPGS + nPanels;
   // phases: e^{ik1.Re}/de (GEMM form) or the line tables, and e^{-ik1.Rp}/dp
+  int nLinesB = 0;
   if (line) {
+    // lines of the batch: consecutive outer points with equal (j1, j2)
     const int m0 = p.mesh[0], m1 = p.mesh[1], m2 = p.mesh[2];
-    int j0, j1, j2;
-    mesh_coords(L->active[k1s[0]], m0, m1, j0, j1, j2);
-    L->hphLine.resize(L->nRe);
-    for (int i = 0; i < L->nRe; ++i) {
-      const double2 a = twiddle((long long)j1 * L->reF1[i], m1);
-      const double2 b = twiddle((long long)j2 * L->reF2[i], m2);
-      const double d = L->reInvDeg[i];
-      L->hphLine[i] = make_double2((a.x * b.x - a.y * b.y) * d, (a.x * b.y + a.y * b.x) * d);
+    std::vector<int>& lk = L->hLineKK;
+    lk.clear();
+    for (int i = 0; i < nkk; ++i)
+      if (i == 0 || L->active[k1s[i]] / m0 != L->active[k1s[i - 1]] / m0) lk.push_back(i);
+    nLinesB = int(lk.size());
+    lk.push_back(nkk);
+    PHX_REQUIRE(nLinesB <= kLinesPerBatch, "line batch: too many mesh lines");
+    L->hphLine.resize(size_t(nLinesB) * L->nRe);
+    for (int l = 0; l < nLinesB; ++l) {
+      int j0, j1, j2;
+      mesh_coords(L->active[k1s[lk[l]]], m0, m1, j0, j1, j2);
+      for (int i = 0; i < L->nRe; ++i) {
+        const double2 a = twiddle((long long)j1 * L->reF1[i], m1);
+        const double2 b = twiddle((long long)j2 * L->reF2[i], m2);
+        const double d = L->reInvDeg[i];
+        L->hphLine[size_t(l) * L->nRe + i] =
+            make_double2((a.x * b.x - a.y * b.y) * d, (a.x * b.y + a.y * b.x) * d);
+      }
     }
     L->hwLine.resize(size_t(nkk) * L->nGe);
     for (int i = 0; i < nkk; ++i) {
       int x, y, z;
       mesh_coords(L->active[k1s[i]], m0, m1, x, y, z);
-      PHX_REQUIRE(y == j1 && z == j2, "line batch: outer points on different mesh lines");
       for (int g = 0; g < L->nGe; ++g)
         L->hwLine[size_t(i) * L->nGe + g] = twiddle((long long)x * L->grpF0e[g], m0);
     }
+    L->lineKKD.reserve(lk.size());
+    PHX_HIP(hipMemcpyAsync(L->lineKKD.p, lk.data(), lk.size() * sizeof(int), hipMemcpyHostToDevice, s));
     L->wLineD.reserve(L->hwLine.size());
     PHX_HIP(hipMemcpyAsync(L->wLineD.p, L->hwLine.data(), L->hwLine.size() * sizeof(double2),
                            hipMemcpyHostToDevice, s));
+    const size_t nph = L->hphLine.size();
     if (L->precision == PHX_FP64) {
-      L->phLineD.reserve(L->nRe);
-      PHX_HIP(hipMemcpyAsync(L->phLineD.p, L->hphLine.data(), L->nRe * sizeof(double2),
+      L->phLineD.reserve(nph);
+      PHX_HIP(hipMemcpyAsync(L->phLineD.p, L->hphLine.data(), nph * sizeof(double2),
                              hipMemcpyHostToDevice, s));
     } else {
-      L->hphLineF.resize(L->nRe);
-      for (int i = 0; i < L->nRe; ++i)
+      L->hphLineF.resize(nph);
+      for (size_t i = 0; i < nph; ++i)
         L->hphLineF[i] = make_float2(float(L->hphLine[i].x), float(L->hphLine[i].y));
-      L->phLineF.reserve(L->nRe);
-      PHX_HIP(hipMemcpyAsync(L->phLineF.p, L->hphLineF.data(), L->nRe * sizeof(float2),
+      L->phLineF.reserve(nph);
+      PHX_HIP(hipMemcpyAsync(L->phLineF.p, L->hphLineF.data(), nph * sizeof(float2),
                              hipMemcpyHostToDevice, s));
     }
   } else {
@@ -1005,12 +1022,14 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cp.nGe = L->nGe;
     cp.reOrder = L->reOrderD.p;
     cp.reGroupEnd = L->reGroupEndD.p;
+    cp.nLinesB = nLinesB;
+    cp.lineKK = L->lineKKD.p;
     cp.wLine = L->wLineD.p;
   }
   if (L->precision == PHX_FP64) {
     cp.B = L->Bd.p;
     if (line) {
-      L->TgD.reserve(size_t(L->nGe) * cp.N);
+      L->TgD.reserve(size_t(nLinesB) * L->nGe * cp.N);
       cp.phLine = L->phLineD.p;
       cp.Tg = L->TgD.p;
       launch_cache_line<double, double>(cp, s);
@@ -1020,7 +1039,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   } else {
     cp.B = L->Bf.p;
     if (line) {
-      L->TgF.reserve(size_t(L->nGe) * cp.N);
+      L->TgF.reserve(size_t(nLinesB) * L->nGe * cp.N);
       cp.phLine = L->phLineF.p;
       cp.Tg = L->TgF.p;
       launch_cache_line<float, float>(cp, s);
@@ -1126,22 +1145,38 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.screen = L->screen;
     cq.qWlo = L->qWlo.p;
     cq.qWhi = L->qWhi.p;
+    // panels in chunks: the stage buffers T1 / T2 hold one chunk
+    const int chunk = std::min(nPanels, L->panelChunk);
     if (L->precision == PHX_FP64) {
-      L->T1d.reserve(sp.t1PanelStride * nPanels);
-      L->T2d.reserve(sp.t2PanelStride * nPanels);
-      sp.B = L->Bd.p;
+      L->T1d.reserve(sp.t1PanelStride * chunk);
+      L->T2d.reserve(sp.t2PanelStride * chunk);
       sp.T1 = L->T1d.p;
       sp.T2 = L->T2d.p;
       cq.T2 = L->T2d.p;
-      launch_separable<double>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
     } else {
-      L->T1f.reserve(sp.t1PanelStride * nPanels);
-      L->T2f.reserve(sp.t2PanelStride * nPanels);
-      sp.B = L->Bf.p;
+      L->T1f.reserve(sp.t1PanelStride * chunk);
+      L->T2f.reserve(sp.t2PanelStride * chunk);
       sp.T1 = L->T1f.p;
       sp.T2 = L->T2f.p;
       cq.T2 = L->T2f.p;
-      launch_separable<float>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
+    }
+    double* const partial0 = cq.partial;
+    for (int p0 = 0; p0 < nPanels; p0 += chunk) {
+      const int n = std::min(chunk, nPanels - p0);
+      sp.nPanels = n;
+      cq.nPanels = n;
+      cq.panelKK = dPKK + p0;
+      cq.panelB1 = dPB1 + p0;
+      cq.panelGStart = dPGS + p0;
+      cq.panelGSize = dPGZ + p0;
+      cq.partial = partial0 + size_t(p0) * nMtiles * kGMax * kCMax;
+      if (L->precision == PHX_FP64) {
+        sp.B = L->Bd.p + size_t(p0) * pstride;
+        launch_separable<double>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
+      } else {
+        sp.B = L->Bf.p + size_t(p0) * pstride;
+        launch_separable<float>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
+      }
     }
   }
   if (L->timing) {
@@ -1210,16 +1245,30 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
       return L->active[a] / m0 < L->active[b] / m0;  // mesh index / m0 = line (j1, j2)
     });
+    // up to kLinesPerBatch lines (kLinePointsMax points) share one gW stream
+    std::vector<int> batch;
+    int linesIn = 0;
+    auto flush = [&] {
+      if (!batch.empty()) run(batch.data(), int(batch.size()), true);
+      batch.clear();
+      linesIn = 0;
+    };
     for (size_t i = 0; i < order.size();) {
       size_t j = i;
       while (j < order.size() && L->active[order[j]] / m0 == L->active[order[i]] / m0) ++j;
       if (int(j - i) >= L->lineMin) {
-        for (size_t c = i; c < j; c += kLineMax) run(order.data() + c, int(std::min<size_t>(kLineMax, j - c)), true);
+        for (size_t c = i; c < j; c += kLinePointsMax) {
+          const size_t n = std::min<size_t>(kLinePointsMax, j - c);
+          if (linesIn == kLinesPerBatch || batch.size() + n > size_t(kLinePointsMax)) flush();
+          batch.insert(batch.end(), order.begin() + c, order.begin() + c + n);
+          ++linesIn;
+        }
       } else {
         rest.insert(rest.end(), order.begin() + i, order.begin() + j);
       }
       i = j;
     }
+    flush();
     k1list = rest.data();
     nk1 = int(rest.size());
   }
@@ -1486,6 +1535,14 @@ int phx_lw_set_line_batching(phx_lw* h, int min_points) {
     PHX_REQUIRE(min_points == 0 || h->reLattice,
                 "phx_lw_set_line_batching: the electronic R vectors are not lattice vectors");
     h->lineMin = min_points;
+  });
+}
+
+int phx_lw_set_panel_chunk(phx_lw* h, int panels) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_REQUIRE(panels >= 1, "phx_lw_set_panel_chunk: panels must be >= 1");
+    h->panelChunk = panels;
   });
 }
 

@@ -778,6 +778,363 @@ inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int m0, int nw, int n
          size_t(m0) * sizeof(SepXShared) + size_t(m0 + 1) * sizeof(int) + 64;
 }
 
+// ---- last stage on MFMA
+// Same result as k_sep_x.  Per workgroup (one (jy, jz) line of mesh points and
+// one panel), the kept rows jx are compacted, then for every tile of 16 of them
+//   g3(jx, c) = sum_g w_x^{jx n0(g)} T2(g, jy, jz, c)
+// runs as a real GEMM on v_mfma_*_16x16x4 in the transform precision:
+//   A(m, 2g + s) = (Re, Im) w(jx_m, g),   B(2g + s, 2c + t) = [[Xr, Xi], [-Xi, Xr]](s, t),
+// A from a twiddle table in LDS, B straight from T2 (L2).  The epilogue gives
+// TPR threads to each row (thread sub = mode nu).  Workgroups are ordered
+// with the panels innermost, so that consecutive workgroups share the rows'
+// U2 and the q-line's phonons in L2.
+constexpr int kXmNtw = 6;   // n-tiles (16 real columns) per wave: 2 * nm * nw <= 384
+constexpr int kXmKsc = 3;   // MFMA k-steps per B chunk
+
+struct SepXmRow {            // per kept row, staged in LDS
+  int jx, row, iq, nb2, u2off, soff;
+};
+
+template <typename T, bool REDUCE, int NMX, int CM>
+__global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Vec2<T>::type T2v;
+  typedef typename Mfma4<T>::acc_t acc_t;
+  constexpr int TPR = NMX <= 16 ? 16 : 32;  // epilogue threads per row
+  constexpr int RPP = 256 / TPR;            // rows per epilogue pass
+  const int nLinesYZ = p.m1 * p.m2;
+  const int nTilesTot = p.nPanels * nLinesYZ;
+  const int bid = blockIdx.x;
+  const int perXcd = gridDim.x / 8;
+  const int Lx = (bid % 8) * perXcd + bid / 8;
+  if (Lx >= nTilesTot) return;
+  const int yz = Lx / p.nPanels, panel = Lx % p.nPanels;
+  const int slot = panel * nLinesYZ + yz;  // partial-sum slot (panel-major, as k_collect reads it)
+  const int jy = yz % p.m1, jz = yz / p.m1;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
+  const int nm = p.nm, nw = p.nw, ncol = nm * nw;
+  const int NWP = 2 * nw + 1, LDG = nm * NWP;  // g3 row layout: (nu, 2 w2 + ri), padded per nu
+  const int kk = p.panelKK[panel];
+  const int b1p = p.panelB1[panel];
+  const int nG0 = p.nG0, nG0p = nG0 + 1;
+
+  // LDS carve-up
+  SepXmRow* rows = reinterpret_cast<SepXmRow*>(smem);                          // m0
+  int* wcnt = reinterpret_cast<int*>(rows + p.m0);                             // 8
+  T2v* Wk = reinterpret_cast<T2v*>(smem + (size_t(p.m0) * sizeof(SepXmRow) + 8 * sizeof(int) + 15) / 16 * 16);
+  unsigned char* dyn = reinterpret_cast<unsigned char*>(Wk + size_t(p.m0) * nG0p);
+  dyn = smem + (size_t(dyn - smem) + 15) / 16 * 16;
+  T* g3 = reinterpret_cast<T*>(dyn);                                           // 16 x LDG
+  double* gs = reinterpret_cast<double*>(dyn);                                 // aliases g3 (RPP x nm x nb2max)
+  double2* sU2 = reinterpret_cast<double2*>(dyn + (size_t(16) * LDG * sizeof(T) + 15) / 16 * 16);  // 16 x nw x nb2max
+  double2* gp = sU2 + size_t(16) * nw * p.nb2max;                              // RPP x nm x nb2max
+  double* se2 = reinterpret_cast<double*>(gp + size_t(RPP) * nm * p.nb2max);   // 16 x nb2max
+  double* sw3 = se2 + size_t(16) * p.nb2max;                                   // 16 x nm
+
+  // ---- rows of the line, energy screen, compaction (ascending jx)
+  int nkeep = 0;
+  for (int j0 = 0; j0 < p.m0; j0 += 256) {
+    const int jx = j0 + t;
+    SepXmRow ri{jx, -1, 0, 0, 0, 0};
+    if (jx < p.m0) {
+      const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
+      int row = p.meshToRow[mesh];
+      if (row < p.rowLo || row >= p.rowHi) row = -1;
+      if (row >= 0) {
+        ri.iq = mesh_diff_index(mesh, p.kkMesh[kk], p.m0, p.m1, p.m2);
+        ri.nb2 = p.rowNb2[row];
+        ri.u2off = p.rowU2off[row];
+        ri.soff = p.rowStateOff[row];
+        bool keep = true;
+        if (p.screen > 0.) {
+          const double wlo = p.qWlo[ri.iq], whi = p.qWhi[ri.iq];
+          const double e2lo = p.e2pool[ri.soff], e2hi = p.e2pool[ri.soff + ri.nb2 - 1];
+          const double lim = p.screen / p.invSigma;
+          const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
+          keep = false;
+          if (whi >= wlo) {  // at least one mode above the cutoff
+            for (int gi = 0; gi < p.panelGSize[panel]; ++gi) {
+              const double a_lo = e1[gi] - e2hi + wlo, a_hi = e1[gi] - e2lo + whi;
+              const double b_lo = e1[gi] - e2hi - whi, b_hi = e1[gi] - e2lo - wlo;
+              keep |= (a_hi > -lim && a_lo < lim) || (b_hi > -lim && b_lo < lim);
+            }
+          }
+        }
+        if (keep) ri.row = row;
+      }
+    }
+    const bool keep = ri.row >= 0;
+    const unsigned long long mask = __ballot(keep);
+    if (lane == 0) wcnt[wave] = __popcll(mask);
+    __syncthreads();
+    int off = nkeep, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      if (w < wave) off += wcnt[w];
+      tot += wcnt[w];
+    }
+    if (keep) rows[off + __popcll(mask & ((1ull << lane) - 1ull))] = ri;
+    nkeep += tot;
+    __syncthreads();
+  }
+  if (nkeep == 0) {
+    if (REDUCE) {
+      double part[kGMax][CM];
+      for (int gi = 0; gi < kGMax; ++gi)
+        for (int c = 0; c < CM; ++c) part[gi][c] = 0.;
+      reduce_partials<CM>(p, slot, smem, part);
+    }
+    return;
+  }
+  // twiddles of the kept rows: Wk(k, g) = w_x^{jx_k n0(g)}, zero column g = nG0 pads the K loop
+  for (int i = t; i < nkeep * nG0p; i += 256) {
+    const int k = i / nG0p, g = i % nG0p;
+    T2v w{T(0), T(0)};
+    if (g < nG0) {
+      const double2 d = p.tw0[(rows[k].jx * p.g0val[g]) % p.m0];
+      w = T2v{T(d.x), T(d.y)};
+    }
+    Wk[i] = w;
+  }
+
+  double part[kGMax][CM];
+#pragma unroll
+  for (int gi = 0; gi < kGMax; ++gi)
+#pragma unroll
+    for (int c = 0; c < CM; ++c) part[gi][c] = 0.;
+  const int nCalc = p.nCalc;
+  const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
+  const double* e1 = p.e1pool + p.kkStateOff[kk] + gstart;
+  const double inv_g = 1. / double(float(gsize));
+  const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
+                 (size_t(jy) * p.m2 + jz) * ncol;
+  const size_t gStride = size_t(p.m1) * p.m2 * ncol;  // T2v elements between groups
+  const int KS = (2 * nG0 + 3) / 4;                    // k-steps of 4 real K
+  const int nNt = (2 * ncol + 15) / 16;                // n-tiles in use
+
+  for (int m0r = 0; m0r < nkeep; m0r += 16) {
+    const int nr = min(16, nkeep - m0r);
+    // ---- stage the tile's row data (U2, e2, omega) in LDS
+    for (int i = t; i < 16 * nw * p.nb2max; i += 256) {
+      const int r = i / (nw * p.nb2max), o = i % (nw * p.nb2max);
+      double2 v = make_double2(0., 0.);
+      if (r < nr) {
+        const SepXmRow& ri = rows[m0r + r];
+        if (o < nw * ri.nb2) v = p.U2pool[ri.u2off + o];
+      }
+      sU2[i] = v;
+    }
+    for (int i = t; i < 16 * p.nb2max; i += 256) {
+      const int r = i / p.nb2max, b = i % p.nb2max;
+      double v = 0.;
+      if (r < nr && b < rows[m0r + r].nb2) v = p.e2pool[rows[m0r + r].soff + b];
+      se2[i] = v;
+    }
+    for (int i = t; i < 16 * nm; i += 256) {
+      const int r = i / nm, nu = i % nm;
+      sw3[i] = r < nr ? p.Wpool[size_t(rows[m0r + r].iq) * nm + nu] : 0.;
+    }
+    __syncthreads();  // Wk (first tile) and the previous tile's epilogue are done
+
+    // ---- g3 on MFMA: wave owns n-tiles wave*kXmNtw .. +kXmNtw
+    acc_t acc[kXmNtw];
+#pragma unroll
+    for (int j = 0; j < kXmNtw; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[j][i] = T(0);
+    const bool rowOk = lr < nr;
+    const T2v* wrow = Wk + size_t(m0r + (rowOk ? lr : 0)) * nG0p;
+    for (int ks0 = 0; ks0 < KS; ks0 += kXmKsc) {
+      T bv[kXmKsc][kXmNtw];
+      T av[kXmKsc];
+#pragma unroll
+      for (int q = 0; q < kXmKsc; ++q) {
+        const int k = 4 * (ks0 + q) + lg, g = k >> 1, kc = k & 1;
+        const bool gOk = g < nG0 && ks0 + q < KS;
+        const T2v w = wrow[gOk ? g : nG0];
+        av[q] = (rowOk && gOk) ? (kc ? w.y : w.x) : T(0);
+#pragma unroll
+        for (int j = 0; j < kXmNtw; ++j) {
+          const int n = (wave * kXmNtw + j) * 16 + lr, c = n >> 1, ri = n & 1;
+          T v = T(0);
+          if (gOk && c < ncol) {
+            const T2v x = X[size_t(g) * gStride + c];
+            v = ri ? (kc ? x.x : x.y) : (kc ? -x.y : x.x);
+          }
+          bv[q][j] = v;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kXmKsc; ++q)
+#pragma unroll
+        for (int j = 0; j < kXmNtw; ++j) acc[j] = Mfma4<T>::mac(av[q], bv[q][j], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kXmNtw; ++j) {
+      const int n = (wave * kXmNtw + j) * 16 + lr, c = n >> 1, ri = n & 1;
+      if (c < ncol && (wave * kXmNtw + j) < nNt) {
+        const int nu = c / nw, w2 = c - nu * nw;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) g3[Mfma4<T>::row(lg, i) * LDG + nu * NWP + 2 * w2 + ri] = acc[j][i];
+      }
+    }
+    __syncthreads();
+
+    // ---- epilogue, RPP rows per pass
+    for (int pr = 0; pr < 16; pr += RPP) {
+      const int r = pr + t / TPR, sub = t % TPR, rl = t / TPR;
+      const bool rowOn = r < nr;
+      const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
+      const int nb2 = rowOn ? ri.nb2 : 0;
+      const bool modeOn = rowOn && sub < nm;
+      double2 e3c[NMX];
+      double w3 = 0., n3[CM];
+#pragma unroll
+      for (int c = 0; c < CM; ++c) n3[c] = 0.;
+      if (modeOn) {
+        const double2* e3 = p.E3pool + size_t(ri.iq) * nm * nm + size_t(nm) * sub;
+#pragma unroll
+        for (int nu = 0; nu < NMX; ++nu)
+          if (nu < nm) e3c[nu] = e3[nu];
+        w3 = sw3[r * nm + sub];
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+          if (c < nCalc) n3[c] = p.bose[(size_t(ri.iq) * nm + sub) * nCalc + c];
+      }
+      // g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2), thread nu = sub
+      if (modeOn) {
+        const T* gr = g3 + r * LDG + sub * NWP;
+        for (int b2 = 0; b2 < nb2; ++b2) {
+          const double2* u = sU2 + (size_t(r) * p.nb2max + b2) * nw;
+          double sx = 0., sy = 0.;
+          for (int w2 = 0; w2 < nw; ++w2) {
+            const double xr = double(gr[2 * w2]), xi = double(gr[2 * w2 + 1]);
+            sx += u[w2].x * xr + u[w2].y * xi;
+            sy += u[w2].x * xi - u[w2].y * xr;
+          }
+          gp[(rl * nm + sub) * p.nb2max + b2] = make_double2(sx, sy);
+        }
+      }
+      __syncthreads();
+      // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub; |g|^2 in registers
+      double v[kSepNb2];
+#pragma unroll
+      for (int b = 0; b < kSepNb2; ++b) v[b] = 0.;
+      if (modeOn) {
+#pragma unroll
+        for (int b2 = 0; b2 < kSepNb2; ++b2) {
+          if (b2 < nb2) {
+            double sx = 0., sy = 0.;
+#pragma unroll
+            for (int nu = 0; nu < NMX; ++nu)
+              if (nu < nm) {
+                const double2 x = gp[(rl * nm + nu) * p.nb2max + b2];
+                sx += x.x * e3c[nu].x - x.y * e3c[nu].y;
+                sy += x.x * e3c[nu].y + x.y * e3c[nu].x;
+              }
+            v[b2] = sx * sx + sy * sy;
+          }
+        }
+        if (!REDUCE) {
+          const int nb1 = p.kkNb1[kk];
+          for (int b2 = 0; b2 < nb2; ++b2)
+            p.out[p.rowOutOff[ri.row] + b1p + size_t(nb1) * (b2 + size_t(nb2) * sub)] = v[b2];
+        }
+      }
+      if (REDUCE) {
+        // symmetrizeCoupling along b2 (scattering.cpp:1250-1284), in registers
+        if (modeOn) {
+          const double* e2r = se2 + r * p.nb2max;
+          for (int b = 0; b < nb2;) {
+            int deg = 0;
+            for (int i = b; i < nb2; ++i) {
+              if (fabs(e2r[b] - e2r[i]) < 1.0e-6) deg++;
+              else break;
+            }
+            if (deg > 1) {
+              double s2 = 0.;
+#pragma unroll
+              for (int i = 0; i < kSepNb2; ++i)
+                if (i >= b && i < b + deg) s2 += v[i] / double(float(deg));
+#pragma unroll
+              for (int i = 0; i < kSepNb2; ++i)
+                if (i >= b && i < b + deg) v[i] = s2;
+            }
+            b += deg;
+          }
+        }
+        // ... and along nu (scattering.cpp:1286-1316), through LDS when modes are degenerate
+        int mydeg = 1, mystart = sub;
+        if (modeOn) {
+          const double* wr = sw3 + r * nm;
+          for (int s0 = 0; s0 < nm;) {
+            int deg = 0;
+            for (int i = s0; i < nm; ++i) {
+              if (fabs(wr[s0] - wr[i]) < 1.0e-6) deg++;
+              else break;
+            }
+            if (sub >= s0 && sub < s0 + deg) {
+              mystart = s0;
+              mydeg = deg;
+            }
+            s0 += deg;
+          }
+        }
+        if (__syncthreads_or(mydeg > 1)) {  // g3 is dead here: gs aliases it
+          if (modeOn)
+            for (int b2 = 0; b2 < nb2; ++b2) gs[(rl * nm + sub) * p.nb2max + b2] = v[b2];
+          __syncthreads();
+          if (modeOn && mydeg > 1)
+            for (int b2 = 0; b2 < nb2; ++b2) {
+              double s2 = 0.;
+              for (int i = 0; i < mydeg; ++i)
+                s2 += gs[(rl * nm + mystart + i) * p.nb2max + b2] / double(float(mydeg));
+#pragma unroll
+              for (int b = 0; b < kSepNb2; ++b)
+                if (b == b2) v[b] = s2;
+            }
+        }
+        // Gaussian-delta weights (el_scattering.cpp:277-380)
+        if (modeOn && w3 >= p.cutoff) {
+#pragma unroll
+          for (int b2 = 0; b2 < kSepNb2; ++b2) {
+            if (b2 >= nb2) break;
+            const double en2 = se2[r * p.nb2max + b2];
+            const double cpl = v[b2] * inv_g;
+            const int is2 = ri.soff + b2;
+#pragma unroll
+            for (int gi = 0; gi < kGMax; ++gi) {
+              if (gi >= gsize) break;
+              const double d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
+              const double d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+              if (d1 <= 0. && d2 <= 0.) continue;
+#pragma unroll
+              for (int c = 0; c < CM; ++c) {
+                if (c >= nCalc) break;
+                const double f2 = p.fermi[size_t(c) * p.numStates + is2];
+                part[gi][c] += cpl * ((f2 + n3[c]) * d1 + (1. - f2 + n3[c]) * d2) * p.norm / w3 *
+                               3.14159265358979323846;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (REDUCE) reduce_partials<CM>(p, slot, smem, part);
+}
+
+inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb2max, int rpp) {
+  auto up = [](size_t b) { return (b + 15) / 16 * 16; };
+  size_t b = up(size_t(m0) * sizeof(SepXmRow) + 8 * sizeof(int));
+  b = up(b + size_t(m0) * (nG0 + 1) * 2 * elem);
+  const size_t g3 = std::max(size_t(16) * nm * (2 * nw + 1) * elem, size_t(rpp) * nm * nb2max * sizeof(double));
+  b += up(g3);
+  b += size_t(16) * nw * nb2max * sizeof(double2) + size_t(rpp) * nm * nb2max * sizeof(double2) +
+       size_t(16) * nb2max * sizeof(double) + size_t(16) * nm * sizeof(double);
+  return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
+}
+
 // maxLineE / maxGrpE: largest number of entries of a line / group (LDS sizing).
 template <typename T>
 void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int maxLineE,
@@ -787,10 +1144,7 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
   if (sp.nPanels <= 0) return;
   const size_t ldsZ = size_t(maxLineE) * sp.ncol * sizeof(double2);
   const size_t ldsY = size_t(maxGrpE) * sp.ncol * sizeof(double2);
-  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
-                               size_t(kGMax) * kCMax * 4 * sizeof(double));
-  PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024 && ldsX <= 160 * 1024,
-              "separable transform: LDS budget exceeded");
+  PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024, "separable transform: LDS budget exceeded");
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_z<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(ldsZ, 16))));
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_y<T>),
@@ -801,20 +1155,43 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
   PHX_HIP(hipGetLastError());
   const int tiles = sp.nPanels * sp.my * sp.mz;
   const int grid = ceil_div(tiles, 8) * 8;
+  const int nmb = p.nm <= 12 ? 12 : p.nm <= 16 ? 16 : p.nm <= 24 ? 24 : kSepNm;
+  const bool one = p.nCalc == 1;
+  if (2 * sp.ncol <= 4 * kXmNtw * 16) {
+    const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8);
+    PHX_REQUIRE(ldsXm <= 160 * 1024, "separable transform: LDS budget exceeded");
+    auto go = [&](auto kern) {
+      PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsXm)));
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsXm, s, p);
+    };
+#define PHX_XM(NMX)                                                                 \
+    if (nmb == NMX) {                                                               \
+      if (!reduce) go(k_sep_xm<T, false, NMX, 1>);                                  \
+      else if (one) go(k_sep_xm<T, true, NMX, 1>);                                  \
+      else go(k_sep_xm<T, true, NMX, kCMax>);                                       \
+    }
+    PHX_XM(12) PHX_XM(16) PHX_XM(24) PHX_XM(32)
+#undef PHX_XM
+    PHX_HIP(hipGetLastError());
+    return;
+  }
+  const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
+                               size_t(kGMax) * kCMax * 4 * sizeof(double));
+  PHX_REQUIRE(ldsX <= 160 * 1024, "separable transform: LDS budget exceeded");
   auto go = [&](auto kern) {
     PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsX)));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsX, s, p);
   };
-  const int nmb = p.nm <= 12 ? 12 : p.nm <= 18 ? 18 : kSepNm;
-  const bool one = p.nCalc == 1;
+  const int nmo = p.nm <= 12 ? 12 : p.nm <= 18 ? 18 : kSepNm;
   if (reduce) {
-    if (nmb == 12) one ? go(k_sep_x<T, true, 12, 1>) : go(k_sep_x<T, true, 12, kCMax>);
-    else if (nmb == 18) one ? go(k_sep_x<T, true, 18, 1>) : go(k_sep_x<T, true, 18, kCMax>);
+    if (nmo == 12) one ? go(k_sep_x<T, true, 12, 1>) : go(k_sep_x<T, true, 12, kCMax>);
+    else if (nmo == 18) one ? go(k_sep_x<T, true, 18, 1>) : go(k_sep_x<T, true, 18, kCMax>);
     else one ? go(k_sep_x<T, true, kSepNm, 1>) : go(k_sep_x<T, true, kSepNm, kCMax>);
   } else {
-    if (nmb == 12) go(k_sep_x<T, false, 12, 1>);
-    else if (nmb == 18) go(k_sep_x<T, false, 18, 1>);
+    if (nmo == 12) go(k_sep_x<T, false, 12, 1>);
+    else if (nmo == 18) go(k_sep_x<T, false, 18, 1>);
     else go(k_sep_x<T, false, kSepNm, 1>);
   }
   PHX_HIP(hipGetLastError());

@@ -232,20 +232,20 @@ template <> struct MfmaC<float> {
   __device__ static int row(int lk, int i) { return 4 * lk + i; }
 };
 
-// Panel epilogue shared by the cacheElPh kernels.  Cb holds g1 of 16 outer
-// points (kbase .. kbase+15) for the 256 columns c0 .. c0+255 of block (rp, nu):
+// Panel epilogue shared by the cacheElPh kernels.  Cb holds g1 of up to 16
+// outer points (kbase .. min(kbase+16, kend)-1) for the 256 columns c0 .. c0+255 of block (rp, nu):
 // real parts in rows 0..15, imaginary parts in rows 16..31.  Applies the U1
 // rotation over the w1 range of those columns and the factor s(k1, Rp), and
 // writes (c0 == 0) or adds (later column passes) the panel entries.  Each
 // output (b1, w2) is owned by the same thread in every pass.
 template <typename T, typename OutT>
 __device__ __forceinline__ void cache_panel_epilogue(const CacheParams& p, const T* Cb, int c0,
-                                                     int kbase, int rp, int nu) {
+                                                     int kbase, int rp, int nu, int kend) {
   const int t = threadIdx.x;
   OutT* Bout = static_cast<OutT*>(p.B);
   for (int kl = 0; kl < 16; ++kl) {
     const int kk = kbase + kl;
-    if (kk >= p.nkk) break;
+    if (kk >= kend) break;
     const int nb1 = p.nb1[kk];
     const double2* U1 = p.U1pool + p.U1off[kk];
     const double2 s = p.phP[size_t(kk) * p.nRp + rp];
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(256, 2) k_cache(CacheParams p) {
         }
       }
       __syncthreads();
-      cache_panel_epilogue<T, OutT>(p, Cb, c0, 16 * q, rp, nu);
+      cache_panel_epilogue<T, OutT>(p, Cb, c0, 16 * q, rp, nu, p.nkk);
     }
   }
 }
@@ -383,50 +383,75 @@ void launch_cache(const CacheParams& p, hipStream_t s) {
 template void launch_cache<double, double>(const CacheParams&, hipStream_t);
 template void launch_cache<float, float>(const CacheParams&, hipStream_t);
 
-// ------------------------------------------------------------------ cacheElPh on a mesh line
+// ------------------------------------------------------------------ cacheElPh on mesh lines
 // Outer points that share the mesh line (j1, j2) and differ in j0 factor the
 // phase: with n_d = b_d.R_e / 2pi,
 //   e^{ik1.R_e} = e^{2 pi i j0 n0/m0} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}.
-// Stage a (HBM stream over gW, once per line):
-//   Tg(g, c) = sum_{R_e in group g} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}/de gW(R_e, c)
+// Stage a (one HBM stream over gW for up to kLinesPerBatch lines):
+//   Tg(l, g, c) = sum_{R_e in group g} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}/de gW(R_e, c)
 // with groups of equal n0 mod m0 (about 2 * 10 + 1 of them for a 10^3 R_e grid).
-// Stage b (per outer point): g1(k1, c) = sum_g e^{2 pi i j0 n0(g)/m0} Tg(g, c),
+// Stage b (per outer point): g1(k1, c) = sum_g e^{2 pi i j0 n0(g)/m0} Tg(l(k1), g, c),
 // then the same U1 / s(k1, Rp) panel epilogue as k_cache.
-constexpr int kLineDepth = 8;  // gW rows in flight per thread
+constexpr int kLineDepth = 4;  // gW rows in flight per thread (16 bytes each)
 
-template <typename T>
+// NL lines, CPT adjacent columns per thread (one 16-byte load per R_e for
+// CPT * sizeof(T2) == 16).
+template <typename T, int NL, int CPT>
 __global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
   typedef typename Vec2<T>::type T2;
+  typedef T vec __attribute__((ext_vector_type(2 * CPT)));
   constexpr int D = kLineDepth;
-  const size_t col = size_t(blockIdx.x) * 256 + threadIdx.x;
-  const bool ok = col < p.N;
+  const size_t col0 = (size_t(blockIdx.x) * 256 + threadIdx.x) * CPT;
+  const bool ok = col0 < p.N;  // N is a multiple of CPT
   const T2* G = static_cast<const T2*>(p.gW);
   const T2* ph = static_cast<const T2*>(p.phLine);
   T2* out = static_cast<T2*>(p.Tg);
-  T2 cur[D], nxt[D];
-  auto load = [&](T2(&b)[D], int i0) {
+  vec cur[D], nxt[D];
+  auto load = [&](vec(&b)[D], int i0) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int i = i0 + u;
-      b[u] = (ok && i < p.nRe) ? G[size_t(p.reOrder[i]) * p.N + col] : T2{T(0), T(0)};
+      if (ok && i < p.nRe) b[u] = *reinterpret_cast<const vec*>(G + size_t(p.reOrder[i]) * p.N + col0);
+      else b[u] = vec(T(0));
     }
   };
   load(cur, 0);
-  T ar = T(0), ai = T(0);
+  T ar[NL][CPT], ai[NL][CPT];
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) ar[l][c] = ai[l][c] = T(0);
   for (int i0 = 0; i0 < p.nRe; i0 += D) {
     load(nxt, i0 + D);
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int i = i0 + u;
       if (i < p.nRe) {
-        const T2 f = ph[i];
-        ar += f.x * cur[u].x - f.y * cur[u].y;
-        ai += f.x * cur[u].y + f.y * cur[u].x;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+          const T2 f = ph[size_t(l) * p.nRe + i];
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            ar[l][c] += f.x * cur[u][2 * c] - f.y * cur[u][2 * c + 1];
+            ai[l][c] += f.x * cur[u][2 * c + 1] + f.y * cur[u][2 * c];
+          }
+        }
         const int g = p.reGroupEnd[i];
         if (g >= 0) {
-          if (ok) out[size_t(g) * p.N + col] = T2{ar, ai};
-          ar = T(0);
-          ai = T(0);
+#pragma unroll
+          for (int l = 0; l < NL; ++l) {
+            if (ok) {
+              vec o;
+#pragma unroll
+              for (int c = 0; c < CPT; ++c) {
+                o[2 * c] = ar[l][c];
+                o[2 * c + 1] = ai[l][c];
+              }
+              *reinterpret_cast<vec*>(out + (size_t(l) * p.nGe + g) * p.N + col0) = o;
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) ar[l][c] = ai[l][c] = T(0);
+          }
         }
       }
     }
@@ -436,7 +461,7 @@ __global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
 }
 
 // One workgroup per (Rp, nu) block of nw^2 columns, in passes of 256 columns
-// and 16 outer points.
+// and, per line, of 16 outer points.
 template <typename T, typename OutT>
 __global__ void __launch_bounds__(256) k_cache_line_b(CacheParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -449,40 +474,64 @@ __global__ void __launch_bounds__(256) k_cache_line_b(CacheParams p) {
     const int cl = c0 + t;
     const bool ok = cl < E;
     const size_t colg = (size_t(rp) * p.nm + nu) * E + cl;
-    for (int kbase = 0; kbase < p.nkk; kbase += 16) {
-      T accr[16], acci[16];
+    for (int l = 0; l < p.nLinesB; ++l) {
+      const int kend = p.lineKK[l + 1];
+      for (int kbase = p.lineKK[l]; kbase < kend; kbase += 16) {
+        T accr[16], acci[16];
 #pragma unroll
-      for (int kl = 0; kl < 16; ++kl) accr[kl] = acci[kl] = T(0);
-      for (int g = 0; g < p.nGe; ++g) {
-        const T2 v = ok ? Tg[size_t(g) * p.N + colg] : T2{T(0), T(0)};
+        for (int kl = 0; kl < 16; ++kl) accr[kl] = acci[kl] = T(0);
+        for (int g = 0; g < p.nGe; ++g) {
+          const T2 v = ok ? Tg[(size_t(l) * p.nGe + g) * p.N + colg] : T2{T(0), T(0)};
+#pragma unroll
+          for (int kl = 0; kl < 16; ++kl) {
+            const int kk = min(kbase + kl, kend - 1);
+            const double2 w = p.wLine[size_t(kk) * p.nGe + g];
+            const T wr = T(w.x), wi = T(w.y);
+            accr[kl] += wr * v.x - wi * v.y;
+            acci[kl] += wr * v.y + wi * v.x;
+          }
+        }
+        __syncthreads();
 #pragma unroll
         for (int kl = 0; kl < 16; ++kl) {
-          const int kk = min(kbase + kl, p.nkk - 1);
-          const double2 w = p.wLine[size_t(kk) * p.nGe + g];
-          const T wr = T(w.x), wi = T(w.y);
-          accr[kl] += wr * v.x - wi * v.y;
-          acci[kl] += wr * v.y + wi * v.x;
+          Cb[kl * 256 + t] = accr[kl];
+          Cb[(16 + kl) * 256 + t] = acci[kl];
         }
+        __syncthreads();
+        cache_panel_epilogue<T, OutT>(p, Cb, c0, kbase, rp, nu, kend);
       }
-      __syncthreads();
-#pragma unroll
-      for (int kl = 0; kl < 16; ++kl) {
-        Cb[kl * 256 + t] = accr[kl];
-        Cb[(16 + kl) * 256 + t] = acci[kl];
-      }
-      __syncthreads();
-      cache_panel_epilogue<T, OutT>(p, Cb, c0, kbase, rp, nu);
     }
   }
+}
+
+template <typename T, int NL>
+static void launch_line_a(const CacheParams& p, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (p.N % 2 == 0) {
+      hipLaunchKernelGGL((k_cache_line_a<T, NL, 2>), dim3(ceil_div((long long)p.N / 2, 256)), dim3(256),
+                         0, s, p);
+      PHX_HIP(hipGetLastError());
+      return;
+    }
+  }
+  {
+    hipLaunchKernelGGL((k_cache_line_a<T, NL, 1>), dim3(ceil_div((long long)p.N, 256)), dim3(256), 0, s, p);
+  }
+  PHX_HIP(hipGetLastError());
 }
 
 template <typename T, typename OutT>
 void launch_cache_line(const CacheParams& p, hipStream_t s) {
   PHX_REQUIRE(p.nw <= 32, "cacheElPh kernel supports nw <= 32");
   PHX_REQUIRE(p.nGe >= 1 && p.N == size_t(p.nRp) * p.nm * p.nw * p.nw, "cacheElPh line: bad sizes");
+  PHX_REQUIRE(p.nLinesB >= 1 && p.nLinesB <= kLinesPerBatch, "cacheElPh line: 1 to 4 lines per batch");
   if (p.nkk <= 0) return;
-  hipLaunchKernelGGL(k_cache_line_a<T>, dim3(ceil_div((long long)p.N, 256)), dim3(256), 0, s, p);
-  PHX_HIP(hipGetLastError());
+  switch (p.nLinesB) {
+    case 1: launch_line_a<T, 1>(p, s); break;
+    case 2: launch_line_a<T, 2>(p, s); break;
+    case 3: launch_line_a<T, 3>(p, s); break;
+    default: launch_line_a<T, 4>(p, s); break;
+  }
   const size_t lds = size_t(2) * 16 * 256 * sizeof(T);
   auto kb = k_cache_line_b<T, OutT>;
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kb),

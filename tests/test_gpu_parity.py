@@ -211,6 +211,7 @@ def test_line_batching_vs_oracle(precision, tol, transform):
     ref = orc.linewidths()
     lw = P.ElectronLinewidths(p, gW=gW, precision=precision, k1_batch=7, transform=transform)
     lw.set_line_batching(4)
+    lw.set_panel_chunk(5)  # several separable launches per batch
     lines = lw.active // p.mesh[0]
     counts = np.bincount(lines)
     say("  lines with >= 4 points:", int((counts >= 4).sum()), "of", int((counts > 0).sum()))
@@ -246,4 +247,6 @@ def test_line_batching_argument_validation():
     s = P.ElectronLinewidths(p, gW=p.gw_host(), precision="fp64")
     with pytest.raises(P.PhxError):
         s.set_line_batching(-1)
+    with pytest.raises(P.PhxError):
+        s.set_panel_chunk(0)
     s.set_line_batching(0)
