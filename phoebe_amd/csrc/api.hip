@@ -504,6 +504,7 @@ struct LwHandle {
   // energy-conservation screen of the mesh transform (units of sigma, 0 = off)
   double screen = 0.;
   DevBuf<double> qWlo, qWhi;
+  DevBuf<double2> rowE2range;  // per active point: lowest and highest windowed band energy
   // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -628,6 +629,13 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   h2d(L->rowU2off, L->u2off, s);
   h2d(L->rowStateOff, L->stateOff, s);
   h2d(L->e2pool, L->stateE, s);
+  {
+    std::vector<double2> er(na);
+    for (int a = 0; a < na; ++a)
+      er[a] = make_double2(L->stateE[L->stateOff[a]], L->stateE[L->stateOff[a] + L->nb[a] - 1]);
+    h2d(L->rowE2range, er, s);
+    PHX_HIP(hipStreamSynchronize(s));
+  }
   h2d(L->gStartD, L->gStart, s);
   h2d(L->gSizeD, L->gSize, s);
   {
@@ -1145,6 +1153,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.screen = L->screen;
     cq.qWlo = L->qWlo.p;
     cq.qWhi = L->qWhi.p;
+    cq.rowE2range = L->rowE2range.p;
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
     if (L->precision == PHX_FP64) {
@@ -1204,6 +1213,8 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cl.nMtiles = nMtiles;
   cl.nCalc = L->nCalc;
   cl.numStates = L->numStates;
+  cl.nb1Max = 0;
+  for (int i = 0; i < nkk; ++i) cl.nb1Max = std::max(cl.nb1Max, L->nb[k1s[i]]);
   cl.kkNb1 = dNb1;
   cl.kkPanelBase = dPbase;
   cl.kkStateOff = dKKState;

@@ -64,6 +64,7 @@ struct CoupParams {
   int rowLo, rowHi;                       // rows (active indices) taken by the mesh transform
   double screen;                          // energy screen in units of sigma (0: off)
   const double* qWlo; const double* qWhi; // per q: lowest / highest omega >= cutoff
+  const double2* rowE2range;              // per row: (lowest, highest) band energy
 };
 
 template <typename T> struct Vec2;
@@ -802,6 +803,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   typedef typename Mfma4<T>::acc_t acc_t;
   constexpr int TPR = NMX <= 16 ? 16 : 32;  // epilogue threads per row
   constexpr int RPP = 256 / TPR;            // rows per epilogue pass
+  constexpr int KSB = sizeof(T) == 4 ? 12 : 6;  // k-steps of B held in registers
   const int nLinesYZ = p.m1 * p.m2;
   const int nTilesTot = p.nPanels * nLinesYZ;
   const int bid = blockIdx.x;
@@ -817,11 +819,14 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   const int kk = p.panelKK[panel];
   const int b1p = p.panelB1[panel];
   const int nG0 = p.nG0, nG0p = nG0 + 1;
+  const int kkMesh = p.kkMesh[kk];
 
   // LDS carve-up
   SepXmRow* rows = reinterpret_cast<SepXmRow*>(smem);                          // m0
   int* wcnt = reinterpret_cast<int*>(rows + p.m0);                             // 8
-  T2v* Wk = reinterpret_cast<T2v*>(smem + (size_t(p.m0) * sizeof(SepXmRow) + 8 * sizeof(int) + 15) / 16 * 16);
+  int* sg0 = wcnt + 8;                                                         // nG0
+  T2v* stw = reinterpret_cast<T2v*>(smem + (size_t(p.m0) * sizeof(SepXmRow) + (8 + nG0) * sizeof(int) + 15) / 16 * 16);
+  T2v* Wk = stw + p.m0;                                                        // m0 x nG0p
   unsigned char* dyn = reinterpret_cast<unsigned char*>(Wk + size_t(p.m0) * nG0p);
   dyn = smem + (size_t(dyn - smem) + 15) / 16 * 16;
   T* g3 = reinterpret_cast<T*>(dyn);                                           // 16 x LDG
@@ -831,6 +836,13 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   double* se2 = reinterpret_cast<double*>(gp + size_t(RPP) * nm * p.nb2max);   // 16 x nb2max
   double* sw3 = se2 + size_t(16) * p.nb2max;                                   // 16 x nm
 
+  // twiddles and group coordinates
+  for (int i = t; i < p.m0; i += 256) {
+    const double2 d = p.tw0[i];
+    stw[i] = T2v{T(d.x), T(d.y)};
+  }
+  for (int i = t; i < nG0; i += 256) sg0[i] = p.g0val[i];
+
   // ---- rows of the line, energy screen, compaction (ascending jx)
   int nkeep = 0;
   for (int j0 = 0; j0 < p.m0; j0 += 256) {
@@ -838,17 +850,22 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
     SepXmRow ri{jx, -1, 0, 0, 0, 0};
     if (jx < p.m0) {
       const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
+      ri.iq = mesh_diff_index(mesh, kkMesh, p.m0, p.m1, p.m2);
+      double wlo = 0., whi = 0.;
+      if (p.screen > 0.) {
+        wlo = p.qWlo[ri.iq];
+        whi = p.qWhi[ri.iq];
+      }
       int row = p.meshToRow[mesh];
       if (row < p.rowLo || row >= p.rowHi) row = -1;
       if (row >= 0) {
-        ri.iq = mesh_diff_index(mesh, p.kkMesh[kk], p.m0, p.m1, p.m2);
         ri.nb2 = p.rowNb2[row];
         ri.u2off = p.rowU2off[row];
         ri.soff = p.rowStateOff[row];
         bool keep = true;
         if (p.screen > 0.) {
-          const double wlo = p.qWlo[ri.iq], whi = p.qWhi[ri.iq];
-          const double e2lo = p.e2pool[ri.soff], e2hi = p.e2pool[ri.soff + ri.nb2 - 1];
+          const double2 er = p.rowE2range[row];
+          const double e2lo = er.x, e2hi = er.y;
           const double lim = p.screen / p.invSigma;
           const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
           keep = false;
@@ -888,13 +905,9 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   // twiddles of the kept rows: Wk(k, g) = w_x^{jx_k n0(g)}, zero column g = nG0 pads the K loop
   for (int i = t; i < nkeep * nG0p; i += 256) {
     const int k = i / nG0p, g = i % nG0p;
-    T2v w{T(0), T(0)};
-    if (g < nG0) {
-      const double2 d = p.tw0[(rows[k].jx * p.g0val[g]) % p.m0];
-      w = T2v{T(d.x), T(d.y)};
-    }
-    Wk[i] = w;
+    Wk[i] = g < nG0 ? stw[(rows[k].jx * sg0[g]) % p.m0] : T2v{T(0), T(0)};
   }
+  __syncthreads();
 
   double part[kGMax][CM];
 #pragma unroll
@@ -913,45 +926,20 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 
   for (int m0r = 0; m0r < nkeep; m0r += 16) {
     const int nr = min(16, nkeep - m0r);
-    // ---- stage the tile's row data (U2, e2, omega) in LDS
-    for (int i = t; i < 16 * nw * p.nb2max; i += 256) {
-      const int r = i / (nw * p.nb2max), o = i % (nw * p.nb2max);
-      double2 v = make_double2(0., 0.);
-      if (r < nr) {
-        const SepXmRow& ri = rows[m0r + r];
-        if (o < nw * ri.nb2) v = p.U2pool[ri.u2off + o];
-      }
-      sU2[i] = v;
-    }
-    for (int i = t; i < 16 * p.nb2max; i += 256) {
-      const int r = i / p.nb2max, b = i % p.nb2max;
-      double v = 0.;
-      if (r < nr && b < rows[m0r + r].nb2) v = p.e2pool[rows[m0r + r].soff + b];
-      se2[i] = v;
-    }
-    for (int i = t; i < 16 * nm; i += 256) {
-      const int r = i / nm, nu = i % nm;
-      sw3[i] = r < nr ? p.Wpool[size_t(rows[m0r + r].iq) * nm + nu] : 0.;
-    }
-    __syncthreads();  // Wk (first tile) and the previous tile's epilogue are done
-
-    // ---- g3 on MFMA: wave owns n-tiles wave*kXmNtw .. +kXmNtw
+    const bool rowOk = lr < nr;
+    const T2v* wrow = Wk + size_t(m0r + (rowOk ? lr : 0)) * nG0p;
     acc_t acc[kXmNtw];
 #pragma unroll
     for (int j = 0; j < kXmNtw; ++j)
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[j][i] = T(0);
-    const bool rowOk = lr < nr;
-    const T2v* wrow = Wk + size_t(m0r + (rowOk ? lr : 0)) * nG0p;
-    for (int ks0 = 0; ks0 < KS; ks0 += kXmKsc) {
-      T bv[kXmKsc][kXmNtw];
-      T av[kXmKsc];
+    for (int kb = 0; kb < KS; kb += KSB) {
+      // B operands of KSB k-steps straight from T2 (one round trip), then the MFMAs
+      T bv[KSB][kXmNtw];
 #pragma unroll
-      for (int q = 0; q < kXmKsc; ++q) {
-        const int k = 4 * (ks0 + q) + lg, g = k >> 1, kc = k & 1;
-        const bool gOk = g < nG0 && ks0 + q < KS;
-        const T2v w = wrow[gOk ? g : nG0];
-        av[q] = (rowOk && gOk) ? (kc ? w.y : w.x) : T(0);
+      for (int q = 0; q < KSB; ++q) {
+        const int k = 4 * (kb + q) + lg, g = k >> 1, kc = k & 1;
+        const bool gOk = g < nG0 && kb + q < KS;
 #pragma unroll
         for (int j = 0; j < kXmNtw; ++j) {
           const int n = (wave * kXmNtw + j) * 16 + lr, c = n >> 1, ri = n & 1;
@@ -963,10 +951,38 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           bv[q][j] = v;
         }
       }
+      if (kb == 0) {
+        // the tile's row data (U2, e2, omega) into LDS while the B loads are in flight;
+        // the previous tile's epilogue ended with a barrier
+        for (int i = t; i < 16 * nw * p.nb2max; i += 256) {
+          const int r = i / (nw * p.nb2max), o = i % (nw * p.nb2max);
+          double2 v = make_double2(0., 0.);
+          if (r < nr) {
+            const SepXmRow& rr = rows[m0r + r];
+            if (o < nw * rr.nb2) v = p.U2pool[rr.u2off + o];
+          }
+          sU2[i] = v;
+        }
+        for (int i = t; i < 16 * p.nb2max; i += 256) {
+          const int r = i / p.nb2max, b = i % p.nb2max;
+          double v = 0.;
+          if (r < nr && b < rows[m0r + r].nb2) v = p.e2pool[rows[m0r + r].soff + b];
+          se2[i] = v;
+        }
+        for (int i = t; i < 16 * nm; i += 256) {
+          const int r = i / nm, nu = i % nm;
+          sw3[i] = r < nr ? p.Wpool[size_t(rows[m0r + r].iq) * nm + nu] : 0.;
+        }
+      }
 #pragma unroll
-      for (int q = 0; q < kXmKsc; ++q)
+      for (int q = 0; q < KSB; ++q) {
+        const int k = 4 * (kb + q) + lg, g = k >> 1, kc = k & 1;
+        const bool gOk = g < nG0 && kb + q < KS;
+        const T2v w = wrow[gOk ? g : nG0];
+        const T av = (rowOk && gOk) ? (kc ? w.y : w.x) : T(0);
 #pragma unroll
-        for (int j = 0; j < kXmNtw; ++j) acc[j] = Mfma4<T>::mac(av[q], bv[q][j], acc[j]);
+        for (int j = 0; j < kXmNtw; ++j) acc[j] = Mfma4<T>::mac(av, bv[q][j], acc[j]);
+      }
     }
 #pragma unroll
     for (int j = 0; j < kXmNtw; ++j) {
@@ -986,7 +1002,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
       const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
       const int nb2 = rowOn ? ri.nb2 : 0;
       const bool modeOn = rowOn && sub < nm;
-      double2 e3c[NMX];
+      T2v e3c[NMX];
       double w3 = 0., n3[CM];
 #pragma unroll
       for (int c = 0; c < CM; ++c) n3[c] = 0.;
@@ -994,7 +1010,10 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
         const double2* e3 = p.E3pool + size_t(ri.iq) * nm * nm + size_t(nm) * sub;
 #pragma unroll
         for (int nu = 0; nu < NMX; ++nu)
-          if (nu < nm) e3c[nu] = e3[nu];
+          if (nu < nm) {
+            const double2 d = e3[nu];
+            e3c[nu] = T2v{T(d.x), T(d.y)};
+          }
         w3 = sw3[r * nm + sub];
 #pragma unroll
         for (int c = 0; c < CM; ++c)
@@ -1028,8 +1047,9 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
             for (int nu = 0; nu < NMX; ++nu)
               if (nu < nm) {
                 const double2 x = gp[(rl * nm + nu) * p.nb2max + b2];
-                sx += x.x * e3c[nu].x - x.y * e3c[nu].y;
-                sy += x.x * e3c[nu].y + x.y * e3c[nu].x;
+                const double er = double(e3c[nu].x), ei = double(e3c[nu].y);
+                sx += x.x * er - x.y * ei;
+                sy += x.x * ei + x.y * er;
               }
             v[b2] = sx * sx + sy * sy;
           }
@@ -1093,13 +1113,16 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
                 if (b == b2) v[b] = s2;
             }
         }
-        // Gaussian-delta weights (el_scattering.cpp:277-380)
+        // Gaussian-delta weights (el_scattering.cpp:277-380); b2 loop kept rolled
         if (modeOn && w3 >= p.cutoff) {
+#pragma unroll 1
+          for (int b2 = 0; b2 < nb2; ++b2) {
+            double vb = 0.;
 #pragma unroll
-          for (int b2 = 0; b2 < kSepNb2; ++b2) {
-            if (b2 >= nb2) break;
+            for (int b = 0; b < kSepNb2; ++b)
+              if (b == b2) vb = v[b];
             const double en2 = se2[r * p.nb2max + b2];
-            const double cpl = v[b2] * inv_g;
+            const double cpl = vb * inv_g;
             const int is2 = ri.soff + b2;
 #pragma unroll
             for (int gi = 0; gi < kGMax; ++gi) {
@@ -1126,8 +1149,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 
 inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb2max, int rpp) {
   auto up = [](size_t b) { return (b + 15) / 16 * 16; };
-  size_t b = up(size_t(m0) * sizeof(SepXmRow) + 8 * sizeof(int));
-  b = up(b + size_t(m0) * (nG0 + 1) * 2 * elem);
+  size_t b = up(size_t(m0) * sizeof(SepXmRow) + (8 + nG0) * sizeof(int));
+  b = up(b + size_t(m0) * 2 * elem + size_t(m0) * (nG0 + 1) * 2 * elem);
   const size_t g3 = std::max(size_t(16) * nm * (2 * nw + 1) * elem, size_t(rpp) * nm * nb2max * sizeof(double));
   b += up(g3);
   b += size_t(16) * nw * nb2max * sizeof(double2) + size_t(rpp) * nm * nb2max * sizeof(double2) +

@@ -557,27 +557,43 @@ void launch_to_float(const double* a, float* b, size_t n, hipStream_t s) {
 
 // ------------------------------------------------------------------ collect
 
-__global__ void k_collect(CollectParams p) {
-  const int kk = blockIdx.x;
-  if (kk >= p.nkk) return;
-  const int nb1 = p.kkNb1[kk];
-  for (int i = threadIdx.x; i < nb1 * p.nCalc; i += blockDim.x) {
-    const int b1 = i / p.nCalc, c = i % p.nCalc;
-    const int pan = p.kkPanelBase[kk] + b1;
-    const int gs = p.panelGStart[pan], gz = p.panelGSize[pan];
-    double sum = 0.;
-    for (int j = 0; j < gz; ++j) {
-      const int src = p.kkPanelBase[kk] + gs + j;
-      for (int mt = 0; mt < p.nMtiles; ++mt)
-        sum += p.partial[(size_t(src) * p.nMtiles + mt) * kGMax * kCMax + (b1 - gs) * kCMax + c];
-    }
-    p.linewidth[size_t(c) * p.numStates + p.kkStateOff[kk] + b1] += sum;
+// One workgroup per (k1, b1): 256 threads stride over the partial slots of the
+// panels of b1's degenerate group, then a fixed-order tree sum (deterministic).
+__global__ void __launch_bounds__(256) k_collect(CollectParams p) {
+  __shared__ double red[4][kCMax];
+  const int kk = blockIdx.x, b1 = blockIdx.y;
+  if (kk >= p.nkk || b1 >= p.kkNb1[kk]) return;  // uniform per workgroup
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int pan = p.kkPanelBase[kk] + b1;
+  const int gs = p.panelGStart[pan], gz = p.panelGSize[pan];
+  double sum[kCMax] = {0., 0., 0., 0.};
+  const int n = gz * p.nMtiles;
+  for (int i = t; i < n; i += 256) {
+    const int j = i / p.nMtiles, mt = i - j * p.nMtiles;
+    const int src = p.kkPanelBase[kk] + gs + j;
+    const double2* q = reinterpret_cast<const double2*>(
+        p.partial + (size_t(src) * p.nMtiles + mt) * kGMax * kCMax + (b1 - gs) * kCMax);
+    const double2 v0 = q[0], v1 = q[1];
+    sum[0] += v0.x;
+    sum[1] += v0.y;
+    sum[2] += v1.x;
+    sum[3] += v1.y;
   }
+#pragma unroll
+  for (int c = 0; c < kCMax; ++c) {
+    double v = sum[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (t < p.nCalc)
+    p.linewidth[size_t(t) * p.numStates + p.kkStateOff[kk] + b1] +=
+        ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]));
 }
 
 void launch_collect(const CollectParams& p, hipStream_t s) {
   if (p.nkk <= 0) return;
-  hipLaunchKernelGGL(k_collect, dim3(p.nkk), dim3(64), 0, s, p);
+  hipLaunchKernelGGL(k_collect, dim3(p.nkk, p.nb1Max), dim3(256), 0, s, p);
   PHX_HIP(hipGetLastError());
 }
 

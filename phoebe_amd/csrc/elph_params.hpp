@@ -43,6 +43,7 @@ struct CacheParams {
 // Deterministic sum of the per-tile partial linewidths of one batch.
 struct CollectParams {
   int nkk, nMtiles, nCalc, numStates;
+  int nb1Max;  // largest nb1 of the batch
   const int* kkNb1; const int* kkPanelBase; const int* kkStateOff;
   const int* panelGStart; const int* panelGSize;
   const double* partial;

@@ -291,3 +291,25 @@ def config(name: str, **overrides) -> Problem:
     kw = dict(CONFIGS[name])
     kw.update(overrides)
     return make_problem(name=name, **kw)
+
+
+def eig_family(kind: str, n: int, seed: int = 1001, rgrid: int = 8):
+    """Inputs of the batched eigensolve microbenchmark: an fcc cell with R on
+    WS(rgrid^3), and either H(R) for n Wannier functions ("el", same
+    distribution as the el-ph configs) or force constants for n = 3 * natoms
+    modes ("ph", natoms <= 12).  Returns dict(recip, R, weight, coef, masses)."""
+    rng = np.random.default_rng(seed + 7919 * n + (0 if kind == "el" else 104729))
+    A = _fcc(10.26)
+    R, deg = ws_vectors((rgrid,) * 3, A)
+    ang = 1.0 / DISTANCE_BOHR_TO_ANG
+    if kind == "el":
+        return dict(recip=reciprocal_cell(A), R=R, weight=deg,
+                    coef=_hermitian_h0r(rng, R, deg, n, 2.0 * ang), masses=None)
+    natoms = n // 3
+    frac = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25], [0.5, 0.0, 0.0], [0.75, 0.25, 0.25],
+                     [0.0, 0.5, 0.0], [0.25, 0.75, 0.25], [0.0, 0.0, 0.5], [0.25, 0.25, 0.75],
+                     [0.5, 0.5, 0.0], [0.75, 0.75, 0.25], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    tau = (A @ frac[:natoms].T).T
+    mat = _force_constants(rng, R, natoms, tau, 1.5 * ang)
+    return dict(recip=reciprocal_cell(A), R=R, weight=1.0 / deg, coef=np.ascontiguousarray(mat),
+                masses=np.full(n, 28.0855 * MASS_AMU_TO_RY))
