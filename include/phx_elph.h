@@ -194,6 +194,44 @@ int phx_lw_set_line_batching(phx_lw* h, int min_points);
  * Its two stage buffers hold one chunk: (lines * mz + groups * my * mz) *
  * nm * nw complex values per panel.  No counterpart in the reference. */
 int phx_lw_set_panel_chunk(phx_lw* h, int panels);
+/* ---- Learned q-block screen (separable transform with the energy screen on).
+ * No counterpart in the reference, which evaluates every (k1, k2) pair
+ * (el_scattering.cpp:161-386).  A q-block is one panel (k1, b1) times one
+ * (jy, jz) line of inner points.  A 16 -> 64 -> 64 -> 1 MLP with BF16 operands
+ * and FP32 accumulation predicts ln of the block's contribution to the
+ * linewidths of b1's degenerate group.  Blocks with a row within 3 sigma of
+ * resonance are always computed; the others are skipped when their predicted
+ * contribution, raised by the 99.9th-percentile under-prediction seen in
+ * training, stays below threshold times the panel's predicted total.
+ * audit_frac of the skipped blocks are recomputed exactly, and a block found
+ * above 10 x the threshold makes the batch recompute every skipped block.
+ *
+ * Collect training samples: runs the exact last stage on a random fraction
+ * of the tail blocks (rows inside the energy screen, none within 3 sigma)
+ * for the given outer points (all
+ * when k1_list is null) and appends their features and log contributions.
+ * The linewidths are not touched.  Returns the total sample count. */
+int phx_lw_screen_collect(phx_lw* h, int nk1, const int* k1_list, double sample_frac,
+                          unsigned long long seed);
+/* Train the network on the collected samples on the GPU (Adam, MSE on the
+ * standardized log contribution); loss_first_last receives the full-set loss
+ * before and after training. */
+int phx_lw_screen_train(phx_lw* h, int epochs, int batch, double lr, unsigned long long seed,
+                        double* loss_first_last);
+int phx_lw_set_learned_screen(phx_lw* h, int on, double threshold, double audit_frac,
+                              unsigned long long seed);
+/* out8: blocks inside the energy screen, learned skips, audited, batches that
+ * fell back, max audited contribution / threshold, mean audited contribution /
+ * threshold, training samples, under-prediction margin (ln units). */
+int phx_lw_screen_stats(phx_lw* h, double* out8);
+int phx_lw_screen_reset_stats(phx_lw* h);
+/* Read (set = 0) or install (set != 0) the phx_mlp_num_params() FP32 network parameters. */
+int phx_lw_screen_params(phx_lw* h, float* params, int set);
+int phx_mlp_num_params(void);
+/* Loss, gradients and outputs of the network for n samples (X: n x 16,
+ * row-major; t: n targets), computed by the GPU forward / backward kernels. */
+int phx_mlp_gradients(const float* params, const float* X, const float* t, int n, float* grads,
+                      float* loss, float* y);
 /* Per-kernel timing with HIP events recorded on the handle's stream around
  * every batch; turning it on (or off) clears the accumulators. */
 int phx_lw_set_timing(phx_lw* h, int on);

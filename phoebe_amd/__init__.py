@@ -86,6 +86,16 @@ EXPORTS = {
     "phx_lw_set_screen": (C.c_int, [C.c_void_p, C.c_double]),
     "phx_lw_set_line_batching": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_panel_chunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_screen_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_ulonglong]),
+    "phx_lw_screen_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
+                                      C.c_void_p]),
+    "phx_lw_set_learned_screen": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_ulonglong]),
+    "phx_lw_screen_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phx_lw_screen_reset_stats": (C.c_int, [C.c_void_p]),
+    "phx_lw_screen_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "phx_mlp_num_params": (C.c_int, []),
+    "phx_mlp_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
     "phx_lw_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_stream": (C.c_void_p, [C.c_void_p]),
     "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
@@ -166,6 +176,20 @@ def ph_bloch_diag(qc, R, w, mat2R, masses, mass_scaling=True):
                                    _p(_f64(mat2R)), _p(_f64(masses)), int(bool(mass_scaling)),
                                    _p(om), _p(ev)))
     return om, ev.view(np.complex128).reshape(nq, nm, nm).transpose(0, 2, 1)
+
+
+def mlp_gradients(params, X, t):
+    """(loss, grads, y) of the screen's MLP on the GPU forward / backward kernels."""
+    n = len(t)
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    Xc = np.ascontiguousarray(X, dtype=np.float32)
+    tc = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(lib().phx_mlp_num_params(), dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    y = np.zeros(n, dtype=np.float32)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _check(lib().phx_mlp_gradients(vp(P), vp(Xc), vp(tc), n, vp(g), vp(loss), vp(y)))
+    return float(loss[0]), g, y
 
 
 # ------------------------------------------------------------------ InteractionElPhWan
@@ -349,6 +373,48 @@ class ElectronLinewidths:
     def set_panel_chunk(self, panels):
         """Panels per separable-transform launch (bounds its stage buffers)."""
         _check(lib().phx_lw_set_panel_chunk(self._h, int(panels)))
+
+    # ---- learned q-block screen
+    def screen_collect(self, k1_list=None, sample_frac=0.02, seed=1):
+        """Exact contributions of a random sample of q-blocks for training; returns the sample count."""
+        k1 = None if k1_list is None else np.ascontiguousarray(k1_list, dtype=np.int32)
+        n = lib().phx_lw_screen_collect(self._h, 0 if k1 is None else len(k1),
+                                        None if k1 is None else k1.ctypes.data_as(_ip),
+                                        float(sample_frac), int(seed))
+        if n < 0:
+            raise PhxError(lib().phx_last_error().decode())
+        return n
+
+    def screen_train(self, epochs=200, batch=512, lr=2e-3, seed=1):
+        """Train the MLP on the GPU; returns (loss before, loss after) on the standardized targets."""
+        out = np.zeros(2)
+        _check(lib().phx_lw_screen_train(self._h, int(epochs), int(batch), float(lr), int(seed), _p(out)))
+        return float(out[0]), float(out[1])
+
+    def set_learned_screen(self, on=True, threshold=1e-8, audit_frac=0.01, seed=7):
+        _check(lib().phx_lw_set_learned_screen(self._h, int(bool(on)), float(threshold),
+                                               float(audit_frac), int(seed)))
+
+    def screen_stats(self):
+        out = np.zeros(8)
+        _check(lib().phx_lw_screen_stats(self._h, _p(out)))
+        keys = ("blocks_in_screen", "learned_skips", "audited", "fallbacks", "audit_max_ratio",
+                "audit_mean_ratio", "train_samples", "margin")
+        return {k: (int(v) if i < 4 or i == 6 else float(v)) for i, (k, v) in enumerate(zip(keys, out))}
+
+    def screen_reset_stats(self):
+        _check(lib().phx_lw_screen_reset_stats(self._h))
+
+    def screen_params(self, params=None):
+        """Return the trained network's parameters, or install `params`."""
+        n = lib().phx_mlp_num_params()
+        if params is None:
+            out = np.zeros(n, dtype=np.float32)
+            _check(lib().phx_lw_screen_params(self._h, out.ctypes.data_as(C.c_void_p), 0))
+            return out
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == n
+        _check(lib().phx_lw_screen_params(self._h, p.ctypes.data_as(C.c_void_p), 1))
 
     def set_timing(self, on=True):
         _check(lib().phx_lw_set_timing(self._h, int(bool(on))))

@@ -8,12 +8,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "../../include/phx_elph.h"
 #include "coupling_kernel.hpp"
 #include "phx_common.hpp"
+#include "screen.hpp"
 
 namespace phx {
 
@@ -48,6 +50,10 @@ extern template void launch_separable<double>(const SepParams&, const CoupParams
                                               hipStream_t);
 extern template void launch_separable<float>(const SepParams&, const CoupParams&, bool, int, int,
                                              hipStream_t);
+extern template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t);
+extern template void launch_separable_zy<float>(const SepParams&, int, int, hipStream_t);
+extern template void launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t);
+extern template void launch_separable_x<float>(const SepParams&, const CoupParams&, bool, hipStream_t);
 int coupling_rows_per_tile_f64(int ldb);
 
 
@@ -501,6 +507,22 @@ struct LwHandle {
   std::vector<int> hLineKK;
   DevBuf<int> lineKKD;
   std::vector<float2> hphLineF;
+  // learned q-block screen (separable transform with the energy screen on)
+  int lsMode = 0;                  // 0 off, 1 apply the trained network
+  bool lsCollect = false;          // collecting training samples (no linewidth accumulation)
+  double lsThr = 1e-8, lsAudit = 0.01, lsFailFactor = 10., lsSample = 0.02;
+  uint64_t lsSeed = 0x5EEDull, lsBatch = 0;
+  bool mlpTrained = false;
+  DevBuf<float> mlpP, mlpG, mlpM1, mlpM2, mlpScratch, mlpLoss, lsFeat, lsPred, trXd, trTd, trXb, trTb;
+  DevBuf<int> lsNscr, lsSlots, lsFail;
+  DevBuf<unsigned char> lsMaskC, lsMaskA, lsMaskS;
+  DevBuf<double> lsThrAbs, lsPanelStats, lsStats, partialAudit;
+  DevBuf<long long> lsCounts;      // [blocks in the energy screen, learned skips, audits, fallbacks]
+  std::vector<float> trX, trT;     // collected training samples
+  std::vector<int> hNscr, hSel;
+  std::vector<unsigned char> hMask;
+  double lsLoss[2] = {0., 0.};
+  double lsMargin = 0.;            // 99.9th percentile of (target - prediction) on the training set
   // energy-conservation screen of the mesh transform (units of sigma, 0 = off)
   double screen = 0.;
   DevBuf<double> qWlo, qWhi;
@@ -884,6 +906,141 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   PHX_HIP(hipStreamSynchronize(s));
 }
 
+// Learned q-block screen for one panel chunk of the separable transform (the
+// z / y stages have run).  Collect mode: the exact last stage on a random
+// sample of the blocks inside the energy screen, whose features and log
+// contributions are appended to the training set (host).  Apply mode:
+// features -> MLP -> skip / audit / compute masks; exact stage on the computed
+// blocks, exact stage on the audited blocks into a side buffer, audit, and a
+// gated exact stage on every skipped block when the audit fails.
+template <typename XStage>
+static void learned_screen_chunk(LwHandle* L, const CoupParams& cq, XStage&& xstage, int panelBase,
+                                 int nPan, int nLinesYZ, int k2b, int k2e) {
+  const phx_problem& p = L->prob;
+  hipStream_t s = L->stream;
+  const int slots = nPan * nLinesYZ;
+  L->lsFeat.reserve(size_t(slots) * kMlpIn);
+  L->lsNscr.reserve(slots);
+  FeatParams fp{};
+  fp.nPanels = nPan;
+  fp.m0 = p.mesh[0];
+  fp.m1 = p.mesh[1];
+  fp.m2 = p.mesh[2];
+  fp.panelKK = cq.panelKK;
+  fp.panelB1 = cq.panelB1;
+  fp.panelGStart = cq.panelGStart;
+  fp.panelGSize = cq.panelGSize;
+  fp.kkMesh = cq.kkMesh;
+  fp.kkStateOff = cq.kkStateOff;
+  fp.epool = L->e2pool.p;
+  fp.meshToRow = L->meshToRow.p;
+  fp.rowLo = k2b;
+  fp.rowHi = k2e;
+  fp.rowE2range = L->rowE2range.p;
+  fp.qWlo = L->qWlo.p;
+  fp.qWhi = L->qWhi.p;
+  fp.mu = p.chemPots[0];
+  fp.escale = 0.5 * (p.winMax - p.winMin);
+  fp.wscale = 0.;
+  for (double w : L->omegaAll) fp.wscale = std::max(fp.wscale, w);
+  if (!(fp.wscale > 0.)) fp.wscale = 1.;
+  fp.sigma = p.sigma;
+  fp.screen = L->screen;
+  fp.feat = L->lsFeat.p;
+  fp.nscreen = L->lsNscr.p;
+  launch_block_features(fp, s);
+  L->lsMaskC.reserve(slots);
+  if (L->lsCollect) {
+    // training sample: blocks inside the energy screen drawn with probability lsSample
+    L->hNscr.resize(slots);
+    PHX_HIP(hipMemcpyAsync(L->hNscr.data(), L->lsNscr.p, slots * sizeof(int), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    L->hMask.assign(slots, 0);
+    L->hSel.clear();
+    // tail blocks only (rows inside the energy screen, none within 3 sigma): the
+    // blocks the network may skip
+    for (int i = 0; i < slots; ++i)
+      if ((L->hNscr[i] & 0xffff) > 0 && (L->hNscr[i] >> 16) == 0 &&
+          screen_draw(L->lsSeed ^ 0xC011EC7ull, L->lsBatch, panelBase + i / nLinesYZ, nLinesYZ,
+                      i % nLinesYZ) < L->lsSample) {
+        L->hMask[i] = 1;
+        L->hSel.push_back(i);
+      }
+    const int nSel = int(L->hSel.size());
+    PHX_HIP(hipMemcpyAsync(L->lsMaskC.p, L->hMask.data(), slots, hipMemcpyHostToDevice, s));
+    CoupParams c1 = cq;
+    c1.slotOn = L->lsMaskC.p;
+    c1.slotZero = 1;
+    xstage(c1);
+    if (nSel == 0) return;
+    L->lsSlots.reserve(nSel);
+    PHX_HIP(hipMemcpyAsync(L->lsSlots.p, L->hSel.data(), nSel * sizeof(int), hipMemcpyHostToDevice, s));
+    L->trXb.reserve(size_t(nSel) * kMlpIn);
+    L->trTb.reserve(nSel);
+    launch_gather_targets(nSel, L->lsSlots.p, cq.partial, L->lsFeat.p, L->trXb.p, L->trTb.p, s);
+    const size_t o = L->trT.size();
+    L->trX.resize(L->trX.size() + size_t(nSel) * kMlpIn);
+    L->trT.resize(o + nSel);
+    PHX_HIP(hipMemcpyAsync(L->trX.data() + o * kMlpIn, L->trXb.p, size_t(nSel) * kMlpIn * sizeof(float),
+                           hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(L->trT.data() + o, L->trTb.p, nSel * sizeof(float), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  // inference and decisions
+  L->lsPred.reserve(slots);
+  L->mlpScratch.reserve(mlp_scratch_floats(slots));
+  mlp_forward(L->mlpP.p, L->lsFeat.p, slots, L->mlpScratch.p, L->lsPred.p, s);
+  L->lsMaskA.reserve(slots);
+  L->lsMaskS.reserve(slots);
+  L->lsThrAbs.reserve(nPan);
+  L->lsPanelStats.reserve(2 * size_t(nPan));
+  DecideParams dp{};
+  dp.nPanels = nPan;
+  dp.nLinesYZ = nLinesYZ;
+  dp.panelBase = panelBase;
+  dp.pred = L->lsPred.p;
+  dp.nscreen = L->lsNscr.p;
+  dp.thr = L->lsThr;
+  dp.auditFrac = L->lsAudit;
+  dp.seed = L->lsSeed;
+  dp.batchId = L->lsBatch;
+  dp.margin = L->lsMargin;
+  dp.computeMask = L->lsMaskC.p;
+  dp.auditMask = L->lsMaskA.p;
+  dp.skipMask = L->lsMaskS.p;
+  dp.thrAbs = L->lsThrAbs.p;
+  dp.counts = L->lsCounts.p;
+  launch_screen_decide(dp, s);
+  CoupParams c1 = cq;
+  c1.slotOn = L->lsMaskC.p;
+  c1.slotZero = 1;
+  xstage(c1);
+  L->partialAudit.reserve(size_t(slots) * kGMax * kCMax);
+  CoupParams c2 = cq;
+  c2.slotOn = L->lsMaskA.p;
+  c2.slotZero = 1;
+  c2.partial = L->partialAudit.p;
+  xstage(c2);
+  AuditParams ap{};
+  ap.nPanels = nPan;
+  ap.nLinesYZ = nLinesYZ;
+  ap.auditMask = L->lsMaskA.p;
+  ap.thrAbs = L->lsThrAbs.p;
+  ap.partialAudit = L->partialAudit.p;
+  ap.partial = cq.partial;
+  ap.failFactor = L->lsFailFactor;
+  ap.fail = L->lsFail.p;
+  ap.stats = L->lsStats.p;
+  launch_screen_audit(ap, L->lsPanelStats.p, s);
+  CoupParams c3 = cq;
+  c3.slotOn = L->lsMaskS.p;
+  c3.slotZero = 0;
+  c3.gate = L->lsFail.p;
+  xstage(c3);
+  launch_fail_fold(L->lsFail.p, L->lsCounts.p + 3, s);
+}
+
 // Enqueue one batch of outer points.  line: every k1 of the batch lies on one
 // mesh line (same j1, j2) and cacheElPh runs in its line form.
 static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, bool line) {
@@ -1179,13 +1336,20 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
       cq.panelGStart = dPGS + p0;
       cq.panelGSize = dPGZ + p0;
       cq.partial = partial0 + size_t(p0) * nMtiles * kGMax * kCMax;
-      if (L->precision == PHX_FP64) {
-        sp.B = L->Bd.p + size_t(p0) * pstride;
-        launch_separable<double>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
-      } else {
-        sp.B = L->Bf.p + size_t(p0) * pstride;
-        launch_separable<float>(sp, cq, true, L->maxLineE, L->maxGrpE, s);
+      const bool f64 = L->precision == PHX_FP64;
+      if (f64) sp.B = L->Bd.p + size_t(p0) * pstride;
+      else sp.B = L->Bf.p + size_t(p0) * pstride;
+      auto xstage = [&](const CoupParams& c) {
+        if (f64) launch_separable_x<double>(sp, c, true, s);
+        else launch_separable_x<float>(sp, c, true, s);
+      };
+      if (f64) launch_separable_zy<double>(sp, L->maxLineE, L->maxGrpE, s);
+      else launch_separable_zy<float>(sp, L->maxLineE, L->maxGrpE, s);
+      if (!L->lsCollect && !(L->lsMode && L->mlpTrained)) {
+        xstage(cq);
+        continue;
       }
+      learned_screen_chunk(L, cq, xstage, p0, n, nMtiles, k2b, k2e);
     }
   }
   if (L->timing) {
@@ -1207,6 +1371,14 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
       }
     }
     L->couplingFlops += f * nPanels;
+  }
+  if (L->lsCollect || (L->lsMode && L->mlpTrained)) ++L->lsBatch;
+  if (L->lsCollect) {  // training samples only: the linewidths are left untouched
+    if (L->timing) {
+      PHX_HIP(hipEventRecord(L->ev[3], s));
+      L->evPending = true;
+    }
+    return (long long)nkk * M;
   }
   CollectParams cl{};
   cl.nkk = nkk;
@@ -1554,6 +1726,223 @@ int phx_lw_set_panel_chunk(phx_lw* h, int panels) {
     PHX_REQUIRE(h, "null handle");
     PHX_REQUIRE(panels >= 1, "phx_lw_set_panel_chunk: panels must be >= 1");
     h->panelChunk = panels;
+  });
+}
+
+// ---- learned screen: training on the collected samples
+static void ls_alloc(LwHandle* L) {
+  hipStream_t s = L->stream;
+  if (!L->mlpP.p) {
+    L->mlpP.alloc(kMlpParams);
+    L->mlpG.alloc(kMlpParams);
+    L->mlpM1.alloc(kMlpParams);
+    L->mlpM2.alloc(kMlpParams);
+    L->mlpLoss.alloc(4);
+  }
+  if (!L->lsCounts.p) {
+    L->lsCounts.alloc(4);
+    L->lsStats.alloc(2);
+    L->lsFail.alloc(1);
+    launch_fill<int>(reinterpret_cast<int*>(L->lsCounts.p), 8, 0, s);
+    launch_fill<double>(L->lsStats.p, 2, 0., s);
+    launch_fill<int>(L->lsFail.p, 1, 0, s);
+  }
+}
+
+static void ls_train(LwHandle* L, int epochs, int batch, float lr, uint64_t seed) {
+  hipStream_t s = L->stream;
+  const int n = int(L->trT.size());
+  PHX_REQUIRE(n >= 16, "learned screen: collect at least 16 training samples first");
+  PHX_REQUIRE(epochs >= 1 && batch >= 1 && lr > 0.f, "learned screen: bad training settings");
+  ls_alloc(L);
+  double mean = 0., var = 0.;
+  for (float t : L->trT) mean += t;
+  mean /= n;
+  for (float t : L->trT) var += (t - mean) * (t - mean);
+  const double sd = std::max(std::sqrt(var / n), 1e-6);
+  std::vector<float> tn(n);
+  for (int i = 0; i < n; ++i) tn[i] = float((L->trT[i] - mean) / sd);
+  // Glorot-uniform weights, zero biases
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> U(-1., 1.);
+  std::vector<float> P(kMlpParams, 0.f);
+  auto init = [&](int off, int fin, int fout) {
+    const double a = std::sqrt(6. / (fin + fout));
+    for (int i = 0; i < fin * fout; ++i) P[off + i] = float(a * U(rng));
+  };
+  init(kMlpOffW1, kMlpIn, kMlpHid);
+  init(kMlpOffW2, kMlpHid, kMlpHid);
+  init(kMlpOffW3, kMlpHid, 1);
+  PHX_HIP(hipMemcpyAsync(L->mlpP.p, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice, s));
+  launch_fill<float>(L->mlpM1.p, kMlpParams, 0.f, s);
+  launch_fill<float>(L->mlpM2.p, kMlpParams, 0.f, s);
+  L->trXd.reserve(size_t(n) * kMlpIn);
+  L->trTd.reserve(n);
+  L->mlpScratch.reserve(mlp_scratch_floats(std::max(n, batch)));
+  std::vector<float> Xp(size_t(n) * kMlpIn), Tp(n);
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  auto full_loss = [&]() {
+    PHX_HIP(hipMemcpyAsync(L->trXd.p, L->trX.data(), size_t(n) * kMlpIn * sizeof(float),
+                           hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(L->trTd.p, tn.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    mlp_backward(L->mlpP.p, L->trXd.p, L->trTd.p, n, L->mlpScratch.p, L->mlpG.p, L->mlpLoss.p, s);
+    float v = 0.f;
+    PHX_HIP(hipMemcpyAsync(&v, L->mlpLoss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    return double(v);
+  };
+  L->lsLoss[0] = full_loss();
+  int step = 0;
+  for (int e = 0; e < epochs; ++e) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    for (int i = 0; i < n; ++i) {
+      std::copy(L->trX.begin() + size_t(perm[i]) * kMlpIn, L->trX.begin() + size_t(perm[i] + 1) * kMlpIn,
+                Xp.begin() + size_t(i) * kMlpIn);
+      Tp[i] = tn[perm[i]];
+    }
+    PHX_HIP(hipMemcpyAsync(L->trXd.p, Xp.data(), Xp.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(L->trTd.p, Tp.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    for (int b = 0; b < n; b += batch) {
+      const int nb = std::min(batch, n - b);
+      mlp_backward(L->mlpP.p, L->trXd.p + size_t(b) * kMlpIn, L->trTd.p + b, nb, L->mlpScratch.p,
+                   L->mlpG.p, L->mlpLoss.p, s);
+      mlp_adam(L->mlpP.p, L->mlpG.p, L->mlpM1.p, L->mlpM2.p, ++step, lr, s);
+    }
+    PHX_HIP(hipStreamSynchronize(s));  // the host staging arrays are refilled next epoch
+  }
+  L->lsLoss[1] = full_loss();
+  {
+    // under-prediction margin: ln-residuals target - prediction (standardized units)
+    std::vector<float> yv(n);
+    PHX_HIP(hipMemcpy(yv.data(), L->mlpScratch.p + size_t(n) * (4 * kMlpHid + 1), n * sizeof(float),
+                      hipMemcpyDeviceToHost));
+    std::vector<double> r(n);
+    for (int i = 0; i < n; ++i) r[i] = (double(tn[i]) - yv[i]) * sd;
+    std::sort(r.begin(), r.end());
+    L->lsMargin = std::max(0., r[std::min(n - 1, int(0.999 * n))]);
+  }
+  // fold the target normalization into the output layer: the network predicts ln c
+  PHX_HIP(hipMemcpy(P.data(), L->mlpP.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < kMlpHid; ++i) P[kMlpOffW3 + i] = float(P[kMlpOffW3 + i] * sd);
+  P[kMlpOffB3] = float(P[kMlpOffB3] * sd + mean);
+  PHX_HIP(hipMemcpy(L->mlpP.p, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  L->mlpTrained = true;
+}
+
+int phx_lw_screen_collect(phx_lw* h, int nk1, const int* k1_list, double sample_frac,
+                          unsigned long long seed) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_REQUIRE(h->transform == PHX_TRANSFORM_SEPARABLE && h->screen > 0.,
+                "learned screen: needs the separable transform with the energy screen on");
+    PHX_REQUIRE(sample_frac > 0. && sample_frac <= 1., "learned screen: sample fraction in (0, 1]");
+    ls_alloc(h);
+    h->lsSample = sample_frac;
+    h->lsSeed = seed;
+    h->lsCollect = true;
+    try {
+      lw_run(h, nk1, k1_list, 0, -1);
+      PHX_HIP(hipStreamSynchronize(h->stream));
+    } catch (...) {
+      h->lsCollect = false;
+      throw;
+    }
+    h->lsCollect = false;
+  }) < 0 ? -1 : int(h->trT.size());
+}
+
+int phx_lw_screen_train(phx_lw* h, int epochs, int batch, double lr, unsigned long long seed,
+                        double* loss_first_last) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    ls_train(h, epochs, batch, float(lr), seed);
+    if (loss_first_last) {
+      loss_first_last[0] = h->lsLoss[0];
+      loss_first_last[1] = h->lsLoss[1];
+    }
+  });
+}
+
+int phx_lw_set_learned_screen(phx_lw* h, int on, double threshold, double audit_frac,
+                              unsigned long long seed) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_REQUIRE(!on || h->mlpTrained, "learned screen: train the network first");
+    PHX_REQUIRE(threshold > 0. && threshold < 1. && audit_frac >= 0. && audit_frac <= 1.,
+                "learned screen: threshold in (0, 1), audit fraction in [0, 1]");
+    ls_alloc(h);
+    h->lsMode = on ? 1 : 0;
+    h->lsThr = threshold;
+    h->lsAudit = audit_frac;
+    h->lsSeed = seed;
+  });
+}
+
+int phx_lw_screen_stats(phx_lw* h, double* out8) {
+  return guard([&] {
+    PHX_REQUIRE(h && out8, "null argument");
+    for (int i = 0; i < 8; ++i) out8[i] = 0.;
+    if (h->lsCounts.p) {
+      long long c[4];
+      double st[2];
+      PHX_HIP(hipStreamSynchronize(h->stream));
+      PHX_HIP(hipMemcpy(c, h->lsCounts.p, sizeof(c), hipMemcpyDeviceToHost));
+      PHX_HIP(hipMemcpy(st, h->lsStats.p, sizeof(st), hipMemcpyDeviceToHost));
+      for (int i = 0; i < 4; ++i) out8[i] = double(c[i]);
+      out8[4] = st[0];
+      out8[5] = c[2] > 0 ? st[1] / double(c[2]) : 0.;
+    }
+    out8[6] = double(h->trT.size());
+    out8[7] = h->lsMargin;
+  });
+}
+
+int phx_lw_screen_reset_stats(phx_lw* h) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    ls_alloc(h);
+    launch_fill<int>(reinterpret_cast<int*>(h->lsCounts.p), 8, 0, h->stream);
+    launch_fill<double>(h->lsStats.p, 2, 0., h->stream);
+    PHX_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int phx_lw_screen_params(phx_lw* h, float* params, int set) {
+  return guard([&] {
+    PHX_REQUIRE(h && params, "null argument");
+    ls_alloc(h);
+    if (set) {
+      PHX_HIP(hipMemcpy(h->mlpP.p, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+      h->mlpTrained = true;
+    } else {
+      PHX_REQUIRE(h->mlpTrained, "learned screen: no trained network");
+      PHX_HIP(hipMemcpy(params, h->mlpP.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int phx_mlp_num_params(void) { return kMlpParams; }
+
+int phx_mlp_gradients(const float* params, const float* X, const float* t, int n, float* grads,
+                      float* loss, float* y) {
+  return guard([&] {
+    PHX_REQUIRE(params && X && t && n > 0, "phx_mlp_gradients: bad arguments");
+    hipStream_t s = 0;
+    DevBuf<float> P, Xd, Td, G, sc, l;
+    h2d(P, params, kMlpParams, s);
+    h2d(Xd, X, size_t(n) * kMlpIn, s);
+    h2d(Td, t, size_t(n), s);
+    G.alloc(kMlpParams);
+    sc.alloc(mlp_scratch_floats(n));
+    l.alloc(1);
+    mlp_backward(P.p, Xd.p, Td.p, n, sc.p, G.p, l.p, s);
+    PHX_HIP(hipStreamSynchronize(s));
+    if (grads) PHX_HIP(hipMemcpy(grads, G.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) PHX_HIP(hipMemcpy(loss, l.p, sizeof(float), hipMemcpyDeviceToHost));
+    // y (network output) sits after H1, H2, dZ2, dZ1 and dy in the scratch
+    if (y) PHX_HIP(hipMemcpy(y, sc.p + size_t(n) * (4 * kMlpHid + 1), n * sizeof(float),
+                             hipMemcpyDeviceToHost));
   });
 }
 

@@ -65,6 +65,10 @@ struct CoupParams {
   double screen;                          // energy screen in units of sigma (0: off)
   const double* qWlo; const double* qWhi; // per q: lowest / highest omega >= cutoff
   const double2* rowE2range;              // per row: (lowest, highest) band energy
+  // learned screen (k_sep_xm): per (panel, yz) slot masks
+  const unsigned char* slotOn;            // compute only slots with slotOn != 0 (null: all)
+  int slotZero;                           // off slots: 1 = write zero partials, 0 = leave untouched
+  const int* gate;                        // run only when *gate != 0 (null: always)
 };
 
 template <typename T> struct Vec2;
@@ -814,6 +818,11 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   const int slot = panel * nLinesYZ + yz;  // partial-sum slot (panel-major, as k_collect reads it)
   const int jy = yz % p.m1, jz = yz / p.m1;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
+  if (p.gate && *p.gate == 0) return;
+  if (p.slotOn && !p.slotOn[slot]) {
+    if (REDUCE && p.slotZero && t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
+    return;
+  }
   const int nm = p.nm, nw = p.nw, ncol = nm * nw;
   const int NWP = 2 * nw + 1, LDG = nm * NWP;  // g3 row layout: (nu, 2 w2 + ri), padded per nu
   const int kk = p.panelKK[panel];
@@ -1160,10 +1169,7 @@ inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb
 
 // maxLineE / maxGrpE: largest number of entries of a line / group (LDS sizing).
 template <typename T>
-void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int maxLineE,
-                      int maxGrpE, hipStream_t s) {
-  PHX_REQUIRE(p.nm <= kSepNm && p.nb2max <= kSepNb2,
-              "separable transform supports nm <= 32 and at most 8 bands per inner point");
+void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStream_t s) {
   if (sp.nPanels <= 0) return;
   const size_t ldsZ = size_t(maxLineE) * sp.ncol * sizeof(double2);
   const size_t ldsY = size_t(maxGrpE) * sp.ncol * sizeof(double2);
@@ -1176,6 +1182,13 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
   PHX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_sep_y<T>, dim3(sp.nG0, sp.mz, sp.nPanels), dim3(256), ldsY, s, sp);
   PHX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, hipStream_t s) {
+  PHX_REQUIRE(p.nm <= kSepNm && p.nb2max <= kSepNb2,
+              "separable transform supports nm <= 32 and at most 8 bands per inner point");
+  if (sp.nPanels <= 0) return;
   const int tiles = sp.nPanels * sp.my * sp.mz;
   const int grid = ceil_div(tiles, 8) * 8;
   const int nmb = p.nm <= 12 ? 12 : p.nm <= 16 ? 16 : p.nm <= 24 ? 24 : kSepNm;
@@ -1199,6 +1212,7 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
     PHX_HIP(hipGetLastError());
     return;
   }
+  PHX_REQUIRE(!p.slotOn && !p.gate, "separable transform: slot masks need 2 nm nw <= 384");
   const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
                                size_t(kGMax) * kCMax * 4 * sizeof(double));
   PHX_REQUIRE(ldsX <= 160 * 1024, "separable transform: LDS budget exceeded");
@@ -1218,6 +1232,13 @@ void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int
     else go(k_sep_x<T, false, kSepNm, 1>);
   }
   PHX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int maxLineE,
+                      int maxGrpE, hipStream_t s) {
+  launch_separable_zy<T>(sp, maxLineE, maxGrpE, s);
+  launch_separable_x<T>(sp, p, reduce, s);
 }
 
 }  // namespace phx
