@@ -44,6 +44,8 @@ FORMS = {  # name: (precision, transform, energy screen in sigma, line batching 
     "sep64s": ("fp64", "separable", 7.0, 0),
     "line32s": ("fp32", "separable", 7.0, 8),
     "line64s": ("fp64", "separable", 7.0, 8),
+    "line32s6": ("fp32", "separable", 6.0, 8),
+    "line3s6": ("bf16x3", "separable", 6.0, 8),
 }
 
 
@@ -129,7 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--k1-batch", type=int, default=32)
-    ap.add_argument("--chosen", default="line32s", choices=sorted(k for k in FORMS if k != "fp64"))
+    ap.add_argument("--chosen", default="line32s6", choices=sorted(k for k in FORMS if k != "fp64"))
     ap.add_argument("--cpu-k1", type=int, default=1, help="outer points in the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

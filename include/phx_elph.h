@@ -27,7 +27,10 @@ extern "C" {
 #endif
 
 /* precision of the Fourier-transform and rotation GEMMs; phases,
- * eigensolves and the linewidth accumulation are always FP64 */
+ * eigensolves and the linewidth accumulation are always FP64.  PHX_BF16X3
+ * keeps FP32 data and runs the last mesh-transform GEMMs as split BF16
+ * (hi * hi + hi * lo + lo * hi, FP32 accumulation); linewidth builder with the
+ * separable transform only. */
 enum { PHX_FP64 = 0, PHX_FP32 = 1, PHX_BF16X3 = 2 };
 
 /* how the R_ph sum of calcCouplingSquared is evaluated in the linewidth builder:

@@ -558,7 +558,9 @@ static void lw_harvest_timing(LwHandle* L) {
 static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch,
                      int transform) {
   const phx_problem& p = *P;
-  PHX_REQUIRE(precision == PHX_FP64 || precision == PHX_FP32, "phx_lw_setup: precision not available");
+  PHX_REQUIRE(precision == PHX_FP64 || precision == PHX_FP32 ||
+                  (precision == PHX_BF16X3 && transform == PHX_TRANSFORM_SEPARABLE),
+              "phx_lw_setup: precision not available (split-BF16x3 runs with the separable transform)");
   PHX_REQUIRE(p.nw >= 1 && p.nw <= 32 && p.nm >= 1 && p.nm <= 36, "phx_lw_setup: unsupported sizes");
   PHX_REQUIRE(p.nCalc >= 1 && p.nCalc <= kCMax, "phx_lw_setup: 1..4 (T, mu) calculations supported");
   PHX_REQUIRE(p.nw <= 44 && p.nm <= 44, "phx_lw_setup: eigensolver size");
@@ -1311,6 +1313,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.qWlo = L->qWlo.p;
     cq.qWhi = L->qWhi.p;
     cq.rowE2range = L->rowE2range.p;
+    cq.bf16x3 = L->precision == PHX_BF16X3;
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
     if (L->precision == PHX_FP64) {

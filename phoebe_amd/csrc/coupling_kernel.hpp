@@ -66,6 +66,7 @@ struct CoupParams {
   const double* qWlo; const double* qWhi; // per q: lowest / highest omega >= cutoff
   const double2* rowE2range;              // per row: (lowest, highest) band energy
   // learned screen (k_sep_xm): per (panel, yz) slot masks
+  int bf16x3;                             // x-stage GEMMs in split-BF16x3 (FP32 data)
   const unsigned char* slotOn;            // compute only slots with slotOn != 0 (null: all)
   int slotZero;                           // off slots: 1 = write zero partials, 0 = leave untouched
   const int* gate;                        // run only when *gate != 0 (null: always)
@@ -787,27 +788,67 @@ inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int m0, int nw, int n
 // Same result as k_sep_x.  Per workgroup (one (jy, jz) line of mesh points and
 // one panel), the kept rows jx are compacted, then for every tile of 16 of them
 //   g3(jx, c) = sum_g w_x^{jx n0(g)} T2(g, jy, jz, c)
-// runs as a real GEMM on v_mfma_*_16x16x4 in the transform precision:
-//   A(m, 2g + s) = (Re, Im) w(jx_m, g),   B(2g + s, 2c + t) = [[Xr, Xi], [-Xi, Xr]](s, t),
-// A from a twiddle table in LDS, B straight from T2 (L2).  The epilogue gives
+// runs as two real GEMMs on v_mfma_*_16x16x4 in the transform precision,
+// sharing B(2g + s, c) = (Re, Im) X(g, c):
+//   Re g3 = A1 B with A1(m, 2g + s) = (Re w, -Im w),  Im g3 = A2 B with A2 = (Im w, Re w),
+// w = w(jx_m, g) from a twiddle table in LDS, B straight from T2 (L2).  The epilogue gives
 // TPR threads to each row (thread sub = mode nu).  Workgroups are ordered
 // with the panels innermost, so that consecutive workgroups share the rows'
 // U2 and the q-line's phonons in L2.
-constexpr int kXmNtw = 6;   // n-tiles (16 real columns) per wave: 2 * nm * nw <= 384
+constexpr int kXmNt = 3;    // complex column tiles (16 columns) per wave: nm * nw <= 192
 constexpr int kXmKsc = 3;   // MFMA k-steps per B chunk
 
 struct SepXmRow {            // per kept row, staged in LDS
   int jx, row, iq, nb2, u2off, soff;
 };
 
-template <typename T, bool REDUCE, int NMX, int CM>
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+
+// Row data of a 16-row tile into LDS: U2 (16 x nw x nb2max), e2 (16 x nb2max), omega (16 x nm).
+__device__ inline void sep_xm_stage_rows(const CoupParams& p, const SepXmRow* rows, int m0r, int nr,
+                                         int nw, int nm, double2* sU2, double* se2, double* sw3) {
+  const int t = threadIdx.x;
+  for (int i = t; i < 16 * nw * p.nb2max; i += 256) {
+    const int r = i / (nw * p.nb2max), o = i % (nw * p.nb2max);
+    double2 v = make_double2(0., 0.);
+    if (r < nr) {
+      const SepXmRow& rr = rows[m0r + r];
+      if (o < nw * rr.nb2) v = p.U2pool[rr.u2off + o];
+    }
+    sU2[i] = v;
+  }
+  for (int i = t; i < 16 * p.nb2max; i += 256) {
+    const int r = i / p.nb2max, b = i % p.nb2max;
+    double v = 0.;
+    if (r < nr && b < rows[m0r + r].nb2) v = p.e2pool[rows[m0r + r].soff + b];
+    se2[i] = v;
+  }
+  for (int i = t; i < 16 * nm; i += 256) {
+    const int r = i / nm, nu = i % nm;
+    sw3[i] = r < nr ? p.Wpool[size_t(rows[m0r + r].iq) * nm + nu] : 0.;
+  }
+}
+
+// Split-BF16 operands: x ~ hi + lo with hi = bf16(x), lo = bf16(x - hi).
+__device__ inline void bf16_split4(const float (&x)[4], bf16x4_t& hi, bf16x4_t& lo) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const short h = bf16_bits(x[j]);
+    hi[j] = h;
+    lo[j] = bf16_bits(x[j] - bf16_value(h));
+  }
+}
+
+// BF3: the x-stage GEMMs in split-BF16x3 (hi*hi + hi*lo + lo*hi on
+// v_mfma_f32_16x16x16_bf16, FP32 accumulation); T must be float.
+template <typename T, bool REDUCE, int NMX, int CM, bool BF3 = false>
 __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
   typedef typename Mfma4<T>::acc_t acc_t;
   constexpr int TPR = NMX <= 16 ? 16 : 32;  // epilogue threads per row
   constexpr int RPP = 256 / TPR;            // rows per epilogue pass
-  constexpr int KSB = sizeof(T) == 4 ? 12 : 6;  // k-steps of B held in registers
+  constexpr int KSB = sizeof(T) == 4 ? 6 : 3;   // k-steps of B held in registers
   const int nLinesYZ = p.m1 * p.m2;
   const int nTilesTot = p.nPanels * nLinesYZ;
   const int bid = blockIdx.x;
@@ -931,75 +972,106 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
                  (size_t(jy) * p.m2 + jz) * ncol;
   const size_t gStride = size_t(p.m1) * p.m2 * ncol;  // T2v elements between groups
   const int KS = (2 * nG0 + 3) / 4;                    // k-steps of 4 real K
-  const int nNt = (2 * ncol + 15) / 16;                // n-tiles in use
 
   for (int m0r = 0; m0r < nkeep; m0r += 16) {
     const int nr = min(16, nkeep - m0r);
     const bool rowOk = lr < nr;
     const T2v* wrow = Wk + size_t(m0r + (rowOk ? lr : 0)) * nG0p;
-    acc_t acc[kXmNtw];
+    // Cr = A1 B, Ci = A2 B with B(2g + s, c) = (Xr, Xi)(g, c), A1(m, 2g + s) = (Wr, -Wi),
+    // A2(m, 2g + s) = (Wi, Wr); wave owns complex column tiles wave*kXmNt .. +kXmNt
+    acc_t accr[kXmNt], acci[kXmNt];
 #pragma unroll
-    for (int j = 0; j < kXmNtw; ++j)
+    for (int j = 0; j < kXmNt; ++j)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[j][i] = T(0);
+      for (int i = 0; i < 4; ++i) accr[j][i] = acci[j][i] = T(0);
+    if constexpr (BF3) {
+      sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, sU2, se2, sw3);
+      // 16 real K per step: lane (lr, lg) holds k = 16 kb + 4 lg + j, i.e. groups
+      // g = 8 kb + 2 lg and g + 1, (Re, Im) each
+      for (int kb = 0; kb < (2 * nG0 + 15) / 16; ++kb) {
+        const int gA = 8 * kb + 2 * lg;
+        const T2v w0 = wrow[gA < nG0 ? gA : nG0], w1 = wrow[gA + 1 < nG0 ? gA + 1 : nG0];
+        float a1[4] = {w0.x, -w0.y, w1.x, -w1.y}, a2[4] = {w0.y, w0.x, w1.y, w1.x};
+        if (!rowOk)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) a1[j] = a2[j] = 0.f;
+        bf16x4_t a1h, a1l, a2h, a2l;
+        bf16_split4(a1, a1h, a1l);
+        bf16_split4(a2, a2h, a2l);
+#pragma unroll
+        for (int j = 0; j < kXmNt; ++j) {
+          const int c = (wave * kXmNt + j) * 16 + lr;
+          float b[4] = {0.f, 0.f, 0.f, 0.f};
+          if (c < ncol) {
+            if (gA < nG0) {
+              const T2v x = X[size_t(gA) * gStride + c];
+              b[0] = x.x;
+              b[1] = x.y;
+            }
+            if (gA + 1 < nG0) {
+              const T2v x = X[size_t(gA + 1) * gStride + c];
+              b[2] = x.x;
+              b[3] = x.y;
+            }
+          }
+          bf16x4_t bh, bl;
+          bf16_split4(b, bh, bl);
+          accr[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1h, bh, accr[j], 0, 0, 0);
+          accr[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1h, bl, accr[j], 0, 0, 0);
+          accr[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1l, bh, accr[j], 0, 0, 0);
+          acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2h, bh, acci[j], 0, 0, 0);
+          acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2h, bl, acci[j], 0, 0, 0);
+          acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2l, bh, acci[j], 0, 0, 0);
+        }
+      }
+    } else
     for (int kb = 0; kb < KS; kb += KSB) {
-      // B operands of KSB k-steps straight from T2 (one round trip), then the MFMAs
-      T bv[KSB][kXmNtw];
+      // B operands of KSB k-steps straight from T2, then the MFMAs
+      T bv[KSB][kXmNt];
 #pragma unroll
       for (int q = 0; q < KSB; ++q) {
         const int k = 4 * (kb + q) + lg, g = k >> 1, kc = k & 1;
         const bool gOk = g < nG0 && kb + q < KS;
 #pragma unroll
-        for (int j = 0; j < kXmNtw; ++j) {
-          const int n = (wave * kXmNtw + j) * 16 + lr, c = n >> 1, ri = n & 1;
+        for (int j = 0; j < kXmNt; ++j) {
+          const int c = (wave * kXmNt + j) * 16 + lr;
           T v = T(0);
           if (gOk && c < ncol) {
             const T2v x = X[size_t(g) * gStride + c];
-            v = ri ? (kc ? x.x : x.y) : (kc ? -x.y : x.x);
+            v = kc ? x.y : x.x;
           }
           bv[q][j] = v;
         }
       }
-      if (kb == 0) {
-        // the tile's row data (U2, e2, omega) into LDS while the B loads are in flight;
-        // the previous tile's epilogue ended with a barrier
-        for (int i = t; i < 16 * nw * p.nb2max; i += 256) {
-          const int r = i / (nw * p.nb2max), o = i % (nw * p.nb2max);
-          double2 v = make_double2(0., 0.);
-          if (r < nr) {
-            const SepXmRow& rr = rows[m0r + r];
-            if (o < nw * rr.nb2) v = p.U2pool[rr.u2off + o];
-          }
-          sU2[i] = v;
-        }
-        for (int i = t; i < 16 * p.nb2max; i += 256) {
-          const int r = i / p.nb2max, b = i % p.nb2max;
-          double v = 0.;
-          if (r < nr && b < rows[m0r + r].nb2) v = p.e2pool[rows[m0r + r].soff + b];
-          se2[i] = v;
-        }
-        for (int i = t; i < 16 * nm; i += 256) {
-          const int r = i / nm, nu = i % nm;
-          sw3[i] = r < nr ? p.Wpool[size_t(rows[m0r + r].iq) * nm + nu] : 0.;
-        }
-      }
+      // the tile's row data into LDS while the B loads are in flight; the
+      // previous tile's epilogue ended with a barrier
+      if (kb == 0) sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, sU2, se2, sw3);
 #pragma unroll
       for (int q = 0; q < KSB; ++q) {
         const int k = 4 * (kb + q) + lg, g = k >> 1, kc = k & 1;
         const bool gOk = g < nG0 && kb + q < KS;
         const T2v w = wrow[gOk ? g : nG0];
-        const T av = (rowOk && gOk) ? (kc ? w.y : w.x) : T(0);
+        const bool on = rowOk && gOk;
+        const T a1 = on ? (kc ? -w.y : w.x) : T(0);
+        const T a2 = on ? (kc ? w.x : w.y) : T(0);
 #pragma unroll
-        for (int j = 0; j < kXmNtw; ++j) acc[j] = Mfma4<T>::mac(av, bv[q][j], acc[j]);
+        for (int j = 0; j < kXmNt; ++j) {
+          accr[j] = Mfma4<T>::mac(a1, bv[q][j], accr[j]);
+          acci[j] = Mfma4<T>::mac(a2, bv[q][j], acci[j]);
+        }
       }
     }
 #pragma unroll
-    for (int j = 0; j < kXmNtw; ++j) {
-      const int n = (wave * kXmNtw + j) * 16 + lr, c = n >> 1, ri = n & 1;
-      if (c < ncol && (wave * kXmNtw + j) < nNt) {
+    for (int j = 0; j < kXmNt; ++j) {
+      const int c = (wave * kXmNt + j) * 16 + lr;
+      if (c < ncol) {
         const int nu = c / nw, w2 = c - nu * nw;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) g3[Mfma4<T>::row(lg, i) * LDG + nu * NWP + 2 * w2 + ri] = acc[j][i];
+        for (int i = 0; i < 4; ++i) {
+          T* dst = g3 + Mfma4<T>::row(lg, i) * LDG + nu * NWP + 2 * w2;
+          dst[0] = accr[j][i];
+          dst[1] = acci[j][i];
+        }
       }
     }
     __syncthreads();
@@ -1193,7 +1265,11 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
   const int grid = ceil_div(tiles, 8) * 8;
   const int nmb = p.nm <= 12 ? 12 : p.nm <= 16 ? 16 : p.nm <= 24 ? 24 : kSepNm;
   const bool one = p.nCalc == 1;
-  if (2 * sp.ncol <= 4 * kXmNtw * 16) {
+  constexpr bool bf3t = sizeof(T) == 4;  // split-BF16x3 exists for the FP32 data path
+  const bool bf3 = p.bf16x3 != 0;
+  PHX_REQUIRE(!bf3 || (bf3t && reduce && sp.ncol <= 4 * kXmNt * 16),
+              "split-BF16x3: FP32 data, the linewidth reduction and nm nw <= 192");
+  if (sp.ncol <= 4 * kXmNt * 16) {
     const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8);
     PHX_REQUIRE(ldsXm <= 160 * 1024, "separable transform: LDS budget exceeded");
     auto go = [&](auto kern) {
@@ -1204,6 +1280,8 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
 #define PHX_XM(NMX)                                                                 \
     if (nmb == NMX) {                                                               \
       if (!reduce) go(k_sep_xm<T, false, NMX, 1>);                                  \
+      else if (bf3 && one) go(k_sep_xm<T, true, NMX, 1, bf3t>);                      \
+      else if (bf3) go(k_sep_xm<T, true, NMX, kCMax, bf3t>);                         \
       else if (one) go(k_sep_xm<T, true, NMX, 1>);                                  \
       else go(k_sep_xm<T, true, NMX, kCMax>);                                       \
     }
@@ -1212,7 +1290,7 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
     PHX_HIP(hipGetLastError());
     return;
   }
-  PHX_REQUIRE(!p.slotOn && !p.gate, "separable transform: slot masks need 2 nm nw <= 384");
+  PHX_REQUIRE(!p.slotOn && !p.gate, "separable transform: slot masks need nm nw <= 192");
   const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
                                size_t(kGMax) * kCMax * 4 * sizeof(double));
   PHX_REQUIRE(ldsX <= 160 * 1024, "separable transform: LDS budget exceeded");

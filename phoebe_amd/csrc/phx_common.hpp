@@ -86,4 +86,17 @@ __host__ __device__ inline int mesh_diff_index(int i2, int i1, int m0, int m1, i
   return dz * m0 * m1 + dy * m0 + dx;
 }
 
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet, infinities exact).
+__host__ __device__ inline short bf16_bits(float x) {
+  union { float f; unsigned u; } v{x};
+  unsigned u = v.u;
+  if ((u & 0x7f800000u) == 0x7f800000u) return short((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+__host__ __device__ inline float bf16_value(short h) {
+  union { unsigned u; float f; } v{unsigned(static_cast<unsigned short>(h)) << 16};
+  return v.f;
+}
+
 }  // namespace phx

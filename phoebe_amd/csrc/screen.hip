@@ -14,15 +14,6 @@ namespace phx {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 bits, round to nearest even (NaN kept quiet, infinities exact).
-__host__ __device__ inline short to_bf16(float x) {
-  union { float f; unsigned u; } v{x};
-  unsigned u = v.u;
-  if ((u & 0x7f800000u) == 0x7f800000u) return short((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return short(u >> 16);
-}
-
 // ------------------------------------------------------------------ GEMM
 // Workgroup tile 64 x 64: wave w owns rows 16w..16w+15 and four 16-column
 // tiles.  v_mfma_f32_16x16x16_bf16: A lane l holds A[l%16][4(l/16)+j], B lane l
@@ -41,7 +32,7 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(MlpGemm g) {
       const int k = k0 + 4 * lg + j;
       float v = 0.f;
       if (m < g.M && k < g.K) v = g.ta ? g.A[size_t(k) * g.lda + m] : g.A[size_t(m) * g.lda + k];
-      a[j] = to_bf16(v);
+      a[j] = bf16_bits(v);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -52,7 +43,7 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(MlpGemm g) {
         const int k = k0 + 4 * lg + j;
         float v = 0.f;
         if (n < g.N && k < g.K) v = g.tb ? g.B[size_t(n) * g.ldb + k] : g.B[size_t(k) * g.ldb + n];
-        b[j] = to_bf16(v);
+        b[j] = bf16_bits(v);
       }
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
     }

@@ -173,8 +173,9 @@ def test_separable_k2_range_and_subsets():
     assert rel_err(got[mask], ref[mask]).max() < 1e-10
 
 
+@pytest.mark.parametrize("nsigma", [6.0, 7.0])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
-def test_energy_screen_keeps_parity(precision, tol):
+def test_energy_screen_keeps_parity(precision, tol, nsigma):
     """Rows skipped by the 7-sigma energy screen carry only Gaussian tails below
     exp(-49) of their peak: the linewidths still match the oracle."""
     say("energy screen", precision)
@@ -182,7 +183,7 @@ def test_energy_screen_keeps_parity(precision, tol):
     gW = p.gw_host()
     ref = O.Oracle(p, gW.reshape(-1)).linewidths()
     s = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="separable")
-    s.set_screen(7.0)
+    s.set_screen(nsigma)
     s.run()
     err = rel_err(s.linewidths(), ref)
     say("  max rel err", err.max())
@@ -250,3 +251,20 @@ def test_line_batching_argument_validation():
     with pytest.raises(P.PhxError):
         s.set_panel_chunk(0)
     s.set_line_batching(0)
+
+
+@pytest.mark.parametrize("mesh", [(5, 5, 5), (6, 4, 5)])
+def test_split_bf16x3_separable_tracks_fp64(mesh):
+    """Split-BF16x3 x-stage: linewidths within the split-BF16 error level of FP64."""
+    say("bf16x3", mesh)
+    p = synth.make_problem(2, 6, 3, 3, mesh, seed=21, name="bf3")
+    gW = p.gw_host()
+    ref = O.Oracle(p, gW.reshape(-1)).linewidths()
+    s = P.ElectronLinewidths(p, gW=gW, precision="bf16x3", transform="separable")
+    s.set_screen(6.0)
+    s.run()
+    err = rel_err(s.linewidths(), ref)
+    say("  max rel err", err.max())
+    assert err.max() < 1e-3
+    with pytest.raises(P.PhxError):
+        P.ElectronLinewidths(p, gW=gW, precision="bf16x3", transform="direct")
