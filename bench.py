@@ -74,22 +74,46 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
     lw.sync()
     elapsed = time.perf_counter() - t0
     kt = lw.kernel_times()
+    st = lw.stage_times()
     gam = lw.linewidths(average=False)
     log(f"[{form}] {len(k1_batches)} run calls in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
-    return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, linewidths=gam,
-                    k1=int(sum(len(b) for b in k1_batches)))
+    return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, stage=st, linewidths=gam,
+                    k1=int(sum(len(b) for b in k1_batches)), active=lw.num_active)
 
 
-def roofline(kt, form):
+def traffic_per_dispatch(kernel, workload):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_dispatch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def roofline(res, form, workload):
+    """Dominant kernel: k_coupling (direct form) or k_sep_xm (separable form),
+    algorithmic FLOPs of its transform GEMMs over the rows it computed, per
+    launch, against the MFMA peak of the precision."""
     precision, transform = FORMS[form][:2]
-    ach = kt["coupling_flops"] / (kt["coupling_ms"] * 1e-3) / 1e12 if kt["coupling_ms"] > 0 else 0.0
     peak = PEAK_TFLOPS[precision]
-    kern = "k_coupling" if transform == "direct" else "k_sep_z + k_sep_y + k_sep_x"
-    return {"bound": "mfma" if transform == "direct" else "valu", "kernel": kern,
-            "achieved": round(ach, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(ach / peak, 4), "traffic": None,
-            "avg_launch_ms": round(kt["coupling_ms"] / max(kt["batches"], 1), 4)}
+    if transform == "direct":
+        kt = res["kernels"]
+        ms, launches, flops = kt["coupling_ms"], max(kt["batches"], 1), kt["coupling_flops"]
+        kern = "k_coupling"
+    else:
+        st = res["stage"]
+        ms, launches, flops = st["ms"], max(st["launches"], 1), st["flops"]
+        kern = "k_sep_xm"
+    ach = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    return {"bound": "mfma", "kernel": kern, "achieved": round(ach, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(ach / peak, 4),
+            "traffic": traffic_per_dispatch(kern, workload),
+            "avg_launch_ms": round(ms / launches, 4), "launches": launches,
+            "flops_per_launch": flops / launches}
 
 
 def cpu_baseline(lw64, prob, k1_batches, gam64, k1_count):
@@ -134,6 +158,8 @@ def main():
     ap.add_argument("--chosen", default="line32s6", choices=sorted(k for k in FORMS if k != "fp64"))
     ap.add_argument("--cpu-k1", type=int, default=1, help="outer points in the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-chosen", action="store_true",
+                    help="skip the plain FP64 run and the CPU baseline (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k1 sharding "
@@ -165,27 +191,33 @@ def main():
         return [timed[i:i + B] for i in range(0, len(timed), B)], [warm[i * B:(i + 1) * B] for i in range(warmup)]
 
     # plain FP64
-    lw64, r64 = run_form(P, prob, "fp64", make_batches, args.warmup, B)
-    na = lw64.num_active
-    batches, _ = make_batches(lw64.active, False, 0)
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            cpu = cpu_baseline(lw64, prob, batches, r64["linewidths"], args.cpu_k1)
-        except Exception as e:  # reported, not fatal for the GPU numbers
-            log(f"[cpu] baseline failed: {e}")
-            cpu = {"error": str(e)}
-    rf64 = roofline(r64["kernels"], "fp64")
-    del lw64
+    r64, rf64, cpu = None, None, None
+    if not args.only_chosen:
+        lw64, r64 = run_form(P, prob, "fp64", make_batches, args.warmup, B)
+        batches, _ = make_batches(lw64.active, False, 0)
+        if not args.no_cpu_baseline:
+            try:
+                cpu = cpu_baseline(lw64, prob, batches, r64["linewidths"], args.cpu_k1)
+            except Exception as e:  # reported, not fatal for the GPU numbers
+                log(f"[cpu] baseline failed: {e}")
+                cpu = {"error": str(e)}
+        rf64 = roofline(r64, "fp64", args.config)
+        del lw64
 
     lwc, rc = run_form(P, prob, args.chosen, make_batches, args.warmup, B)
     del lwc
-    # per-linewidth error of the chosen form against FP64, over the processed outer states
-    g64, gc = r64["linewidths"], rc["linewidths"]
-    mask = g64 != 0
-    relerr = np.abs(gc[mask] - g64[mask]) / np.abs(g64[mask])
-    v64 = r64["pairs"] / r64["seconds"]
+    # HBM reference on this GPU: a plain 4 GiB device copy (8 GiB of traffic)
+    copy_ms = P.stream_copy_ms(4 << 30, reps=5)
+    copy_gbps = 2 * (4 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"[copy] 4 GiB device copy {copy_ms:.3f} ms: {copy_gbps:.0f} GB/s")
     vc = rc["pairs"] / rc["seconds"]
+    v64, relerr = None, None
+    if r64 is not None:
+        # per-linewidth error of the chosen form against FP64, over the processed outer states
+        g64, gc = r64["linewidths"], rc["linewidths"]
+        mask = g64 != 0
+        relerr = np.abs(gc[mask] - g64[mask]) / np.abs(g64[mask])
+        v64 = r64["pairs"] / r64["seconds"]
     out = {
         "metric": METRIC,
         "value": vc,
@@ -204,17 +236,18 @@ def main():
                                f"{prob.mesh[0]}^3 k/q mesh, 0.5 eV window",
                    "k1_per_step": int(rc["k1"] // args.steps) if FORMS[args.chosen][3] else B,
                    "k2_per_k1": int(rc["pairs"] // rc["k1"]),
-                   "active_k": int(na), "chosen_form": args.chosen,
+                   "active_k": int(rc["active"]), "chosen_form": args.chosen,
                    "chosen_precision": FORMS[args.chosen][0],
                    "chosen_transform": FORMS[args.chosen][1],
                    "chosen_screen_sigma": FORMS[args.chosen][2],
                    "chosen_line_batching": FORMS[args.chosen][3]},
-        "plain_fp64": {"value": v64, "ms_per_step": r64["seconds"] * 1e3 / args.steps,
-                       "roofline": rf64},
-        "speedup_vs_fp64": vc / v64,
-        "max_linewidth_rel_err_vs_fp64": float(relerr.max()) if relerr.size else None,
-        "linewidths_checked": int(relerr.size),
-        "roofline": roofline(rc["kernels"], args.chosen),
+        "plain_fp64": None if r64 is None else {
+            "value": v64, "ms_per_step": r64["seconds"] * 1e3 / args.steps, "roofline": rf64},
+        "speedup_vs_fp64": None if v64 is None else vc / v64,
+        "max_linewidth_rel_err_vs_fp64": float(relerr.max()) if relerr is not None and relerr.size else None,
+        "linewidths_checked": 0 if relerr is None else int(relerr.size),
+        "roofline": roofline(rc, args.chosen, args.config),
+        "hbm_copy_GBps": copy_gbps,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -235,6 +235,14 @@ int phx_mlp_num_params(void);
  * row-major; t: n targets), computed by the GPU forward / backward kernels. */
 int phx_mlp_gradients(const float* params, const float* X, const float* t, int n, float* grads,
                       float* loss, float* y);
+/* Timing of the separable transform's last stage (k_sep_xm), with HIP events
+ * around each of its launches while timing is on: out4 = {milliseconds,
+ * launches, rows computed (after the screens), FLOPs of its mesh-transform
+ * GEMMs for those rows = 8 * rows * groups * nm * nw}. */
+int phx_lw_stage_times(phx_lw* h, double* out4);
+/* A plain device-to-device copy of `bytes` (float4 grid-stride kernel), timed
+ * over `reps` launches: the HBM reference of the roofline. */
+int phx_stream_copy(unsigned long long bytes, int reps, double* ms_per_copy);
 /* Per-kernel timing with HIP events recorded on the handle's stream around
  * every batch; turning it on (or off) clears the accumulators. */
 int phx_lw_set_timing(phx_lw* h, int on);

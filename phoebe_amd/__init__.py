@@ -94,6 +94,8 @@ EXPORTS = {
     "phx_lw_screen_reset_stats": (C.c_int, [C.c_void_p]),
     "phx_lw_screen_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "phx_mlp_num_params": (C.c_int, []),
+    "phx_lw_stage_times": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phx_stream_copy": (C.c_int, [C.c_ulonglong, C.c_int, C.c_void_p]),
     "phx_mlp_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
     "phx_lw_kernel_times": (C.c_int, [C.c_void_p, _dp]),
@@ -176,6 +178,13 @@ def ph_bloch_diag(qc, R, w, mat2R, masses, mass_scaling=True):
                                    _p(_f64(mat2R)), _p(_f64(masses)), int(bool(mass_scaling)),
                                    _p(om), _p(ev)))
     return om, ev.view(np.complex128).reshape(nq, nm, nm).transpose(0, 2, 1)
+
+
+def stream_copy_ms(nbytes, reps=5):
+    """Average milliseconds of a plain device copy of nbytes (HBM reference)."""
+    out = np.zeros(1)
+    _check(lib().phx_stream_copy(int(nbytes), int(reps), _p(out)))
+    return float(out[0])
 
 
 def mlp_gradients(params, X, t):
@@ -418,6 +427,12 @@ class ElectronLinewidths:
 
     def set_timing(self, on=True):
         _check(lib().phx_lw_set_timing(self._h, int(bool(on))))
+
+    def stage_times(self):
+        """Last separable stage: dict(ms, launches, rows, flops)."""
+        out = np.zeros(4)
+        _check(lib().phx_lw_stage_times(self._h, _p(out)))
+        return dict(ms=out[0], launches=int(out[1]), rows=int(out[2]), flops=out[3])
 
     def kernel_times(self):
         """dict: cache_ms, coupling_ms, collect_ms, batches, coupling_flops."""

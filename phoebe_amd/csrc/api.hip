@@ -534,9 +534,16 @@ struct LwHandle {
   double kernelMs[3] = {0., 0., 0.};
   long long kernelLaunches = 0;
   double couplingFlops = 0.;  // algorithmic FLOPs of the coupling launches timed
+  // last-stage launches of the separable transform: event pairs, summed at the next sync
+  std::vector<hipEvent_t> xev;
+  size_t xevUsed = 0;
+  double xMs = 0.;
+  long long xLaunches = 0;
+  DevBuf<unsigned long long> xKept;  // rows computed by the last stage (timed launches)
   ~LwHandle() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : xev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -553,6 +560,13 @@ static void lw_harvest_timing(LwHandle* L) {
   L->kernelMs[2] += c;
   L->kernelLaunches += 1;
   L->evPending = false;
+  for (size_t i = 0; i + 1 < L->xevUsed; i += 2) {
+    float x = 0.f;
+    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 1]));
+    L->xMs += x;
+    L->xLaunches += 1;
+  }
+  L->xevUsed = 0;
 }
 
 static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch,
@@ -1342,9 +1356,23 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
       const bool f64 = L->precision == PHX_FP64;
       if (f64) sp.B = L->Bd.p + size_t(p0) * pstride;
       else sp.B = L->Bf.p + size_t(p0) * pstride;
-      auto xstage = [&](const CoupParams& c) {
+      auto xstage = [&](const CoupParams& c0) {
+        CoupParams c = c0;
+        if (L->timing) {
+          while (L->xev.size() < L->xevUsed + 2) {
+            hipEvent_t e;
+            PHX_HIP(hipEventCreate(&e));
+            L->xev.push_back(e);
+          }
+          c.keptRows = L->xKept.p;
+          PHX_HIP(hipEventRecord(L->xev[L->xevUsed], s));
+        }
         if (f64) launch_separable_x<double>(sp, c, true, s);
         else launch_separable_x<float>(sp, c, true, s);
+        if (L->timing) {
+          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 1], s));
+          L->xevUsed += 2;
+        }
       };
       if (f64) launch_separable_zy<double>(sp, L->maxLineE, L->maxGrpE, s);
       else launch_separable_zy<float>(sp, L->maxLineE, L->maxGrpE, s);
@@ -1960,6 +1988,12 @@ int phx_lw_set_timing(phx_lw* h, int on) {
     h->kernelMs[0] = h->kernelMs[1] = h->kernelMs[2] = 0.;
     h->kernelLaunches = 0;
     h->couplingFlops = 0.;
+    h->xMs = 0.;
+    h->xLaunches = 0;
+    h->xevUsed = 0;
+    if (!h->xKept.p) h->xKept.alloc(1);
+    launch_fill<int>(reinterpret_cast<int*>(h->xKept.p), 2, 0, h->stream);
+    PHX_HIP(hipStreamSynchronize(h->stream));
   });
 }
 
@@ -1973,6 +2007,50 @@ int phx_lw_kernel_times(phx_lw* h, double* out5) {
     out5[2] = h->kernelMs[2];
     out5[3] = double(h->kernelLaunches);
     out5[4] = h->couplingFlops;
+  });
+}
+
+int phx_lw_stage_times(phx_lw* h, double* out4) {
+  return guard([&] {
+    PHX_REQUIRE(h && out4, "null argument");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    lw_harvest_timing(h);
+    unsigned long long kept = 0;
+    if (h->xKept.p) PHX_HIP(hipMemcpy(&kept, h->xKept.p, sizeof(kept), hipMemcpyDeviceToHost));
+    out4[0] = h->xMs;
+    out4[1] = double(h->xLaunches);
+    out4[2] = double(kept);
+    // complex multiply-adds of the last-stage transform for the rows computed
+    out4[3] = 8. * double(kept) * h->nG0 * double(h->nm) * h->nw;
+  });
+}
+
+__global__ void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
+int phx_stream_copy(unsigned long long bytes, int reps, double* ms_per_copy) {
+  return guard([&] {
+    PHX_REQUIRE(bytes >= 16 && reps >= 1 && ms_per_copy, "phx_stream_copy: bad arguments");
+    const size_t n = size_t(bytes / 16);
+    DevBuf<float4> a(n), b(n);
+    hipStream_t s = 0;
+    launch_fill<float>(reinterpret_cast<float*>(a.p), 4 * n, 1.f, s);
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, s, a.p, b.p, n);
+    PHX_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, s, a.p, b.p, n);
+    PHX_HIP(hipEventRecord(e1, s));
+    PHX_HIP(hipEventSynchronize(e1));
+    PHX_HIP(hipGetLastError());
+    float ms = 0.f;
+    PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = double(ms) / reps;
   });
 }
 

@@ -70,6 +70,7 @@ struct CoupParams {
   const unsigned char* slotOn;            // compute only slots with slotOn != 0 (null: all)
   int slotZero;                           // off slots: 1 = write zero partials, 0 = leave untouched
   const int* gate;                        // run only when *gate != 0 (null: always)
+  unsigned long long* keptRows;           // rows computed by the last stage are added here (null: off)
 };
 
 template <typename T> struct Vec2;
@@ -943,6 +944,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
     nkeep += tot;
     __syncthreads();
   }
+  if (t == 0 && p.keptRows && nkeep > 0) atomicAdd(p.keptRows, (unsigned long long)nkeep);
   if (nkeep == 0) {
     if (REDUCE) {
       double part[kGMax][CM];
