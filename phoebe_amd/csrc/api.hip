@@ -527,6 +527,8 @@ struct LwHandle {
   double screen = 0.;
   DevBuf<double> qWlo, qWhi;
   DevBuf<double2> rowE2range;  // per active point: lowest and highest windowed band energy
+  DevBuf<int4> meshRow;        // per mesh point: (active row or -1, nb, U pool offset, state offset)
+  DevBuf<unsigned long long> keepMask;  // last-stage row masks of one panel chunk
   // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -886,6 +888,10 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     std::vector<int> m2r(nk, -1);
     for (int a = 0; a < na; ++a) m2r[L->active[a]] = a;
     h2d(L->meshToRow, m2r, s);
+    std::vector<int4> mr(nk, make_int4(-1, 0, 0, 0));
+    for (int a = 0; a < na; ++a) mr[L->active[a]] = make_int4(a, L->nb[a], L->u2off[a], L->stateOff[a]);
+    h2d(L->meshRow, mr, s);
+    PHX_HIP(hipStreamSynchronize(s));
   }
   // ---- R_e grouped by n0 mod m0 for the mesh-line form of cacheElPh
   {
@@ -1328,6 +1334,11 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.qWhi = L->qWhi.p;
     cq.rowE2range = L->rowE2range.p;
     cq.bf16x3 = L->precision == PHX_BF16X3;
+    cq.meshRow = L->meshRow.p;
+    if (p.mesh[0] <= 64) {
+      L->keepMask.reserve(size_t(std::min(nPanels, L->panelChunk)) * nMtiles);
+      cq.keepMask = L->keepMask.p;
+    }
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
     if (L->precision == PHX_FP64) {

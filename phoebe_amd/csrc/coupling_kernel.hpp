@@ -71,6 +71,9 @@ struct CoupParams {
   int slotZero;                           // off slots: 1 = write zero partials, 0 = leave untouched
   const int* gate;                        // run only when *gate != 0 (null: always)
   unsigned long long* keptRows;           // rows computed by the last stage are added here (null: off)
+  // precomputed row lists of the last stage (m0 <= 64)
+  unsigned long long* keepMask;           // per slot: bit jx set = row jx kept (null: screen in k_sep_xm)
+  const int4* meshRow;                    // per mesh point: (row, nb2, u2off, stateOff), row -1 if inactive
 };
 
 template <typename T> struct Vec2;
@@ -894,12 +897,50 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   }
   for (int i = t; i < nG0; i += 256) sg0[i] = p.g0val[i];
 
+  const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
+                 (size_t(jy) * p.m2 + jz) * ncol;
+  const size_t gStride = size_t(p.m1) * p.m2 * ncol;  // T2v elements between groups
+  const int KS = (2 * nG0 + 3) / 4;                    // k-steps of 4 real K
+  // B operands of every k-step, loaded once for all tiles when they fit in registers
+  constexpr int KSR = (BF3 || sizeof(T) != 4) ? 1 : 12;  // FP32: up to 24 x-groups
+  const bool bReg = !BF3 && KS <= KSR;
+  T bAll[KSR][kXmNt];
+  if (bReg) {
+#pragma unroll
+    for (int q = 0; q < KSR; ++q) {
+      const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+      const bool gOk = g < nG0 && q < KS;
+#pragma unroll
+      for (int j = 0; j < kXmNt; ++j) {
+        const int c = (wave * kXmNt + j) * 16 + lr;
+        T v = T(0);
+        if (gOk && c < ncol) {
+          const T2v x = X[size_t(g) * gStride + c];
+          v = kc ? x.y : x.x;
+        }
+        bAll[q][j] = v;
+      }
+    }
+  }
+
   // ---- rows of the line, energy screen, compaction (ascending jx)
   int nkeep = 0;
+  if (p.keepMask) {
+    // kept rows from the screen kernel: one load per kept row
+    const unsigned long long mask = p.keepMask[slot];
+    nkeep = __popcll(mask);
+    for (int jx = t; jx < p.m0; jx += 256)
+      if ((mask >> jx) & 1ull) {
+        const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
+        const int4 mr = p.meshRow[mesh];
+        rows[__popcll(mask & ((1ull << jx) - 1ull))] =
+            SepXmRow{jx, mr.x, mesh_diff_index(mesh, kkMesh, p.m0, p.m1, p.m2), mr.y, mr.z, mr.w};
+      }
+    __syncthreads();
+  } else
   for (int j0 = 0; j0 < p.m0; j0 += 256) {
     const int jx = j0 + t;
-    SepXmRow ri{jx, -1, 0, 0, 0, 0};
-    if (jx < p.m0) {
+    SepXmRow ri{jx, -1, 0, 0, 0, 0};    if (jx < p.m0) {
       const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
       ri.iq = mesh_diff_index(mesh, kkMesh, p.m0, p.m1, p.m2);
       double wlo = 0., whi = 0.;
@@ -970,15 +1011,33 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
   const double* e1 = p.e1pool + p.kkStateOff[kk] + gstart;
   const double inv_g = 1. / double(float(gsize));
-  const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
-                 (size_t(jy) * p.m2 + jz) * ncol;
-  const size_t gStride = size_t(p.m1) * p.m2 * ncol;  // T2v elements between groups
-  const int KS = (2 * nG0 + 3) / 4;                    // k-steps of 4 real K
 
   for (int m0r = 0; m0r < nkeep; m0r += 16) {
     const int nr = min(16, nkeep - m0r);
     const bool rowOk = lr < nr;
     const T2v* wrow = Wk + size_t(m0r + (rowOk ? lr : 0)) * nG0p;
+    // phonon eigenvector column, Bose factors of the epilogue row (r, mode sub)
+    T2v e3c[NMX];
+    double n3[CM];
+    auto load_e3 = [&](int pr) {
+      const int r = pr + t / TPR, sub = t % TPR;
+#pragma unroll
+      for (int c = 0; c < CM; ++c) n3[c] = 0.;
+      if (r < nr && sub < nm) {
+        const int iq = rows[m0r + r].iq;
+        const double2* e3 = p.E3pool + size_t(iq) * nm * nm + size_t(nm) * sub;
+#pragma unroll
+        for (int nu = 0; nu < NMX; ++nu)
+          if (nu < nm) {
+            const double2 d = e3[nu];
+            e3c[nu] = T2v{T(d.x), T(d.y)};
+          }
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+          if (c < p.nCalc) n3[c] = p.bose[(size_t(iq) * nm + sub) * p.nCalc + c];
+      }
+    };
+    load_e3(0);
     // Cr = A1 B, Ci = A2 B with B(2g + s, c) = (Xr, Xi)(g, c), A1(m, 2g + s) = (Wr, -Wi),
     // A2(m, 2g + s) = (Wi, Wr); wave owns complex column tiles wave*kXmNt .. +kXmNt
     acc_t accr[kXmNt], acci[kXmNt];
@@ -1024,6 +1083,23 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2h, bh, acci[j], 0, 0, 0);
           acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2h, bl, acci[j], 0, 0, 0);
           acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2l, bh, acci[j], 0, 0, 0);
+        }
+      }
+    } else if (bReg) {
+      sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, sU2, se2, sw3);
+#pragma unroll
+      for (int q = 0; q < KSR; ++q) {
+        if (q >= KS) break;
+        const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+        const bool gOk = g < nG0;
+        const T2v w = wrow[gOk ? g : nG0];
+        const bool on = rowOk && gOk;
+        const T a1 = on ? (kc ? -w.y : w.x) : T(0);
+        const T a2 = on ? (kc ? w.x : w.y) : T(0);
+#pragma unroll
+        for (int j = 0; j < kXmNt; ++j) {
+          accr[j] = Mfma4<T>::mac(a1, bAll[q][j], accr[j]);
+          acci[j] = Mfma4<T>::mac(a2, bAll[q][j], acci[j]);
         }
       }
     } else
@@ -1078,30 +1154,15 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
     }
     __syncthreads();
 
-    // ---- epilogue, RPP rows per pass
+    // ---- epilogue, RPP rows per pass (the first pass's phonon data was prefetched)
     for (int pr = 0; pr < 16; pr += RPP) {
       const int r = pr + t / TPR, sub = t % TPR, rl = t / TPR;
       const bool rowOn = r < nr;
       const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
       const int nb2 = rowOn ? ri.nb2 : 0;
       const bool modeOn = rowOn && sub < nm;
-      T2v e3c[NMX];
-      double w3 = 0., n3[CM];
-#pragma unroll
-      for (int c = 0; c < CM; ++c) n3[c] = 0.;
-      if (modeOn) {
-        const double2* e3 = p.E3pool + size_t(ri.iq) * nm * nm + size_t(nm) * sub;
-#pragma unroll
-        for (int nu = 0; nu < NMX; ++nu)
-          if (nu < nm) {
-            const double2 d = e3[nu];
-            e3c[nu] = T2v{T(d.x), T(d.y)};
-          }
-        w3 = sw3[r * nm + sub];
-#pragma unroll
-        for (int c = 0; c < CM; ++c)
-          if (c < nCalc) n3[c] = p.bose[(size_t(ri.iq) * nm + sub) * nCalc + c];
-      }
+      if (pr > 0) load_e3(pr);
+      const double w3 = modeOn ? sw3[r * nm + sub] : 0.;
       // g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2), thread nu = sub
       if (modeOn) {
         const T* gr = g3 + r * LDG + sub * NWP;
@@ -1230,6 +1291,44 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   if (REDUCE) reduce_partials<CM>(p, slot, smem, part);
 }
 
+// Row lists of the last stage: for every (panel, yz) slot, bit jx of
+// keepMask[slot] is set when mesh point (jx, jy, jz) is an inner point of the
+// range and passes the energy screen (same interval test as k_sep_x; every
+// valid row when the screen is off).  One thread per slot, m0 <= 64.
+static __global__ void __launch_bounds__(256) k_sep_screen(CoupParams p) {
+  const int nLinesYZ = p.m1 * p.m2;
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= p.nPanels * nLinesYZ) return;
+  const int panel = slot / nLinesYZ, yz = slot % nLinesYZ;
+  const int jy = yz % p.m1, jz = yz / p.m1;
+  const int kk = p.panelKK[panel];
+  const int kmesh = p.kkMesh[kk];
+  const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
+  const int gsz = p.panelGSize[panel];
+  const double lim = p.screen / p.invSigma;
+  unsigned long long mask = 0ull;
+  for (int jx = 0; jx < p.m0; ++jx) {
+    const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
+    const int row = p.meshRow[mesh].x;
+    if (row < p.rowLo || row >= p.rowHi) continue;
+    bool keep = true;
+    if (p.screen > 0.) {
+      const int iq = mesh_diff_index(mesh, kmesh, p.m0, p.m1, p.m2);
+      const double wlo = p.qWlo[iq], whi = p.qWhi[iq];
+      const double2 er = p.rowE2range[row];
+      keep = false;
+      if (whi >= wlo)  // at least one mode above the cutoff
+        for (int gi = 0; gi < gsz; ++gi) {
+          const double a_lo = e1[gi] - er.y + wlo, a_hi = e1[gi] - er.x + whi;
+          const double b_lo = e1[gi] - er.y - whi, b_hi = e1[gi] - er.x - wlo;
+          keep |= (a_hi > -lim && a_lo < lim) || (b_hi > -lim && b_lo < lim);
+        }
+    }
+    if (keep) mask |= 1ull << jx;
+  }
+  p.keepMask[slot] = mask;
+}
+
 inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb2max, int rpp) {
   auto up = [](size_t b) { return (b + 15) / 16 * 16; };
   size_t b = up(size_t(m0) * sizeof(SepXmRow) + (8 + nG0) * sizeof(int));
@@ -1274,6 +1373,11 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
   if (sp.ncol <= 4 * kXmNt * 16) {
     const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8);
     PHX_REQUIRE(ldsXm <= 160 * 1024, "separable transform: LDS budget exceeded");
+    if (p.keepMask) {
+      PHX_REQUIRE(p.m0 <= 64 && p.meshRow, "separable transform: row masks need m0 <= 64");
+      hipLaunchKernelGGL(k_sep_screen, dim3(ceil_div(tiles, 256)), dim3(256), 0, s, p);
+      PHX_HIP(hipGetLastError());
+    }
     auto go = [&](auto kern) {
       PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsXm)));
