@@ -808,6 +808,13 @@ struct SepXmRow {            // per kept row, staged in LDS
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 
+// Orders LDS accesses between the lanes of one wavefront (no block barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row data of a 16-row tile into LDS: U2 (16 x nw x nb2max), e2 (16 x nb2max), omega (16 x nm).
 __device__ inline void sep_xm_stage_rows(const CoupParams& p, const SepXmRow* rows, int m0r, int nr,
                                          int nw, int nm, double2* sU2, double* se2, double* sw3) {
@@ -884,7 +891,6 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   unsigned char* dyn = reinterpret_cast<unsigned char*>(Wk + size_t(p.m0) * nG0p);
   dyn = smem + (size_t(dyn - smem) + 15) / 16 * 16;
   T* g3 = reinterpret_cast<T*>(dyn);                                           // 16 x LDG
-  double* gs = reinterpret_cast<double*>(dyn);                                 // aliases g3 (RPP x nm x nb2max)
   double2* sU2 = reinterpret_cast<double2*>(dyn + (size_t(16) * LDG * sizeof(T) + 15) / 16 * 16);  // 16 x nw x nb2max
   double2* gp = sU2 + size_t(16) * nw * p.nb2max;                              // RPP x nm x nb2max
   double* se2 = reinterpret_cast<double*>(gp + size_t(RPP) * nm * p.nb2max);   // 16 x nb2max
@@ -1163,7 +1169,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
       const bool modeOn = rowOn && sub < nm;
       if (pr > 0) load_e3(pr);
       const double w3 = modeOn ? sw3[r * nm + sub] : 0.;
-      // g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2), thread nu = sub
+      // g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2), thread nu = sub.  Each wave
+      // owns whole rows from here on: only wave-level ordering is needed.
       if (modeOn) {
         const T* gr = g3 + r * LDG + sub * NWP;
         for (int b2 = 0; b2 < nb2; ++b2) {
@@ -1177,7 +1184,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           gp[(rl * nm + sub) * p.nb2max + b2] = make_double2(sx, sy);
         }
       }
-      __syncthreads();
+      wave_sync();
       // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub; |g|^2 in registers
       double v[kSepNb2];
 #pragma unroll
@@ -1243,15 +1250,17 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
             s0 += deg;
           }
         }
-        if (__syncthreads_or(mydeg > 1)) {  // g3 is dead here: gs aliases it
+        if (__any(mydeg > 1)) {  // g' of the row is dead here: its slots hold |g|^2
+          double* gs = reinterpret_cast<double*>(gp);
+          wave_sync();
           if (modeOn)
-            for (int b2 = 0; b2 < nb2; ++b2) gs[(rl * nm + sub) * p.nb2max + b2] = v[b2];
-          __syncthreads();
+            for (int b2 = 0; b2 < nb2; ++b2) gs[2 * ((rl * nm + sub) * p.nb2max + b2)] = v[b2];
+          wave_sync();
           if (modeOn && mydeg > 1)
             for (int b2 = 0; b2 < nb2; ++b2) {
               double s2 = 0.;
               for (int i = 0; i < mydeg; ++i)
-                s2 += gs[(rl * nm + mystart + i) * p.nb2max + b2] / double(float(mydeg));
+                s2 += gs[2 * ((rl * nm + mystart + i) * p.nb2max + b2)] / double(float(mydeg));
 #pragma unroll
               for (int b = 0; b < kSepNb2; ++b)
                 if (b == b2) v[b] = s2;
@@ -1285,8 +1294,9 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           }
         }
       }
-      __syncthreads();
+      wave_sync();  // the next pass reuses this wave's g' rows
     }
+    __syncthreads();  // the next tile rewrites g3, U2, e2, omega
   }
   if (REDUCE) reduce_partials<CM>(p, slot, smem, part);
 }
