@@ -869,6 +869,13 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     }
     L->nLines = int(lineE.size());
     L->nG0 = int(grpE.size());
+    {
+      // the z / y stages stage one line / group of panel columns in LDS
+      const size_t ncol = size_t(nm) * nw;
+      PHX_REQUIRE(size_t(L->maxLineE) * ncol * 16 <= 160 * 1024 && size_t(L->maxGrpE) * ncol * 16 <= 160 * 1024 &&
+                      (ncol <= 192 || (size_t(L->nG0) * ncol * (precision == PHX_FP64 ? 16 : 8) <= 96 * 1024)),
+                  "phx_lw_setup: panels too wide for the separable transform (use the direct transform)");
+    }
     h2d(L->lineStart, ls, s);
     h2d(L->lineF2, lf2, s);
     h2d(L->lineRp, lrp, s);

@@ -71,6 +71,7 @@ struct CoupParams {
   int slotZero;                           // off slots: 1 = write zero partials, 0 = leave untouched
   const int* gate;                        // run only when *gate != 0 (null: always)
   unsigned long long* keptRows;           // rows computed by the last stage are added here (null: off)
+  int wideCols;                           // k_coupling_wide: real columns per mode window (multiple of 16)
   // precomputed row lists of the last stage (m0 <= 64)
   unsigned long long* keepMask;           // per slot: bit jx set = row jx kept (null: screen in k_sep_xm)
   const int4* meshRow;                    // per mesh point: (row, nb2, u2off, stateOff), row -1 if inactive
@@ -111,35 +112,47 @@ inline size_t epilogue_lds_bytes(int elem, int ldb, int nm, int nb2max) {
   return size_t(16) * ldb * elem + size_t(16) * nm * nb2max * (sizeof(double2) + sizeof(double));
 }
 
-// Epilogue for the 16 rows whose g3 blocks are in LDS (row r at g3 + r*PWp,
-// (re, im) of column (nu, w2) at 2(nu*nw + w2)).  Thread t serves row t/16;
-// myRow < 0 marks an empty slot.  Must be called by all 256 threads.
-template <typename T, bool REDUCE>
-__device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned char* scratch,
-                           int panel, int kk, int b1p, int myRow, int myIq,
-                           double (&part)[kGMax][kCMax]) {
+// Electron rotation of the modes nuLo .. nuHi-1 for the 16 rows whose g3
+// blocks are in LDS (row r at g3 + r*PWp, (re, im) of column (nu, w2) at
+// 2((nu - nuLo)*nw + w2)) into g'(nu, b2) in scratch.  Thread t serves row t/16.
+template <typename T>
+__device__ void epilogue16_u2(const CoupParams& p, const T* g3, int PWp, unsigned char* scratch,
+                              int myRow, int nuLo, int nuHi) {
   const int t = threadIdx.x, r = t >> 4, sub = t & 15;
-  const int nm = p.nm, nw = p.nw;
+  const int nw = p.nw;
+  const int stride = p.nm * p.nb2max;
+  double2* gp = reinterpret_cast<double2*>(scratch) + size_t(r) * stride;
+  const bool valid = myRow >= 0;
+  const int nb2 = valid ? p.rowNb2[myRow] : 0;
+  const T* g = g3 + size_t(r) * PWp;
+  const double2* U2 = p.U2pool + (valid ? p.rowU2off[myRow] : 0);
+  for (int o = sub; o < (nuHi - nuLo) * nb2; o += 16) {
+    const int nl = o / nb2, b2 = o % nb2;
+    double2 s = make_double2(0., 0.);
+    for (int w2 = 0; w2 < nw; ++w2) {
+      const double2 u = U2[w2 + nw * b2];
+      const double xr = double(g[2 * (nl * nw + w2)]), xi = double(g[2 * (nl * nw + w2) + 1]);
+      s.x += u.x * xr + u.y * xi;
+      s.y += u.x * xi - u.y * xr;
+    }
+    gp[(nuLo + nl) * nb2 + b2] = s;
+  }
+}
+
+// Rest of the epilogue once g'(nu, b2) of every mode is in scratch: phonon
+// rotation, |g|^2, then the store or the symmetrization and the weights.
+// Must be called by all 256 threads.
+template <bool REDUCE>
+__device__ void epilogue16_tail(const CoupParams& p, unsigned char* scratch, int panel, int kk, int b1p,
+                                int myRow, int myIq, double (&part)[kGMax][kCMax]) {
+  const int t = threadIdx.x, r = t >> 4, sub = t & 15;
+  const int nm = p.nm;
   const int stride = nm * p.nb2max;
   double2* gp = reinterpret_cast<double2*>(scratch) + size_t(r) * stride;
   double* gs = reinterpret_cast<double*>(reinterpret_cast<double2*>(scratch) + size_t(16) * stride) +
                size_t(r) * stride;
   const bool valid = myRow >= 0;
   const int nb2 = valid ? p.rowNb2[myRow] : 0;
-  const T* g = g3 + size_t(r) * PWp;
-  const double2* U2 = p.U2pool + (valid ? p.rowU2off[myRow] : 0);
-  for (int o = sub; o < nm * nb2; o += 16) {
-    const int nu = o / nb2, b2 = o % nb2;
-    double2 s = make_double2(0., 0.);
-    for (int w2 = 0; w2 < nw; ++w2) {
-      const double2 u = U2[w2 + nw * b2];
-      const double xr = double(g[2 * (nu * nw + w2)]), xi = double(g[2 * (nu * nw + w2) + 1]);
-      s.x += u.x * xr + u.y * xi;
-      s.y += u.x * xi - u.y * xr;
-    }
-    gp[nu * nb2 + b2] = s;
-  }
-  __syncthreads();
   const double2* e3 = p.E3pool + size_t(myIq) * nm * nm;
   for (int o = sub; o < nm * nb2; o += 16) {
     const int nu2 = o / nb2, b2 = o % nb2;
@@ -224,6 +237,18 @@ __device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned c
       }
     }
   }
+}
+
+// Epilogue for the 16 rows whose full g3 rows are in LDS (row r at g3 + r*PWp,
+// (re, im) of column (nu, w2) at 2(nu*nw + w2)).  Thread t serves row t/16;
+// myRow < 0 marks an empty slot.  Must be called by all 256 threads.
+template <typename T, bool REDUCE>
+__device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned char* scratch,
+                           int panel, int kk, int b1p, int myRow, int myIq,
+                           double (&part)[kGMax][kCMax]) {
+  epilogue16_u2<T>(p, g3, PWp, scratch, myRow, 0, p.nm);
+  __syncthreads();
+  epilogue16_tail<REDUCE>(p, scratch, panel, kk, b1p, myRow, myIq, part);
 }
 
 // Block reduction of the per-thread partial linewidths into partial[L]
@@ -381,6 +406,102 @@ __global__ void __launch_bounds__(256, 2) k_coupling(CoupParams p) {
   if (REDUCE) reduce_partials<kCMax>(p, L, smem, part);
 }
 
+// Wide panels (2 nm nw > 576): 16-row tiles, the panel columns in windows of
+// whole modes (<= 384 real columns), each window's MFMA product rotated by U2
+// into g'(nu, b2) before the next; the phonon rotation and the weights run once
+// all modes are in.
+constexpr int kWideNtw = 6;
+template <typename T, bool REDUCE>
+__global__ void __launch_bounds__(256, 1) k_coupling_wide(CoupParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NTW = kWideNtw;
+  const int WCOL = p.wideCols;  // <= 4 * NTW * 16
+  const int nTilesTot = p.nPanels * p.nMtiles;
+  const int bid = blockIdx.x;
+  const int perXcd = gridDim.x / 8;
+  const int L = (bid % 8) * perXcd + bid / 8;
+  if (L >= nTilesTot) return;
+  const int panel = L / p.nMtiles, mtile = L % p.nMtiles;
+  const int row0 = mtile * 16;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lk = lane >> 4;
+  typedef typename Mfma4<T>::acc_t acc_t;
+  constexpr int LDA = kKC + 1, KS = kKC / 4, AE = 16 * kKC / 256;
+  T* As = reinterpret_cast<T*>(smem);                                    // 16 x LDA
+  T* g3 = As + 16 * LDA;                                                 // 16 x WCOL
+  unsigned char* scratch = reinterpret_cast<unsigned char*>(g3 + 16 * WCOL);
+  scratch = smem + (size_t(scratch - smem) + 15) / 16 * 16;
+  const T* A = static_cast<const T*>(p.A) + size_t(row0) * p.lda;
+  const T* B = static_cast<const T*>(p.B) + size_t(panel) * p.panelStride;
+  const int kk = p.panelKK[panel];
+  const int b1p = p.panelB1[panel];
+  const int row = row0 + (t >> 4);
+  const int myRow = row < p.M ? row : -1;
+  const int nuPerWin = max(1, (WCOL / 2) / p.nw);  // WCOL >= 2 nw
+  for (int nu0 = 0; nu0 < p.nm; nu0 += nuPerWin) {
+    const int nu1 = min(p.nm, nu0 + nuPerWin);
+    const int c0 = 2 * nu0 * p.nw, width = 2 * (nu1 - nu0) * p.nw;
+    const int ntiles = (width + 15) / 16;
+    acc_t acc[NTW];
+#pragma unroll
+    for (int j = 0; j < NTW; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[j][i] = 0;
+    for (int kc = 0; kc < p.Kp; kc += kKC) {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < AE; ++e) {
+        const int i = t + 256 * e, r = i / kKC, k = i % kKC;
+        As[r * LDA + k] = A[size_t(r) * p.lda + kc + k];
+      }
+      T b[KS][NTW];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const T* Bk = B + size_t(kc + ks * 4 + lk) * p.ldb + c0 + lr;
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+          const int col = (wave + 4 * j) * 16 + lr;
+          b[ks][j] = (wave + 4 * j < ntiles && col < width) ? Bk[(wave + 4 * j) * 16] : T(0);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const T a = As[lr * LDA + ks * 4 + lk];
+#pragma unroll
+        for (int j = 0; j < NTW; ++j)
+          if (wave + 4 * j < ntiles) acc[j] = Mfma4<T>::mac(a, b[ks][j], acc[j]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NTW; ++j) {
+      const int tile = wave + 4 * j;
+      if (tile < ntiles)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) g3[Mfma4<T>::row(lk, i) * WCOL + tile * 16 + lr] = acc[j][i];
+    }
+    __syncthreads();
+    epilogue16_u2<T>(p, g3, WCOL, scratch, myRow, nu0, nu1);
+  }
+  __syncthreads();
+  int myIq = 0;
+  if (myRow >= 0)
+    myIq = p.qmode ? mesh_diff_index(p.rowMesh[myRow], p.kkMesh[kk], p.m0, p.m1, p.m2) : p.rowE3[myRow];
+  double part[kGMax][kCMax];
+#pragma unroll
+  for (int gi = 0; gi < kGMax; ++gi)
+#pragma unroll
+    for (int c = 0; c < kCMax; ++c) part[gi][c] = 0.;
+  epilogue16_tail<REDUCE>(p, scratch, panel, kk, b1p, myRow, myIq, part);
+  if (REDUCE) reduce_partials<kCMax>(p, L, smem, part);
+}
+
+inline size_t coupling_wide_lds_bytes(int elem, int nm, int nb2max, int wcol) {
+  const size_t a = (size_t(16) * (kKC + 1) + size_t(16) * wcol) * elem;
+  return std::max((a + 15) / 16 * 16 + size_t(16) * nm * nb2max * (sizeof(double2) + sizeof(double)),
+                  size_t(kGMax) * kCMax * 4 * sizeof(double));
+}
+
 inline size_t coupling_lds_bytes(int elem, int MT, int ldb, int nm, int nb2max, bool reduce) {
   (void)reduce;
   const int ROWS = 16 * MT;
@@ -412,12 +533,36 @@ static void dispatch_ntw(int ntw, const CoupParams& p, int grid, size_t lds, hip
   }
 }
 
-inline int coupling_rows_per_tile(int ldb) { return ldb <= 384 ? 32 : 16; }
+inline int coupling_rows_per_tile(int ldb) { return ldb <= 384 ? 32 : 16; }  // wide panels: 16
 
 template <typename T>
 void launch_coupling(const CoupParams& p, bool reduce, hipStream_t s) {
   PHX_REQUIRE(p.nm <= kNmMax, "coupling kernel supports nm <= 36");
   PHX_REQUIRE(p.ldb % 16 == 0 && p.Kp % kKC == 0, "coupling kernel: bad padding");
+  if (p.ldb > 9 * 64) {  // wider than one pass of the tiled kernel
+    PHX_REQUIRE(p.nMtiles * 16 >= p.M, "coupling kernel: wide panels use 16-row tiles");
+    const int tiles = p.nPanels * p.nMtiles;
+    if (tiles <= 0) return;
+    // widest mode window (whole modes, <= 384 real columns) that fits the LDS budget
+    CoupParams q = p;
+    q.wideCols = 0;
+    size_t lds = 0;
+    for (int nuw = std::max(1, 192 / p.nw); nuw >= 1; --nuw) {
+      q.wideCols = int(round_up(size_t(2) * nuw * p.nw, 16));
+      lds = coupling_wide_lds_bytes(sizeof(T), p.nm, p.nb2max, q.wideCols);
+      if (lds <= 160 * 1024) break;
+    }
+    PHX_REQUIRE(lds <= 160 * 1024, "coupling kernel: nm x bands per inner point too large for LDS");
+    auto go = [&](auto kern) {
+      PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+      hipLaunchKernelGGL(kern, dim3(ceil_div(tiles, 8) * 8), dim3(256), lds, s, q);
+    };
+    if (reduce) go(k_coupling_wide<T, true>);
+    else go(k_coupling_wide<T, false>);
+    PHX_HIP(hipGetLastError());
+    return;
+  }
   const int MT = coupling_rows_per_tile(p.ldb) / 16;
   const int ntw = ceil_div(p.ldb / 16, 4);
   const int tiles = p.nPanels * p.nMtiles;

@@ -232,8 +232,12 @@ template <> struct MfmaC<float> {
   __device__ static int row(int lk, int i) { return 4 * lk + i; }
 };
 
+// Column passes of the cacheElPh kernels cover whole w1 blocks: (256 / nw) * nw columns.
+__host__ __device__ inline int cache_pass_cols(int nw) { return (256 / nw) * nw; }
+
 // Panel epilogue shared by the cacheElPh kernels.  Cb holds g1 of up to 16
-// outer points (kbase .. min(kbase+16, kend)-1) for the 256 columns c0 .. c0+255 of block (rp, nu):
+// outer points (kbase .. min(kbase+16, kend)-1) for the columns c0 .. c0+255 of block (rp, nu),
+// of which the pass owns c0 .. c0 + cache_pass_cols(nw) - 1 (whole w1 blocks):
 // real parts in rows 0..15, imaginary parts in rows 16..31.  Applies the U1
 // rotation over the w1 range of those columns and the factor s(k1, Rp), and
 // writes (c0 == 0) or adds (later column passes) the panel entries.  Each
@@ -249,8 +253,8 @@ __device__ __forceinline__ void cache_panel_epilogue(const CacheParams& p, const
     const int nb1 = p.nb1[kk];
     const double2* U1 = p.U1pool + p.U1off[kk];
     const double2 s = p.phP[size_t(kk) * p.nRp + rp];
-    // columns of this pass: w1 in [c0/nw, (c0+256)/nw), all w2
-    const int w1lo = c0 / p.nw, w1hi = min(p.nw, (c0 + 256) / p.nw);
+    // columns of this pass: w1 in [c0/nw, (c0 + pass)/nw), all w2
+    const int w1lo = c0 / p.nw, w1hi = min(p.nw, (c0 + cache_pass_cols(p.nw)) / p.nw);
     for (int o = t; o < nb1 * p.nw; o += 256) {
       const int b1 = o / p.nw, w2 = o % p.nw;
       double2 v = make_double2(0., 0.);
@@ -294,7 +298,7 @@ __global__ void __launch_bounds__(256, 2) k_cache(CacheParams p) {
   T* Cb = reinterpret_cast<T*>(smem);               // epilogue: 2 x 16 x 256
   const int nchunk = (p.nRe + 7) / 8;
 
-  for (int c0 = 0; c0 < E; c0 += 256) {
+  for (int c0 = 0; c0 < E; c0 += cache_pass_cols(p.nw)) {
     acc_t acc[MTc][4];
 #pragma unroll
     for (int m = 0; m < MTc; ++m)
@@ -470,7 +474,7 @@ __global__ void __launch_bounds__(256) k_cache_line_b(CacheParams p) {
   const int E = p.nw * p.nw, t = threadIdx.x;
   T* Cb = reinterpret_cast<T*>(smem);  // 2 x 16 x 256
   const T2* Tg = static_cast<const T2*>(p.Tg);
-  for (int c0 = 0; c0 < E; c0 += 256) {
+  for (int c0 = 0; c0 < E; c0 += cache_pass_cols(p.nw)) {
     const int cl = c0 + t;
     const bool ok = cl < E;
     const size_t colg = (size_t(rp) * p.nm + nu) * E + cl;

@@ -268,3 +268,21 @@ def test_split_bf16x3_separable_tracks_fp64(mesh):
     assert err.max() < 1e-3
     with pytest.raises(P.PhxError):
         P.ElectronLinewidths(p, gW=gW, precision="bf16x3", transform="direct")
+
+
+@pytest.mark.parametrize("natoms,nw", [(8, 14), (12, 18)])
+def test_wide_panels_direct_vs_oracle(natoms, nw):
+    """nm * nw beyond one pass of the tiled kernel (mode windows of the wide kernel)."""
+    say("wide panels", natoms, nw)
+    p = synth.make_problem(natoms, nw, 2, 2, 3, seed=71 + natoms, name="wide")
+    gW = p.gw_host()
+    orc = O.Oracle(p, gW.reshape(-1))
+    ref = orc.linewidths()
+    for precision, tol in (("fp64", 1e-10), ("fp32", 1e-4)):
+        for lines in (0, 1):  # GEMM and mesh-line forms of cacheElPh (nw need not divide 256)
+            lw = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="direct")
+            lw.set_line_batching(lines)
+            assert lw.run() == orc.last_pairs
+            err = rel_err(lw.linewidths(), ref)
+            say("  ", precision, "lines", lines, "states", lw.num_states, "max rel err", err.max())
+            assert err.max() < tol
