@@ -183,6 +183,25 @@ int phx_lw_reset(phx_lw* h);
  * exp(-nsigma^2) of their peak.  0 turns it off (the default); otherwise
  * nsigma >= 6.  No counterpart in the reference, which evaluates every pair. */
 int phx_lw_set_screen(phx_lw* h, double nsigma);
+/* ---- Sweeps without host round trips.  Prepare every batch of a sweep over
+ * k1_list (all active points when null; same batching as phx_lw_run) with
+ * its own device metadata and all scratch sized; returns the batch count.
+ * No counterpart in the reference, whose k1 loop is serial on the host. */
+long long phx_lw_plan(phx_lw* h, int nk1, const int* k1_list, int k2_begin, int k2_end);
+/* Enqueue the prepared sweep on the handle's stream without host
+ * synchronization; graph != 0 captures it into a hipGraph on first use and
+ * replays that graph.  Handles run concurrently on their own streams.
+ * Returns the (k1, k2) pair count. */
+long long phx_lw_plan_run(phx_lw* h, int graph);
+/* Batch shapes: outer points per GEMM-form batch (1..32, the cacheElPh GEMM
+ * M tile is 2 x k1_batch) and mesh lines per line batch (1..4). */
+int phx_lw_set_batching(phx_lw* h, int k1_batch, int lines_per_batch);
+/* Autotuner: times full runs over the sample k1_list for k1 batches {8, 16,
+ * 32}, lines per batch {1, 2, 4} (line batching on) and panels per separable
+ * launch {16, 48, 128}, one knob at a time, keeps the fastest; the
+ * linewidths are restored.  out4 = {k1_batch, lines_per_batch, panel_chunk,
+ * best seconds}.  No counterpart in the reference. */
+int phx_lw_autotune(phx_lw* h, int nk1, const int* k1_list, double* out4);
 /* Mesh-line batching of the outer points: the points of a phx_lw_run call
  * that share a mesh line (equal j1, j2 of k1 = sum_d j_d/m_d b_d) are
  * processed together, up to 64 per batch, when the line holds at least

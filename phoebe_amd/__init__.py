@@ -95,6 +95,10 @@ EXPORTS = {
     "phx_lw_screen_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "phx_mlp_num_params": (C.c_int, []),
     "phx_lw_stage_times": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phx_lw_plan": (C.c_longlong, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int]),
+    "phx_lw_plan_run": (C.c_longlong, [C.c_void_p, C.c_int]),
+    "phx_lw_set_batching": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "phx_lw_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "phx_stream_copy": (C.c_int, [C.c_ulonglong, C.c_int, C.c_void_p]),
     "phx_mlp_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),
@@ -363,6 +367,35 @@ class ElectronLinewidths:
         if pairs < 0:
             raise PhxError(lib().phx_last_error().decode())
         return pairs
+
+    def plan(self, k1_list=None, k2_range=None):
+        """Prepare a whole sweep (device metadata per batch); returns the batch count."""
+        k1 = None if k1_list is None else np.ascontiguousarray(k1_list, dtype=np.int32)
+        b, e = (0, -1) if k2_range is None else k2_range
+        n = lib().phx_lw_plan(self._h, 0 if k1 is None else len(k1),
+                              None if k1 is None else k1.ctypes.data_as(_ip), int(b), int(e))
+        if n < 0:
+            raise PhxError(lib().phx_last_error().decode())
+        return n
+
+    def plan_run(self, graph=True):
+        """Enqueue the planned sweep (as one captured hipGraph when graph); asynchronous."""
+        pairs = lib().phx_lw_plan_run(self._h, int(bool(graph)))
+        if pairs < 0:
+            raise PhxError(lib().phx_last_error().decode())
+        return pairs
+
+    def set_batching(self, k1_batch=32, lines_per_batch=4):
+        _check(lib().phx_lw_set_batching(self._h, int(k1_batch), int(lines_per_batch)))
+
+    def autotune(self, k1_list=None):
+        """Pick k1 batch, lines per batch and panel chunk by timing; returns them and the time."""
+        k1 = None if k1_list is None else np.ascontiguousarray(k1_list, dtype=np.int32)
+        out = np.zeros(4)
+        _check(lib().phx_lw_autotune(self._h, 0 if k1 is None else len(k1),
+                                     None if k1 is None else k1.ctypes.data_as(_ip), _p(out)))
+        return dict(k1_batch=int(out[0]), lines_per_batch=int(out[1]), panel_chunk=int(out[2]),
+                    seconds=float(out[3]))
 
     def sync(self):
         _check(lib().phx_lw_sync(self._h))

@@ -3,6 +3,7 @@
 // kernels of eig.hip, bloch.hip and elph.hip on one HIP stream per handle.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <thread>
 #include <cmath>
 #include <cstdlib>
@@ -448,6 +449,24 @@ static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int
 }
 
 // ============================================================ linewidth builder
+// One batch of outer points, prepared on the host: its device metadata lives
+// here (one object per batch of a captured sweep), its host copies stay
+// alive until the uploads have run.
+struct BatchMeta {
+  int nkk = 0, nPanels = 0, nMtiles = 0, M = 0, k2b = 0, k2e = 0, nLinesB = 0, nb1Max = 0, nb2max = 1;
+  bool line = false;
+  double flops = 0.;  // algorithmic FLOPs of the coupling stage (timing)
+  // [U1off nkk][nb1 nkk][panelBase nkk][kkMesh nkk][kkStateOff nkk][panelKK P][panelB1 P][panelGStart P][panelGSize P]
+  std::vector<int> hmeta, hLineKK;
+  std::vector<double> hk1;
+  std::vector<double2> hwLine, hphLine;
+  std::vector<float2> hphLineF;
+  DevBuf<int> meta, lineKK;
+  DevBuf<double> k1;
+  DevBuf<double2> wLine, phLineD;
+  DevBuf<float2> phLineF;
+};
+
 struct LwHandle {
   phx_problem prob{};
   int precision = PHX_FP64, k1Batch = 16;
@@ -480,11 +499,11 @@ struct LwHandle {
   DevBuf<double> Bd;
   DevBuf<float> Bf;
   DevBuf<double> partial;
-  DevBuf<int> bmeta;
-  DevBuf<double> bk1;
   DevBuf<double2> phE, phP;
-  std::vector<int> hmeta;
-  std::vector<double> hk1;
+  BatchMeta cur;                    // the batch being run by phx_lw_run
+  std::vector<BatchMeta> plan;      // a prepared sweep (phx_lw_plan)
+  hipGraphExec_t graphExec = nullptr;
+  long long planPairs = 0;
   // separable mesh transform of the R_ph sum (transform == PHX_TRANSFORM_SEPARABLE)
   int transform = 0;
   int nLines = 0, nG0 = 0, maxLineE = 0, maxGrpE = 0;
@@ -496,17 +515,15 @@ struct LwHandle {
   // batched together when the line holds at least lineMin of them (0 = off)
   int lineMin = 0;
   int panelChunk = 48;     // panels per separable-transform launch (bounds T1 / T2)
+  int linesPerBatch = kLinesPerBatch;  // mesh lines per line batch (1 .. kLinesPerBatch)
   bool reLattice = false;  // every R_e is a lattice vector (required by the line form)
   int nGe = 0;
   std::vector<int> reF1, reF2, grpF0e;  // n1, n2 mod m of the grouped R_e; n0 mod m0 per group
   std::vector<double> reInvDeg;         // 1/de of the grouped R_e
   DevBuf<int> reOrderD, reGroupEndD;
-  DevBuf<double2> phLineD, wLineD, TgD;
-  DevBuf<float2> phLineF, TgF;
-  std::vector<double2> hphLine, hwLine;
-  std::vector<int> hLineKK;
-  DevBuf<int> lineKKD;
-  std::vector<float2> hphLineF;
+  DevBuf<double2> TgD;
+  DevBuf<float2> TgF;
+
   // learned q-block screen (separable transform with the energy screen on)
   int lsMode = 0;                  // 0 off, 1 apply the trained network
   bool lsCollect = false;          // collecting training samples (no linewidth accumulation)
@@ -543,6 +560,7 @@ struct LwHandle {
   long long xLaunches = 0;
   DevBuf<unsigned long long> xKept;  // rows computed by the last stage (timed launches)
   ~LwHandle() {
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : xev) (void)hipEventDestroy(e);
@@ -1070,9 +1088,11 @@ static void learned_screen_chunk(LwHandle* L, const CoupParams& cq, XStage&& xst
   launch_fail_fold(L->lsFail.p, L->lsCounts.p + 3, s);
 }
 
-// Enqueue one batch of outer points.  line: every k1 of the batch lies on one
-// mesh line (same j1, j2) and cacheElPh runs in its line form.
-static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, bool line) {
+// Host side of one batch of outer points: metadata and line tables into b
+// (uploaded on the handle's stream), every scratch buffer the launch needs
+// sized.  line: the outer points lie on at most kLinesPerBatch mesh lines and
+// cacheElPh runs in its line form.
+static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, bool line, BatchMeta& b) {
   const phx_problem& p = L->prob;
   hipStream_t s = L->stream;
   const int nw = L->nw, nm = L->nm;
@@ -1093,10 +1113,17 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   const int M = k2e - k2b;
   const int nMtiles = L->transform == PHX_TRANSFORM_DIRECT ? ceil_div(M, L->rows)
                                                          : p.mesh[1] * p.mesh[2];
+  b.nkk = nkk;
+  b.nPanels = nPanels;
+  b.nMtiles = nMtiles;
+  b.M = M;
+  b.k2b = k2b;
+  b.k2e = k2e;
+  b.line = line;
   L->partial.reserve(size_t(L->maxPanels) * std::max(nMtiles, 1) * kGMax * kCMax);
   // metadata: [U1off nkk][nb1 nkk][panelBase nkk][kkMesh nkk][kkStateOff nkk]
   //           [panelKK P][panelB1 P][panelGStart P][panelGSize P]
-  std::vector<int>& m = L->hmeta;
+  std::vector<int>& m = b.hmeta;
   m.assign(5 * size_t(nkk) + 4 * size_t(nPanels), 0);
   int* U1off = m.data();
   int* nb1 = U1off + nkk;
@@ -1107,7 +1134,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   int* pB1 = pKK + nPanels;
   int* pGS = pB1 + nPanels;
   int* pGZ = pGS + nPanels;
-  std::vector<double>& k1c = L->hk1;
+  std::vector<double>& k1c = b.hk1;
   k1c.assign(3 * size_t(nkk), 0.);
   int pan = 0;
   for (int i = 0; i < nkk; ++i) {
@@ -1125,11 +1152,129 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
       pGZ[pan] = L->gSize[L->stateOff[a] + b];
     }
   }
-  L->bmeta.reserve(m.size());
-  PHX_HIP(hipMemcpyAsync(L->bmeta.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  L->bk1.reserve(k1c.size());
-  PHX_HIP(hipMemcpyAsync(L->bk1.p, k1c.data(), k1c.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  const int* dm = L->bmeta.p;
+  b.meta.reserve(m.size());
+  PHX_HIP(hipMemcpyAsync(b.meta.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  b.k1.reserve(k1c.size());
+  PHX_HIP(hipMemcpyAsync(b.k1.p, k1c.data(), k1c.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  // line tables (line form) or the phase table size (GEMM form)
+  int nLinesB = 0;
+  if (line) {
+    // lines of the batch: consecutive outer points with equal (j1, j2)
+    const int m0 = p.mesh[0], m1 = p.mesh[1], m2 = p.mesh[2];
+    std::vector<int>& lk = b.hLineKK;
+    lk.clear();
+    for (int i = 0; i < nkk; ++i)
+      if (i == 0 || L->active[k1s[i]] / m0 != L->active[k1s[i - 1]] / m0) lk.push_back(i);
+    nLinesB = int(lk.size());
+    lk.push_back(nkk);
+    PHX_REQUIRE(nLinesB <= kLinesPerBatch, "line batch: too many mesh lines");
+    b.hphLine.resize(size_t(nLinesB) * L->nRe);
+    for (int l = 0; l < nLinesB; ++l) {
+      int j0, j1, j2;
+      mesh_coords(L->active[k1s[lk[l]]], m0, m1, j0, j1, j2);
+      for (int i = 0; i < L->nRe; ++i) {
+        const double2 ty = twiddle((long long)j1 * L->reF1[i], m1);
+        const double2 tz = twiddle((long long)j2 * L->reF2[i], m2);
+        const double d = L->reInvDeg[i];
+        b.hphLine[size_t(l) * L->nRe + i] =
+            make_double2((ty.x * tz.x - ty.y * tz.y) * d, (ty.x * tz.y + ty.y * tz.x) * d);
+      }
+    }
+    b.hwLine.resize(size_t(nkk) * L->nGe);
+    for (int i = 0; i < nkk; ++i) {
+      int x, y, z;
+      mesh_coords(L->active[k1s[i]], m0, m1, x, y, z);
+      for (int g = 0; g < L->nGe; ++g)
+        b.hwLine[size_t(i) * L->nGe + g] = twiddle((long long)x * L->grpF0e[g], m0);
+    }
+    b.lineKK.reserve(lk.size());
+    PHX_HIP(hipMemcpyAsync(b.lineKK.p, lk.data(), lk.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    b.wLine.reserve(b.hwLine.size());
+    PHX_HIP(hipMemcpyAsync(b.wLine.p, b.hwLine.data(), b.hwLine.size() * sizeof(double2),
+                           hipMemcpyHostToDevice, s));
+    const size_t nph = b.hphLine.size();
+    if (L->precision == PHX_FP64) {
+      b.phLineD.reserve(nph);
+      PHX_HIP(hipMemcpyAsync(b.phLineD.p, b.hphLine.data(), nph * sizeof(double2),
+                             hipMemcpyHostToDevice, s));
+    } else {
+      b.hphLineF.resize(nph);
+      for (size_t i = 0; i < nph; ++i)
+        b.hphLineF[i] = make_float2(float(b.hphLine[i].x), float(b.hphLine[i].y));
+      b.phLineF.reserve(nph);
+      PHX_HIP(hipMemcpyAsync(b.phLineF.p, b.hphLineF.data(), nph * sizeof(float2),
+                             hipMemcpyHostToDevice, s));
+    }
+    const size_t N = size_t(L->nRp) * nm * nw * nw;
+    if (L->precision == PHX_FP64) L->TgD.reserve(size_t(nLinesB) * L->nGe * N);
+    else L->TgF.reserve(size_t(nLinesB) * L->nGe * N);
+  } else {
+    L->phE.reserve(size_t(nkk) * L->nRe);
+  }
+  b.nLinesB = nLinesB;
+  L->phP.reserve(size_t(nkk) * L->nRp);
+  b.nb2max = 1;
+  for (int a = k2b; a < k2e; ++a) b.nb2max = std::max(b.nb2max, L->nb[a]);
+  b.nb1Max = 0;
+  for (int i = 0; i < nkk; ++i) b.nb1Max = std::max(b.nb1Max, L->nb[k1s[i]]);
+  // scratch of the separable transform: one panel chunk of stage buffers, row masks,
+  // and the learned screen's per-block arrays
+  if (L->transform == PHX_TRANSFORM_SEPARABLE && M > 0) {
+    const int chunk = std::min(nPanels, L->panelChunk);
+    const size_t ncol = size_t(nm) * nw;
+    const size_t t1 = size_t(L->nLines) * p.mesh[2] * ncol * 2, t2 = size_t(L->nG0) * p.mesh[1] * p.mesh[2] * ncol * 2;
+    if (L->precision == PHX_FP64) {
+      L->T1d.reserve(t1 * chunk);
+      L->T2d.reserve(t2 * chunk);
+    } else {
+      L->T1f.reserve(t1 * chunk);
+      L->T2f.reserve(t2 * chunk);
+    }
+    const size_t slots = size_t(chunk) * nMtiles;
+    if (p.mesh[0] <= 64) L->keepMask.reserve(slots);
+    if (L->lsMode && L->mlpTrained) {
+      L->lsFeat.reserve(slots * kMlpIn);
+      L->lsNscr.reserve(slots);
+      L->lsMaskC.reserve(slots);
+      L->lsMaskA.reserve(slots);
+      L->lsMaskS.reserve(slots);
+      L->lsPred.reserve(slots);
+      L->mlpScratch.reserve(mlp_scratch_floats(int(slots)));
+      L->lsThrAbs.reserve(chunk);
+      L->lsPanelStats.reserve(2 * size_t(chunk));
+      L->partialAudit.reserve(slots * kGMax * kCMax);
+    }
+  }
+  // algorithmic FLOPs of the coupling stage (per (pair, b1): transform, rotations, |g|^2)
+  double f = 0.;
+  if (L->transform == PHX_TRANSFORM_DIRECT) {
+    for (int a = k2b; a < k2e; ++a) {
+      const double nb2 = L->nb[a];
+      f += 8. * L->nRp * nm * nw + 8. * nm * nw * nb2 + 8. * nm * nm * nb2 + 3. * nm * nb2;
+    }
+  } else {
+    const double ncol = double(nm) * nw, mx = p.mesh[0], my = p.mesh[1], mz = p.mesh[2];
+    f = 8. * ncol * (double(L->nRp) * mz + double(L->nLines) * my * mz + double(L->nG0) * mx * my * mz);
+    for (int a = k2b; a < k2e; ++a) {
+      const double nb2 = L->nb[a];
+      f += 8. * nm * nw * nb2 + 8. * nm * nm * nb2 + 3. * nm * nb2;
+    }
+  }
+  b.flops = f * nPanels;
+}
+
+// Device side of one prepared batch: every launch on the handle's stream, no
+// allocation, upload or host synchronization (so that a sweep can be captured
+// into a graph).
+static long long lw_launch(LwHandle* L, const BatchMeta& b) {
+  const phx_problem& p = L->prob;
+  hipStream_t s = L->stream;
+  const int nw = L->nw, nm = L->nm;
+  const int nkk = b.nkk, nPanels = b.nPanels, nMtiles = b.nMtiles, M = b.M, k2b = b.k2b, k2e = b.k2e;
+  const int nLinesB = b.nLinesB;
+  const bool line = b.line;
+  const size_t pstride = size_t(L->Kp) * L->ldb;
+  const int* dm = b.meta.p;
   const int* dU1off = dm;
   const int* dNb1 = dm + nkk;
   const int* dPbase = dm + 2 * nkk;
@@ -1139,61 +1284,9 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   const int* dPB1 = dPKK + nPanels;
   const int* dPGS = dPB1 + nPanels;
   const int* dPGZ = dPGS + nPanels;
-  // phases: e^{ik1.Re}/de (GEMM form) or the line tables, and e^{-ik1.Rp}/dp
-  int nLinesB = 0;
-  if (line) {
-    // lines of the batch: consecutive outer points with equal (j1, j2)
-    const int m0 = p.mesh[0], m1 = p.mesh[1], m2 = p.mesh[2];
-    std::vector<int>& lk = L->hLineKK;
-    lk.clear();
-    for (int i = 0; i < nkk; ++i)
-      if (i == 0 || L->active[k1s[i]] / m0 != L->active[k1s[i - 1]] / m0) lk.push_back(i);
-    nLinesB = int(lk.size());
-    lk.push_back(nkk);
-    PHX_REQUIRE(nLinesB <= kLinesPerBatch, "line batch: too many mesh lines");
-    L->hphLine.resize(size_t(nLinesB) * L->nRe);
-    for (int l = 0; l < nLinesB; ++l) {
-      int j0, j1, j2;
-      mesh_coords(L->active[k1s[lk[l]]], m0, m1, j0, j1, j2);
-      for (int i = 0; i < L->nRe; ++i) {
-        const double2 a = twiddle((long long)j1 * L->reF1[i], m1);
-        const double2 b = twiddle((long long)j2 * L->reF2[i], m2);
-        const double d = L->reInvDeg[i];
-        L->hphLine[size_t(l) * L->nRe + i] =
-            make_double2((a.x * b.x - a.y * b.y) * d, (a.x * b.y + a.y * b.x) * d);
-      }
-    }
-    L->hwLine.resize(size_t(nkk) * L->nGe);
-    for (int i = 0; i < nkk; ++i) {
-      int x, y, z;
-      mesh_coords(L->active[k1s[i]], m0, m1, x, y, z);
-      for (int g = 0; g < L->nGe; ++g)
-        L->hwLine[size_t(i) * L->nGe + g] = twiddle((long long)x * L->grpF0e[g], m0);
-    }
-    L->lineKKD.reserve(lk.size());
-    PHX_HIP(hipMemcpyAsync(L->lineKKD.p, lk.data(), lk.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    L->wLineD.reserve(L->hwLine.size());
-    PHX_HIP(hipMemcpyAsync(L->wLineD.p, L->hwLine.data(), L->hwLine.size() * sizeof(double2),
-                           hipMemcpyHostToDevice, s));
-    const size_t nph = L->hphLine.size();
-    if (L->precision == PHX_FP64) {
-      L->phLineD.reserve(nph);
-      PHX_HIP(hipMemcpyAsync(L->phLineD.p, L->hphLine.data(), nph * sizeof(double2),
-                             hipMemcpyHostToDevice, s));
-    } else {
-      L->hphLineF.resize(nph);
-      for (size_t i = 0; i < nph; ++i)
-        L->hphLineF[i] = make_float2(float(L->hphLine[i].x), float(L->hphLine[i].y));
-      L->phLineF.reserve(nph);
-      PHX_HIP(hipMemcpyAsync(L->phLineF.p, L->hphLineF.data(), nph * sizeof(float2),
-                             hipMemcpyHostToDevice, s));
-    }
-  } else {
-    L->phE.reserve(size_t(nkk) * L->nRe);
-    launch_phase_table(nkk, L->nRe, L->bk1.p, L->Re.p, L->invDegE.p, 1.0, L->phE.p, s);
-  }
-  L->phP.reserve(size_t(nkk) * L->nRp);
-  launch_phase_table(nkk, L->nRp, L->bk1.p, L->Rp.p, L->invDegP.p, -1.0, L->phP.p, s);
+  // phases: e^{ik1.Re}/de (GEMM form), e^{-ik1.Rp}/dp
+  if (!line) launch_phase_table(nkk, L->nRe, b.k1.p, L->Re.p, L->invDegE.p, 1.0, L->phE.p, s);
+  launch_phase_table(nkk, L->nRp, b.k1.p, L->Rp.p, L->invDegP.p, -1.0, L->phP.p, s);
   if (L->timing) PHX_HIP(hipEventRecord(L->ev[0], s));
   // cacheElPh for the whole batch
   CacheParams cp{};
@@ -1217,14 +1310,13 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cp.reOrder = L->reOrderD.p;
     cp.reGroupEnd = L->reGroupEndD.p;
     cp.nLinesB = nLinesB;
-    cp.lineKK = L->lineKKD.p;
-    cp.wLine = L->wLineD.p;
+    cp.lineKK = b.lineKK.p;
+    cp.wLine = b.wLine.p;
   }
   if (L->precision == PHX_FP64) {
     cp.B = L->Bd.p;
     if (line) {
-      L->TgD.reserve(size_t(nLinesB) * L->nGe * cp.N);
-      cp.phLine = L->phLineD.p;
+      cp.phLine = b.phLineD.p;
       cp.Tg = L->TgD.p;
       launch_cache_line<double, double>(cp, s);
     } else {
@@ -1233,8 +1325,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   } else {
     cp.B = L->Bf.p;
     if (line) {
-      L->TgF.reserve(size_t(nLinesB) * L->nGe * cp.N);
-      cp.phLine = L->phLineF.p;
+      cp.phLine = b.phLineF.p;
       cp.Tg = L->TgF.p;
       launch_cache_line<float, float>(cp, s);
     } else {
@@ -1290,9 +1381,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cq.norm = 1. / (double(p.mesh[0]) * p.mesh[1] * p.mesh[2]);
   cq.cutoff = phonon_cutoff();
   cq.partial = L->partial.p;
-  int nb2max = 1;
-  for (int a = k2b; a < k2e; ++a) nb2max = std::max(nb2max, L->nb[a]);
-  cq.nb2max = nb2max;
+  cq.nb2max = b.nb2max;
   if (L->transform == PHX_TRANSFORM_DIRECT) {
     if (L->precision == PHX_FP64) {
       cq.A = L->Ad.p + size_t(k2b) * L->Kp;
@@ -1342,21 +1431,14 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
     cq.rowE2range = L->rowE2range.p;
     cq.bf16x3 = L->precision == PHX_BF16X3;
     cq.meshRow = L->meshRow.p;
-    if (p.mesh[0] <= 64) {
-      L->keepMask.reserve(size_t(std::min(nPanels, L->panelChunk)) * nMtiles);
-      cq.keepMask = L->keepMask.p;
-    }
+    if (p.mesh[0] <= 64) cq.keepMask = L->keepMask.p;
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
     if (L->precision == PHX_FP64) {
-      L->T1d.reserve(sp.t1PanelStride * chunk);
-      L->T2d.reserve(sp.t2PanelStride * chunk);
       sp.T1 = L->T1d.p;
       sp.T2 = L->T2d.p;
       cq.T2 = L->T2d.p;
     } else {
-      L->T1f.reserve(sp.t1PanelStride * chunk);
-      L->T2f.reserve(sp.t2PanelStride * chunk);
       sp.T1 = L->T1f.p;
       sp.T2 = L->T2f.p;
       cq.T2 = L->T2f.p;
@@ -1403,23 +1485,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   }
   if (L->timing) {
     PHX_HIP(hipEventRecord(L->ev[2], s));
-    double f = 0.;
-    if (L->transform == PHX_TRANSFORM_DIRECT) {
-      // algorithmic FLOPs per (pair, b1): R_ph transform, electron and phonon rotations, |g|^2
-      for (int a = k2b; a < k2e; ++a) {
-        const double nb2 = L->nb[a];
-        f += 8. * L->nRp * nm * nw + 8. * nm * nw * nb2 + 8. * nm * nm * nb2 + 3. * nm * nb2;
-      }
-    } else {
-      // three lattice-direction transforms over the whole mesh, then the rotations per pair
-      const double ncol = double(nm) * nw, mx = p.mesh[0], my = p.mesh[1], mz = p.mesh[2];
-      f = 8. * ncol * (double(L->nRp) * mz + double(L->nLines) * my * mz + double(L->nG0) * mx * my * mz);
-      for (int a = k2b; a < k2e; ++a) {
-        const double nb2 = L->nb[a];
-        f += 8. * nm * nw * nb2 + 8. * nm * nm * nb2 + 3. * nm * nb2;
-      }
-    }
-    L->couplingFlops += f * nPanels;
+    L->couplingFlops += b.flops;
   }
   if (L->lsCollect || (L->lsMode && L->mlpTrained)) ++L->lsBatch;
   if (L->lsCollect) {  // training samples only: the linewidths are left untouched
@@ -1434,8 +1500,7 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   cl.nMtiles = nMtiles;
   cl.nCalc = L->nCalc;
   cl.numStates = L->numStates;
-  cl.nb1Max = 0;
-  for (int i = 0; i < nkk; ++i) cl.nb1Max = std::max(cl.nb1Max, L->nb[k1s[i]]);
+  cl.nb1Max = b.nb1Max;
   cl.kkNb1 = dNb1;
   cl.kkPanelBase = dPbase;
   cl.kkStateOff = dKKState;
@@ -1451,9 +1516,23 @@ static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e
   return (long long)nkk * M;
 }
 
-static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2e) {
-  if (k2e < 0) k2e = L->na;
-  PHX_REQUIRE(k2b >= 0 && k2b <= k2e && k2e <= L->na, "phx_lw_run: bad k2 range");
+
+// Prepare and enqueue one batch (phx_lw_run's path: the caller waits for the
+// previous batch before its metadata object is reused).
+static long long lw_batch(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, bool line) {
+  lw_prepare(L, k1s, nkk, k2b, k2e, line, L->cur);
+  return lw_launch(L, L->cur);
+}
+
+struct BatchSpec {
+  std::vector<int> k1;
+  bool line;
+};
+
+// The batches of a sweep over the outer points k1list (all when null): whole
+// mesh lines with at least lineMin points go into line batches of up to
+// linesPerBatch lines (kLinePointsMax points); the rest into GEMM batches of k1Batch.
+static std::vector<BatchSpec> lw_batches(LwHandle* L, int nk1, const int* k1list) {
   std::vector<int> all;
   if (!k1list) {
     all.resize(L->na);
@@ -1462,14 +1541,7 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
     nk1 = L->na;
   }
   for (int i = 0; i < nk1; ++i) PHX_REQUIRE(k1list[i] >= 0 && k1list[i] < L->na, "phx_lw_run: bad k1");
-  long long pairs = 0;
-  auto run = [&](const int* k1s, int n, bool line) {
-    // the metadata buffers are reused per batch: wait for the previous batch
-    PHX_HIP(hipStreamSynchronize(L->stream));
-    lw_harvest_timing(L);
-    pairs += lw_batch(L, k1s, n, k2b, k2e, line);
-  };
-  // outer points on well-filled mesh lines (same j1, j2): one line batch each
+  std::vector<BatchSpec> out;
   std::vector<int> rest;
   if (L->lineMin > 0) {
     const int m0 = L->prob.mesh[0];
@@ -1477,11 +1549,10 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
       return L->active[a] / m0 < L->active[b] / m0;  // mesh index / m0 = line (j1, j2)
     });
-    // up to kLinesPerBatch lines (kLinePointsMax points) share one gW stream
     std::vector<int> batch;
     int linesIn = 0;
     auto flush = [&] {
-      if (!batch.empty()) run(batch.data(), int(batch.size()), true);
+      if (!batch.empty()) out.push_back(BatchSpec{batch, true});
       batch.clear();
       linesIn = 0;
     };
@@ -1491,7 +1562,7 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
       if (int(j - i) >= L->lineMin) {
         for (size_t c = i; c < j; c += kLinePointsMax) {
           const size_t n = std::min<size_t>(kLinePointsMax, j - c);
-          if (linesIn == kLinesPerBatch || batch.size() + n > size_t(kLinePointsMax)) flush();
+          if (linesIn == L->linesPerBatch || batch.size() + n > size_t(kLinePointsMax)) flush();
           batch.insert(batch.end(), order.begin() + c, order.begin() + c + n);
           ++linesIn;
         }
@@ -1504,8 +1575,74 @@ static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2
     k1list = rest.data();
     nk1 = int(rest.size());
   }
-  for (int i = 0; i < nk1; i += L->k1Batch) run(k1list + i, std::min(L->k1Batch, nk1 - i), false);
+  for (int i = 0; i < nk1; i += L->k1Batch)
+    out.push_back(BatchSpec{std::vector<int>(k1list + i, k1list + std::min(nk1, i + L->k1Batch)), false});
+  return out;
+}
+
+static long long lw_run(LwHandle* L, int nk1, const int* k1list, int k2b, int k2e) {
+  if (k2e < 0) k2e = L->na;
+  PHX_REQUIRE(k2b >= 0 && k2b <= k2e && k2e <= L->na, "phx_lw_run: bad k2 range");
+  long long pairs = 0;
+  for (const BatchSpec& b : lw_batches(L, nk1, k1list)) {
+    // the metadata object is reused per batch: wait for the previous batch
+    PHX_HIP(hipStreamSynchronize(L->stream));
+    lw_harvest_timing(L);
+    pairs += lw_batch(L, b.k1.data(), int(b.k1.size()), k2b, k2e, b.line);
+  }
   return pairs;
+}
+
+// A whole sweep prepared up front (one metadata object per batch).
+static void lw_plan(LwHandle* L, int nk1, const int* k1list, int k2b, int k2e) {
+  if (k2e < 0) k2e = L->na;
+  PHX_REQUIRE(k2b >= 0 && k2b <= k2e && k2e <= L->na, "phx_lw_plan: bad k2 range");
+  PHX_REQUIRE(!L->lsCollect, "phx_lw_plan: not while collecting screen samples");
+  if (L->graphExec) {
+    PHX_HIP(hipGraphExecDestroy(L->graphExec));
+    L->graphExec = nullptr;
+  }
+  PHX_HIP(hipStreamSynchronize(L->stream));
+  L->plan.clear();
+  L->planPairs = 0;
+  const std::vector<BatchSpec> specs = lw_batches(L, nk1, k1list);
+  L->plan.resize(specs.size());
+  for (size_t i = 0; i < specs.size(); ++i) {
+    lw_prepare(L, specs[i].k1.data(), int(specs[i].k1.size()), k2b, k2e, specs[i].line, L->plan[i]);
+    L->planPairs += (long long)specs[i].k1.size() * (k2e - k2b);
+  }
+  PHX_HIP(hipStreamSynchronize(L->stream));  // uploads and buffer fills done
+}
+
+// Enqueue the prepared sweep; with graph != 0 it is captured into a hipGraph on
+// first use and replayed as one graph launch.  Asynchronous on the handle's stream.
+static long long lw_plan_run(LwHandle* L, int graph) {
+  PHX_REQUIRE(!L->plan.empty() || L->planPairs == 0, "phx_lw_plan_run: no plan");
+  if (!graph) {
+    for (const BatchMeta& b : L->plan) lw_launch(L, b);
+    return L->planPairs;
+  }
+  if (!L->graphExec) {
+    const bool timing = L->timing;
+    L->timing = false;
+    hipGraph_t g = nullptr;
+    PHX_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeRelaxed));
+    try {
+      for (const BatchMeta& b : L->plan) lw_launch(L, b);
+    } catch (...) {
+      (void)hipStreamEndCapture(L->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      L->timing = timing;
+      throw;
+    }
+    PHX_HIP(hipStreamEndCapture(L->stream, &g));
+    L->timing = timing;
+    hipError_t e = hipGraphInstantiate(&L->graphExec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PHX_HIP(e);
+  }
+  PHX_HIP(hipGraphLaunch(L->graphExec, L->stream));
+  return L->planPairs;
 }
 
 static void lw_get(LwHandle* L, double* out, int average) {
@@ -1733,6 +1870,84 @@ long long phx_lw_run(phx_lw* h, int nk1, const int* k1list, int k2b, int k2e) {
     pairs = lw_run(h, nk1, k1list, k2b, k2e);
   });
   return rc ? rc : pairs;
+}
+
+long long phx_lw_plan(phx_lw* h, int nk1, const int* k1_list, int k2_begin, int k2_end) {
+  int rc = guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    lw_plan(h, nk1, k1_list, k2_begin, k2_end);
+  });
+  return rc < 0 ? rc : (long long)h->plan.size();
+}
+
+long long phx_lw_plan_run(phx_lw* h, int graph) {
+  long long pairs = 0;
+  int rc = guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    pairs = lw_plan_run(h, graph);
+  });
+  return rc < 0 ? rc : pairs;
+}
+
+// Autotuner of the batch-level tile shapes: k1 batch (GEMM-form M tile), mesh
+// lines per line batch, panels per separable launch.  Coordinate descent on
+// the wall time of full runs over the sample k1_list; the handle's linewidths
+// are restored afterwards.  out4 = {k1_batch, lines_per_batch, panel_chunk, best seconds}.
+static void lw_autotune(LwHandle* L, int nk1, const int* k1list, double* out4) {
+  hipStream_t s = L->stream;
+  PHX_HIP(hipStreamSynchronize(s));
+  DevBuf<double> keep(L->lw.n);
+  PHX_HIP(hipMemcpyAsync(keep.p, L->lw.p, L->lw.bytes(), hipMemcpyDeviceToDevice, s));
+  auto timeit = [&]() {
+    PHX_HIP(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    lw_run(L, nk1, k1list, 0, -1);
+    PHX_HIP(hipStreamSynchronize(s));
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  timeit();  // buffers grown to size
+  double best = timeit();
+  auto tune = [&](int* knob, std::initializer_list<int> values) {
+    int bestv = *knob;
+    for (int v : values) {
+      if (v == bestv) continue;
+      *knob = v;
+      const double t = timeit();
+      if (t < best) {
+        best = t;
+        bestv = v;
+      }
+    }
+    *knob = bestv;
+  };
+  tune(&L->k1Batch, {8, 16, 32});
+  if (L->lineMin > 0) tune(&L->linesPerBatch, {1, 2, 4});
+  if (L->transform == PHX_TRANSFORM_SEPARABLE) tune(&L->panelChunk, {16, 48, 128});
+  PHX_HIP(hipMemcpyAsync(L->lw.p, keep.p, L->lw.bytes(), hipMemcpyDeviceToDevice, s));
+  PHX_HIP(hipStreamSynchronize(s));
+  out4[0] = L->k1Batch;
+  out4[1] = L->linesPerBatch;
+  out4[2] = L->panelChunk;
+  out4[3] = best;
+}
+
+int phx_lw_autotune(phx_lw* h, int nk1, const int* k1_list, double* out4) {
+  return guard([&] {
+    PHX_REQUIRE(h && out4, "null argument");
+    PHX_REQUIRE(!h->lsCollect, "phx_lw_autotune: not while collecting screen samples");
+    lw_autotune(h, nk1, k1_list, out4);
+  });
+}
+
+int phx_lw_set_batching(phx_lw* h, int k1_batch, int lines_per_batch) {
+  return guard([&] {
+    PHX_REQUIRE(h, "null handle");
+    PHX_REQUIRE(k1_batch >= 1 && k1_batch <= kCacheNKK && lines_per_batch >= 1 &&
+                    lines_per_batch <= kLinesPerBatch,
+                "phx_lw_set_batching: k1_batch in 1..32, lines_per_batch in 1..4");
+    h->k1Batch = k1_batch;
+    h->linesPerBatch = lines_per_batch;
+  });
 }
 
 int phx_lw_get_linewidths(phx_lw* h, double* out, int average) {

@@ -313,3 +313,34 @@ def eig_family(kind: str, n: int, seed: int = 1001, rgrid: int = 8):
     mat = _force_constants(rng, R, natoms, tau, 1.5 * ang)
     return dict(recip=reciprocal_cell(A), R=R, weight=1.0 / deg, coef=np.ascontiguousarray(mat),
                 masses=np.full(n, 28.0855 * MASS_AMU_TO_RY))
+
+
+_WS_COUNT = {}
+
+
+def _ws_count(grid: int) -> int:
+    if grid not in _WS_COUNT:
+        _WS_COUNT[grid] = len(ws_vectors((grid,) * 3, _fcc(10.26))[0])
+    return _WS_COUNT[grid]
+
+
+def uneven_materials(count: int, seed: int = 1003, max_gw_bytes: float = 1.0e9,
+                     max_mesh: int = 24):
+    """Materials of the uneven-batching workload: Nw uniform in 4..32, atoms in
+    1..12 (3..36 modes), R shells uniform in 5^3..9^3 (electron and phonon
+    drawn separately), cubic k/q meshes 8^3..max_mesh^3, config-0 distributions
+    (make_problem).  Draws whose FP32 coupling tensor exceeds max_gw_bytes are
+    redrawn (a per-material device-memory budget), so the sample is the
+    distribution conditioned on that size."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        nw = int(rng.integers(4, 33))
+        natoms = int(rng.integers(1, 13))
+        re, rp = int(rng.integers(5, 10)), int(rng.integers(5, 10))
+        mesh = int(rng.integers(8, max_mesh + 1))
+        mseed = int(rng.integers(1, 2**31))
+        if _ws_count(re) * _ws_count(rp) * 3 * natoms * nw * nw * 8 > max_gw_bytes:
+            continue
+        out.append(make_problem(natoms, nw, re, rp, mesh, seed=mseed, name=f"uneven{len(out)}"))
+    return out
