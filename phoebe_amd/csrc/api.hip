@@ -888,10 +888,15 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     L->nLines = int(lineE.size());
     L->nG0 = int(grpE.size());
     {
+      int nbMax = 0;
+      for (int n : L->nb) nbMax = std::max(nbMax, n);
+      PHX_REQUIRE(nm <= kSepNm && nbMax <= kSepNb2,
+                  "phx_lw_setup: the separable transform supports nm <= 32 and at most 8 windowed "
+                  "bands per point (use the direct transform)");
       // the z / y stages stage one line / group of panel columns in LDS
       const size_t ncol = size_t(nm) * nw;
       PHX_REQUIRE(size_t(L->maxLineE) * ncol * 16 <= 160 * 1024 && size_t(L->maxGrpE) * ncol * 16 <= 160 * 1024 &&
-                      (ncol <= 192 || (size_t(L->nG0) * ncol * (precision == PHX_FP64 ? 16 : 8) <= 96 * 1024)),
+                      (ncol <= 384 || (size_t(L->nG0) * ncol * (precision == PHX_FP64 ? 16 : 8) <= 96 * 1024)),
                   "phx_lw_setup: panels too wide for the separable transform (use the direct transform)");
     }
     h2d(L->lineStart, ls, s);

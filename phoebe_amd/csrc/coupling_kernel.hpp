@@ -944,7 +944,7 @@ inline size_t sep_x_lds_bytes(int elem, int nG0, int ncol, int m0, int nw, int n
 // TPR threads to each row (thread sub = mode nu).  Workgroups are ordered
 // with the panels innermost, so that consecutive workgroups share the rows'
 // U2 and the q-line's phonons in L2.
-constexpr int kXmNt = 3;    // complex column tiles (16 columns) per wave: nm * nw <= 192
+// kXmNt: complex column tiles (16 columns) per wave: 3 for nm * nw <= 192, 6 for <= 384
 constexpr int kXmKsc = 3;   // MFMA k-steps per B chunk
 
 struct SepXmRow {            // per kept row, staged in LDS
@@ -997,14 +997,14 @@ __device__ inline void bf16_split4(const float (&x)[4], bf16x4_t& hi, bf16x4_t& 
 
 // BF3: the x-stage GEMMs in split-BF16x3 (hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x16_bf16, FP32 accumulation); T must be float.
-template <typename T, bool REDUCE, int NMX, int CM, bool BF3 = false>
+template <typename T, bool REDUCE, int NMX, int CM, bool BF3 = false, int kXmNt = 3>
 __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
   typedef typename Mfma4<T>::acc_t acc_t;
   constexpr int TPR = NMX <= 16 ? 16 : 32;  // epilogue threads per row
   constexpr int RPP = 256 / TPR;            // rows per epilogue pass
-  constexpr int KSB = sizeof(T) == 4 ? 6 : 3;   // k-steps of B held in registers
+  constexpr int KSB = (sizeof(T) == 4 && kXmNt <= 3) ? 6 : 3;   // k-steps of B held in registers
   const int nLinesYZ = p.m1 * p.m2;
   const int nTilesTot = p.nPanels * nLinesYZ;
   const int bid = blockIdx.x;
@@ -1053,7 +1053,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   const size_t gStride = size_t(p.m1) * p.m2 * ncol;  // T2v elements between groups
   const int KS = (2 * nG0 + 3) / 4;                    // k-steps of 4 real K
   // B operands of every k-step, loaded once for all tiles when they fit in registers
-  constexpr int KSR = (BF3 || sizeof(T) != 4) ? 1 : 12;  // FP32: up to 24 x-groups
+  constexpr int KSR = (BF3 || sizeof(T) != 4 || kXmNt > 3) ? 1 : 12;  // FP32: up to 24 x-groups
   const bool bReg = !BF3 && KS <= KSR;
   T bAll[KSR][kXmNt];
   if (bReg) {
@@ -1523,9 +1523,10 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
   const bool one = p.nCalc == 1;
   constexpr bool bf3t = sizeof(T) == 4;  // split-BF16x3 exists for the FP32 data path
   const bool bf3 = p.bf16x3 != 0;
-  PHX_REQUIRE(!bf3 || (bf3t && reduce && sp.ncol <= 4 * kXmNt * 16),
-              "split-BF16x3: FP32 data, the linewidth reduction and nm nw <= 192");
-  if (sp.ncol <= 4 * kXmNt * 16) {
+  PHX_REQUIRE(!bf3 || (bf3t && reduce && sp.ncol <= 4 * 6 * 16),
+              "split-BF16x3: FP32 data, the linewidth reduction and nm nw <= 384");
+  if (reduce && sp.ncol <= 4 * 6 * 16) {
+    const bool wide = sp.ncol > 4 * 3 * 16;
     const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8);
     PHX_REQUIRE(ldsXm <= 160 * 1024, "separable transform: LDS budget exceeded");
     if (p.keepMask) {
@@ -1540,18 +1541,24 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
     };
 #define PHX_XM(NMX)                                                                 \
     if (nmb == NMX) {                                                               \
-      if (!reduce) go(k_sep_xm<T, false, NMX, 1>);                                  \
-      else if (bf3 && one) go(k_sep_xm<T, true, NMX, 1, bf3t>);                      \
-      else if (bf3) go(k_sep_xm<T, true, NMX, kCMax, bf3t>);                         \
-      else if (one) go(k_sep_xm<T, true, NMX, 1>);                                  \
-      else go(k_sep_xm<T, true, NMX, kCMax>);                                       \
+      if (wide) {                                                                   \
+        if (bf3 && one) go(k_sep_xm<T, true, NMX, 1, bf3t, 6>);                      \
+        else if (bf3) go(k_sep_xm<T, true, NMX, kCMax, bf3t, 6>);                    \
+        else if (one) go(k_sep_xm<T, true, NMX, 1, false, 6>);                       \
+        else go(k_sep_xm<T, true, NMX, kCMax, false, 6>);                            \
+      } else {                                                                      \
+        if (bf3 && one) go(k_sep_xm<T, true, NMX, 1, bf3t>);                         \
+        else if (bf3) go(k_sep_xm<T, true, NMX, kCMax, bf3t>);                       \
+        else if (one) go(k_sep_xm<T, true, NMX, 1>);                                 \
+        else go(k_sep_xm<T, true, NMX, kCMax>);                                      \
+      }                                                                             \
     }
     PHX_XM(12) PHX_XM(16) PHX_XM(24) PHX_XM(32)
 #undef PHX_XM
     PHX_HIP(hipGetLastError());
     return;
   }
-  PHX_REQUIRE(!p.slotOn && !p.gate, "separable transform: slot masks need nm nw <= 192");
+  PHX_REQUIRE(!p.slotOn && !p.gate, "separable transform: slot masks need nm nw <= 384");
   const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
                                size_t(kGMax) * kCMax * 4 * sizeof(double));
   PHX_REQUIRE(ldsX <= 160 * 1024, "separable transform: LDS budget exceeded");

@@ -19,10 +19,15 @@ def say(*a):
 
 
 def _handle(p):
-    sep = p.nm * p.nw <= 192
-    lw = P.ElectronLinewidths(p, gW=None, precision="fp32", transform="separable" if sep else "direct")
-    if sep:
-        lw.set_screen(6.0)
+    lw = None
+    if p.nm * p.nw <= 192:
+        try:  # the separable transform needs nm <= 32 and <= 8 windowed bands per point
+            lw = P.ElectronLinewidths(p, gW=None, precision="fp32", transform="separable")
+            lw.set_screen(6.0)
+        except P.PhxError:
+            lw = None
+    if lw is None:
+        lw = P.ElectronLinewidths(p, gW=None, precision="fp32", transform="direct")
     lw.set_line_batching(4)
     return lw
 
