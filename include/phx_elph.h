@@ -134,6 +134,50 @@ int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double max_mem_
 
 int phx_elph_destroy(phx_elph* h);
 
+/* ---------------------------------------------------------------- el-ph file
+ * Phoebe's prefix.phoebe.elph.hdf5, read and written without libhdf5 (the
+ * library's own HDF5 subset: root-group datasets, contiguous or compact
+ * storage, no filters).  Host code only; no GPU needed. */
+typedef struct phx_elph_file phx_elph_file;
+
+/* parseHDF5 + parseHeaderHDF5 (interaction_elph_parsing.cpp:626-670, 161-233):
+ * opens the file, decides the format (a file without /fileFormat is format 1,
+ * :651-663) and reads the header datasets.  A missing file fails with the
+ * reference's message (:629-636). */
+int phx_elph_file_open(phx_elph_file** f, const char* path);
+
+/* ints13 = {fileFormat, numElectrons, numSpin, numElBands (nw), numPhModes (nm),
+ *           nRe, nRp, kMesh[3], qMesh[3]}  (meshes 0 when the file has none). */
+int phx_elph_file_header(phx_elph_file* f, int* ints13);
+
+/* elBravaisVectors / elDegeneracies / phBravaisVectors / phDegeneracies
+ * (interaction_elph_parsing.cpp:199-211): elR nRe x 3, phR nRp x 3 (Bohr);
+ * any pointer may be NULL. */
+int phx_elph_file_bravais(phx_elph_file* f, double* elR, double* elDeg, double* phR,
+                          double* phDeg);
+
+/* gW of the lattice vectors R_e in [re_begin, re_begin + re_count), in the
+ * phx_elph_create layout: the slice a rank or pool selects
+ * (format 1: interaction_elph_parsing.cpp:416-425, 495-502; format 2: :576-582). */
+int phx_elph_file_read_gw(phx_elph_file* f, int re_begin, int re_count, double* out);
+
+int phx_elph_file_close(phx_elph_file* f);
+
+/* writeElPhCouplingHDF5v1 / v2 + writeHeaderHDF5 (qe_to_phoebe_utils.cpp:446-626,
+ * 628-720, 377-443): format 1 (one /gWannier) or 2 (/gWannier_<irE>).
+ *   ints12 = {numElectrons, numSpin, nw, nm, nRe, nRp, kMesh[3], qMesh[3]};
+ *   elR nRe x 3, phR nRp x 3, gW in the phx_elph_create layout.
+ * An existing file at path is replaced (qe_to_phoebe_utils.cpp:469-474). */
+int phx_elph_file_write(const char* path, int format, const int* ints12, const double* elR,
+                        const double* elDeg, const double* phR, const double* phDeg,
+                        const double* gW);
+
+/* InteractionElPhWan::parse (interaction_elph_parsing.cpp:672-692) followed by
+ * the constructor: the coupling tensor is streamed from the file into device
+ * memory through two pinned staging buffers (reads overlap the copies; FP32
+ * handles convert each chunk on the device), so no host copy of gW is made. */
+int phx_elph_create_from_file(phx_elph** h, const char* path, int precision);
+
 /* ---------------------------------------------------------------- linewidths
  * Whole electron linewidth calculation of ElScatteringMatrix::builder,
  * switchCase 2 (src/bte/el_scattering.cpp:38-473), inner == outer band
@@ -162,6 +206,13 @@ typedef struct phx_lw phx_lw;
 /* Band structures, window, phonons on the q-set, device-resident operands. */
 int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batch,
                  int transform);
+/* Same, with the coupling (nRe, nRp, Re, degE, Rp, degP, gW) taken from Phoebe's
+ * el-ph file instead of prob: the builder's interaction comes from
+ * InteractionElPhWan::parse (interaction_elph_parsing.cpp:672-692).  The file's
+ * numElBands / numPhModes must equal prob->nw / prob->nm; gW is streamed from the
+ * file into device memory. */
+int phx_lw_setup_file(phx_lw** h, const phx_problem* prob, const char* elph_path, int precision,
+                      int k1_batch, int transform);
 int phx_lw_num_active(phx_lw* h);
 int phx_lw_num_states(phx_lw* h);
 /* active points (full-mesh indices), band extrema, energies of the states */

@@ -10,7 +10,10 @@ classes mirror the reference interface of the hot path:
 * el_bloch_diag / ph_bloch_diag (kokkosBatchedDiagonalizeFromCoordinates,
   src/harmonic/harmonic.h:87-91);
 * ElectronLinewidths  (ElScatteringMatrix::builder, switchCase 2,
-  src/bte/el_scattering.cpp:38-473).
+  src/bte/el_scattering.cpp:38-473);
+* ElPhFile / write_elph_file  (Phoebe's prefix.phoebe.elph.hdf5,
+  interaction_elph_parsing.cpp:161-670, qe_to_phoebe_utils.cpp:377-720),
+  read and written by the library's own HDF5 subset (no libhdf5, no GPU).
 
 The HIP library is required: importing works without it (so that build
 machines can import the package), but every compute call raises if the
@@ -107,6 +110,15 @@ EXPORTS = {
     "phx_lw_copy_gw": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_destroy": (C.c_int, [C.c_void_p]),
     "phx_synth_gw_host": (C.c_int, [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_ulonglong]),
+    "phx_elph_file_open": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p]),
+    "phx_elph_file_header": (C.c_int, [C.c_void_p, _ip]),
+    "phx_elph_file_bravais": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    "phx_elph_file_read_gw": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    "phx_elph_file_close": (C.c_int, [C.c_void_p]),
+    "phx_elph_file_write": (C.c_int, [C.c_char_p, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp]),
+    "phx_elph_create_from_file": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
+    "phx_lw_setup_file": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_char_p,
+                                    C.c_int, C.c_int, C.c_int]),
 }
 
 
@@ -205,6 +217,79 @@ def mlp_gradients(params, X, t):
     return float(loss[0]), g, y
 
 
+# ------------------------------------------------------------------ el-ph file
+HEADER_KEYS = ("fileFormat", "numElectrons", "numSpin", "numElBands", "numPhModes", "nRe", "nRp")
+
+
+def _path(path):
+    return os.fsencode(os.fspath(path))
+
+
+class ElPhFile:
+    """Phoebe's el-ph coupling file, prefix.phoebe.elph.hdf5, formats 1 and 2.
+
+    Mirrors parseHDF5 / parseHeaderHDF5 / parseHDF5V1 / parseHDF5V2
+    (interaction_elph_parsing.cpp:161-670).  Host code only.
+    """
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+        self._h = C.c_void_p()
+        _check(lib().phx_elph_file_open(C.byref(self._h), _path(path)))
+        v = np.zeros(13, dtype=np.int32)
+        _check(lib().phx_elph_file_header(self._h, v.ctypes.data_as(_ip)))
+        self.header = {k: int(x) for k, x in zip(HEADER_KEYS, v[:7])}
+        self.header["kMesh"] = tuple(int(x) for x in v[7:10])
+        self.header["qMesh"] = tuple(int(x) for x in v[10:13])
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().phx_elph_file_close(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def bravais(self):
+        """(elR (nRe, 3), elDeg (nRe,), phR (nRp, 3), phDeg (nRp,)), Bohr."""
+        h = self.header
+        eR, eD = np.zeros((h["nRe"], 3)), np.zeros(h["nRe"])
+        pR, pD = np.zeros((h["nRp"], 3)), np.zeros(h["nRp"])
+        _check(lib().phx_elph_file_bravais(self._h, _p(eR), _p(eD), _p(pR), _p(pD)))
+        return eR, eD, pR, pD
+
+    def read_gw(self, re_begin=0, re_count=None):
+        """gW of R_e in [re_begin, re_begin + re_count) as complex
+        (re_count, nRp, nm, nw, nw) (the device layout of phx_elph_create)."""
+        h = self.header
+        if re_count is None:
+            re_count = h["nRe"] - re_begin
+        out = np.zeros((re_count, h["nRp"], h["numPhModes"], h["numElBands"], h["numElBands"]),
+                       dtype=np.complex128)
+        _check(lib().phx_elph_file_read_gw(self._h, int(re_begin), int(re_count),
+                                           _p(out.view(np.float64))))
+        return out
+
+
+def write_elph_file(path, gW, elR, elDeg, phR, phDeg, numElectrons, numSpin=1,
+                    kMesh=(0, 0, 0), qMesh=(0, 0, 0), fileFormat=1):
+    """writeElPhCouplingHDF5v1 / v2 (qe_to_phoebe_utils.cpp:446-720): gW complex
+    (nRe, nRp, nm, nw, nw); elR (nRe, 3), phR (nRp, 3)."""
+    g = np.ascontiguousarray(gW, dtype=np.complex128)
+    nRe, nRp, nm, nw, nw2 = g.shape
+    eR, eD, pR, pD = (_f64(a) for a in (elR, elDeg, phR, phDeg))
+    if nw2 != nw or eR.shape != (nRe, 3) or pR.shape != (nRp, 3) or eD.size != nRe or pD.size != nRp:
+        raise PhxError("write_elph_file: inconsistent shapes")
+    v = np.array([numElectrons, numSpin, nw, nm, nRe, nRp, *kMesh, *qMesh], dtype=np.int32)
+    _check(lib().phx_elph_file_write(_path(path), int(fileFormat), v.ctypes.data_as(_ip), _p(eR),
+                                     _p(eD), _p(pR), _p(pD), _p(g.view(np.float64))))
+
+
 # ------------------------------------------------------------------ InteractionElPhWan
 class InteractionElPhWan:
     """Wannier electron-phonon coupling on the GPU (interaction_elph.h:30-219)."""
@@ -215,6 +300,15 @@ class InteractionElPhWan:
         """couplingWannier: complex (nRe, nRp, nm, nw, nw) (the reference's
         device view of its Eigen tensor); or synthetic=(nw, nm, decay, seed)."""
         self._h = C.c_void_p()
+        if isinstance(couplingWannier, (str, bytes, os.PathLike)):
+            # InteractionElPhWan::parse (interaction_elph_parsing.cpp:672-692)
+            prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+            _check(lib().phx_elph_create_from_file(C.byref(self._h), _path(couplingWannier), prec))
+            dims = self.getCouplingDimensions()
+            self.numElBands, self.numPhBands = int(dims[3]), int(dims[2])
+            self._cache = []
+            self._nb1 = None
+            return
         elR, phR = _f64(elBravaisVectors), _f64(phBravaisVectors)
         elD, phD = _f64(elDegeneracies), _f64(phDegeneracies)
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -233,6 +327,11 @@ class InteractionElPhWan:
         self.numElBands, self.numPhBands = int(dims[3]), int(dims[2])
         self._cache = []
         self._nb1 = None
+
+    @classmethod
+    def parse(cls, path, precision="fp64"):
+        """The interaction read from Phoebe's el-ph file, gW streamed into HBM."""
+        return cls(os.fspath(path), precision=precision)
 
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -326,13 +425,21 @@ class ElectronLinewidths:
     Gaussian-delta reduction per outer point, degeneracy averaging.
     """
 
-    def __init__(self, prob, gW=None, precision="fp64", k1_batch=32, transform="direct"):
+    def __init__(self, prob, gW=None, precision="fp64", k1_batch=32, transform="direct",
+                 elph_file=None):
+        """gW: host coupling (complex (nRe, nRp, nm, nw, nw)); None with elph_file
+        unset uses the problem's synthetic recipe; elph_file: path of Phoebe's
+        el-ph file, whose R vectors and gW replace the problem's."""
         self.prob = prob
         self._s, self._keep = make_problem_struct(prob, gW)
         self._h = C.c_void_p()
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         tr = TRANSFORMS[transform] if isinstance(transform, str) else int(transform)
-        _check(lib().phx_lw_setup(C.byref(self._h), C.byref(self._s), prec, int(k1_batch), tr))
+        if elph_file is not None:
+            _check(lib().phx_lw_setup_file(C.byref(self._h), C.byref(self._s), _path(elph_file),
+                                           prec, int(k1_batch), tr))
+        else:
+            _check(lib().phx_lw_setup(C.byref(self._h), C.byref(self._s), prec, int(k1_batch), tr))
         self.num_active = lib().phx_lw_num_active(self._h)
         self.num_states = lib().phx_lw_num_states(self._h)
         self.active = np.zeros(self.num_active, dtype=np.int32)

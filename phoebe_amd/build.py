@@ -2,7 +2,9 @@
 
     python -m phoebe_amd.build
 
-compiles phoebe_amd/csrc/*.hip with hipcc into phoebe_amd/libphx_elph.so.
+compiles phoebe_amd/csrc/*.hip with hipcc and the host-only
+phoebe_amd/csrc/*.cpp (the el-ph file reader / writer) with g++ into
+phoebe_amd/libphx_elph.so.
 """
 from __future__ import annotations
 
@@ -21,6 +23,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
 
 
 def build(verbose: bool = False) -> str:
@@ -34,13 +37,19 @@ def build(verbose: bool = False) -> str:
         cmd = ["hipcc", "-c", *[f for f in FLAGS if f != "-shared"], s, "-o", o]
         jobs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
         objs.append(o)
+    for s in sorted(glob.glob(os.path.join(HERE, "csrc", "*.cpp"))):
+        o = os.path.join(odir, os.path.basename(s) + ".o")
+        cmd = ["g++", "-c", *HOST_FLAGS, s, "-o", o]
+        jobs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+        objs.append(o)
+        srcs = srcs + [s]
     err = None
     for j, s in zip(jobs, srcs):
         out, _ = j.communicate()
         if verbose and out:
             print(out)
         if j.returncode != 0:
-            err = f"hipcc failed on {s}:\n{out}"
+            err = f"compile failed on {s}:\n{out}"
     if err:
         raise RuntimeError(err)
     tmp = LIB + ".tmp"

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <random>
 #include <string>
@@ -15,6 +16,7 @@
 
 #include "../../include/phx_elph.h"
 #include "coupling_kernel.hpp"
+#include "elph_file.hpp"
 #include "phx_common.hpp"
 #include "screen.hpp"
 
@@ -108,6 +110,63 @@ static void h2d(DevBuf<T>& d, const T* h, size_t n, hipStream_t s = 0) {
 template <typename T>
 static void h2d(DevBuf<T>& d, const std::vector<T>& v, hipStream_t s = 0) {
   h2d(d, v.data(), v.size(), s);
+}
+
+// Writes elements [first, first + n) of the flattened coupling tensor, as
+// interleaved (re, im) doubles, into a host buffer.
+using GwReader = std::function<void(uint64_t first, uint64_t n, double* dst)>;
+
+// Coupling tensor from a host-side reader into device memory: two pinned
+// staging buffers, the read of one chunk overlapping the copy of the other;
+// FP32 handles (precision != FP64) convert each chunk on the device, so no
+// full FP64 copy is ever resident.
+static void upload_gw_stream(const GwReader& rd, size_t n, int precision, DevBuf<double2>& gW,
+                             DevBuf<float2>& gWf, hipStream_t s) {
+  const size_t chunk = std::max<size_t>(1, std::min<size_t>(n, size_t(4) << 20));  // 64 MiB
+  struct Staging {
+    double2* pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Staging() {
+      for (int b = 0; b < 2; ++b) {
+        if (ev[b]) (void)hipEventSynchronize(ev[b]);
+        if (ev[b]) (void)hipEventDestroy(ev[b]);
+        if (pin[b]) (void)hipHostFree(pin[b]);
+      }
+    }
+  } st;
+  for (int b = 0; b < 2; ++b) {
+    PHX_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.pin[b]), chunk * sizeof(double2),
+                          hipHostMallocDefault));
+    PHX_HIP(hipEventCreateWithFlags(&st.ev[b], hipEventDisableTiming));
+  }
+  DevBuf<double2> stage;
+  if (precision == PHX_FP64) {
+    gWf.release();
+    gW.alloc(n);
+  } else {
+    gW.release();
+    gWf.alloc(n);
+    stage.alloc(2 * chunk);
+  }
+  bool used[2] = {false, false};
+  size_t it = 0;
+  for (size_t first = 0; first < n; first += chunk, ++it) {
+    const int b = int(it & 1);
+    const size_t k = std::min(chunk, n - first);
+    if (used[b]) PHX_HIP(hipEventSynchronize(st.ev[b]));  // pin[b] and stage[b] are free again
+    rd(first, k, reinterpret_cast<double*>(st.pin[b]));
+    if (precision == PHX_FP64) {
+      PHX_HIP(hipMemcpyAsync(gW.p + first, st.pin[b], k * sizeof(double2), hipMemcpyHostToDevice, s));
+    } else {
+      double2* sb = stage.p + size_t(b) * chunk;
+      PHX_HIP(hipMemcpyAsync(sb, st.pin[b], k * sizeof(double2), hipMemcpyHostToDevice, s));
+      launch_to_float(reinterpret_cast<const double*>(sb), reinterpret_cast<float*>(gWf.p + first),
+                      2 * k, s);
+    }
+    PHX_HIP(hipEventRecord(st.ev[b], s));
+    used[b] = true;
+  }
+  PHX_HIP(hipStreamSynchronize(s));
 }
 
 // constants.h:9-57 (restated)
@@ -468,6 +527,7 @@ struct BatchMeta {
 };
 
 struct LwHandle {
+  std::vector<double> ownT, ownMu;  // copies of the problem's temperatures and chemical potentials
   phx_problem prob{};
   int precision = PHX_FP64, k1Batch = 16;
   hipStream_t stream = nullptr;
@@ -590,7 +650,7 @@ static void lw_harvest_timing(LwHandle* L) {
 }
 
 static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch,
-                     int transform) {
+                     int transform, const ElphFile* file = nullptr) {
   const phx_problem& p = *P;
   PHX_REQUIRE(precision == PHX_FP64 || precision == PHX_FP32 ||
                   (precision == PHX_BF16X3 && transform == PHX_TRANSFORM_SEPARABLE),
@@ -599,6 +659,11 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   PHX_REQUIRE(p.nCalc >= 1 && p.nCalc <= kCMax, "phx_lw_setup: 1..4 (T, mu) calculations supported");
   PHX_REQUIRE(p.nw <= 44 && p.nm <= 44, "phx_lw_setup: eigensolver size");
   L->prob = p;
+  // the (T, mu) lists are read again after set-up: keep copies
+  L->ownT.assign(p.temperatures, p.temperatures + p.nCalc);
+  L->ownMu.assign(p.chemPots, p.chemPots + p.nCalc);
+  L->prob.temperatures = L->ownT.data();
+  L->prob.chemPots = L->ownMu.data();
   L->precision = precision;
   L->k1Batch = std::min(kCacheNKK, std::max(1, k1Batch));
   PHX_REQUIRE(transform == PHX_TRANSFORM_DIRECT || transform == PHX_TRANSFORM_SEPARABLE,
@@ -802,7 +867,10 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     h2d(L->invDegP, ip, s);
   }
   const size_t gwN = size_t(p.nRe) * p.nRp * nm * nw * nw;
-  if (p.gW) {
+  if (file) {
+    upload_gw_stream([file](uint64_t f0, uint64_t k, double* d) { file->read_gw(f0, k, d); }, gwN,
+                     precision, L->gW, L->gWf, s);
+  } else if (p.gW) {
     h2d(L->gW, reinterpret_cast<const double2*>(p.gW), gwN, s);
     if (precision != PHX_FP64) {
       L->gWf.alloc(gwN);
@@ -1683,6 +1751,9 @@ using namespace phx;
 // ============================================================ C ABI
 struct phx_elph : phx::ElphHandle {};
 struct phx_lw : phx::LwHandle {};
+struct phx_elph_file : phx::ElphFile {
+  using phx::ElphFile::ElphFile;
+};
 
 extern "C" {
 
@@ -1774,6 +1845,90 @@ int phx_elph_create_synthetic(phx_elph** h, int nw, int nm, int nRp, int nRe, co
   });
 }
 
+int phx_elph_create_from_file(phx_elph** h, const char* path, int precision) {
+  return guard([&] {
+    PHX_REQUIRE(h && path, "phx_elph_create_from_file: null argument");
+    phx::ElphFile f(path);
+    const phx::ElphHeader& hd = f.header();
+    std::vector<double> eR(3 * size_t(hd.nRe)), eD(size_t(hd.nRe)), pR(3 * size_t(hd.nRp)),
+        pD(size_t(hd.nRp));
+    f.bravais(eR.data(), eD.data(), pR.data(), pD.data());
+    auto* H = new phx_elph();
+    try {
+      elph_common_init(H, hd.nw, hd.nm, hd.nRp, hd.nRe, eR.data(), eD.data(), pR.data(), pD.data(),
+                       precision);
+      upload_gw_stream([&f](uint64_t f0, uint64_t k, double* d) { f.read_gw(f0, k, d); },
+                       f.gw_elements(), precision, H->gW, H->gWf, H->stream);
+    } catch (...) {
+      delete H;
+      throw;
+    }
+    *h = H;
+  });
+}
+
+int phx_elph_file_open(phx_elph_file** f, const char* path) {
+  return guard([&] {
+    PHX_REQUIRE(f && path, "phx_elph_file_open: null argument");
+    *f = nullptr;
+    *f = new phx_elph_file(path);
+  });
+}
+
+int phx_elph_file_header(phx_elph_file* f, int* v) {
+  return guard([&] {
+    PHX_REQUIRE(f && v, "phx_elph_file_header: null argument");
+    const phx::ElphHeader& h = f->header();
+    const int a[13] = {h.fileFormat, h.numElectrons, h.numSpin, h.nw, h.nm, h.nRe, h.nRp,
+                       h.kMesh[0], h.kMesh[1], h.kMesh[2], h.qMesh[0], h.qMesh[1], h.qMesh[2]};
+    std::memcpy(v, a, sizeof(a));
+  });
+}
+
+int phx_elph_file_bravais(phx_elph_file* f, double* elR, double* elDeg, double* phR,
+                          double* phDeg) {
+  return guard([&] {
+    PHX_REQUIRE(f, "phx_elph_file_bravais: null handle");
+    f->bravais(elR, elDeg, phR, phDeg);
+  });
+}
+
+int phx_elph_file_read_gw(phx_elph_file* f, int re_begin, int re_count, double* out) {
+  return guard([&] {
+    PHX_REQUIRE(f && out, "phx_elph_file_read_gw: null argument");
+    const int nRe = f->header().nRe;
+    PHX_REQUIRE(re_begin >= 0 && re_count >= 0 && re_begin <= nRe && re_count <= nRe - re_begin,
+                "phx_elph_file_read_gw: R_e range outside the file");
+    const uint64_t per = f->gw_elements_per_re();
+    f->read_gw(per * uint64_t(re_begin), per * uint64_t(re_count), out);
+  });
+}
+
+int phx_elph_file_close(phx_elph_file* f) {
+  return guard([&] { delete f; });
+}
+
+int phx_elph_file_write(const char* path, int format, const int* v, const double* elR,
+                        const double* elDeg, const double* phR, const double* phDeg,
+                        const double* gW) {
+  return guard([&] {
+    PHX_REQUIRE(path && v, "phx_elph_file_write: null argument");
+    phx::ElphHeader h;
+    h.fileFormat = format;
+    h.numElectrons = v[0];
+    h.numSpin = v[1];
+    h.nw = v[2];
+    h.nm = v[3];
+    h.nRe = v[4];
+    h.nRp = v[5];
+    for (int i = 0; i < 3; ++i) {
+      h.kMesh[i] = v[6 + i];
+      h.qMesh[i] = v[9 + i];
+    }
+    phx::write_elph_file(path, h, elR, elDeg, phR, phDeg, gW);
+  });
+}
+
 int phx_elph_cache(phx_elph* h, const double* U1, int nb1, const double* k1) {
   return guard([&] {
     PHX_REQUIRE(h && U1 && k1, "phx_elph_cache: null argument");
@@ -1843,6 +1998,42 @@ int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batc
       delete L;
       throw;
     }
+    *h = L;
+  });
+}
+
+int phx_lw_setup_file(phx_lw** h, const phx_problem* prob, const char* elph_path, int precision,
+                      int k1_batch, int transform) {
+  return guard([&] {
+    PHX_REQUIRE(h && prob && elph_path, "phx_lw_setup_file: null argument");
+    phx::ElphFile f(elph_path);
+    const phx::ElphHeader& hd = f.header();
+    PHX_REQUIRE(hd.nw == prob->nw && hd.nm == prob->nm,
+                "phx_lw_setup_file: the el-ph file's numElBands / numPhModes (" +
+                    std::to_string(hd.nw) + ", " + std::to_string(hd.nm) +
+                    ") differ from the band structures (" + std::to_string(prob->nw) + ", " +
+                    std::to_string(prob->nm) + ")");
+    std::vector<double> eR(3 * size_t(hd.nRe)), eD(size_t(hd.nRe)), pR(3 * size_t(hd.nRp)),
+        pD(size_t(hd.nRp));
+    f.bravais(eR.data(), eD.data(), pR.data(), pD.data());
+    phx_problem q = *prob;
+    q.nRe = hd.nRe;
+    q.nRp = hd.nRp;
+    q.Re = eR.data();
+    q.degE = eD.data();
+    q.Rp = pR.data();
+    q.degP = pD.data();
+    q.gW = nullptr;
+    q.gwDecay = nullptr;
+    auto* L = new phx_lw();
+    try {
+      lw_setup(L, &q, precision, k1_batch, transform, &f);
+    } catch (...) {
+      delete L;
+      throw;
+    }
+    // the stored problem must not point at the arrays above
+    L->prob.Re = L->prob.degE = L->prob.Rp = L->prob.degP = nullptr;
     *h = L;
   });
 }
