@@ -134,6 +134,51 @@ int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double max_mem_
 
 int phx_elph_destroy(phx_elph* h);
 
+/* ---------------------------------------------------------------- harmonic models
+ * The inputs of the band structures, read from the files Phoebe reads, with
+ * the set-up work of the reference's constructors.  Host code, except the
+ * *_diag calls, which run on the GPU. */
+typedef struct phx_ph_model phx_ph_model;
+typedef struct phx_el_model phx_el_model;
+
+/* QEParser::parsePhHarmonic (qe_input_parser.cpp:385-562) + the PhononH0
+ * constructor (phonon_h0.cpp:18-235): QE force constants (q2r.x), the acoustic
+ * sum rule ("" none, "simple"; "crystal" is not supported), the Wigner-Seitz
+ * reorder (:426-535, 672-771) and the long-range set-up (:53-160). */
+int phx_ph_model_read_qe(phx_ph_model** m, const char* fc_path, const char* sum_rule);
+/* dims8 = {natoms, nm, nR, nG, hasDielectric, qGrid[3]} */
+int phx_ph_model_dims(phx_ph_model* m, int* dims8);
+/* cell9 / recip9 column-major (columns a_i / b_i, Bohr / Bohr^-1), volume,
+ * positions natoms x 3 (Bohr), masses nm (Ry mass units); any may be NULL. */
+int phx_ph_model_crystal(phx_ph_model* m, double* cell9, double* recip9, double* volume,
+                         double* positions, double* masses);
+/* R nR x 3, weights nR, mat2R nR x nm x nm (R, iAt*3+i, jAt*3+j); any may be NULL. */
+int phx_ph_model_force_constants(phx_ph_model* m, double* R, double* weights, double* mat2R);
+/* born natoms x 9, dielectric 9, gVectors nG x 3, lrConstant natoms x 9, norm_gmax2 = {lrNorm, gMax} */
+int phx_ph_model_long_range(phx_ph_model* m, double* born, double* dielectric, double* gVectors,
+                            double* lrConstant, double* norm_gmax2);
+/* PhononH0::kokkosBatchedDiagonalizeFromCoordinates (phonon_h0_kokkos.cpp:177-207)
+ * with the long-range term: omega nq x nm, eigvecs nq x (nm x nm) column-major,
+ * divided by sqrt(M) when mass_scaling. */
+int phx_ph_model_diag(phx_ph_model* m, int nq, const double* q, int mass_scaling, double* omega,
+                      double* eigvecs);
+int phx_ph_model_destroy(phx_ph_model* m);
+
+/* QEParser::parseElHarmonicWannier (qe_input_parser.cpp:883-1040): Wannier90
+ * seedname_tb.dat.  cell9: the crystal's direct cell (column-major, Bohr), as
+ * the reference passes the phonon crystal; NULL uses the file's cell. */
+int phx_el_model_read_w90(phx_el_model** m, const char* tb_path, const double* cell9);
+/* dims2 = {nw, nR} */
+int phx_el_model_dims(phx_el_model* m, int* dims2);
+/* cell9, R nR x 3 (Cartesian Bohr), deg nR, h0R nR x nw x nw complex (Ry),
+ * rMatrix 3 x nR x nw x nw complex (Bohr); any may be NULL. */
+int phx_el_model_arrays(phx_el_model* m, double* cell9, double* R, double* deg, double* h0R,
+                        double* rMatrix);
+/* ElectronH0Wannier::kokkosBatchedDiagonalizeFromCoordinates
+ * (electron_h0_wannier.cpp:777-877): energies nk x nw, eigvecs nk x (nw x nw). */
+int phx_el_model_diag(phx_el_model* m, int nk, const double* k, double* energies, double* eigvecs);
+int phx_el_model_destroy(phx_el_model* m);
+
 /* ---------------------------------------------------------------- el-ph file
  * Phoebe's prefix.phoebe.elph.hdf5, read and written without libhdf5 (the
  * library's own HDF5 subset: root-group datasets, contiguous or compact
@@ -199,6 +244,16 @@ typedef struct {
   int nCalc;                 /* (T, mu) pairs, Ry */
   const double *temperatures, *chemPots;
   double sigma, winMin, winMax; /* Gaussian width, energy window (Ry) */
+  /* rigid-ion long-range term of D(q), present when the force constants come
+   * with a dielectric tensor (PhononH0, phonon_h0.cpp:53-160 and
+   * phonon_h0_kokkos.cpp:48-149); nG = 0 leaves it out. */
+  int natoms, nG;
+  const double* atomPos;     /* natoms x 3, Cartesian Bohr */
+  const double* bornCharges; /* natoms x 3 x 3, (atom, i, j), after the sum rule */
+  const double* dielectric;  /* 3 x 3, (i, j) */
+  const double* gVectors;    /* nG x 3 */
+  const double* lrConstant;  /* natoms x 3 x 3, (atom, i, j): added to the diagonal blocks */
+  double lrNorm, gMax;       /* e2 * fourPi / volume (fourPi as constants.h:12 defines it), 14 */
 } phx_problem;
 
 typedef struct phx_lw phx_lw;
@@ -213,6 +268,14 @@ int phx_lw_setup(phx_lw** h, const phx_problem* prob, int precision, int k1_batc
  * file into device memory. */
 int phx_lw_setup_file(phx_lw** h, const phx_problem* prob, const char* elph_path, int precision,
                       int k1_batch, int transform);
+
+/* Points prob's band-structure fields (recip from the phonon crystal, the
+ * electron Hamiltonian, the force constants and the long-range term) at the
+ * models' arrays, which must outlive prob's use; the coupling, mesh, (T, mu),
+ * smearing and window fields are left to the caller.  As in the reference's
+ * apps (electron_wannier_transport_app.cpp:16-20), the electron model should
+ * be read with the phonon crystal's cell. */
+int phx_problem_set_models(phx_problem* prob, phx_el_model* el, phx_ph_model* ph);
 int phx_lw_num_active(phx_lw* h);
 int phx_lw_num_states(phx_lw* h);
 /* active points (full-mesh indices), band extrema, energies of the states */

@@ -264,8 +264,67 @@ int orc_el_diag(int nw, int nR, const double* R, const double* deg, const double
 // Phonon dynamical matrix: phonon_h0_kokkos.cpp:6-46 (FT with e^{-iq.R} w(R)),
 // :150-170 (hermitize, divide by sqrt(M_m M_n)); mat2R is (nR, nm, nm).
 // ---------------------------------------------------------------------------
+// Rigid-ion long-range term (PhononH0::addLongRangeTerm, phonon_h0.cpp:537-670,
+// the CPU form of QE's rgd_blk): constant part on the diagonal blocks, then
+// the G-space sum with k = G + q,
+//   dyn(i, j, na, nb) += normG(k) e^{i (tau_na - tau_nb).k} kZ(i, na) kZ(j, nb),
+//   kZ(i, nb) = sum_l k_l born(nb, l, i),  normG = norm e^{-k.eps.k / 4} / (k.eps.k)
+//   for 0 < k.eps.k < 4 gMax.
+struct orc_long_range {
+  int nat, nG;
+  const double *tau, *born, *eps, *G, *cst;  // nat x 3, nat x 9, 9, nG x 3, nat x 9
+  double norm, gMax;
+};
+
+static void orc_add_long_range(int nm, const orc_long_range& lr, const double* q, cd* dyn /* (m, n) row-major */) {
+  const int nat = lr.nat;
+  for (int na = 0; na < nat; ++na)
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) dyn[size_t(na * 3 + i) * nm + na * 3 + j] += lr.cst[9 * na + 3 * i + j];
+  std::vector<double> kZ(size_t(3) * nat);
+  std::vector<cd> ph(size_t(nat) * nat);
+  for (int ig = 0; ig < lr.nG; ++ig) {
+    double k[3];
+    for (int i = 0; i < 3; ++i) k[i] = lr.G[3 * ig + i] + q[i];
+    double geg = 0.;
+    for (int i = 0; i < 3; ++i) {
+      double t = 0.;
+      for (int j = 0; j < 3; ++j) t += lr.eps[3 * i + j] * k[j];
+      geg += k[i] * t;
+    }
+    if (!(geg > 0. && geg < 4. * lr.gMax)) continue;
+    const double normG = lr.norm * std::exp(-geg * 0.25) / geg;
+    for (int i = 0; i < 3; ++i)
+      for (int nb = 0; nb < nat; ++nb)
+        kZ[size_t(i) * nat + nb] = k[0] * lr.born[9 * nb + 0 + i] + k[1] * lr.born[9 * nb + 3 + i] +
+                                   k[2] * lr.born[9 * nb + 6 + i];
+    for (int nb = 0; nb < nat; ++nb)
+      for (int na = 0; na < nat; ++na) {
+        double arg = 0.;
+        for (int i = 0; i < 3; ++i) arg += (lr.tau[3 * na + i] - lr.tau[3 * nb + i]) * k[i];
+        ph[size_t(na) * nat + nb] = cd(std::cos(arg), std::sin(arg));
+      }
+    for (int nb = 0; nb < nat; ++nb)
+      for (int na = 0; na < nat; ++na)
+        for (int j = 0; j < 3; ++j)
+          for (int i = 0; i < 3; ++i)
+            dyn[size_t(na * 3 + i) * nm + nb * 3 + j] +=
+                normG * ph[size_t(na) * nat + nb] * cd(kZ[size_t(i) * nat + na] * kZ[size_t(j) * nat + nb], 0.);
+  }
+}
+
+static void orc_ph_dynamical_lr(int nm, int nR, const double* R, const double* w, const double* mat2R,
+                                const double* masses, const double* q, const orc_long_range* lr,
+                                double* D);
+
 void orc_ph_dynamical(int nm, int nR, const double* R, const double* w, const double* mat2R,
                       const double* masses, const double* q, double* D) {
+  orc_ph_dynamical_lr(nm, nR, R, w, mat2R, masses, q, nullptr, D);
+}
+
+static void orc_ph_dynamical_lr(int nm, int nR, const double* R, const double* w, const double* mat2R,
+                                const double* masses, const double* q, const orc_long_range* lr,
+                                double* D) {
   std::vector<cd> phases(nR);
   for (int iR = 0; iR < nR; ++iR) {
     double arg = q[0] * R[3 * iR] + q[1] * R[3 * iR + 1] + q[2] * R[3 * iR + 2];
@@ -278,6 +337,7 @@ void orc_ph_dynamical(int nm, int nR, const double* R, const double* w, const do
       for (int iR = 0; iR < nR; ++iR) tmp += phases[iR] * mat2R[(size_t(iR) * nm + m) * nm + n];
       dyn[size_t(m) * nm + n] = tmp;
     }
+  if (lr && lr->nG > 0) orc_add_long_range(nm, *lr, q, dyn.data());
   for (int n = 0; n < nm; ++n)
     for (int m = 0; m < nm; ++m) {
       cd x = 0.5 * (dyn[size_t(m) * nm + n] + std::conj(dyn[size_t(n) * nm + m]));
@@ -288,9 +348,10 @@ void orc_ph_dynamical(int nm, int nR, const double* R, const double* w, const do
 
 // Frequencies omega = sign(l) sqrt|l| and mass-scaled eigenvectors e/sqrt(M)
 // (phonon_h0_kokkos.cpp:177-219).
-int orc_ph_diag(int nm, int nR, const double* R, const double* w, const double* mat2R,
-                const double* masses, const double* q, double* omega, double* eigvecs) {
-  orc_ph_dynamical(nm, nR, R, w, mat2R, masses, q, eigvecs);
+int orc_ph_diag_lr(int nm, int nR, const double* R, const double* w, const double* mat2R,
+                   const double* masses, const double* q, const orc_long_range* lr, double* omega,
+                   double* eigvecs) {
+  orc_ph_dynamical_lr(nm, nR, R, w, mat2R, masses, q, lr, eigvecs);
   if (orc_zheev(nm, eigvecs, omega) < 0) return -1;
   for (int m = 0; m < nm; ++m)
     omega[m] = omega[m] > 0. ? std::sqrt(omega[m]) : -std::sqrt(-omega[m]);
@@ -300,6 +361,11 @@ int orc_ph_diag(int nm, int nR, const double* R, const double* w, const double* 
       st(eigvecs, i, ld(eigvecs, i) / std::sqrt(masses[m]));
     }
   return 0;
+}
+
+int orc_ph_diag(int nm, int nR, const double* R, const double* w, const double* mat2R,
+                const double* masses, const double* q, double* omega, double* eigvecs) {
+  return orc_ph_diag_lr(nm, nR, R, w, mat2R, masses, q, nullptr, omega, eigvecs);
 }
 
 // ---------------------------------------------------------------------------
@@ -508,6 +574,9 @@ struct orc_problem {
   int nCalc;
   const double *temperatures, *chemPots;
   double sigma, winMin, winMax;
+  int natoms, nG;  // long-range term (nG = 0: none), same fields as phx_problem
+  const double *atomPos, *bornCharges, *dielectric, *gVectors, *lrConstant;
+  double lrNorm, gMax;
 };
 
 struct orc_state {
@@ -644,8 +713,11 @@ void* orc_setup(const orc_problem* prob) {
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : fail)
   for (int iq = 0; iq < s->nk; ++iq) {
     if (!s->qInSet[iq]) continue;
-    fail += orc_ph_diag(p.nm, p.nRf, p.Rf, p.wf, p.mat2R, p.masses, &s->kcart[3 * size_t(iq)],
-                        &s->omega[size_t(iq) * p.nm], &s->phev[2 * size_t(iq) * p.nm * p.nm]) != 0;
+    const orc_long_range lr{p.natoms, p.nG, p.atomPos, p.bornCharges, p.dielectric, p.gVectors,
+                            p.lrConstant, p.lrNorm, p.gMax};
+    fail += orc_ph_diag_lr(p.nm, p.nRf, p.Rf, p.wf, p.mat2R, p.masses, &s->kcart[3 * size_t(iq)],
+                           p.nG > 0 ? &lr : nullptr, &s->omega[size_t(iq) * p.nm],
+                           &s->phev[2 * size_t(iq) * p.nm * p.nm]) != 0;
   }
   if (fail) {
     delete s;

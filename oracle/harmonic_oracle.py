@@ -1,0 +1,295 @@
+"""Oracle (test infrastructure only): numpy restatement of the harmonic inputs
+of the el-ph path, so that the library's C++ readers (phoebe_amd/csrc/
+harmonic_io.cpp) can be checked against an independent reading.
+
+Only tests/ may import this module.
+
+* QE force constants (q2r.x prefix.fc) and the PhononH0 set-up:
+  QEParser::parsePhHarmonic (qe_input_parser.cpp:385-562), the "simple"
+  acoustic sum rule (phonon_h0_sum_rule.cpp:44-76), Wigner-Seitz weights
+  (phonon_h0.cpp:426-535), the reorder into (R, weight, block) triples
+  (:672-771), the long-range set-up (:53-160) and the CPU dynamical matrix
+  (:773-813 short range, :537-670 long range).
+* Wannier90 seedname_tb.dat: QEParser::parseElHarmonicWannier
+  (qe_input_parser.cpp:883-1040) and H(k) (electron_h0_wannier.cpp:150-190).
+
+Units: Bohr, Ry, Ry mass units (as the files and the reference use them).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BOHR_SI = 0.52917721067e-10
+DISTANCE_BOHR_TO_ANG = BOHR_SI / 1.0e-10
+RYDBERG_SI = 4.359744650e-18 / 2.0
+ENERGY_RY_TO_EV = RYDBERG_SI / 1.6021766208e-19
+E2 = 2.0
+FOUR_PI_AS_DEFINED = 2.0 * np.pi    # constants.h:12 sets fourPi = 2 pi
+G_MAX = 14.0
+HBAR_SI = 6.62607015e-34 / (2 * np.pi)
+VELOCITY_RY_TO_SI = BOHR_SI * RYDBERG_SI / HBAR_SI
+
+
+def _num(tok: str) -> float:
+    return float(tok.replace("D", "E").replace("d", "e"))
+
+
+def qe_cell(ibrav: int, celldm, cell0=None) -> np.ndarray:
+    """Columns a1, a2, a3 (Bohr) for the QE lattices the reader supports."""
+    a = celldm[0]
+    if ibrav == 0:
+        A = np.array(cell0, dtype=float)
+        return A * a if a != 0 else A
+    if ibrav == 1:
+        return a * np.eye(3)
+    if ibrav == 2:
+        return a / 2 * np.array([[-1, 0, 1], [0, 1, 1], [-1, 1, 0]], dtype=float).T
+    if ibrav == 3:
+        return a / 2 * np.array([[1, 1, 1], [-1, 1, 1], [-1, -1, 1]], dtype=float).T
+    if ibrav == -3:
+        return a / 2 * np.array([[-1, 1, 1], [1, -1, 1], [1, 1, -1]], dtype=float).T
+    if ibrav == 6:
+        return np.diag([a, a, a * celldm[2]])
+    if ibrav == 8:
+        return np.diag([a, a * celldm[1], a * celldm[2]])
+    raise ValueError(f"ibrav {ibrav} not restated in the oracle")
+
+
+def read_qe_fc(path: str) -> dict:
+    """The raw content of a q2r.x force-constant file."""
+    lines = open(path).read().splitlines()
+    it = iter(lines)
+    head = next(it).split()
+    ntyp, nat, ibrav = int(head[0]), int(head[1]), int(head[2])
+    celldm = [_num(x) for x in head[3:9]]
+    cell0 = None
+    if ibrav == 0:
+        cell0 = np.array([[_num(x) for x in next(it).split()[:3]] for _ in range(3)]).T
+    A = qe_cell(ibrav, celldm, cell0)
+    alat = celldm[0] if celldm[0] != 0 else np.linalg.norm(A[:, 0])
+    names, smass = [], []
+    for _ in range(ntyp):
+        ln = next(it)
+        q1 = ln.index("'")
+        q2 = ln.index("'", q1 + 1)
+        names.append(ln[q1 + 1:q2].strip())
+        smass.append(_num(ln[q2 + 1:].split()[0]))
+    species, tau = [], []
+    for _ in range(nat):
+        f = next(it).split()
+        species.append(int(f[1]) - 1)
+        tau.append([alat * _num(x) for x in f[2:5]])
+    eps = np.zeros((3, 3))
+    born = np.zeros((nat, 3, 3))
+    if next(it).strip() == "T":
+        eps = np.array([[_num(x) for x in next(it).split()[:3]] for _ in range(3)])
+        for a in range(nat):
+            next(it)
+            born[a] = [[_num(x) for x in next(it).split()[:3]] for _ in range(3)]
+    grid = tuple(int(x) for x in next(it).split()[:3])
+    g0, g1, g2 = grid
+    # fc[i, j, n1, n2, n3, na, nb]
+    fc = np.zeros((3, 3, g0, g1, g2, nat, nat))
+    for i in range(3):
+        for j in range(3):
+            for na in range(nat):
+                for nb in range(nat):
+                    lab = [int(x) for x in next(it).split()[:4]]
+                    assert lab == [i + 1, j + 1, na + 1, nb + 1], lab
+                    for n3 in range(g2):
+                        for n2 in range(g1):
+                            for n1 in range(g0):
+                                f = next(it).split()
+                                assert [int(x) for x in f[:3]] == [n1 + 1, n2 + 1, n3 + 1]
+                                fc[i, j, n1, n2, n3, na, nb] = _num(f[3])
+    return dict(ibrav=ibrav, celldm=celldm, cell=A, names=names, species_masses=np.array(smass),
+                species=np.array(species), tau=np.array(tau), eps=eps, born=born, grid=grid, fc=fc)
+
+
+def sum_rule_simple(d: dict) -> dict:
+    """phonon_h0_sum_rule.cpp:44-76: mean Born charge removed; the self term
+    absorbs the row sums of the force constants."""
+    out = dict(d)
+    born = d["born"].copy()
+    born -= born.sum(axis=0, keepdims=True) / born.shape[0]
+    fc = d["fc"].copy()
+    nat = fc.shape[-1]
+    rows = fc.sum(axis=(2, 3, 4, 6))            # (i, j, na)
+    for na in range(nat):
+        fc[:, :, 0, 0, 0, na, na] -= rows[:, :, na]
+    out["born"], out["fc"] = born, fc
+    return out
+
+
+def ws_reorder(d: dict):
+    """(R (nR, 3), weights (nR,), mat2R (nR, nm, nm)) in the reference's order:
+    n3, n2, n1 outer to inner over [-2g, 2g], then nb, then na."""
+    A, tau, fc = d["cell"], d["tau"], d["fc"]
+    g = np.array(d["grid"])
+    nat = len(tau)
+    sup = A * g[None, :]
+    rng = np.arange(-2, 3)
+    ii, jj, kk = np.meshgrid(rng, rng, rng, indexing="ij")
+    rws = (np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1) @ sup.T)
+    rws = rws[(rws ** 2).sum(1) > 1e-6]
+    half = (rws ** 2).sum(1) / 2
+    R, W, M = [], [], []
+    nm = 3 * nat
+    b = 2 * g
+    for n3 in range(-b[2], b[2] + 1):
+        for n2 in range(-b[1], b[1] + 1):
+            for n1 in range(-b[0], b[0] + 1):
+                base = n1 * A[:, 0] + n2 * A[:, 1] + n3 * A[:, 2]
+                for nb in range(nat):
+                    for na in range(nat):
+                        r = base + (tau[na] - tau[nb])
+                        ck = rws @ r - half
+                        if np.any(ck > 1e-5):
+                            continue
+                        w = 1.0 / (1 + int(np.sum(np.abs(ck) <= 1e-5)))
+                        m = np.zeros((nm, nm))
+                        m[3 * na:3 * na + 3, 3 * nb:3 * nb + 3] = fc[:, :, n1 % g[0], n2 % g[1], n3 % g[2], na, nb]
+                        R.append(base.copy())
+                        W.append(w)
+                        M.append(m)
+    return np.array(R), np.array(W), np.array(M)
+
+
+def ws_weight_brute(r: np.ndarray, sup: np.ndarray, span: int = 3, tol: float = 1e-5) -> float:
+    """The Wigner-Seitz weight from its definition: 1/N when r is among the N
+    shortest of its images r + L (L a supercell vector; ties within 2 tol in
+    squared length, the reference's tolerance), else 0."""
+    rng = np.arange(-span, span + 1)
+    ii, jj, kk = np.meshgrid(rng, rng, rng, indexing="ij")
+    L = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1) @ sup.T
+    d2 = ((r[None, :] + L) ** 2).sum(1)
+    r2 = float(r @ r)
+    if np.any(d2 < r2 - 2 * tol):
+        return 0.0
+    return 1.0 / int(np.sum(np.abs(d2 - r2) <= 2 * tol))
+
+
+def long_range_setup(d: dict):
+    """G vectors (nG, 3) and the constant term (nat, 3, 3) (phonon_h0.cpp:53-160)."""
+    A, eps, born, tau = d["cell"], d["eps"], d["born"], d["tau"]
+    B = 2 * np.pi * np.linalg.inv(A).T
+    vol = abs(np.linalg.det(A))
+    norm = E2 * FOUR_PI_AS_DEFINED / vol
+    cutoff = 4 * G_MAX
+    nrx = [int(np.sqrt(cutoff) / np.linalg.norm(B[:, c])) + 1 if d["grid"][c] > 1 else 0 for c in range(3)]
+    G = np.array([m1 * B[:, 0] + m2 * B[:, 1] + m3 * B[:, 2]
+                  for m1 in range(-nrx[0], nrx[0] + 1)
+                  for m2 in range(-nrx[1], nrx[1] + 1)
+                  for m3 in range(-nrx[2], nrx[2] + 1)])
+    nat = len(tau)
+    cst = np.zeros((nat, 3, 3))
+    geg = np.einsum("gi,ij,gj->g", G, eps, G)
+    ok = (geg > 0) & (geg < 4 * G_MAX)
+    for g, gg in zip(G[ok], geg[ok]):
+        normG = norm * np.exp(-gg / 4) / gg
+        gZ = np.einsum("k,akj->aj", g, born)               # (atom, i)
+        cosm = np.cos((tau[:, None, :] - tau[None, :, :]) @ g)   # (na, nb)
+        fnAt = cosm @ gZ                                    # (na, j)
+        cst += -normG * gZ[:, :, None] * fnAt[:, None, :]
+    return G, cst, norm
+
+
+def dynamical_matrix(R, W, M, q, masses=None, lr=None):
+    """D(q) (nm, nm): sum_R w e^{-iq.R} M(R) + long range, hermitized, / sqrt(M M)."""
+    D = np.tensordot(W * np.exp(-1j * (R @ q)), M, axes=(0, 0))
+    if lr is not None:
+        G, cst, norm, eps, born, tau = lr
+        nat = len(tau)
+        for a in range(nat):
+            D[3 * a:3 * a + 3, 3 * a:3 * a + 3] += cst[a]
+        K = G + q[None, :]
+        geg = np.einsum("gi,ij,gj->g", K, eps, K)
+        ok = (geg > 0) & (geg < 4 * G_MAX)
+        for k, gg in zip(K[ok], geg[ok]):
+            normG = norm * np.exp(-gg / 4) / gg
+            kZ = np.einsum("l,ali->ai", k, born).reshape(-1)          # (a*3 + i)
+            ph = np.exp(1j * (tau @ k))
+            v = np.repeat(ph, 3) * kZ
+            D += normG * np.outer(v, v.conj())
+    D = 0.5 * (D + D.conj().T)
+    if masses is not None:
+        D = D / np.sqrt(np.outer(masses, masses))
+    return D
+
+
+def frequencies(D):
+    w, v = np.linalg.eigh(D)
+    return np.sign(w) * np.sqrt(np.abs(w)), v
+
+
+def read_w90_tb(path: str, cell=None) -> dict:
+    """seedname_tb.dat: cell (Bohr, columns), R (Cartesian Bohr), deg,
+    h0R (nR, nw, nw) with the block's lines in (i, j) order, i slowest, in Ry."""
+    with open(path) as fh:
+        lines = [ln for ln in fh.read().splitlines()]
+    p = 1
+    A = np.array([[float(x) for x in lines[p + i].split()[:3]] for i in range(3)]).T / DISTANCE_BOHR_TO_ANG
+    p += 3
+    nw = int(lines[p].split()[0])
+    nR = int(lines[p + 1].split()[0])
+    p += 2
+    deg = []
+    while len(deg) < nR:
+        deg += [float(x) for x in lines[p].split()]
+        p += 1
+    rows = [ln.split() for ln in lines[p:] if ln.strip()]
+    Rc, h0R, labels = [], np.zeros((nR, nw, nw), dtype=complex), np.zeros((nR, nw * nw, 2), dtype=int)
+    q = 0
+    for iR in range(nR):
+        Rc.append([float(x) for x in rows[q][:3]])
+        q += 1
+        blk = rows[q:q + nw * nw]
+        q += nw * nw
+        vals = np.array([[float(r[2]), float(r[3])] for r in blk])
+        labels[iR] = [[int(r[0]), int(r[1])] for r in blk]
+        h0R[iR] = ((vals[:, 0] / ENERGY_RY_TO_EV) + 1j * (vals[:, 1] / ENERGY_RY_TO_EV)).reshape(nw, nw)
+    Rc = np.array(Rc)
+    use = A if cell is None else np.asarray(cell)
+    return dict(cell=use, file_cell=A, nw=nw, Rcrys=Rc, R=Rc @ use.T, deg=np.array(deg), h0R=h0R,
+                labels=labels)
+
+
+def hamiltonian(tb: dict, k: np.ndarray) -> np.ndarray:
+    ph = np.exp(1j * (tb["R"] @ k)) / tb["deg"]
+    return np.tensordot(ph, tb["h0R"], axes=(0, 0))
+
+
+def phonon_velocities(R, W, M, q, masses, lr=None, delta=1.0e-8, deg_tol_cm=1.0e-4):
+    """Velocity operator (nm, nm, 3) of PhononH0::diagonalizeVelocityFromCoordinates
+    (phonon_h0.cpp:895-1070): central differences of sqrt(D) (eigenvectors
+    without the mass scaling), hermitized, rotated to the eigenbasis at q;
+    the block of each degenerate group (frequencies within 1e-4 cm^-1) is
+    diagonalized per Cartesian direction; zero at Gamma."""
+    ry_to_cmm1 = 1.0e10 * (1.0 / (6.62607015e-34 / (2 * np.pi) / 4.359744650e-18 * 1.0e12) / 2.0 / (2 * np.pi)) / 299792458.0
+    w, U = frequencies(dynamical_matrix(R, W, M, q, masses, lr))
+    nm = len(w)
+    v = np.zeros((nm, nm, 3), dtype=complex)
+    for i in range(3):
+        dq = np.zeros(3)
+        dq[i] = delta
+        wp, Up = frequencies(dynamical_matrix(R, W, M, q + dq, masses, lr))
+        wm, Um = frequencies(dynamical_matrix(R, W, M, q - dq, masses, lr))
+        der = ((Up * wp) @ Up.conj().T - (Um * wm) @ Um.conj().T) / (2 * delta)
+        der = 0.5 * (der + der.conj().T)
+        v[:, :, i] = U.conj().T @ der @ U
+    ib = 0
+    while ib < nm:
+        s = 1
+        while ib + s < nm and abs(w[ib] - w[ib + s]) <= deg_tol_cm / ry_to_cmm1:
+            s += 1
+        if s > 1:
+            for i in range(3):
+                sub = v[ib:ib + s, ib:ib + s, i]
+                sub = 0.5 * (sub + sub.conj().T)
+                _, X = np.linalg.eigh(sub)
+                sub = X.conj().T @ sub @ X
+                v[ib:ib + s, ib:ib + s, i] = 0.5 * (sub + sub.conj().T)
+        ib += s
+    if np.linalg.norm(q) < 1e-6:
+        v[:] = 0
+    return w, v

@@ -29,6 +29,9 @@ class OrcProblem(C.Structure):
         ("degP", _dp), ("gW", _dp),
         ("nCalc", C.c_int), ("temperatures", _dp), ("chemPots", _dp),
         ("sigma", C.c_double), ("winMin", C.c_double), ("winMax", C.c_double),
+        ("natoms", C.c_int), ("nG", C.c_int), ("atomPos", _dp), ("bornCharges", _dp),
+        ("dielectric", _dp), ("gVectors", _dp), ("lrConstant", _dp),
+        ("lrNorm", C.c_double), ("gMax", C.c_double),
     ]
 
 
@@ -224,6 +227,13 @@ class Oracle:
         s.nCalc = len(prob.temperatures)
         s.temperatures, s.chemPots = _p(keep(prob.temperatures)), _p(keep(prob.chem_pots))
         s.sigma, s.winMin, s.winMax = prob.sigma, prob.win_min, prob.win_max
+        lr = getattr(prob, "long_range", None)
+        if lr is not None and lr["nG"] > 0:
+            s.natoms, s.nG = int(lr["natoms"]), int(lr["nG"])
+            s.atomPos, s.bornCharges = _p(keep(lr["positions"])), _p(keep(lr["born"]))
+            s.dielectric, s.gVectors = _p(keep(lr["dielectric"])), _p(keep(lr["gVectors"]))
+            s.lrConstant = _p(keep(lr["constant"]))
+            s.lrNorm, s.gMax = float(lr["norm"]), float(lr["gMax"])
         self._s = s
         self.h = L.orc_setup(C.byref(s))
         if not self.h:

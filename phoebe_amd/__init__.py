@@ -53,6 +53,9 @@ class PhxProblem(C.Structure):
         ("degP", _dp), ("gW", _dp), ("gwDecay", _dp), ("gwSeed", C.c_ulonglong),
         ("nCalc", C.c_int), ("temperatures", _dp), ("chemPots", _dp),
         ("sigma", C.c_double), ("winMin", C.c_double), ("winMax", C.c_double),
+        ("natoms", C.c_int), ("nG", C.c_int), ("atomPos", _dp), ("bornCharges", _dp),
+        ("dielectric", _dp), ("gVectors", _dp), ("lrConstant", _dp),
+        ("lrNorm", C.c_double), ("gMax", C.c_double),
     ]
 
 
@@ -119,6 +122,19 @@ EXPORTS = {
     "phx_elph_create_from_file": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
     "phx_lw_setup_file": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_char_p,
                                     C.c_int, C.c_int, C.c_int]),
+    "phx_ph_model_read_qe": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p, C.c_char_p]),
+    "phx_ph_model_dims": (C.c_int, [C.c_void_p, _ip]),
+    "phx_ph_model_crystal": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "phx_ph_model_force_constants": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "phx_ph_model_long_range": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "phx_ph_model_diag": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "phx_ph_model_destroy": (C.c_int, [C.c_void_p]),
+    "phx_el_model_read_w90": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p, _dp]),
+    "phx_el_model_dims": (C.c_int, [C.c_void_p, _ip]),
+    "phx_el_model_arrays": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "phx_el_model_diag": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "phx_el_model_destroy": (C.c_int, [C.c_void_p]),
+    "phx_problem_set_models": (C.c_int, [C.POINTER(PhxProblem), C.c_void_p, C.c_void_p]),
 }
 
 
@@ -408,8 +424,15 @@ def make_problem_struct(prob, gW=None):
         g = gW.view(np.float64) if gW.dtype == np.complex128 else gW
         keep.append(g)
         s.gW = _p(g)
-    s.gwDecay = k(prob.gw_decay)
-    s.gwSeed = int(prob.gw_seed)
+    if getattr(prob, "gw_decay", None) is not None:
+        s.gwDecay = k(prob.gw_decay)
+    s.gwSeed = int(getattr(prob, "gw_seed", 0))
+    lr = getattr(prob, "long_range", None)
+    if lr is not None and lr["nG"] > 0:
+        s.natoms, s.nG = int(lr["natoms"]), int(lr["nG"])
+        s.atomPos, s.bornCharges = k(lr["positions"]), k(lr["born"])
+        s.dielectric, s.gVectors, s.lrConstant = k(lr["dielectric"]), k(lr["gVectors"]), k(lr["constant"])
+        s.lrNorm, s.gMax = float(lr["norm"]), float(lr["gMax"])
     s.nCalc = len(prob.temperatures)
     s.temperatures, s.chemPots = k(prob.temperatures), k(prob.chem_pots)
     s.sigma, s.winMin, s.winMax = prob.sigma, prob.win_min, prob.win_max

@@ -17,6 +17,7 @@
 #include "../../include/phx_elph.h"
 #include "coupling_kernel.hpp"
 #include "elph_file.hpp"
+#include "harmonic_io.hpp"
 #include "phx_common.hpp"
 #include "screen.hpp"
 
@@ -27,6 +28,8 @@ void launch_jacobi(int N, int M, double2* A, double* W, int* info, const double*
                    hipStream_t stream);
 void launch_bloch(bool phonon, int nk, int nw, int nR, const double* kc, const double* R,
                   const double* rw, const void* coef, double2* out, hipStream_t stream);
+void launch_ph_long_range(int nq, const double* qc, const LongRange& lr, double2* D,
+                          hipStream_t stream);
 void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
                      hipStream_t s);
 void launch_synth_gw_f32(float2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
@@ -111,6 +114,34 @@ template <typename T>
 static void h2d(DevBuf<T>& d, const std::vector<T>& v, hipStream_t s = 0) {
   h2d(d, v.data(), v.size(), s);
 }
+
+// Device copy of the long-range term's data (LongRange holds its pointers).
+struct DevLongRange {
+  DevBuf<double> tau, born, eps, G, cst;
+  LongRange lr;
+  bool on() const { return lr.nat > 0 && lr.nG > 0; }
+  void upload(int nat, int nG, const double* tau_, const double* born_, const double* eps_,
+              const double* G_, const double* cst_, double norm, double gMax, hipStream_t s) {
+    lr = LongRange();
+    if (nat <= 0 || nG <= 0) return;
+    PHX_REQUIRE(tau_ && born_ && eps_ && G_ && cst_, "long-range term: null array");
+    PHX_REQUIRE(nat <= 12, "long-range term: at most 12 atoms");
+    h2d(tau, tau_, size_t(nat) * 3, s);
+    h2d(born, born_, size_t(nat) * 9, s);
+    h2d(eps, eps_, 9, s);
+    h2d(G, G_, size_t(nG) * 3, s);
+    h2d(cst, cst_, size_t(nat) * 9, s);
+    lr.nat = nat;
+    lr.nG = nG;
+    lr.tau = tau.p;
+    lr.born = born.p;
+    lr.eps = eps.p;
+    lr.G = G.p;
+    lr.cst = cst.p;
+    lr.norm = norm;
+    lr.gMax = gMax;
+  }
+};
 
 // Writes elements [first, first + n) of the flattened coupling tensor, as
 // interleaved (re, im) doubles, into a host buffer.
@@ -268,7 +299,7 @@ static void zheev_batched(int M, int N, double* A, double* W, int* info) {
 
 static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, const double* R,
                        const double* rw, const void* coef, const double* masses, int massScaling,
-                       double* energies, double* eigvecs) {
+                       double* energies, double* eigvecs, const PhononModel* lrModel = nullptr) {
   hipStream_t s = 0;
   DevBuf<double> dk, dR, drw, dm;
   h2d(dk, k, size_t(nk) * 3);
@@ -284,6 +315,13 @@ static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, cons
   launch_bloch(phonon, nk, n, nR, dk.p, dR.p, drw.p,
                phonon ? static_cast<const void*>(dcoefr.p) : static_cast<const void*>(dcoefc.p),
                H.p, s);
+  DevLongRange dlr;
+  if (phonon && lrModel && lrModel->hasDielectric) {
+    const PhononModel& m = *lrModel;
+    dlr.upload(m.nat, m.nG, m.positions.data(), m.born.data(), m.dielectric, m.gVectors.data(),
+               m.lrConstant.data(), m.lrNorm, m.gMax, s);
+    if (dlr.on()) launch_ph_long_range(nk, dk.p, dlr.lr, H.p, s);
+  }
   const double* mptr = nullptr;
   std::vector<double> ones;
   if (phonon) {
@@ -815,6 +853,13 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     DevBuf<double> w(size_t(nq) * nm);
     DevBuf<int> qinfo(nq);
     launch_bloch(true, nq, nm, p.nRf, dq.p, dRf.p, dwf.p, dmat.p, D.p, s);
+    DevLongRange dlr;
+    if (p.nG > 0) {
+      PHX_REQUIRE(3 * p.natoms == nm, "phx_problem: natoms * 3 != nm with a long-range term");
+      dlr.upload(p.natoms, p.nG, p.atomPos, p.bornCharges, p.dielectric, p.gVectors, p.lrConstant,
+                 p.lrNorm, p.gMax, s);
+      launch_ph_long_range(nq, dq.p, dlr.lr, D.p, s);
+    }
     launch_jacobi(nm, nq, D.p, w.p, qinfo.p, dmass.p, s);
     launch_scatter_ph(nq, nm, dql.p, w.p, D.p, L->Wall.p, L->E3all.p, s);
     std::vector<int> qinf(nq);
@@ -1754,6 +1799,12 @@ struct phx_lw : phx::LwHandle {};
 struct phx_elph_file : phx::ElphFile {
   using phx::ElphFile::ElphFile;
 };
+struct phx_ph_model {
+  phx::PhononModel m;
+};
+struct phx_el_model {
+  phx::ElectronModel m;
+};
 
 extern "C" {
 
@@ -1842,6 +1893,167 @@ int phx_elph_create_synthetic(phx_elph** h, int nw, int nm, int nRp, int nRe, co
       throw;
     }
     *h = H;
+  });
+}
+
+int phx_ph_model_read_qe(phx_ph_model** m, const char* fc_path, const char* sum_rule) {
+  return guard([&] {
+    PHX_REQUIRE(m && fc_path, "phx_ph_model_read_qe: null argument");
+    *m = nullptr;
+    auto* M = new phx_ph_model();
+    try {
+      M->m = phx::read_qe_fc(fc_path, sum_rule ? sum_rule : "");
+      PHX_REQUIRE(M->m.nm <= 36, "phx_ph_model_read_qe: at most 12 atoms");
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *m = M;
+  });
+}
+
+int phx_ph_model_dims(phx_ph_model* m, int* d) {
+  return guard([&] {
+    PHX_REQUIRE(m && d, "phx_ph_model_dims: null argument");
+    const phx::PhononModel& x = m->m;
+    const int v[8] = {x.nat, x.nm, x.nR, x.nG, x.hasDielectric ? 1 : 0, x.qGrid[0], x.qGrid[1], x.qGrid[2]};
+    std::memcpy(d, v, sizeof(v));
+  });
+}
+
+int phx_ph_model_crystal(phx_ph_model* m, double* cell9, double* recip9, double* volume,
+                         double* positions, double* masses) {
+  return guard([&] {
+    PHX_REQUIRE(m, "phx_ph_model_crystal: null handle");
+    const phx::PhononModel& x = m->m;
+    if (cell9) std::memcpy(cell9, x.cell, sizeof(x.cell));
+    if (recip9) std::memcpy(recip9, x.recip, sizeof(x.recip));
+    if (volume) *volume = x.volume;
+    if (positions) std::copy(x.positions.begin(), x.positions.end(), positions);
+    if (masses) std::copy(x.masses.begin(), x.masses.end(), masses);
+  });
+}
+
+int phx_ph_model_force_constants(phx_ph_model* m, double* R, double* weights, double* mat2R) {
+  return guard([&] {
+    PHX_REQUIRE(m, "phx_ph_model_force_constants: null handle");
+    const phx::PhononModel& x = m->m;
+    if (R) std::copy(x.R.begin(), x.R.end(), R);
+    if (weights) std::copy(x.weights.begin(), x.weights.end(), weights);
+    if (mat2R) std::copy(x.mat2R.begin(), x.mat2R.end(), mat2R);
+  });
+}
+
+int phx_ph_model_long_range(phx_ph_model* m, double* born, double* dielectric, double* gVectors,
+                            double* lrConstant, double* norm_gmax2) {
+  return guard([&] {
+    PHX_REQUIRE(m, "phx_ph_model_long_range: null handle");
+    const phx::PhononModel& x = m->m;
+    if (born) std::copy(x.born.begin(), x.born.end(), born);
+    if (dielectric) std::memcpy(dielectric, x.dielectric, sizeof(x.dielectric));
+    if (gVectors) std::copy(x.gVectors.begin(), x.gVectors.end(), gVectors);
+    if (lrConstant) std::copy(x.lrConstant.begin(), x.lrConstant.end(), lrConstant);
+    if (norm_gmax2) {
+      norm_gmax2[0] = x.lrNorm;
+      norm_gmax2[1] = x.gMax;
+    }
+  });
+}
+
+int phx_ph_model_diag(phx_ph_model* m, int nq, const double* q, int mass_scaling, double* omega,
+                      double* eigvecs) {
+  return guard([&] {
+    PHX_REQUIRE(m && (nq == 0 || (q && omega && eigvecs)), "phx_ph_model_diag: null argument");
+    PHX_REQUIRE(nq >= 0, "phx_ph_model_diag: nq < 0");
+    const phx::PhononModel& x = m->m;
+    if (nq)
+      bloch_diag(true, nq, q, x.nm, x.nR, x.R.data(), x.weights.data(), x.mat2R.data(),
+                 x.masses.data(), mass_scaling, omega, eigvecs, &x);
+  });
+}
+
+int phx_ph_model_destroy(phx_ph_model* m) {
+  return guard([&] { delete m; });
+}
+
+int phx_el_model_read_w90(phx_el_model** m, const char* tb_path, const double* cell9) {
+  return guard([&] {
+    PHX_REQUIRE(m && tb_path, "phx_el_model_read_w90: null argument");
+    *m = nullptr;
+    auto* M = new phx_el_model();
+    try {
+      M->m = phx::read_w90_tb(tb_path, cell9);
+      PHX_REQUIRE(M->m.nw <= 44, "phx_el_model_read_w90: at most 44 Wannier functions");
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *m = M;
+  });
+}
+
+int phx_el_model_dims(phx_el_model* m, int* d) {
+  return guard([&] {
+    PHX_REQUIRE(m && d, "phx_el_model_dims: null argument");
+    d[0] = m->m.nw;
+    d[1] = m->m.nR;
+  });
+}
+
+int phx_el_model_arrays(phx_el_model* m, double* cell9, double* R, double* deg, double* h0R,
+                        double* rMatrix) {
+  return guard([&] {
+    PHX_REQUIRE(m, "phx_el_model_arrays: null handle");
+    const phx::ElectronModel& x = m->m;
+    if (cell9) std::memcpy(cell9, x.cell, sizeof(x.cell));
+    if (R) std::copy(x.R.begin(), x.R.end(), R);
+    if (deg) std::copy(x.deg.begin(), x.deg.end(), deg);
+    if (h0R) std::copy(x.h0R.begin(), x.h0R.end(), h0R);
+    if (rMatrix) std::copy(x.rMat.begin(), x.rMat.end(), rMatrix);
+  });
+}
+
+int phx_el_model_diag(phx_el_model* m, int nk, const double* k, double* energies, double* eigvecs) {
+  return guard([&] {
+    PHX_REQUIRE(m && (nk == 0 || (k && energies && eigvecs)), "phx_el_model_diag: null argument");
+    PHX_REQUIRE(nk >= 0, "phx_el_model_diag: nk < 0");
+    const phx::ElectronModel& x = m->m;
+    if (nk)
+      bloch_diag(false, nk, k, x.nw, x.nR, x.R.data(), x.deg.data(), x.h0R.data(), nullptr, 0,
+                 energies, eigvecs);
+  });
+}
+
+int phx_el_model_destroy(phx_el_model* m) {
+  return guard([&] { delete m; });
+}
+
+int phx_problem_set_models(phx_problem* prob, phx_el_model* el, phx_ph_model* ph) {
+  return guard([&] {
+    PHX_REQUIRE(prob && el && ph, "phx_problem_set_models: null argument");
+    const phx::PhononModel& f = ph->m;
+    const phx::ElectronModel& e = el->m;
+    std::memcpy(prob->recip, f.recip, sizeof(f.recip));
+    prob->nw = e.nw;
+    prob->nRh = e.nR;
+    prob->Rh = e.R.data();
+    prob->degh = e.deg.data();
+    prob->h0R = e.h0R.data();
+    prob->nm = f.nm;
+    prob->nRf = f.nR;
+    prob->Rf = f.R.data();
+    prob->wf = f.weights.data();
+    prob->mat2R = f.mat2R.data();
+    prob->masses = f.masses.data();
+    prob->natoms = f.nat;
+    prob->nG = f.hasDielectric ? f.nG : 0;
+    prob->atomPos = f.positions.data();
+    prob->bornCharges = f.born.data();
+    prob->dielectric = f.dielectric;
+    prob->gVectors = f.gVectors.data();
+    prob->lrConstant = f.lrConstant.data();
+    prob->lrNorm = f.lrNorm;
+    prob->gMax = f.gMax;
   });
 }
 

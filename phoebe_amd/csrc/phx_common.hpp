@@ -64,6 +64,17 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Rigid-ion long-range term of the dynamical matrix (device pointers).
+struct LongRange {
+  int nat = 0, nG = 0;
+  const double* tau = nullptr;   // nat x 3, Cartesian Bohr
+  const double* born = nullptr;  // nat x 9, (atom, i, j)
+  const double* eps = nullptr;   // 9, (i, j)
+  const double* G = nullptr;     // nG x 3
+  const double* cst = nullptr;   // nat x 9, (atom, i, j)
+  double norm = 0., gMax = 14.;
+};
+
 inline int ceil_div(long long a, long long b) { return int((a + b - 1) / b); }
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 

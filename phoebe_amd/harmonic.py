@@ -1,0 +1,173 @@
+"""Harmonic models read from the files Phoebe reads (library C++ readers).
+
+* PhononH0 - QE force constants (q2r.x prefix.fc): QEParser::parsePhHarmonic
+  (qe_input_parser.cpp:385-562) and the PhononH0 constructor
+  (phonon_h0.cpp:18-235): acoustic sum rule, Wigner-Seitz reorder, rigid-ion
+  long-range set-up.  diag() is kokkosBatchedDiagonalizeFromCoordinates
+  (phonon_h0_kokkos.cpp:177-207) on the GPU, long-range term included.
+* ElectronH0Wannier - Wannier90 seedname_tb.dat: QEParser::parseElHarmonicWannier
+  (qe_input_parser.cpp:883-1040); diag() is the batched H(k) eigensolve
+  (electron_h0_wannier.cpp:777-877) on the GPU.
+* linewidth_problem - the inputs of ElectronLinewidths from these models and
+  Phoebe's el-ph file, as the reference's electron transport app assembles
+  them (electron_wannier_transport_app.cpp:16-20: the phonon crystal is the
+  crystal, the electron Hamiltonian is read with its cell).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+from . import ElPhFile, _check, _dp, _ip, _p, _path, lib
+
+
+class PhononH0:
+    def __init__(self, fc_path, sum_rule="simple"):
+        self._h = C.c_void_p()
+        _check(lib().phx_ph_model_read_qe(C.byref(self._h), _path(fc_path),
+                                          (sum_rule or "").encode()))
+        d = np.zeros(8, dtype=np.int32)
+        _check(lib().phx_ph_model_dims(self._h, d.ctypes.data_as(_ip)))
+        self.natoms, self.nm, self.nR, self.nG = (int(x) for x in d[:4])
+        self.has_dielectric = bool(d[4])
+        self.q_grid = tuple(int(x) for x in d[5:8])
+        cell, recip, vol = np.zeros(9), np.zeros(9), np.zeros(1)
+        self.positions = np.zeros((self.natoms, 3))
+        self.masses = np.zeros(self.nm)
+        _check(lib().phx_ph_model_crystal(self._h, _p(cell), _p(recip), _p(vol), _p(self.positions),
+                                          _p(self.masses)))
+        self.cell = cell.reshape(3, 3, order="F")        # columns a_i
+        self.recip = recip.reshape(3, 3, order="F")      # columns b_i
+        self.volume = float(vol[0])
+        self.R = np.zeros((self.nR, 3))
+        self.weights = np.zeros(self.nR)
+        self.mat2R = np.zeros((self.nR, self.nm, self.nm))
+        _check(lib().phx_ph_model_force_constants(self._h, _p(self.R), _p(self.weights), _p(self.mat2R)))
+        self.born = np.zeros((self.natoms, 3, 3))
+        self.dielectric = np.zeros((3, 3))
+        self.g_vectors = np.zeros((self.nG, 3))
+        self.lr_constant = np.zeros((self.natoms, 3, 3))
+        ng = np.zeros(2)
+        _check(lib().phx_ph_model_long_range(self._h, _p(self.born), _p(self.dielectric),
+                                             _p(self.g_vectors), _p(self.lr_constant), _p(ng)))
+        self.lr_norm, self.g_max = float(ng[0]), float(ng[1])
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().phx_ph_model_destroy(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def long_range(self):
+        return dict(natoms=self.natoms, nG=self.nG if self.has_dielectric else 0,
+                    positions=self.positions, born=self.born, dielectric=self.dielectric,
+                    gVectors=self.g_vectors, constant=self.lr_constant, norm=self.lr_norm,
+                    gMax=self.g_max)
+
+    def diag(self, q, mass_scaling=True):
+        """(omega (nq, nm), V (nq, nm, nm)) with V[i][:, j] the eigenvectors."""
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+        nq = len(q)
+        om = np.zeros((nq, self.nm))
+        ev = np.zeros(2 * nq * self.nm * self.nm)
+        _check(lib().phx_ph_model_diag(self._h, nq, _p(q), int(bool(mass_scaling)), _p(om), _p(ev)))
+        return om, ev.view(np.complex128).reshape(nq, self.nm, self.nm).transpose(0, 2, 1)
+
+
+class ElectronH0Wannier:
+    def __init__(self, tb_path, cell=None):
+        self._h = C.c_void_p()
+        cptr = None
+        if cell is not None:
+            self._cell_in = np.asfortranarray(cell, dtype=np.float64).reshape(-1, order="F").copy()
+            cptr = _p(self._cell_in)
+        _check(lib().phx_el_model_read_w90(C.byref(self._h), _path(tb_path), cptr))
+        d = np.zeros(2, dtype=np.int32)
+        _check(lib().phx_el_model_dims(self._h, d.ctypes.data_as(_ip)))
+        self.nw, self.nR = int(d[0]), int(d[1])
+        cell9 = np.zeros(9)
+        self.R = np.zeros((self.nR, 3))
+        self.deg = np.zeros(self.nR)
+        self.h0R = np.zeros((self.nR, self.nw, self.nw), dtype=np.complex128)
+        self.r_matrix = np.zeros((3, self.nR, self.nw, self.nw), dtype=np.complex128)
+        _check(lib().phx_el_model_arrays(self._h, _p(cell9), _p(self.R), _p(self.deg),
+                                         _p(self.h0R.view(np.float64)), _p(self.r_matrix.view(np.float64))))
+        self.cell = cell9.reshape(3, 3, order="F")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().phx_el_model_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def diag(self, k):
+        """(energies (nk, nw), U (nk, nw, nw)) with U[i][:, b] the eigenvectors."""
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+        nk = len(k)
+        E = np.zeros((nk, self.nw))
+        U = np.zeros(2 * nk * self.nw * self.nw)
+        _check(lib().phx_el_model_diag(self._h, nk, _p(k), _p(E), _p(U)))
+        return E, U.view(np.complex128).reshape(nk, self.nw, self.nw).transpose(0, 2, 1)
+
+
+@dataclasses.dataclass
+class LinewidthProblem:
+    """ElectronLinewidths inputs assembled from the harmonic models and an
+    el-ph file (attribute names of phoebe_amd.synth.Problem)."""
+
+    name: str
+    A: np.ndarray
+    recip: np.ndarray
+    mesh: tuple
+    nw: int
+    Rh: np.ndarray
+    degh: np.ndarray
+    h0R: np.ndarray
+    nm: int
+    Rf: np.ndarray
+    wf: np.ndarray
+    mat2R: np.ndarray
+    masses: np.ndarray
+    Re: np.ndarray
+    degE: np.ndarray
+    Rp: np.ndarray
+    degP: np.ndarray
+    temperatures: np.ndarray
+    chem_pots: np.ndarray
+    sigma: float
+    win_min: float
+    win_max: float
+    long_range: dict
+    elph_path: str
+
+    @property
+    def nRe(self):
+        return len(self.Re)
+
+    @property
+    def nRp(self):
+        return len(self.Rp)
+
+    def gw_host(self):
+        with ElPhFile(self.elph_path) as f:
+            return f.read_gw()
+
+
+def linewidth_problem(el: ElectronH0Wannier, ph: PhononH0, elph_path, mesh, temperatures,
+                      chem_pots, sigma, win_min, win_max, name="files") -> LinewidthProblem:
+    """Temperatures, chemical potentials, sigma and the window in Ry."""
+    with ElPhFile(elph_path) as f:
+        h = f.header
+        if h["numElBands"] != el.nw or h["numPhModes"] != ph.nm:
+            raise ValueError("the el-ph file does not match the harmonic models")
+        Re, degE, Rp, degP = f.bravais()
+    return LinewidthProblem(
+        name=name, A=ph.cell, recip=ph.recip, mesh=tuple(int(x) for x in mesh), nw=el.nw, Rh=el.R,
+        degh=el.deg, h0R=el.h0R, nm=ph.nm, Rf=ph.R, wf=ph.weights, mat2R=ph.mat2R, masses=ph.masses,
+        Re=Re, degE=degE, Rp=Rp, degP=degP,
+        temperatures=np.atleast_1d(np.asarray(temperatures, dtype=float)),
+        chem_pots=np.atleast_1d(np.asarray(chem_pots, dtype=float)), sigma=float(sigma),
+        win_min=float(win_min), win_max=float(win_max), long_range=ph.long_range,
+        elph_path=os.fspath(elph_path))

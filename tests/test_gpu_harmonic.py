@@ -1,0 +1,125 @@
+"""GPU: band structures from the reference's Si inputs (QE force constants
+with the rigid-ion long-range term, Wannier90 tight binding) and a whole Si
+electron-linewidth run from the three files Phoebe reads (fc, tb.dat,
+el-ph HDF5), against the oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import phoebe_amd as P
+from oracle import harmonic_oracle as HO
+from oracle import h5_oracle as H5
+from oracle import oracle as O
+from phoebe_amd import harmonic as Hm
+from tests.helpers import projector_diff, rel_err
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+FC2 = os.path.join(GOLD, "silicon.fc")
+FC4 = os.path.join(GOLD, "444_silicon.fc")
+TB = os.path.join(GOLD, "si_tb.dat")
+TB6 = os.path.join(GOLD, "666_si_tb.dat")
+ELPH = os.path.join(GOLD, "silicon.phoebe.elph.hdf5")
+RY_PER_EV = 1.0 / HO.ENERGY_RY_TO_EV
+KELVIN_TO_RY = 1.38064852e-23 / HO.RYDBERG_SI
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("path,rule", [(FC2, "simple"), (FC4, ""), (FC4, "simple")])
+def test_phonon_model_diag_vs_oracle(path, rule):
+    """D(q) with the long-range term (nonzero without the sum rule) on the GPU
+    against the oracle's CPU form; includes Gamma and a point near it."""
+    say("phonon model", os.path.basename(path), rule or "no sum rule")
+    ph = Hm.PhononH0(path, rule)
+    d = HO.read_qe_fc(path)
+    if rule:
+        d = HO.sum_rule_simple(d)
+    R, W, M = HO.ws_reorder(d)
+    lr = (*HO.long_range_setup(d), d["eps"], d["born"], d["tau"])
+    masses = np.repeat(d["species_masses"][d["species"]], 3)
+    rng = np.random.default_rng(3)
+    qs = np.vstack([np.zeros(3), ph.recip @ [1 / 40, 0, 0], (ph.recip @ rng.uniform(-1, 1, (3, 30))).T])
+    om, V = ph.diag(qs, mass_scaling=False)
+    scale = np.abs(om).max()
+    for i, q in enumerate(qs):
+        w, U = HO.frequencies(HO.dynamical_matrix(R, W, M, q, masses, lr))
+        # compare omega^2 (sign kept): omega = sqrt|lambda| amplifies the
+        # rounding of the zero acoustic eigenvalues at Gamma
+        lam_g, lam_o = np.sign(om[i]) * om[i] ** 2, np.sign(w) * w ** 2
+        assert np.abs(lam_g - lam_o).max() < 1e-12 * scale ** 2, i
+        big = np.abs(w) > 1e-3 * scale
+        assert np.abs(om[i][big] - w[big]).max(initial=0.0) < 1e-10 * scale, i
+        if i > 0:
+            assert projector_diff(V[i], U, w ** 2 * np.sign(w), tol=1e-12 * scale ** 2) < 1e-7, i
+    # mass-scaled eigenvectors: e / sqrt(M)
+    _, Vs = ph.diag(qs[2:4], mass_scaling=True)
+    assert np.allclose(Vs, V[2:4] / np.sqrt(ph.masses)[None, :, None], rtol=0, atol=1e-13)
+
+
+def test_electron_model_diag_vs_oracle_and_hole_mass():
+    """H(k) of 666_si_tb.dat on the GPU against the oracle, and the known
+    answer of test/velocityTest.cpp (WannierH0.Velocity): |2 (E3(dk) - E3(0)) /
+    dk^2| = 686.779 +- 0.02 at dk = 0.00025 Bohr^-1."""
+    say("electron model, hole mass")
+    el = Hm.ElectronH0Wannier(TB6)
+    tb = HO.read_w90_tb(TB6)
+    rng = np.random.default_rng(4)
+    ks = np.vstack([np.zeros(3), [0.00025, 0, 0], rng.uniform(-1, 1, (20, 3))])
+    E, U = el.diag(ks)
+    for i, k in enumerate(ks):
+        e, u = np.linalg.eigh(HO.hamiltonian(tb, k))
+        assert np.abs(E[i] - e).max() < 1e-12 * np.abs(e).max(), i
+        assert projector_diff(U[i], u, e) < 1e-9, i
+    mass = 2 * (E[1, 3] - E[0, 3]) / 0.00025 ** 2
+    say("  mass", mass)
+    assert abs(abs(mass) - 686.779) < 0.02
+
+
+@pytest.mark.parametrize("fc,rule", [(FC2, "simple"), (FC4, "")])
+@pytest.mark.parametrize("transform", ["direct", "separable"])
+def test_si_linewidths_from_reference_files(fc, rule, transform):
+    """Si electron linewidths from silicon.fc / 444_silicon.fc, si_tb.dat (read
+    with the phonon crystal's cell) and silicon.phoebe.elph.hdf5 on a 6^3
+    mesh at 300 K: GPU against the oracle, FP64, 1e-10 per linewidth."""
+    say("Si linewidths", os.path.basename(fc), rule or "no sum rule", transform)
+    ph = Hm.PhononH0(fc, rule)
+    el = Hm.ElectronH0Wannier(TB, ph.cell)
+    # chemical potential at the top of the fourth band on the mesh, 1 eV window around it
+    mesh = (6, 6, 6)
+    pts = np.array([[i / 6, j / 6, k / 6] for k in range(6) for j in range(6) for i in range(6)])
+    E, _ = el.diag((ph.recip @ pts.T).T)
+    vbm = E[:, 3].max()
+    prob = Hm.linewidth_problem(el, ph, ELPH, mesh, temperatures=[300 * KELVIN_TO_RY],
+                                chem_pots=[vbm], sigma=0.05 * RY_PER_EV,
+                                win_min=vbm - 1.0 * RY_PER_EV, win_max=vbm + 1.0 * RY_PER_EV)
+    assert ph.has_dielectric and (prob.long_range["nG"] > 0)
+    gW = H5.read_elph(ELPH)["gW"]
+    orc = O.Oracle(prob, gW.reshape(-1))
+    ref = orc.linewidths()
+    lw = P.ElectronLinewidths(prob, precision="fp64", transform=transform, elph_file=ELPH)
+    assert np.array_equal(lw.active, orc.active)
+    assert np.array_equal(lw.band_lo, orc.band_lo) and np.array_equal(lw.band_hi, orc.band_hi)
+    om_o, _, set_o = orc.phonons()
+    om, set_g = lw.phonons()
+    assert np.array_equal(set_g, set_o)
+    # omega^2 with its sign: the acoustic zeros at Gamma are rounding noise of either sign
+    # (both sides drop them at the 5 cm^-1 cut)
+    sc = np.abs(om_o).max()
+    assert np.abs(np.sign(om) * om ** 2 - np.sign(om_o) * om_o ** 2)[set_o].max() < 1e-12 * sc ** 2
+    pairs = lw.run()
+    assert pairs == orc.last_pairs
+    got = lw.linewidths()
+    nz = ref > 1e-30 * np.abs(ref).max()
+    err = rel_err(got[nz], ref[nz])
+    say("  states", lw.num_states, "pairs", pairs, "nonzero", int(nz.sum()), "max rel err", err.max())
+    assert nz.sum() > 0.5 * ref.size
+    assert err.max() < 1e-10
+    assert np.abs(got[~nz]).max(initial=0.0) <= 1e-30 * np.abs(ref).max()

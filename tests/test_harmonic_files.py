@@ -1,0 +1,235 @@
+"""Harmonic inputs read from the reference's Si test data (copied to
+tests/golden/): QE force constants (silicon.fc: 2x2x2, 444_silicon.fc:
+4x4x4) and Wannier90 tight-binding files (si_tb.dat, 666_si_tb.dat).
+
+The oracle's numpy reading (oracle/harmonic_oracle.py) is pinned by the
+files' own labels, by identities the mathematics fixes and by the known
+answers of the reference's tests (test/velocityTest.cpp); the library's C++
+readers (through the C ABI, host code only) are then compared with it.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import phoebe_amd as P
+from oracle import harmonic_oracle as HO
+from phoebe_amd import harmonic as Hm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+FC2 = os.path.join(GOLD, "silicon.fc")
+FC4 = os.path.join(GOLD, "444_silicon.fc")
+TB = os.path.join(GOLD, "si_tb.dat")
+TB6 = os.path.join(GOLD, "666_si_tb.dat")
+SHA = {
+    "444_silicon.fc": "ae3d10ffcb9a614c03eaa2947669afeb7a4170a4098c34a443ab0cd7f7fab47d",
+    "666_si_tb.dat": "8b4af577e25ee38288843f2a73845e7fd770db9a355b3562b09b9f34e2624406",
+    "si_tb.dat": "528d86c93a1e1bd6de2e5d50dd6ade913f311e30958d3eb749fa20564e24d46c",
+    "silicon.fc": "e50617e1beda37dea8b815f28e0d95635d1ee20678458464d313c213bb1eb003",
+}
+
+
+def _oracle_model(path, sum_rule):
+    d = HO.read_qe_fc(path)
+    if sum_rule == "simple":
+        d = HO.sum_rule_simple(d)
+    R, W, M = HO.ws_reorder(d)
+    return d, R, W, M
+
+
+def test_fixtures_are_the_reference_files():
+    for name, sha in SHA.items():
+        assert hashlib.sha256(open(os.path.join(GOLD, name), "rb").read()).hexdigest() == sha, name
+
+
+# ---------------------------------------------------------------- oracle pins: force constants
+def test_qe_fc_header():
+    d = HO.read_qe_fc(FC2)      # the parser itself asserts every block label and cell index
+    assert (d["ibrav"], d["celldm"][0], d["grid"]) == (2, 10.2, (2, 2, 2))
+    assert d["names"] == ["Si"] and d["species_masses"][0] == 25598.823011949724
+    assert np.allclose(d["tau"], [[0, 0, 0], [2.55, 2.55, 2.55]])
+    assert np.allclose(d["eps"], 58.9157771 * np.eye(3))
+    # fcc cell of a = 10.2 Bohr: |a_i| = a / sqrt(2), volume a^3 / 4
+    assert np.allclose(np.linalg.norm(d["cell"], axis=0), 10.2 / np.sqrt(2))
+    assert abs(abs(np.linalg.det(d["cell"])) - 10.2 ** 3 / 4) < 1e-9
+
+
+@pytest.mark.parametrize("path", [FC2, FC4])
+def test_ws_weights_against_their_definition(path):
+    """Every weight equals the brute-force Wigner-Seitz weight (1/N for the N
+    shortest images of tau_a - tau_b + R in the supercell), and the weights
+    of each atom pair sum to the number of cells."""
+    d, R, W, M = _oracle_model(path, "simple")
+    A, tau, g = d["cell"], d["tau"], np.array(d["grid"])
+    sup = A * g[None, :]
+    nat = len(tau)
+    # which atom pair each entry belongs to: the non-zero block of M
+    pair = []
+    for m in M:
+        nz = np.argwhere(np.abs(m) > 0)
+        pair.append((nz[0][0] // 3, nz[0][1] // 3) if len(nz) else None)
+    tot = np.zeros((nat, nat))
+    for r, w, pr in zip(R, W, pair):
+        if pr is None:
+            continue
+        na, nb = pr
+        assert HO.ws_weight_brute(r + tau[na] - tau[nb], sup) == pytest.approx(w, abs=1e-15)
+        tot[na, nb] += w
+    assert abs(W.sum() - np.prod(g) * nat * nat) < 1e-10
+    assert np.allclose(tot, np.prod(g))
+
+
+@pytest.mark.parametrize("path", [FC2, FC4])
+def test_coarse_grid_identity(path):
+    """On the coarse q grid, the Wigner-Seitz sum of the force constants equals
+    their plain periodic Fourier sum: e^{-iq.R} is the same for every image of
+    a cell, and each cell's image weights add up to one."""
+    d, R, W, M = _oracle_model(path, "")
+    A, fc, g = d["cell"], d["fc"], d["grid"]
+    B = 2 * np.pi * np.linalg.inv(A).T
+    nat = len(d["tau"])
+    rng = np.random.default_rng(0)
+    for _ in range(6):
+        m = rng.integers(0, g)
+        q = B @ (m / np.array(g))
+        ws = np.tensordot(W * np.exp(-1j * (R @ q)), M, axes=(0, 0))
+        plain = np.zeros((3 * nat, 3 * nat), dtype=complex)
+        for n1 in range(g[0]):
+            for n2 in range(g[1]):
+                for n3 in range(g[2]):
+                    ph = np.exp(-1j * q @ (n1 * A[:, 0] + n2 * A[:, 1] + n3 * A[:, 2]))
+                    for na in range(nat):
+                        for nb in range(nat):
+                            plain[3 * na:3 * na + 3, 3 * nb:3 * nb + 3] += ph * fc[:, :, n1, n2, n3, na, nb]
+        assert np.abs(ws - plain).max() < 1e-12 * np.abs(plain).max()
+
+
+def test_simple_sum_rule_gives_acoustic_zeros():
+    """With the simple sum rule the three acoustic frequencies at Gamma vanish
+    and the Born charges sum to zero; without it they do not."""
+    for rule, zero in (("simple", True), ("", False)):
+        d, R, W, M = _oracle_model(FC4, rule)
+        masses = np.repeat(d["species_masses"][d["species"]], 3)
+        lam = np.linalg.eigvalsh(HO.dynamical_matrix(R, W, M, np.zeros(3), masses))
+        low = np.sort(np.abs(lam))[:3]
+        if zero:
+            assert low.max() < 1e-12 * np.abs(lam).max()
+            assert np.abs(d["born"].sum(axis=0)).max() < 1e-14
+        else:
+            assert low.max() > 1e-6 * np.abs(lam).max()
+
+
+def test_long_range_term_keeps_translation_invariance():
+    """The constant part of the long-range term cancels the G-sum's row sums
+    at q = 0: sum_b D_LR(a i, b j) = 0 (no sum rule, so the Born charges are
+    the file's and the term is not zero)."""
+    d = HO.read_qe_fc(FC4)
+    G, cst, norm = HO.long_range_setup(d)
+    assert np.abs(cst).max() > 1e-7
+    nat = len(d["tau"])
+    zero = np.zeros((1, 3))
+    D = HO.dynamical_matrix(zero, np.zeros(1), np.zeros((1, 3 * nat, 3 * nat)), np.zeros(3),
+                            lr=(G, cst, norm, d["eps"], d["born"], d["tau"]))
+    rows = D.reshape(nat, 3, nat, 3).sum(axis=2)
+    assert np.abs(rows).max() < 1e-12 * np.abs(D).max()
+    # a planted sign error in the constant breaks it
+    Dbad = HO.dynamical_matrix(zero, np.zeros(1), np.zeros((1, 3 * nat, 3 * nat)), np.zeros(3),
+                               lr=(G, -cst, norm, d["eps"], d["born"], d["tau"]))
+    assert np.abs(Dbad.reshape(nat, 3, nat, 3).sum(axis=2)).max() > 1e-3 * np.abs(D).max()
+
+
+def test_speed_of_sound_known_answer():
+    """test/velocityTest.cpp (PhononH0.Velocity): 444_silicon.fc with the
+    simple sum rule, the point (1/40, 0, 0) of a 40^3 mesh.  With the
+    reference's velocity operator (central differences of sqrt(D), degenerate
+    blocks diagonalized per direction) the acoustic velocities are parallel to
+    q within 4%, and the smallest |min component| is 2200 +- 200 m/s."""
+    d, R, W, M = _oracle_model(FC4, "simple")
+    masses = np.repeat(d["species_masses"][d["species"]], 3)
+    B = 2 * np.pi * np.linalg.inv(d["cell"]).T
+    q = B @ np.array([1 / 40, 0, 0])
+    lr = (*HO.long_range_setup(d), d["eps"], d["born"], d["tau"])
+    w, v = HO.phonon_velocities(R, W, M, q, masses, lr)
+    gv = np.array([[v[b, b, i].real for i in range(3)] for b in range(3)])
+    for b in range(3):
+        assert abs(abs(gv[b] @ q) / np.linalg.norm(q) / np.linalg.norm(gv[b]) - 1) < 0.04
+    sound = min(abs(gv[b].min()) for b in range(3)) * HO.VELOCITY_RY_TO_SI
+    assert abs(sound - 2200) < 200
+    # the group velocity of the longitudinal mode equals omega / |q| near Gamma
+    assert abs(np.linalg.norm(gv[2]) - w[2] / np.linalg.norm(q)) < 0.01 * np.linalg.norm(gv[2])
+
+
+# ---------------------------------------------------------------- oracle pins: tight binding
+def test_tb_labels_and_layout():
+    """Within each R block the reference fills h0R(iR, i, j) line by line with
+    j fastest; the file's labels run the first index fastest, so line (m, n)
+    lands at h0R(R, n-1, m-1)."""
+    tb = HO.read_w90_tb(TB)
+    nw = tb["nw"]
+    k = np.arange(nw * nw)
+    for lab in tb["labels"]:
+        assert np.array_equal(lab[:, 0], k % nw + 1) and np.array_equal(lab[:, 1], k // nw + 1)
+    assert abs((1 / tb["deg"]).sum() - 8) < 1e-12          # Wannier90's 2x2x2 k grid
+    # Hermitian H(k): the file holds H(-R) = H(R)^dagger
+    for kv in np.random.default_rng(1).uniform(-1, 1, (4, 3)):
+        H = HO.hamiltonian(tb, kv)
+        assert np.abs(H - H.conj().T).max() < 1e-12
+
+
+def test_hole_mass_known_answer():
+    """test/velocityTest.cpp (WannierH0.Velocity): with 666_si_tb.dat and its
+    own cell, 2 (E3(k) - E3(0)) / dk^2 at k = (0.00025, 0, 0) Bohr^-1 is
+    686.779 +- 0.02 in magnitude."""
+    tb = HO.read_w90_tb(TB6)
+    dk = 0.00025
+    e0 = np.linalg.eigvalsh(HO.hamiltonian(tb, np.zeros(3)))
+    e1 = np.linalg.eigvalsh(HO.hamiltonian(tb, np.array([dk, 0, 0])))
+    mass = 2 * (e1[3] - e0[3]) / dk ** 2
+    assert abs(abs(mass) - 686.779) < 0.02
+
+
+# ---------------------------------------------------------------- library vs oracle
+@pytest.mark.parametrize("path,rule", [(FC2, "simple"), (FC4, "simple"), (FC4, "")])
+def test_library_phonon_model_matches_oracle(path, rule):
+    d, R, W, M = _oracle_model(path, rule)
+    ph = Hm.PhononH0(path, rule)
+    assert (ph.natoms, ph.nm, ph.q_grid) == (2, 6, d["grid"])
+    assert np.array_equal(ph.R, R) and np.array_equal(ph.weights, W)
+    assert np.abs(ph.mat2R - M).max() <= 1e-14 * np.abs(M).max()   # sum-rule sums in another order
+    assert np.array_equal(ph.cell, d["cell"])
+    assert np.allclose(ph.recip, 2 * np.pi * np.linalg.inv(d["cell"]).T, rtol=0, atol=1e-15)
+    assert np.array_equal(ph.masses, np.repeat(d["species_masses"][d["species"]], 3))
+    assert np.array_equal(ph.positions, d["tau"])
+    assert ph.has_dielectric and np.array_equal(ph.dielectric, d["eps"])
+    assert np.abs(ph.born - d["born"]).max() < 1e-15
+    G, cst, norm = HO.long_range_setup(d)
+    assert np.array_equal(ph.g_vectors, G) and abs(ph.lr_norm - norm) < 1e-16
+    assert np.abs(ph.lr_constant - cst).max() <= 1e-13 * max(np.abs(cst).max(), 1e-300)
+
+
+@pytest.mark.parametrize("path,cell_from_fc", [(TB, True), (TB6, False)])
+def test_library_tb_model_matches_oracle(path, cell_from_fc):
+    cell = Hm.PhononH0(FC2, "simple").cell if cell_from_fc else None
+    tb = HO.read_w90_tb(path, cell)
+    el = Hm.ElectronH0Wannier(path, cell)
+    assert (el.nw, el.nR) == (tb["nw"], len(tb["deg"]))
+    assert np.array_equal(el.h0R, tb["h0R"]) and np.array_equal(el.deg, tb["deg"])
+    assert np.abs(el.R - tb["R"]).max() < 1e-13
+    assert np.allclose(el.cell, tb["cell"], rtol=0, atol=1e-15)
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(P.PhxError, match="Dynamical matrix file not found"):
+        Hm.PhononH0(tmp_path / "none.fc")
+    with pytest.raises(P.PhxError, match="Wannier H0 file not found"):
+        Hm.ElectronH0Wannier(tmp_path / "none_tb.dat")
+    with pytest.raises(P.PhxError, match="not supported"):
+        Hm.PhononH0(FC2, "crystal")
+    with pytest.raises(P.PhxError, match="invalid Acoustic Sum Rule"):
+        Hm.PhononH0(FC2, "bogus")
+    bad = tmp_path / "cut.fc"
+    bad.write_text("".join(open(FC2).readlines()[:40]))
+    with pytest.raises(P.PhxError, match="unexpected end of file"):
+        Hm.PhononH0(bad)
