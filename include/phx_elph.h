@@ -41,6 +41,10 @@ enum { PHX_FP64 = 0, PHX_FP32 = 1, PHX_BF16X3 = 2 };
  *             (exact, same FP64 accuracy, far fewer operations). */
 enum { PHX_TRANSFORM_DIRECT = 0, PHX_TRANSFORM_SEPARABLE = 1 };
 
+/* where band velocities are set to zero as "Gamma": the reference's batched
+ * GPU forms use |k|^2 <= 1e-6, its single-point CPU forms |k| < 1e-6. */
+enum { PHX_GAMMA_BATCHED = 0, PHX_GAMMA_POINT = 1 };
+
 /* Message of the last error on the calling thread ("" if none). */
 const char* phx_last_error(void);
 
@@ -162,6 +166,17 @@ int phx_ph_model_long_range(phx_ph_model* m, double* born, double* dielectric, d
  * divided by sqrt(M) when mass_scaling. */
 int phx_ph_model_diag(phx_ph_model* m, int nq, const double* q, int mass_scaling, double* omega,
                       double* eigvecs);
+/* PhononH0::kokkosBatchedDiagonalizeWithVelocities (phonon_h0_kokkos.cpp:378-532):
+ * as phx_ph_model_diag with mass scaling, plus the velocity operator
+ * velocities nq x nm x nm x 3 complex, (q, m, n, i) row-major (the reference's
+ * ComplexView4D): central differences (1e-8) of sqrt(D), rotated to the
+ * eigenbasis, zero at Gamma, degenerate blocks (1e-4 cm^-1) diagonalized per
+ * direction (harmonic.cpp:3-203; blocks of up to 8 modes).
+ * gamma_rule: PHX_GAMMA_BATCHED (velocities zero unless |q|^2 > 1e-6, the
+ * batched GPU form) or PHX_GAMMA_POINT (zero when |q| < 1e-6, the single-point
+ * form PhononH0::diagonalizeVelocityFromCoordinates, phonon_h0.cpp:895-1070). */
+int phx_ph_model_diag_velocities(phx_ph_model* m, int nq, const double* q, int gamma_rule,
+                                 double* omega, double* eigvecs, double* velocities);
 int phx_ph_model_destroy(phx_ph_model* m);
 
 /* QEParser::parseElHarmonicWannier (qe_input_parser.cpp:883-1040): Wannier90
@@ -177,6 +192,15 @@ int phx_el_model_arrays(phx_el_model* m, double* cell9, double* R, double* deg, 
 /* ElectronH0Wannier::kokkosBatchedDiagonalizeFromCoordinates
  * (electron_h0_wannier.cpp:777-877): energies nk x nw, eigvecs nk x (nw x nw). */
 int phx_el_model_diag(phx_el_model* m, int nk, const double* k, double* energies, double* eigvecs);
+/* ElectronH0Wannier::kokkosBatchedDiagonalizeWithVelocities
+ * (electron_h0_wannier.cpp:888-999): energies, eigenvectors and the velocity
+ * operator nk x nw x nw x 3 complex, (k, m, n, i) row-major: central
+ * differences (1e-8) of H(k), degenerate blocks (1 micro-eV) diagonalized per
+ * direction (blocks of up to 8 bands).  gamma_rule as for phonons; the
+ * single-point form is ElectronH0Wannier::batchedDiagonalizeWithVelocities
+ * (electron_h0_wannier.cpp:633-760). */
+int phx_el_model_diag_velocities(phx_el_model* m, int nk, const double* k, int gamma_rule,
+                                 double* energies, double* eigvecs, double* velocities);
 int phx_el_model_destroy(phx_el_model* m);
 
 /* ---------------------------------------------------------------- el-ph file

@@ -259,12 +259,14 @@ def hamiltonian(tb: dict, k: np.ndarray) -> np.ndarray:
     return np.tensordot(ph, tb["h0R"], axes=(0, 0))
 
 
-def phonon_velocities(R, W, M, q, masses, lr=None, delta=1.0e-8, deg_tol_cm=1.0e-4):
+def phonon_velocities(R, W, M, q, masses, lr=None, delta=1.0e-8, deg_tol_cm=1.0e-4, gamma_rule="batched"):
     """Velocity operator (nm, nm, 3) of PhononH0::diagonalizeVelocityFromCoordinates
     (phonon_h0.cpp:895-1070): central differences of sqrt(D) (eigenvectors
     without the mass scaling), hermitized, rotated to the eigenbasis at q;
     the block of each degenerate group (frequencies within 1e-4 cm^-1) is
-    diagonalized per Cartesian direction; zero at Gamma."""
+    diagonalized per Cartesian direction; zero at Gamma, which the batched
+    form (phonon_h0_kokkos.cpp:499-506) takes as |q|^2 <= 1e-6 and the
+    single-point form as |q| < 1e-6 (gamma_rule "batched" / "point")."""
     ry_to_cmm1 = 1.0e10 * (1.0 / (6.62607015e-34 / (2 * np.pi) / 4.359744650e-18 * 1.0e12) / 2.0 / (2 * np.pi)) / 299792458.0
     w, U = frequencies(dynamical_matrix(R, W, M, q, masses, lr))
     nm = len(w)
@@ -290,6 +292,48 @@ def phonon_velocities(R, W, M, q, masses, lr=None, delta=1.0e-8, deg_tol_cm=1.0e
                 sub = X.conj().T @ sub @ X
                 v[ib:ib + s, ib:ib + s, i] = 0.5 * (sub + sub.conj().T)
         ib += s
-    if np.linalg.norm(q) < 1e-6:
+    if _is_gamma(q, gamma_rule):
         v[:] = 0
     return w, v
+
+
+def _is_gamma(k, rule):
+    return not (k @ k > 1e-6) if rule == "batched" else bool(np.linalg.norm(k) < 1e-6)
+
+
+def _treat_degenerate(E, v, threshold):
+    n = len(E)
+    ib = 0
+    while ib < n:
+        s = 1
+        while ib + s < n and abs(E[ib] - E[ib + s]) <= threshold:
+            s += 1
+        if s > 1:
+            for i in range(3):
+                sub = v[ib:ib + s, ib:ib + s, i]
+                sub = 0.5 * (sub + sub.conj().T)
+                _, X = np.linalg.eigh(sub)
+                v[ib:ib + s, ib:ib + s, i] = X.conj().T @ sub @ X
+        ib += s
+    return v
+
+
+def electron_velocities(tb: dict, k: np.ndarray, delta: float = 1.0e-8, gamma_rule="batched"):
+    """(E, U, v (nw, nw, 3)) of ElectronH0Wannier::kokkosBatchedDiagonalizeWithVelocities
+    (electron_h0_wannier.cpp:888-999): der_i = (H(k + d) - H(k - d)) / 2d
+    hermitized, v_i = U^H der_i U, zero at Gamma (gamma_rule "batched": |k|^2
+    <= 1e-6; "point", the single-point form electron_h0_wannier.cpp:633-760:
+    |k| < 1e-6), degenerate blocks (1 micro-eV) diagonalized per direction
+    (harmonic.cpp:3-203)."""
+    E, U = np.linalg.eigh(hamiltonian(tb, k))
+    n = len(E)
+    v = np.zeros((n, n, 3), dtype=complex)
+    if not _is_gamma(k, gamma_rule):
+        for i in range(3):
+            dk = np.zeros(3)
+            dk[i] = delta
+            Hp, Hm = hamiltonian(tb, k + dk), hamiltonian(tb, k - dk)
+            der = 0.25 / delta * ((Hp - Hm) + (Hp - Hm).conj().T)
+            v[:, :, i] = U.conj().T @ der @ U
+        _treat_degenerate(E, v, 1.0e-6 / ENERGY_RY_TO_EV)
+    return E, U, v

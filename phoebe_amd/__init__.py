@@ -33,6 +33,7 @@ PHX_FP64, PHX_FP32, PHX_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": PHX_FP64, "fp32": PHX_FP32, "bf16x3": PHX_BF16X3}
 TRANSFORM_DIRECT, TRANSFORM_SEPARABLE = 0, 1
 TRANSFORMS = {"direct": TRANSFORM_DIRECT, "separable": TRANSFORM_SEPARABLE}
+GAMMA_BATCHED, GAMMA_POINT = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -135,6 +136,8 @@ EXPORTS = {
     "phx_el_model_diag": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "phx_el_model_destroy": (C.c_int, [C.c_void_p]),
     "phx_problem_set_models": (C.c_int, [C.POINTER(PhxProblem), C.c_void_p, C.c_void_p]),
+    "phx_ph_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
+    "phx_el_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
 
 

@@ -30,6 +30,11 @@ void launch_bloch(bool phonon, int nk, int nw, int nR, const double* kc, const d
                   const double* rw, const void* coef, double2* out, hipStream_t stream);
 void launch_ph_long_range(int nq, const double* qc, const LongRange& lr, double2* D,
                           hipStream_t stream);
+void launch_vel_operator(bool phonon, int np, int n, double delta, int gammaRule, const double* kc,
+                         const double2* Uc, const double2* M7, const double* W7, const double* masses,
+                         double2* vel, hipStream_t s);
+void launch_vel_degenerate(int np, int n, double threshold, int gammaRule, const double* E,
+                           const double* kc, double2* vel, int* tooBig, hipStream_t s);
 void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
                      hipStream_t s);
 void launch_synth_gw_f32(float2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
@@ -351,6 +356,98 @@ static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, cons
           x[1] *= sc;
         }
   }
+}
+
+// Energies, eigenvectors and the velocity operator at np points
+// (kokkosBatchedDiagonalizeWithVelocities of ElectronH0Wannier,
+// electron_h0_wannier.cpp:888-999, and of PhononH0, phonon_h0_kokkos.cpp:378-532):
+// the Hamiltonians at k and k +- 1e-8 e_i in one Bloch launch, the eigensolves,
+// the velocity operator and the degenerate-block treatment on the device.
+// vel: np x n x n x 3 complex, (p, m, n, i) row-major.  Phonon eigenvectors
+// are returned divided by sqrt(M).
+static void diag_velocities(bool phonon, int np, const double* k, int n, int nR, const double* R,
+                            const double* rw, const void* coef, const double* masses,
+                            const PhononModel* lrModel, int gammaRule, double* energies,
+                            double* eigvecs, double* vel) {
+  PHX_REQUIRE(gammaRule == 0 || gammaRule == 1, "velocities: gamma_rule must be 0 or 1");
+  hipStream_t s = 0;
+  const double delta = 1.0e-8;
+  const double threshold =
+      phonon ? 2.0e-5 * phonon_cutoff()          // 1e-4 cm^-1 (phonon_h0_kokkos.cpp:527)
+             : 0.000001 / units::energyRyToEv;   // 1 micro-eV (electron_h0_wannier.cpp:897)
+  const size_t nn = size_t(n) * n;
+  std::vector<double> k7(size_t(np) * 21);
+  for (int p = 0; p < np; ++p)
+    for (int j = 0; j < 7; ++j) {
+      double* d = &k7[(size_t(p) * 7 + j) * 3];
+      for (int i = 0; i < 3; ++i) d[i] = k[3 * size_t(p) + i];
+      if (j > 0) {
+        const int dir = (j - 1) / 2;
+        if ((j - 1) % 2 == 0) d[dir] += delta;
+        else d[dir] -= delta;
+      }
+    }
+  DevBuf<double> dk7, dkc, dR, drw, dm;
+  h2d(dk7, k7, s);
+  h2d(dkc, k, size_t(np) * 3, s);
+  h2d(dR, R, size_t(nR) * 3, s);
+  h2d(drw, rw, size_t(nR), s);
+  DevBuf<double2> dcoefc;
+  DevBuf<double> dcoefr;
+  if (phonon) {
+    h2d(dcoefr, static_cast<const double*>(coef), size_t(nR) * nn, s);
+    h2d(dm, masses, size_t(n), s);
+  } else {
+    h2d(dcoefc, static_cast<const double2*>(coef), size_t(nR) * nn, s);
+  }
+  const int np7 = 7 * np;
+  DevBuf<double2> H7(size_t(np7) * nn);
+  launch_bloch(phonon, np7, n, nR, dk7.p, dR.p, drw.p,
+               phonon ? static_cast<const void*>(dcoefr.p) : static_cast<const void*>(dcoefc.p), H7.p, s);
+  DevBuf<double2> Uc(size_t(np) * nn), dvel(size_t(np) * nn * 3);
+  DevBuf<double> Wc(size_t(np) * n);
+  DevBuf<int> flag(1);
+  launch_fill<int>(flag.p, 1, 0, s);
+  std::vector<int> inf;
+  if (phonon) {
+    DevLongRange dlr;
+    if (lrModel && lrModel->hasDielectric) {
+      const PhononModel& m = *lrModel;
+      dlr.upload(m.nat, m.nG, m.positions.data(), m.born.data(), m.dielectric, m.gVectors.data(),
+                 m.lrConstant.data(), m.lrNorm, m.gMax, s);
+      if (dlr.on()) launch_ph_long_range(np7, dk7.p, dlr.lr, H7.p, s);
+    }
+    DevBuf<double> W7(size_t(np7) * n);
+    DevBuf<int> info(np7);
+    launch_jacobi(n, np7, H7.p, W7.p, info.p, dm.p, s);
+    launch_vel_operator(true, np, n, delta, gammaRule, dkc.p, nullptr, H7.p, W7.p, dm.p, dvel.p, s);
+    PHX_HIP(hipMemcpy2DAsync(Wc.p, size_t(n) * sizeof(double), W7.p, 7 * size_t(n) * sizeof(double),
+                             size_t(n) * sizeof(double), np, hipMemcpyDeviceToDevice, s));
+    PHX_HIP(hipMemcpy2DAsync(Uc.p, nn * sizeof(double2), H7.p, 7 * nn * sizeof(double2), nn * sizeof(double2),
+                             np, hipMemcpyDeviceToDevice, s));
+    inf.resize(np7);
+    PHX_HIP(hipMemcpyAsync(inf.data(), info.p, info.bytes(), hipMemcpyDeviceToHost, s));
+  } else {
+    PHX_HIP(hipMemcpy2DAsync(Uc.p, nn * sizeof(double2), H7.p, 7 * nn * sizeof(double2), nn * sizeof(double2),
+                             np, hipMemcpyDeviceToDevice, s));
+    DevBuf<int> info(np);
+    launch_jacobi(n, np, Uc.p, Wc.p, info.p, nullptr, s);
+    launch_vel_operator(false, np, n, delta, gammaRule, dkc.p, Uc.p, H7.p, nullptr, nullptr, dvel.p, s);
+    inf.resize(np);
+    PHX_HIP(hipMemcpyAsync(inf.data(), info.p, info.bytes(), hipMemcpyDeviceToHost, s));
+  }
+  launch_vel_degenerate(np, n, threshold, gammaRule, Wc.p, dkc.p, dvel.p, flag.p, s);
+  int tooBig = 0;
+  PHX_HIP(hipMemcpyAsync(&tooBig, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipMemcpyAsync(energies, Wc.p, Wc.bytes(), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipMemcpyAsync(eigvecs, Uc.p, Uc.bytes(), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipMemcpyAsync(vel, dvel.p, dvel.bytes(), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < inf.size(); ++i)
+    if (inf[i] < 0) throw Error(-4, "eigensolve did not converge");
+  if (tooBig)
+    throw Error(-1, "velocities: a degenerate group of " + std::to_string(tooBig) +
+                        " bands (at most 8 are supported)");
 }
 
 // ============================================================ InteractionElPhWan
@@ -1972,6 +2069,19 @@ int phx_ph_model_diag(phx_ph_model* m, int nq, const double* q, int mass_scaling
   });
 }
 
+int phx_ph_model_diag_velocities(phx_ph_model* m, int nq, const double* q, int gamma_rule,
+                                 double* omega, double* eigvecs, double* velocities) {
+  return guard([&] {
+    PHX_REQUIRE(m && (nq == 0 || (q && omega && eigvecs && velocities)),
+                "phx_ph_model_diag_velocities: null argument");
+    PHX_REQUIRE(nq >= 0, "phx_ph_model_diag_velocities: nq < 0");
+    const phx::PhononModel& x = m->m;
+    if (nq)
+      diag_velocities(true, nq, q, x.nm, x.nR, x.R.data(), x.weights.data(), x.mat2R.data(),
+                      x.masses.data(), &x, gamma_rule, omega, eigvecs, velocities);
+  });
+}
+
 int phx_ph_model_destroy(phx_ph_model* m) {
   return guard([&] { delete m; });
 }
@@ -2021,6 +2131,19 @@ int phx_el_model_diag(phx_el_model* m, int nk, const double* k, double* energies
     if (nk)
       bloch_diag(false, nk, k, x.nw, x.nR, x.R.data(), x.deg.data(), x.h0R.data(), nullptr, 0,
                  energies, eigvecs);
+  });
+}
+
+int phx_el_model_diag_velocities(phx_el_model* m, int nk, const double* k, int gamma_rule,
+                                 double* energies, double* eigvecs, double* velocities) {
+  return guard([&] {
+    PHX_REQUIRE(m && (nk == 0 || (k && energies && eigvecs && velocities)),
+                "phx_el_model_diag_velocities: null argument");
+    PHX_REQUIRE(nk >= 0, "phx_el_model_diag_velocities: nk < 0");
+    const phx::ElectronModel& x = m->m;
+    if (nk)
+      diag_velocities(false, nk, k, x.nw, x.nR, x.R.data(), x.deg.data(), x.h0R.data(), nullptr, nullptr,
+                      gamma_rule, energies, eigvecs, velocities);
   });
 }
 

@@ -21,7 +21,9 @@ import os
 
 import numpy as np
 
-from . import ElPhFile, _check, _dp, _ip, _p, _path, lib
+from . import GAMMA_BATCHED, GAMMA_POINT, ElPhFile, _check, _dp, _ip, _p, _path, lib
+
+_GAMMA = {"batched": GAMMA_BATCHED, "point": GAMMA_POINT}
 
 
 class PhononH0:
@@ -76,6 +78,18 @@ class PhononH0:
         _check(lib().phx_ph_model_diag(self._h, nq, _p(q), int(bool(mass_scaling)), _p(om), _p(ev)))
         return om, ev.view(np.complex128).reshape(nq, self.nm, self.nm).transpose(0, 2, 1)
 
+    def diag_velocities(self, q, gamma_rule="batched"):
+        """(omega, V / sqrt(M), velocities (nq, nm, nm, 3)) on the GPU;
+        gamma_rule "batched" (|q|^2 <= 1e-6 is Gamma) or "point" (|q| < 1e-6)."""
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+        nq, n = len(q), self.nm
+        om = np.zeros((nq, n))
+        ev = np.zeros(2 * nq * n * n)
+        vel = np.zeros((nq, n, n, 3), dtype=np.complex128)
+        _check(lib().phx_ph_model_diag_velocities(self._h, nq, _p(q), _GAMMA[gamma_rule], _p(om), _p(ev),
+                                                  _p(vel.view(np.float64))))
+        return om, ev.view(np.complex128).reshape(nq, n, n).transpose(0, 2, 1), vel
+
 
 class ElectronH0Wannier:
     def __init__(self, tb_path, cell=None):
@@ -110,6 +124,17 @@ class ElectronH0Wannier:
         U = np.zeros(2 * nk * self.nw * self.nw)
         _check(lib().phx_el_model_diag(self._h, nk, _p(k), _p(E), _p(U)))
         return E, U.view(np.complex128).reshape(nk, self.nw, self.nw).transpose(0, 2, 1)
+
+    def diag_velocities(self, k, gamma_rule="batched"):
+        """(energies, U, velocities (nk, nw, nw, 3)) on the GPU; gamma_rule as for phonons."""
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+        nk, n = len(k), self.nw
+        E = np.zeros((nk, n))
+        U = np.zeros(2 * nk * n * n)
+        vel = np.zeros((nk, n, n, 3), dtype=np.complex128)
+        _check(lib().phx_el_model_diag_velocities(self._h, nk, _p(k), _GAMMA[gamma_rule], _p(E), _p(U),
+                                                  _p(vel.view(np.float64))))
+        return E, U.view(np.complex128).reshape(nk, n, n).transpose(0, 2, 1), vel
 
 
 @dataclasses.dataclass

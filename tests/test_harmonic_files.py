@@ -190,6 +190,64 @@ def test_hole_mass_known_answer():
     assert abs(abs(mass) - 686.779) < 0.02
 
 
+def test_velocity_known_answers():
+    """test/velocityTest.cpp, WannierH0.Velocity and Velocity2 on 666_si_tb.dat
+    (single-point velocities): the mass from the velocity at k = (0.00025, 0,
+    0) agrees with the one from the energies within 2%, and at k = (0.1, 0.2,
+    0.3) the lowest band's velocity agrees with a forward difference of its
+    energy (1e-4) within 0.1%.  The batched form calls k = (0.00025, 0, 0)
+    Gamma (|k|^2 <= 1e-6) and returns zero there."""
+    tb = HO.read_w90_tb(TB6)
+    dk = 0.00025
+    e0 = np.linalg.eigvalsh(HO.hamiltonian(tb, np.zeros(3)))
+    e1, _, v1 = HO.electron_velocities(tb, np.array([dk, 0, 0]), gamma_rule="point")
+    assert not np.any(HO.electron_velocities(tb, np.array([dk, 0, 0]))[2])
+    mass1 = 2 * (e1[3] - e0[3]) / dk ** 2
+    mass2 = 2 * v1[3, 3, 0].real / dk
+    assert abs((mass1 - mass2) / mass1) < 0.02
+    k1 = np.array([0.1, 0.2, 0.3])
+    ea, _, va = HO.electron_velocities(tb, k1, gamma_rule="point")
+    for i in range(3):
+        kp = k1.copy()
+        kp[i] += 1e-4
+        eb = np.linalg.eigvalsh(HO.hamiltonian(tb, kp))
+        x = (eb[0] - ea[0]) / 1e-4
+        assert abs((x - va[0, 0, i].real) / va[0, 0, i].real) < 0.001
+
+
+def test_velocities_are_energy_gradients():
+    """Hellmann-Feynman: for non-degenerate bands the diagonal of the velocity
+    operator is the gradient of the band energy (central differences), and
+    the velocity operator is Hermitian; a degenerate block's diagonal holds
+    the eigenvalues of the block."""
+    tb = HO.read_w90_tb(TB)
+    rng = np.random.default_rng(7)
+    for k in rng.uniform(-0.8, 0.8, (5, 3)):
+        E, U, v = HO.electron_velocities(tb, k)
+        for i in range(3):
+            assert np.abs(v[:, :, i] - v[:, :, i].conj().T).max() < 1e-9 * np.abs(v).max()
+            dk = np.zeros(3)
+            dk[i] = 1e-5
+            g = (np.linalg.eigvalsh(HO.hamiltonian(tb, k + dk)) - np.linalg.eigvalsh(HO.hamiltonian(tb, k - dk))) / 2e-5
+            sep = np.min(np.abs(np.diff(E)))
+            if sep > 1e-4:
+                assert np.abs(np.diag(v[:, :, i]).real - g).max() < 1e-6 * np.abs(g).max()
+    # along Gamma-L the bands pair up: the block diagonal is basis-independent
+    d = HO.read_qe_fc(FC2)
+    B = 2 * np.pi * np.linalg.inv(d["cell"]).T
+    kx = 0.2 * B[:, 0]
+    E, U, v = HO.electron_velocities(tb, kx)
+    groups = np.split(np.arange(len(E)), np.where(np.diff(E) > 1e-6 / HO.ENERGY_RY_TO_EV)[0] + 1)
+    assert any(len(g) > 1 for g in groups)
+    dH = [(HO.hamiltonian(tb, kx + e) - HO.hamiltonian(tb, kx - e)) / 2e-8 for e in 1e-8 * np.eye(3)]
+    for g in groups:
+        if len(g) > 1:
+            for i in range(3):
+                blk = U[:, g].conj().T @ (0.5 * (dH[i] + dH[i].conj().T)) @ U[:, g]
+                lam = np.linalg.eigvalsh(0.5 * (blk + blk.conj().T))
+                assert np.abs(np.sort(np.diag(v[g][:, g, i]).real) - lam).max() < 1e-6 * np.abs(v).max()
+
+
 # ---------------------------------------------------------------- library vs oracle
 @pytest.mark.parametrize("path,rule", [(FC2, "simple"), (FC4, "simple"), (FC4, "")])
 def test_library_phonon_model_matches_oracle(path, rule):
