@@ -278,7 +278,14 @@ typedef struct {
   const double* gVectors;    /* nG x 3 */
   const double* lrConstant;  /* natoms x 3 x 3, (atom, i, j): added to the diagonal blocks */
   double lrNorm, gMax;       /* e2 * fourPi / volume (fourPi as constants.h:12 defines it), 14 */
+  /* smearing: PHX_SMEAR_GAUSSIAN (width sigma) or PHX_SMEAR_ADAPTIVE
+   * (AdaptiveGaussianDeltaFunction, delta_function.cpp:55-97: per-pair width
+   * from the group velocities of the two electron states, el_scattering.cpp:
+   * 291-295; sigma is then unused and the energy screens are off). */
+  int smearing;
 } phx_problem;
+
+enum { PHX_SMEAR_GAUSSIAN = 0, PHX_SMEAR_ADAPTIVE = 1 };
 
 typedef struct phx_lw phx_lw;
 

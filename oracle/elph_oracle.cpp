@@ -535,6 +535,97 @@ double orc_gaussian_delta(double energy, double sigma) {
   return prefactor * std::exp(-x * x);
 }
 
+// AdaptiveGaussianDeltaFunction (delta_function.cpp:55-97): qTensor = the
+// reciprocal cell with row i divided by mesh(i) (:58-62); width
+// sqrt(sum_i (qTensor.row(i) . v)^2 / 6) with v = v1 - v2, at least 0.1 meV,
+// the Gaussian cut at two widths and divided by erf(2).
+double orc_adaptive_delta(double energy, const double* v, const double* recip, const int* mesh) {
+  if (v[0] == 0. && v[1] == 0. && v[2] == 0. && energy == 0.) return 1.;
+  double sigma = 0.;
+  for (int i = 0; i < 3; ++i) {
+    double d = 0.;
+    for (int j = 0; j < 3; ++j) d += recip[i + 3 * j] / mesh[i] * v[j];
+    sigma += d * d;
+  }
+  sigma = std::sqrt(sigma / 6.);
+  if (sigma == 0.) return 0.;
+  const double energyRyToEv = (4.359744650e-18 / 2.) / 1.6021766208e-19;
+  const double cutoff = 0.0001 / energyRyToEv;
+  if (sigma < cutoff) sigma = cutoff;
+  if (std::abs(energy) > 2. * sigma) return 0.;
+  double x = energy / sigma;
+  if (x > 6.) return 0.;
+  return std::exp(-x * x) / std::sqrt(kPi) / sigma / 0.9953222650189527;
+}
+
+// Group velocities (the diagonal of the velocity operator) of
+// ElectronH0Wannier::batchedDiagonalizeWithVelocities (electron_h0_wannier.cpp:
+// 633-760): central differences (1e-8) of H(k), hermitized, rotated to the
+// eigenbasis; zero when |k| < 1e-6; a degenerate group (1 micro-eV) gets the
+// eigenvalues of its block, ascending.  vg: nw x 3.
+int orc_el_group_velocities(int nw, int nR, const double* R, const double* deg, const double* h0R,
+                            const double* k, double* vg) {
+  const double delta = 1.0e-8;
+  const double energyRyToEv = (4.359744650e-18 / 2.) / 1.6021766208e-19;
+  const double threshold = 0.000001 / energyRyToEv;
+  const size_t nn = size_t(nw) * nw;
+  std::vector<double> U(2 * nn), E(nw);
+  orc_el_hamiltonian(nw, nR, R, deg, h0R, k, U.data());
+  if (orc_zheev(nw, U.data(), E.data()) < 0) return -1;
+  std::fill(vg, vg + 3 * nw, 0.);
+  if (std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]) < 1.0e-6) return 0;
+  std::vector<double> Hp(2 * nn), Hm(2 * nn);
+  std::vector<cd> V(nn * 3);
+  for (int i = 0; i < 3; ++i) {
+    double kp[3] = {k[0], k[1], k[2]}, km[3] = {k[0], k[1], k[2]};
+    kp[i] += delta;
+    km[i] -= delta;
+    orc_el_hamiltonian(nw, nR, R, deg, h0R, kp, Hp.data());
+    orc_el_hamiltonian(nw, nR, R, deg, h0R, km, Hm.data());
+    std::vector<cd> der(nn);
+    for (int n = 0; n < nw; ++n)
+      for (int m = 0; m < nw; ++m) {
+        const size_t a = m + size_t(nw) * n, b = n + size_t(nw) * m;
+        cd x = (cd(Hp[2 * a], Hp[2 * a + 1]) - cd(Hm[2 * a], Hm[2 * a + 1])) / (2. * delta);
+        cd y = (cd(Hp[2 * b], Hp[2 * b + 1]) - cd(Hm[2 * b], Hm[2 * b + 1])) / (2. * delta);
+        der[a] = 0.5 * (x + std::conj(y));
+      }
+    // V_i = U^H der U
+    for (int n = 0; n < nw; ++n)
+      for (int m = 0; m < nw; ++m) {
+        cd acc = 0.;
+        for (int l = 0; l < nw; ++l)
+          for (int l2 = 0; l2 < nw; ++l2)
+            acc += std::conj(cd(U[2 * (l + size_t(nw) * m)], U[2 * (l + size_t(nw) * m) + 1])) *
+                   der[l + size_t(nw) * l2] * cd(U[2 * (l2 + size_t(nw) * n)], U[2 * (l2 + size_t(nw) * n) + 1]);
+        V[(m + size_t(nw) * n) * 3 + i] = acc;
+      }
+  }
+  for (int ib = 0; ib < nw;) {
+    int sz = 1;
+    while (ib + sz < nw && std::abs(E[ib] - E[ib + sz]) <= threshold) ++sz;
+    for (int i = 0; i < 3; ++i) {
+      if (sz == 1) {
+        vg[3 * ib + i] = V[(ib + size_t(nw) * ib) * 3 + i].real();
+        continue;
+      }
+      std::vector<double> blk(2 * size_t(sz) * sz), ev(sz);
+      for (int b = 0; b < sz; ++b)
+        for (int a = 0; a < sz; ++a) {
+          cd x = V[((ib + a) + size_t(nw) * (ib + b)) * 3 + i];
+          cd y = V[((ib + b) + size_t(nw) * (ib + a)) * 3 + i];
+          cd h = 0.5 * (x + std::conj(y));
+          blk[2 * (a + size_t(sz) * b)] = h.real();
+          blk[2 * (a + size_t(sz) * b) + 1] = h.imag();
+        }
+      if (orc_zheev(sz, blk.data(), ev.data()) < 0) return -1;
+      for (int a = 0; a < sz; ++a) vg[3 * (ib + a) + i] = ev[a];
+    }
+    ib += sz;
+  }
+  return 0;
+}
+
 // Particle::getPopulation (particle.cpp:64-96).
 double orc_population(int isBose, double energy, double temperature, double chemPot) {
   double population = 0.;
@@ -577,6 +668,7 @@ struct orc_problem {
   int natoms, nG;  // long-range term (nG = 0: none), same fields as phx_problem
   const double *atomPos, *bornCharges, *dielectric, *gVectors, *lrConstant;
   double lrNorm, gMax;
+  int smearing;    // 0 Gaussian, 1 adaptive Gaussian
 };
 
 struct orc_state {
@@ -588,6 +680,7 @@ struct orc_state {
   std::vector<int> stateOffset;   // per active point, first iBte
   int numStates = 0;
   std::vector<double> energies;   // per state
+  std::vector<double> stateV;     // per state: group velocity (adaptive smearing)
   std::vector<double> eigvecs;    // per active point: (nw, nb) col-major interleaved, packed
   std::vector<size_t> eigOffset;  // in complex elements
   std::vector<int> fullToActive;
@@ -685,6 +778,26 @@ void* orc_setup(const orc_problem* prob) {
   }
   s->eigOffset.push_back(eoff);
   s->stateOffset.push_back(s->numStates);
+  if (p.smearing == 1) {
+    // group velocities of the window's bands (active_bandstructure.cpp:1032-1071)
+    s->stateV.assign(3 * size_t(s->numStates), 0.);
+    const int nact = int(s->active.size());
+    int vfail = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : vfail)
+    for (int a = 0; a < nact; ++a) {
+      std::vector<double> vg(3 * size_t(p.nw));
+      vfail += orc_el_group_velocities(p.nw, p.nRh, p.Rh, p.degh, p.h0R,
+                                       &s->kcart[3 * size_t(s->active[a])], vg.data()) != 0;
+      const int nb = s->bandHi[a] - s->bandLo[a] + 1;
+      for (int b = 0; b < nb; ++b)
+        for (int i = 0; i < 3; ++i)
+          s->stateV[3 * size_t(s->stateOffset[a] + b) + i] = vg[3 * size_t(s->bandLo[a] + b) + i];
+    }
+    if (vfail) {
+      delete s;
+      return nullptr;
+    }
+  }
   // q-set: all folded k2 - k1 (helper_el_scattering.cpp:95-124)
   s->qInSet.assign(s->nk, 0);
   const int na = int(s->active.size());
@@ -846,8 +959,17 @@ long long orc_linewidth_pairs(void* h, int nk1, const int* k1List, int k2Begin, 
           for (int ib3 = 0; ib3 < p.nm; ++ib3) {
             double en3 = w3[ib3];
             if (en3 < phononCutoff) continue;
-            double delta1 = orc_gaussian_delta(en1 - en2 + en3, p.sigma);
-            double delta2 = orc_gaussian_delta(en1 - en2 - en3, p.sigma);
+            double delta1, delta2;
+            if (p.smearing == 1) {
+              double v[3];
+              for (int i = 0; i < 3; ++i)
+                v[i] = s->stateV[3 * size_t(iBte1) + i] - s->stateV[3 * size_t(iBte2) + i];
+              delta1 = orc_adaptive_delta(en1 - en2 + en3, v, p.recip, p.mesh);
+              delta2 = orc_adaptive_delta(en1 - en2 - en3, v, p.recip, p.mesh);
+            } else {
+              delta1 = orc_gaussian_delta(en1 - en2 + en3, p.sigma);
+              delta2 = orc_gaussian_delta(en1 - en2 - en3, p.sigma);
+            }
             if (delta1 <= 0. && delta2 <= 0.) continue;
             double c = cpl[ib1 + size_t(nb1) * (ib2 + size_t(nb2) * ib3)];
             for (int ic = 0; ic < p.nCalc; ++ic) {

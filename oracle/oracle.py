@@ -32,6 +32,7 @@ class OrcProblem(C.Structure):
         ("natoms", C.c_int), ("nG", C.c_int), ("atomPos", _dp), ("bornCharges", _dp),
         ("dielectric", _dp), ("gVectors", _dp), ("lrConstant", _dp),
         ("lrNorm", C.c_double), ("gMax", C.c_double),
+        ("smearing", C.c_int),
     ]
 
 
@@ -68,6 +69,9 @@ def lib():
         L.orc_linewidth_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int, _dp]
         L.orc_degeneracy_average.argtypes = [C.c_void_p, _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_adaptive_delta.restype = C.c_double
+        L.orc_adaptive_delta.argtypes = [C.c_double, _dp, _dp, _ip]
+        L.orc_el_group_velocities.restype = C.c_int
         _lib = L
     return _lib
 
@@ -234,6 +238,7 @@ class Oracle:
             s.dielectric, s.gVectors = _p(keep(lr["dielectric"])), _p(keep(lr["gVectors"]))
             s.lrConstant = _p(keep(lr["constant"]))
             s.lrNorm, s.gMax = float(lr["norm"]), float(lr["gMax"])
+        s.smearing = int(getattr(prob, "smearing", 0))
         self._s = s
         self.h = L.orc_setup(C.byref(s))
         if not self.h:
@@ -287,3 +292,19 @@ class Oracle:
             lib().orc_degeneracy_average(self.h, _p(lw))
         self.last_pairs = pairs
         return lw.reshape(nc, self.num_states)
+
+
+def adaptive_delta(energy, v, recip, mesh):
+    """AdaptiveGaussianDeltaFunction::getSmearing for v = v1 - v2."""
+    rc = np.asfortranarray(recip, dtype=np.float64).reshape(-1, order="F").copy()
+    m = np.ascontiguousarray(mesh, dtype=np.int32)
+    return lib().orc_adaptive_delta(float(energy), _p(_c(v)), _p(rc), m.ctypes.data_as(_ip))
+
+
+def el_group_velocities(prob, k):
+    """(nw, 3) group velocities of the single-point velocity operator at k."""
+    out = np.zeros(3 * prob.nw)
+    rc = lib().orc_el_group_velocities(prob.nw, len(prob.Rh), _p(_c(prob.Rh)), _p(_c(prob.degh)),
+                                       _p(cplx_view(prob.h0R)), _p(_c(k)), _p(out))
+    assert rc == 0
+    return out.reshape(prob.nw, 3)

@@ -34,6 +34,7 @@ PRECISIONS = {"fp64": PHX_FP64, "fp32": PHX_FP32, "bf16x3": PHX_BF16X3}
 TRANSFORM_DIRECT, TRANSFORM_SEPARABLE = 0, 1
 TRANSFORMS = {"direct": TRANSFORM_DIRECT, "separable": TRANSFORM_SEPARABLE}
 GAMMA_BATCHED, GAMMA_POINT = 0, 1
+SMEAR_GAUSSIAN, SMEAR_ADAPTIVE = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -57,6 +58,7 @@ class PhxProblem(C.Structure):
         ("natoms", C.c_int), ("nG", C.c_int), ("atomPos", _dp), ("bornCharges", _dp),
         ("dielectric", _dp), ("gVectors", _dp), ("lrConstant", _dp),
         ("lrNorm", C.c_double), ("gMax", C.c_double),
+        ("smearing", C.c_int),
     ]
 
 
@@ -430,6 +432,7 @@ def make_problem_struct(prob, gW=None):
     if getattr(prob, "gw_decay", None) is not None:
         s.gwDecay = k(prob.gw_decay)
     s.gwSeed = int(getattr(prob, "gw_seed", 0))
+    s.smearing = int(getattr(prob, "smearing", 0))
     lr = getattr(prob, "long_range", None)
     if lr is not None and lr["nG"] > 0:
         s.natoms, s.nG = int(lr["natoms"]), int(lr["nG"])

@@ -663,6 +663,10 @@ struct BatchMeta {
 
 struct LwHandle {
   std::vector<double> ownT, ownMu;  // copies of the problem's temperatures and chemical potentials
+  // adaptive smearing: group velocities of the states and the scaled reciprocal cell
+  int adaptive = 0;
+  DevBuf<double> stateV;
+  double qT[9] = {};
   phx_problem prob{};
   int precision = PHX_FP64, k1Batch = 16;
   hipStream_t stream = nullptr;
@@ -887,6 +891,40 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
   h2d(L->rowU2off, L->u2off, s);
   h2d(L->rowStateOff, L->stateOff, s);
   h2d(L->e2pool, L->stateE, s);
+  // ---- group velocities for adaptive smearing: the single-point velocity
+  //      operator of every active point (buildAsPostprocessing with velocities,
+  //      active_bandstructure.cpp:1032-1071), the window's bands kept
+  PHX_REQUIRE(p.smearing == PHX_SMEAR_GAUSSIAN || p.smearing == PHX_SMEAR_ADAPTIVE,
+              "phx_problem: unknown smearing");
+  if (p.smearing == PHX_SMEAR_ADAPTIVE) {
+    PHX_REQUIRE(nw <= 44, "adaptive smearing: at most 44 bands");
+    std::vector<double> sv(3 * size_t(st), 0.);
+    const int chunk = 2048;
+    std::vector<double> kc, E, U, V;
+    for (int a0 = 0; a0 < na; a0 += chunk) {
+      const int n = std::min(chunk, na - a0);
+      kc.resize(3 * size_t(n));
+      for (int a = 0; a < n; ++a)
+        for (int i = 0; i < 3; ++i) kc[3 * size_t(a) + i] = L->kcart[3 * size_t(L->active[a0 + a]) + i];
+      E.resize(size_t(n) * nw);
+      U.resize(2 * size_t(n) * nw * nw);
+      V.resize(2 * size_t(n) * nw * nw * 3);
+      diag_velocities(false, n, kc.data(), nw, p.nRh, p.Rh, p.degh, p.h0R, nullptr, nullptr,
+                      PHX_GAMMA_POINT, E.data(), U.data(), V.data());
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < L->nb[a0 + a]; ++b) {
+          const int bf = L->lo[a0 + a] + b;
+          for (int i = 0; i < 3; ++i)
+            sv[3 * size_t(L->stateOff[a0 + a] + b) + i] =
+                V[2 * ((((size_t(a) * nw + bf) * nw) + bf) * 3 + i)];
+        }
+    }
+    h2d(L->stateV, sv, s);
+    // qTensor: the reciprocal cell with row i divided by mesh(i) (delta_function.cpp:58-62)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) L->qT[3 * i + j] = p.recip[i + 3 * j] / p.mesh[i];
+    L->adaptive = 1;
+  }
   {
     std::vector<double2> er(na);
     for (int a = 0; a < na; ++a)
@@ -1591,8 +1629,12 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
   cq.numStates = L->numStates;
   cq.bose = L->bose.p;
   cq.nCalc = L->nCalc;
-  cq.invSigma = 1. / p.sigma;
-  cq.deltaPref = 1. / p.sigma / std::sqrt(kPi);
+  cq.invSigma = L->adaptive ? 0. : 1. / p.sigma;
+  cq.deltaPref = L->adaptive ? 0. : 1. / p.sigma / std::sqrt(kPi);
+  cq.adaptive = L->adaptive;
+  cq.stateV = L->stateV.p;
+  for (int i = 0; i < 9; ++i) cq.qT[i] = L->qT[i];
+  cq.minWidth = 0.0001 / units::energyRyToEv;  // delta_function.h:107
   cq.norm = 1. / (double(p.mesh[0]) * p.mesh[1] * p.mesh[2]);
   cq.cutoff = phonon_cutoff();
   cq.partial = L->partial.p;
@@ -2500,6 +2542,8 @@ int phx_lw_set_screen(phx_lw* h, double nsigma) {
   return guard([&] {
     PHX_REQUIRE(h, "null handle");
     PHX_REQUIRE(nsigma == 0. || nsigma >= 6., "phx_lw_set_screen: use 0 (off) or >= 6 sigma");
+    PHX_REQUIRE(nsigma == 0. || !h->adaptive,
+                "phx_lw_set_screen: the energy screen needs a fixed Gaussian width (adaptive smearing is on)");
     PHX_REQUIRE(nsigma == 0. || h->transform == PHX_TRANSFORM_SEPARABLE,
                 "phx_lw_set_screen: the screen runs with the separable transform");
     h->screen = nsigma;

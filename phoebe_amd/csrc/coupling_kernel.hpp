@@ -75,6 +75,11 @@ struct CoupParams {
   // precomputed row lists of the last stage (m0 <= 64)
   unsigned long long* keepMask;           // per slot: bit jx set = row jx kept (null: screen in k_sep_xm)
   const int4* meshRow;                    // per mesh point: (row, nb2, u2off, stateOff), row -1 if inactive
+  // adaptive Gaussian smearing: width from the group velocities of the two states
+  int adaptive;                           // 0: fixed Gaussian (invSigma, deltaPref)
+  const double* stateV;                   // numStates x 3 group velocities
+  double qT[9];                           // reciprocal cell, row i divided by mesh(i), row-major
+  double minWidth;                        // smallest width (0.1 meV)
 };
 
 template <typename T> struct Vec2;
@@ -86,6 +91,41 @@ __device__ inline double gaussian_delta(double energy, double invW, double pref)
   double x = energy * invW;
   if (x > 6.) return 0.;
   return pref * exp(-x * x);
+}
+
+// AdaptiveGaussianDeltaFunction::getSmearing (delta_function.cpp:66-97) with
+// v = v(state 1) - v(state 2) (el_scattering.cpp:291-295): width
+// sqrt(sum_i (qT_i . v)^2 / 6), at least minWidth; cut at 2 widths and
+// renormalized by erf(2).
+struct AdaptiveWidth {
+  double sigma;  // 0: delta is zero; < 0: zero velocity difference
+};
+__device__ inline AdaptiveWidth adaptive_width(const CoupParams& p, int is1, int is2) {
+  const double v0 = p.stateV[3 * is1] - p.stateV[3 * is2];
+  const double v1 = p.stateV[3 * is1 + 1] - p.stateV[3 * is2 + 1];
+  const double v2 = p.stateV[3 * is1 + 2] - p.stateV[3 * is2 + 2];
+  AdaptiveWidth w;
+  if (v0 == 0. && v1 == 0. && v2 == 0.) {
+    w.sigma = -1.;
+    return w;
+  }
+  double s = 0.;
+  for (int i = 0; i < 3; ++i) {
+    const double d = p.qT[3 * i] * v0 + p.qT[3 * i + 1] * v1 + p.qT[3 * i + 2] * v2;
+    s += d * d;
+  }
+  s = sqrt(s / 6.);
+  if (s != 0. && s < p.minWidth) s = p.minWidth;
+  w.sigma = s;
+  return w;
+}
+__device__ inline double adaptive_delta(double energy, AdaptiveWidth w) {
+  if (w.sigma < 0.) return energy == 0. ? 1. : 0.;
+  if (w.sigma == 0.) return 0.;
+  if (fabs(energy) > 2. * w.sigma) return 0.;
+  const double x = energy / w.sigma;
+  if (x > 6.) return 0.;
+  return exp(-x * x) / 1.7724538509055160273 / w.sigma / 0.9953222650189527;
 }
 
 // FP64 / FP32 main loop on v_mfma_*_16x16x4: A lane l holds A[l%16][l/16], B lane l holds
@@ -225,8 +265,15 @@ __device__ void epilogue16_tail(const CoupParams& p, unsigned char* scratch, int
     for (int gi = 0; gi < kGMax; ++gi) {
       if (gi >= gsize) break;
       const double en1 = e1[gstart + gi];
-      const double d1 = gaussian_delta(en1 - en2 + en3, p.invSigma, p.deltaPref);
-      const double d2 = gaussian_delta(en1 - en2 - en3, p.invSigma, p.deltaPref);
+      double d1, d2;
+      if (p.adaptive) {
+        const AdaptiveWidth aw = adaptive_width(p, p.kkStateOff[kk] + gstart + gi, is2);
+        d1 = adaptive_delta(en1 - en2 + en3, aw);
+        d2 = adaptive_delta(en1 - en2 - en3, aw);
+      } else {
+        d1 = gaussian_delta(en1 - en2 + en3, p.invSigma, p.deltaPref);
+        d2 = gaussian_delta(en1 - en2 - en3, p.invSigma, p.deltaPref);
+      }
       if (d1 <= 0. && d2 <= 0.) continue;
 #pragma unroll
       for (int c = 0; c < kCMax; ++c) {
@@ -906,8 +953,15 @@ __global__ void __launch_bounds__(256, 2) k_sep_x(CoupParams p) {
 #pragma unroll
           for (int gi = 0; gi < kGMax; ++gi) {
             if (gi >= gsize) break;
-            const double d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
-            const double d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+            double d1, d2;
+            if (p.adaptive) {
+              const AdaptiveWidth aw = adaptive_width(p, int(e1 - p.e1pool) + gi, is2);
+              d1 = adaptive_delta(e1[gi] - en2 + w3, aw);
+              d2 = adaptive_delta(e1[gi] - en2 - w3, aw);
+            } else {
+              d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
+              d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+            }
             if (d1 <= 0. && d2 <= 0.) continue;
 #pragma unroll
             for (int c = 0; c < CM; ++c) {
@@ -1425,8 +1479,15 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 #pragma unroll
             for (int gi = 0; gi < kGMax; ++gi) {
               if (gi >= gsize) break;
-              const double d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
-              const double d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+              double d1, d2;
+              if (p.adaptive) {
+                const AdaptiveWidth aw = adaptive_width(p, int(e1 - p.e1pool) + gi, is2);
+                d1 = adaptive_delta(e1[gi] - en2 + w3, aw);
+                d2 = adaptive_delta(e1[gi] - en2 - w3, aw);
+              } else {
+                d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
+                d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+              }
               if (d1 <= 0. && d2 <= 0.) continue;
 #pragma unroll
               for (int c = 0; c < CM; ++c) {

@@ -166,6 +166,7 @@ class LinewidthProblem:
     win_max: float
     long_range: dict
     elph_path: str
+    smearing: int = 0            # 0 Gaussian (sigma), 1 adaptive Gaussian
 
     @property
     def nRe(self):
@@ -181,8 +182,10 @@ class LinewidthProblem:
 
 
 def linewidth_problem(el: ElectronH0Wannier, ph: PhononH0, elph_path, mesh, temperatures,
-                      chem_pots, sigma, win_min, win_max, name="files") -> LinewidthProblem:
-    """Temperatures, chemical potentials, sigma and the window in Ry."""
+                      chem_pots, sigma, win_min, win_max, name="files",
+                      smearing="gaussian") -> LinewidthProblem:
+    """Temperatures, chemical potentials, sigma and the window in Ry;
+    smearing "gaussian" (width sigma) or "adaptive"."""
     with ElPhFile(elph_path) as f:
         h = f.header
         if h["numElBands"] != el.nw or h["numPhModes"] != ph.nm:
@@ -195,4 +198,4 @@ def linewidth_problem(el: ElectronH0Wannier, ph: PhononH0, elph_path, mesh, temp
         temperatures=np.atleast_1d(np.asarray(temperatures, dtype=float)),
         chem_pots=np.atleast_1d(np.asarray(chem_pots, dtype=float)), sigma=float(sigma),
         win_min=float(win_min), win_max=float(win_max), long_range=ph.long_range,
-        elph_path=os.fspath(elph_path))
+        elph_path=os.fspath(elph_path), smearing={"gaussian": 0, "adaptive": 1}[smearing])

@@ -123,3 +123,50 @@ def test_si_linewidths_from_reference_files(fc, rule, transform):
     assert nz.sum() > 0.5 * ref.size
     assert err.max() < 1e-10
     assert np.abs(got[~nz]).max(initial=0.0) <= 1e-30 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("transform", ["direct", "separable"])
+def test_si_linewidths_adaptive_smearing(transform):
+    """Adaptive Gaussian smearing (per-pair width from the electrons' group
+    velocities) on the Si inputs: GPU against the oracle at 1e-7; the energy
+    screen is refused because it assumes one width.
+
+    The tolerance is set by the velocities, not by the reduction: they are
+    central differences of H(k) with the reference's step 1e-8, so the
+    rounding of two evaluations of H (sums in different orders) reaches the
+    velocities amplified by 1 / 1e-8, about 1e-8 relative, and the widths and
+    deltas inherit it."""
+    say("Si linewidths, adaptive smearing", transform)
+    ph = Hm.PhononH0(FC4, "")
+    el = Hm.ElectronH0Wannier(TB, ph.cell)
+    mesh = (6, 6, 6)
+    pts = np.array([[i / 6, j / 6, k / 6] for k in range(6) for j in range(6) for i in range(6)])
+    E, _ = el.diag((ph.recip @ pts.T).T)
+    vbm = E[:, 3].max()
+    prob = Hm.linewidth_problem(el, ph, ELPH, mesh, temperatures=[300 * KELVIN_TO_RY, 600 * KELVIN_TO_RY],
+                                chem_pots=[vbm, vbm - 0.1 * RY_PER_EV], sigma=0.05 * RY_PER_EV,
+                                win_min=vbm - 1.0 * RY_PER_EV, win_max=vbm + 1.0 * RY_PER_EV,
+                                smearing="adaptive")
+    gW = H5.read_elph(ELPH)["gW"]
+    orc = O.Oracle(prob, gW.reshape(-1))
+    ref = orc.linewidths()
+    lw = P.ElectronLinewidths(prob, precision="fp64", transform=transform, elph_file=ELPH)
+    if transform == "separable":
+        with pytest.raises(P.PhxError, match="fixed Gaussian width"):
+            lw.set_screen(6.0)
+    pairs = lw.run()
+    assert pairs == orc.last_pairs
+    got = lw.linewidths()
+    nz = ref > 1e-30 * np.abs(ref).max()
+    err = rel_err(got[nz], ref[nz])
+    say("  states", lw.num_states, "nonzero", int(nz.sum()), "max rel err", err.max())
+    assert nz.sum() > 0.3 * ref.size
+    assert err.max() < 1e-7
+    assert np.abs(got[~nz]).max(initial=0.0) <= 1e-30 * np.abs(ref).max()
+    # the widths differ from the fixed Gaussian: the linewidths change
+    gauss = Hm.linewidth_problem(el, ph, ELPH, mesh, temperatures=prob.temperatures,
+                                 chem_pots=prob.chem_pots, sigma=0.05 * RY_PER_EV,
+                                 win_min=prob.win_min, win_max=prob.win_max)
+    lg = P.ElectronLinewidths(gauss, precision="fp64", transform=transform, elph_file=ELPH)
+    lg.run()
+    assert np.abs(lg.linewidths() - got).max() > 1e-3 * np.abs(got).max()

@@ -291,3 +291,51 @@ def test_reader_errors(tmp_path):
     bad.write_text("".join(open(FC2).readlines()[:40]))
     with pytest.raises(P.PhxError, match="unexpected end of file"):
         Hm.PhononH0(bad)
+
+
+# ---------------------------------------------------------------- adaptive smearing (oracle)
+def test_adaptive_delta_closed_forms():
+    """AdaptiveGaussianDeltaFunction: the width is sqrt(sum_i (Q_i . v)^2 / 6)
+    with Q the reciprocal cell whose row i is divided by mesh(i); the delta
+    integrates to one over its [-2 w, 2 w] support (the erf(2) factor); a zero
+    velocity difference gives 1 at zero energy and 0 elsewhere; small widths
+    are raised to 0.1 meV."""
+    from oracle import oracle as O
+    d = HO.read_qe_fc(FC2)
+    B = 2 * np.pi * np.linalg.inv(d["cell"]).T
+    mesh = np.array([4, 6, 5])
+    v = np.array([0.3, -0.7, 0.4])
+    Q = B / mesh[:, None]
+    w = np.sqrt(((Q @ v) ** 2).sum() / 6)
+    assert w > 0.0001 / HO.ENERGY_RY_TO_EV
+    assert O.adaptive_delta(0.0, v, B, mesh) == pytest.approx(1 / np.sqrt(np.pi) / w / 0.9953222650189527,
+                                                              rel=1e-14)
+    # the row-wise scaling is not the column-wise one (b_i / mesh_i) on this mesh
+    w_col = np.sqrt((((B / mesh[None, :]).T @ v) ** 2).sum() / 6)
+    assert abs(w_col - w) > 1e-3 * w
+    e = np.linspace(-2.2 * w, 2.2 * w, 20001)
+    vals = np.array([O.adaptive_delta(x, v, B, mesh) for x in e])
+    assert np.all(vals[np.abs(e) > 2 * w] == 0)
+    assert abs(np.trapezoid(vals, e) - 1) < 2e-3     # the cut at 2 w makes the grid integral converge slowly
+    assert O.adaptive_delta(0.0, np.zeros(3), B, mesh) == 1.0
+    assert O.adaptive_delta(1e-9, np.zeros(3), B, mesh) == 0.0
+    tiny = np.array([1e-9, 0, 0])
+    wmin = 0.0001 / HO.ENERGY_RY_TO_EV
+    assert O.adaptive_delta(0.0, tiny, B, mesh) == pytest.approx(1 / np.sqrt(np.pi) / wmin / 0.9953222650189527,
+                                                                 rel=1e-14)
+
+
+def test_oracle_group_velocities_match_velocity_operator():
+    """The C oracle's group velocities (single-point rule) equal the diagonal of
+    the numpy velocity operator, degenerate blocks included."""
+    from oracle import oracle as O
+    ph = Hm.PhononH0(FC2, "simple")
+    el = Hm.ElectronH0Wannier(TB, ph.cell)
+    tb = HO.read_w90_tb(TB, ph.cell)
+    prob = type("P", (), dict(nw=el.nw, Rh=el.R, degh=el.deg, h0R=el.h0R))
+    for k in [0.2 * ph.recip[:, 0], np.array([0.1, -0.3, 0.2]), np.array([2e-7, 0, 0])]:
+        vg = O.el_group_velocities(prob, k)
+        _, _, v = HO.electron_velocities(tb, k, gamma_rule="point")
+        ref = np.array([[v[b, b, i].real for i in range(3)] for b in range(el.nw)])
+        assert np.abs(vg - ref).max() <= 1e-7 * max(np.abs(ref).max(), 1e-300)
+    assert not np.any(O.el_group_velocities(prob, np.array([2e-7, 0, 0])))
