@@ -563,8 +563,18 @@ double orc_adaptive_delta(double energy, const double* v, const double* recip, c
 // 633-760): central differences (1e-8) of H(k), hermitized, rotated to the
 // eigenbasis; zero when |k| < 1e-6; a degenerate group (1 micro-eV) gets the
 // eigenvalues of its block, ascending.  vg: nw x 3.
+int orc_el_group_velocities_rule(int nw, int nR, const double* R, const double* deg,
+                                 const double* h0R, const double* k, int gammaRule, double* vg);
+
 int orc_el_group_velocities(int nw, int nR, const double* R, const double* deg, const double* h0R,
                             const double* k, double* vg) {
+  return orc_el_group_velocities_rule(nw, nR, R, deg, h0R, k, 1, vg);
+}
+
+// gammaRule 1: zero when |k| < 1e-6 (single-point form); 0: unless |k|^2 >
+// 1e-6 (the batched form, electron_h0_wannier.cpp:978-983).
+int orc_el_group_velocities_rule(int nw, int nR, const double* R, const double* deg,
+                                 const double* h0R, const double* k, int gammaRule, double* vg) {
   const double delta = 1.0e-8;
   const double energyRyToEv = (4.359744650e-18 / 2.) / 1.6021766208e-19;
   const double threshold = 0.000001 / energyRyToEv;
@@ -573,7 +583,8 @@ int orc_el_group_velocities(int nw, int nR, const double* R, const double* deg, 
   orc_el_hamiltonian(nw, nR, R, deg, h0R, k, U.data());
   if (orc_zheev(nw, U.data(), E.data()) < 0) return -1;
   std::fill(vg, vg + 3 * nw, 0.);
-  if (std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]) < 1.0e-6) return 0;
+  const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+  if (gammaRule ? std::sqrt(k2) < 1.0e-6 : !(k2 > 1.0e-6)) return 0;
   std::vector<double> Hp(2 * nn), Hm(2 * nn);
   std::vector<cd> V(nn * 3);
   for (int i = 0; i < 3; ++i) {
@@ -1018,6 +1029,160 @@ int orc_num_threads() {
 #else
   return 1;
 #endif
+}
+
+
+// ---------------------------------------------------------------------------
+// PhElScatteringMatrix::builder (phel_scattering.cpp:43-363): phonon
+// linewidths on the Gamma-centred q mesh qmesh from the electrons of the k mesh
+// (window as the electron builder's), k2 = k1 + q3 with all bands, coupling
+// symmetrized, rate |g|^2 f(1-f)(e1) delta(e1 - e2 + w3) norm / T pi with
+// norm = 1 / prod(kMesh), then degeneracy averaging over the phonon modes
+// (row iCalc = 0, scattering.cpp:1168-1207).  out: nCalc x nq x nm.
+// Returns the number of (k1, q3) pairs, or -1.
+long long orc_phel(const orc_problem* prob, const int* qmesh, int average, double* out) {
+  const orc_problem& p = *prob;
+  const int nw = p.nw, nm = p.nm;
+  const int nk = p.mesh[0] * p.mesh[1] * p.mesh[2];
+  const int nq = qmesh[0] * qmesh[1] * qmesh[2];
+  const double norm = 1. / double(nk);
+  const double phononCutoff = orc_phonon_cutoff();
+  // electrons on the k mesh, window
+  std::vector<double> kcart(3 * size_t(nk)), allE(size_t(nk) * nw), allU(2 * size_t(nk) * nw * nw);
+  int fail = 0;
+  for (int ik = 0; ik < nk; ++ik) {
+    double x[3];
+    orc_mesh_point(ik, p.mesh, x);
+    cart_of(p, x, &kcart[3 * size_t(ik)]);
+    fail += orc_el_diag(nw, p.nRh, p.Rh, p.degh, p.h0R, &kcart[3 * size_t(ik)], &allE[size_t(ik) * nw],
+                        &allU[2 * size_t(ik) * nw * nw]) != 0;
+  }
+  if (fail) return -1;
+  std::vector<int> active, lo, hi;
+  for (int ik = 0; ik < nk; ++ik) {
+    int l = -1, h = -1;
+    for (int ib = 0; ib < nw; ++ib) {
+      double e = allE[size_t(ik) * nw + ib];
+      if (e < p.winMax && e > p.winMin) {
+        if (l < 0) l = ib;
+        h = ib;
+      }
+    }
+    if (l < 0) continue;
+    active.push_back(ik);
+    lo.push_back(l);
+    hi.push_back(h);
+  }
+  // phonons on the q mesh
+  std::vector<double> qcart(3 * size_t(nq)), w(size_t(nq) * nm), ev(2 * size_t(nq) * nm * nm);
+  const orc_long_range lr{p.natoms, p.nG, p.atomPos, p.bornCharges, p.dielectric, p.gVectors,
+                          p.lrConstant, p.lrNorm, p.gMax};
+  for (int iq = 0; iq < nq; ++iq) {
+    double x[3];
+    orc_mesh_point(iq, qmesh, x);
+    cart_of(p, x, &qcart[3 * size_t(iq)]);
+    fail += orc_ph_diag_lr(nm, p.nRf, p.Rf, p.wf, p.mat2R, p.masses, &qcart[3 * size_t(iq)],
+                           p.nG > 0 ? &lr : nullptr, &w[size_t(iq) * nm], &ev[2 * size_t(iq) * nm * nm]) != 0;
+  }
+  if (fail) return -1;
+  std::vector<double> lw(size_t(p.nCalc) * nq * nm, 0.);
+  long long pairs = 0;
+  std::vector<double> cached(2 * size_t(p.nRp) * nm * nw * nw);
+  for (size_t a = 0; a < active.size(); ++a) {
+    const int ik1 = active[a], nb1 = hi[a] - lo[a] + 1;
+    const double* k1 = &kcart[3 * size_t(ik1)];
+    std::vector<double> e1(nb1), U1(2 * size_t(nw) * nb1), v1(3 * size_t(nb1), 0.);
+    for (int b = 0; b < nb1; ++b) {
+      e1[b] = allE[size_t(ik1) * nw + lo[a] + b];
+      for (int iw = 0; iw < nw; ++iw) {
+        const size_t src = 2 * (size_t(ik1) * nw * nw + iw + size_t(nw) * (lo[a] + b));
+        U1[2 * (iw + size_t(nw) * b)] = allU[src];
+        U1[2 * (iw + size_t(nw) * b) + 1] = allU[src + 1];
+      }
+    }
+    if (p.smearing == 1) {
+      std::vector<double> vg(3 * size_t(nw));
+      if (orc_el_group_velocities_rule(nw, p.nRh, p.Rh, p.degh, p.h0R, k1, 1, vg.data()) != 0) return -1;
+      for (int b = 0; b < nb1; ++b)
+        for (int i = 0; i < 3; ++i) v1[3 * b + i] = vg[3 * (lo[a] + b) + i];
+    }
+    orc_cache_elph(nw, nm, p.nRp, p.nRe, p.gW, p.Re, p.degE, U1.data(), nb1, k1, cached.data());
+    std::vector<double> E2(size_t(nq) * nw), U2(2 * size_t(nq) * nw * nw), V2;
+    std::vector<int> nb2s(nq, nw);
+    int f2 = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : f2)
+    for (int q = 0; q < nq; ++q) {
+      double k2[3];
+      for (int i = 0; i < 3; ++i) k2[i] = qcart[3 * size_t(q) + i] + k1[i];
+      f2 += orc_el_diag(nw, p.nRh, p.Rh, p.degh, p.h0R, k2, &E2[size_t(q) * nw],
+                        &U2[2 * size_t(q) * nw * nw]) != 0;
+    }
+    if (f2) return -1;
+    if (p.smearing == 1) {
+      V2.assign(3 * size_t(nq) * nw, 0.);
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : f2)
+      for (int q = 0; q < nq; ++q) {
+        double k2[3];
+        for (int i = 0; i < 3; ++i) k2[i] = qcart[3 * size_t(q) + i] + k1[i];
+        f2 += orc_el_group_velocities_rule(nw, p.nRh, p.Rh, p.degh, p.h0R, k2, 0, &V2[3 * size_t(q) * nw]) != 0;
+      }
+      if (f2) return -1;
+    }
+    std::vector<double> coupling(size_t(nq) * nb1 * nw * nm);
+    orc_coupling_sq(nw, nm, p.nRp, p.Rp, p.degP, cached.data(), nb1, nq, U2.data(), nb2s.data(),
+                    ev.data(), qcart.data(), coupling.data());
+    for (int q = 0; q < nq; ++q) {
+      double* c = &coupling[size_t(q) * nb1 * nw * nm];
+      const double* e2 = &E2[size_t(q) * nw];
+      const double* w3 = &w[size_t(q) * nm];
+      orc_symmetrize_coupling(c, nb1, nw, nm, e1.data(), e2, w3);
+      for (int ib3 = 0; ib3 < nm; ++ib3)
+        for (int ic = 0; ic < p.nCalc; ++ic) {
+          for (int ib1 = 0; ib1 < nb1; ++ib1)
+            for (int ib2 = 0; ib2 < nw; ++ib2) {
+              const double en3 = w3[ib3];
+              double delta = 0.;
+              if (en3 >= phononCutoff) {
+                const double en = e1[ib1] - e2[ib2] + en3;
+                if (p.smearing == 1) {
+                  double v[3];
+                  for (int i = 0; i < 3; ++i) v[i] = v1[3 * ib1 + i] - V2[(size_t(q) * nw + ib2) * 3 + i];
+                  delta = orc_adaptive_delta(en, v, p.recip, p.mesh);
+                } else {
+                  delta = orc_gaussian_delta(en, p.sigma);
+                }
+                delta = std::max(delta, 0.);
+              }
+              // Particle::getPopPopPm1 (particle.cpp:117-128)
+              const double T = p.temperatures[ic];
+              const double x = std::cosh((e1[ib1] - p.chemPots[ic]) / 2. / T);
+              const double F = 0.25 / x / x;
+              const double rate = c[ib1 + size_t(nb1) * (ib2 + size_t(nw) * ib3)] * F * delta * norm / T * kPi;
+              lw[(size_t(ic) * nq + q) * nm + ib3] += rate;
+            }
+        }
+    }
+    pairs += nq;
+  }
+  if (average) {
+    for (int q = 0; q < nq; ++q) {
+      const double* w3 = &w[size_t(q) * nm];
+      std::vector<double> tmp(nm);
+      for (int b1 = 0; b1 < nm; ++b1) {
+        double t2 = 0.;
+        int n = 0;
+        for (int b2 = 0; b2 < nm; ++b2)
+          if (std::abs(w3[b2] - w3[b1]) < 1.0e-6) {
+            n++;
+            t2 += lw[size_t(q) * nm + b2];
+          }
+        tmp[b1] = t2 / float(n);
+      }
+      for (int b1 = 0; b1 < nm; ++b1) lw[size_t(q) * nm + b1] = tmp[b1];
+    }
+  }
+  std::copy(lw.begin(), lw.end(), out);
+  return pairs;
 }
 
 }  // extern "C"

@@ -72,6 +72,8 @@ def lib():
         L.orc_adaptive_delta.restype = C.c_double
         L.orc_adaptive_delta.argtypes = [C.c_double, _dp, _dp, _ip]
         L.orc_el_group_velocities.restype = C.c_int
+        L.orc_phel.restype = C.c_longlong
+        L.orc_phel.argtypes = [C.POINTER(OrcProblem), _ip, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -257,6 +259,18 @@ class Oracle:
         if getattr(self, "h", None):
             lib().orc_free(self.h)
             self.h = None
+
+    def phel_linewidths(self, q_mesh, average=True):
+        """Phonon linewidths (nCalc, nq, nm) of PhElScatteringMatrix::builder
+        on the q mesh, from this problem's electrons, phonons and coupling."""
+        nq = int(np.prod(q_mesh))
+        out = np.zeros(len(self.prob.temperatures) * nq * self.prob.nm)
+        qm = np.ascontiguousarray(q_mesh, dtype=np.int32)
+        pairs = lib().orc_phel(C.byref(self._s), qm.ctypes.data_as(_ip), int(bool(average)), _p(out))
+        if pairs < 0:
+            raise RuntimeError("oracle ph-el failed")
+        self.phel_pairs = pairs
+        return out.reshape(len(self.prob.temperatures), nq, self.prob.nm)
 
     def eigvecs(self, a):
         nb = int(self.band_hi[a] - self.band_lo[a] + 1)

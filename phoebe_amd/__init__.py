@@ -139,6 +139,13 @@ EXPORTS = {
     "phx_el_model_destroy": (C.c_int, [C.c_void_p]),
     "phx_problem_set_models": (C.c_int, [C.POINTER(PhxProblem), C.c_void_p, C.c_void_p]),
     "phx_ph_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
+    "phx_phel_setup": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_void_p, _ip]),
+    "phx_phel_num_active": (C.c_int, [C.c_void_p]),
+    "phx_phel_run": (C.c_longlong, [C.c_void_p, C.c_int, _ip]),
+    "phx_phel_get_linewidths": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "phx_phel_get_phonons": (C.c_int, [C.c_void_p, _dp]),
+    "phx_phel_reset": (C.c_int, [C.c_void_p]),
+    "phx_phel_destroy": (C.c_int, [C.c_void_p]),
     "phx_el_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
 
@@ -632,3 +639,56 @@ def synth_gw_host(prob) -> np.ndarray:
     _check(lib().phx_synth_gw_host(_p(out), prob.nw, prob.nm, prob.nRp, prob.nRe,
                                    _p(_f64(prob.gw_decay)), int(prob.gw_seed)))
     return out
+
+
+# ------------------------------------------------------------------ phonon linewidths
+class PhononElectronLinewidths:
+    """Phonon linewidths from el-ph scattering on the GPU
+    (PhElScatteringMatrix::builder, phel_scattering.cpp:43-363).
+
+    prob: a problem object (phoebe_amd.synth.Problem or
+    harmonic.LinewidthProblem) for the electrons (k mesh, window, T, mu,
+    smearing) and the phonon model; coupling: an InteractionElPhWan;
+    q_mesh: the phonon mesh.
+    """
+
+    def __init__(self, prob, coupling, q_mesh):
+        self.prob = prob
+        self.coupling = coupling          # kept alive with the handle
+        self.q_mesh = tuple(int(x) for x in q_mesh)
+        self._s, self._keep = make_problem_struct(prob)
+        self._h = C.c_void_p()
+        qm = np.array(self.q_mesh, dtype=np.int32)
+        _check(lib().phx_phel_setup(C.byref(self._h), C.byref(self._s), coupling._h,
+                                    qm.ctypes.data_as(_ip)))
+        self.num_active = lib().phx_phel_num_active(self._h)
+        self.nq = int(np.prod(self.q_mesh))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().phx_phel_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def run(self, k1_list=None):
+        if k1_list is None:
+            pairs = lib().phx_phel_run(self._h, -1, None)
+        else:
+            k = np.ascontiguousarray(k1_list, dtype=np.int32)
+            pairs = lib().phx_phel_run(self._h, len(k), k.ctypes.data_as(_ip))
+        if pairs < 0:
+            raise PhxError(lib().phx_last_error().decode())
+        return pairs
+
+    def linewidths(self, average=True):
+        nc = len(self.prob.temperatures)
+        out = np.zeros(nc * self.nq * self.prob.nm)
+        _check(lib().phx_phel_get_linewidths(self._h, _p(out), int(bool(average))))
+        return out.reshape(nc, self.nq, self.prob.nm)
+
+    def phonons(self):
+        out = np.zeros(self.nq * self.prob.nm)
+        _check(lib().phx_phel_get_phonons(self._h, _p(out)))
+        return out.reshape(self.nq, self.prob.nm)
+
+    def reset(self):
+        _check(lib().phx_phel_reset(self._h))

@@ -35,6 +35,22 @@ void launch_vel_operator(bool phonon, int np, int n, double delta, int gammaRule
                          double2* vel, hipStream_t s);
 void launch_vel_degenerate(int np, int n, double threshold, int gammaRule, const double* E,
                            const double* kc, double2* vel, int* tooBig, hipStream_t s);
+struct PhelParams {
+  int nq, nb1, nb2, nm, nCalc;
+  double* g2;
+  const double* e1;
+  const double* E2;
+  const double* W3;
+  const double* F1;
+  const double* invT;
+  double norm, cutoff, invSigma, deltaPref;
+  int adaptive;
+  const double* v1;
+  const double* V2;
+  double qT[9], minWidth;
+  double* lw;
+};
+void launch_phel_reduce(const PhelParams& p, hipStream_t s);
 void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
                      hipStream_t s);
 void launch_synth_gw_f32(float2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
@@ -1902,6 +1918,341 @@ static long long lw_plan_run(LwHandle* L, int graph) {
   return L->planPairs;
 }
 
+
+// ============================================================ PhElScatteringMatrix
+// Phonon linewidths from el-ph scattering (PhElScatteringMatrix::builder,
+// phel_scattering.cpp:43-363).  Outer loop over the active electron points k1
+// of the k mesh (window as the electron builder's); inner "loop" over every
+// phonon wavevector q3 of the q mesh at once: H(k1 + q3) eigensolves (all
+// bands), cacheElPh + calcCouplingSquared with the e^{iq3.Rp} rows computed
+// once, then the symmetrize + rate kernels of phel.hip.
+struct PhelHandle {
+  ElphHandle* H = nullptr;  // coupling (not owned)
+  hipStream_t s = nullptr;
+  int nw = 0, nm = 0, nk = 0, nq = 0, nCalc = 0, na = 0, adaptive = 0;
+  int qmesh[3] = {0, 0, 0};
+  double norm = 0., invSigma = 0., deltaPref = 0., qT[9] = {};
+  std::vector<double> kcart, qcart, omega;   // nk x 3, nq x 3, nq x nm
+  std::vector<int> active, lo, nb;
+  std::vector<double> stateE, U1, v1;          // per active point: energies, (nw x nb) eigvecs, nb x 3
+  std::vector<size_t> stateOff, u1Off;
+  std::vector<double> T, mu;
+  DevBuf<double> Rh, degh, invT, W3, lw, E2, k2, v1d, V2, F1, g2;
+  DevBuf<double2> h0R, E3, U2, U1d;
+  DevBuf<double> A;   // FP64 phase rows
+  DevBuf<float> Af;   // FP32 phase rows
+  DevBuf<int> info, rowU2off, rowNb2, rowE3;
+  DevBuf<long long> rowOutOff;
+  int rowsPad = 0;
+  long long pairs = 0;
+};
+
+static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const int* qmesh) {
+  PHX_REQUIRE(H, "phx_phel_setup: null coupling");
+  PHX_REQUIRE(p.nw == H->nw && p.nm == H->nm, "phx_phel_setup: coupling and band structures differ in size");
+  PHX_REQUIRE(p.nCalc >= 1 && p.nCalc <= kCMax, "phx_phel_setup: 1..4 (T, mu) calculations supported");
+  PHX_REQUIRE(p.smearing == PHX_SMEAR_GAUSSIAN || p.smearing == PHX_SMEAR_ADAPTIVE,
+              "phx_phel_setup: unknown smearing");
+  PHX_REQUIRE(p.smearing == PHX_SMEAR_ADAPTIVE || p.sigma > 0., "phx_phel_setup: sigma must be > 0");
+  for (int i = 0; i < 3; ++i) PHX_REQUIRE(p.mesh[i] >= 1 && qmesh[i] >= 1, "phx_phel_setup: bad meshes");
+  P->H = H;
+  P->s = H->stream;
+  hipStream_t s = P->s;
+  const int nw = p.nw, nm = p.nm;
+  P->nw = nw;
+  P->nm = nm;
+  P->nCalc = p.nCalc;
+  P->nk = p.mesh[0] * p.mesh[1] * p.mesh[2];
+  P->nq = qmesh[0] * qmesh[1] * qmesh[2];
+  for (int i = 0; i < 3; ++i) P->qmesh[i] = qmesh[i];
+  P->T.assign(p.temperatures, p.temperatures + p.nCalc);
+  P->mu.assign(p.chemPots, p.chemPots + p.nCalc);
+  // norm = 1 / prod(kMesh) (phel_scattering.cpp:61)
+  P->norm = 1. / (double(p.mesh[0]) * p.mesh[1] * p.mesh[2]);
+  P->adaptive = p.smearing == PHX_SMEAR_ADAPTIVE;
+  if (!P->adaptive) {
+    P->invSigma = 1. / p.sigma;
+    P->deltaPref = 1. / p.sigma / std::sqrt(kPi);
+  }
+  // ---- electrons on the k mesh and the window
+  P->kcart.resize(3 * size_t(P->nk));
+  for (int ik = 0; ik < P->nk; ++ik) mesh_cart(p.recip, p.mesh, ik, &P->kcart[3 * size_t(ik)]);
+  h2d(P->Rh, p.Rh, size_t(p.nRh) * 3, s);
+  h2d(P->degh, p.degh, size_t(p.nRh), s);
+  h2d(P->h0R, reinterpret_cast<const double2*>(p.h0R), size_t(p.nRh) * nw * nw, s);
+  std::vector<double> Eall(size_t(P->nk) * nw), Uall(2 * size_t(P->nk) * nw * nw);
+  {
+    DevBuf<double> dk;
+    h2d(dk, P->kcart, s);
+    DevBuf<double2> U(size_t(P->nk) * nw * nw);
+    DevBuf<double> E(size_t(P->nk) * nw);
+    DevBuf<int> inf(P->nk);
+    launch_bloch(false, P->nk, nw, p.nRh, dk.p, P->Rh.p, P->degh.p, P->h0R.p, U.p, s);
+    launch_jacobi(nw, P->nk, U.p, E.p, inf.p, nullptr, s);
+    std::vector<int> hi(P->nk);
+    PHX_HIP(hipMemcpyAsync(Eall.data(), E.p, E.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(Uall.data(), U.p, U.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(hi.data(), inf.p, inf.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < P->nk; ++i) PHX_REQUIRE(hi[i] >= 0, "electron eigensolve did not converge");
+  }
+  for (int ik = 0; ik < P->nk; ++ik) {
+    int l = -1, h = -1;
+    for (int ib = 0; ib < nw; ++ib) {
+      const double e = Eall[size_t(ik) * nw + ib];
+      if (e < p.winMax && e > p.winMin) {
+        if (l < 0) l = ib;
+        h = ib;
+      }
+    }
+    if (l < 0) continue;
+    P->active.push_back(ik);
+    P->lo.push_back(l);
+    P->nb.push_back(h - l + 1);
+  }
+  P->na = int(P->active.size());
+  PHX_REQUIRE(P->na > 0, "energy window selects no states");
+  for (int a = 0; a < P->na; ++a) {
+    const int ik = P->active[a], n = P->nb[a];
+    P->stateOff.push_back(P->stateE.size());
+    P->u1Off.push_back(P->U1.size());
+    for (int b = 0; b < n; ++b) {
+      P->stateE.push_back(Eall[size_t(ik) * nw + P->lo[a] + b]);
+      const double* col = &Uall[2 * (size_t(ik) * nw * nw + size_t(nw) * (P->lo[a] + b))];
+      P->U1.insert(P->U1.end(), col, col + 2 * nw);
+    }
+  }
+  if (P->adaptive) {
+    // k1 velocities as the electron band structure holds them (single-point
+    // velocity operator, active_bandstructure.cpp:1032-1071)
+    std::vector<double> kc(3 * size_t(P->na));
+    for (int a = 0; a < P->na; ++a)
+      for (int i = 0; i < 3; ++i) kc[3 * size_t(a) + i] = P->kcart[3 * size_t(P->active[a]) + i];
+    std::vector<double> E(size_t(P->na) * nw), U(2 * size_t(P->na) * nw * nw),
+        V(2 * size_t(P->na) * nw * nw * 3);
+    diag_velocities(false, P->na, kc.data(), nw, p.nRh, p.Rh, p.degh, p.h0R, nullptr, nullptr,
+                    PHX_GAMMA_POINT, E.data(), U.data(), V.data());
+    for (int a = 0; a < P->na; ++a)
+      for (int b = 0; b < P->nb[a]; ++b) {
+        const int bf = P->lo[a] + b;
+        for (int i = 0; i < 3; ++i)
+          P->v1.push_back(V[2 * ((((size_t(a) * nw + bf) * nw) + bf) * 3 + i)]);
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) P->qT[3 * i + j] = p.recip[i + 3 * j] / p.mesh[i];
+  }
+  // ---- phonons on the q mesh (PhononH0 with the long-range term)
+  P->qcart.resize(3 * size_t(P->nq));
+  for (int iq = 0; iq < P->nq; ++iq) mesh_cart(p.recip, qmesh, iq, &P->qcart[3 * size_t(iq)]);
+  {
+    DevBuf<double> dq, dRf, dwf, dmat, dmass;
+    h2d(dq, P->qcart, s);
+    h2d(dRf, p.Rf, size_t(p.nRf) * 3, s);
+    h2d(dwf, p.wf, size_t(p.nRf), s);
+    h2d(dmat, p.mat2R, size_t(p.nRf) * nm * nm, s);
+    h2d(dmass, p.masses, size_t(nm), s);
+    P->E3.alloc(size_t(P->nq) * nm * nm);
+    P->W3.alloc(size_t(P->nq) * nm);
+    DevBuf<int> qinfo(P->nq);
+    launch_bloch(true, P->nq, nm, p.nRf, dq.p, dRf.p, dwf.p, dmat.p, P->E3.p, s);
+    DevLongRange dlr;
+    if (p.nG > 0) {
+      PHX_REQUIRE(3 * p.natoms == nm, "phx_problem: natoms * 3 != nm with a long-range term");
+      dlr.upload(p.natoms, p.nG, p.atomPos, p.bornCharges, p.dielectric, p.gVectors, p.lrConstant,
+                 p.lrNorm, p.gMax, s);
+      launch_ph_long_range(P->nq, dq.p, dlr.lr, P->E3.p, s);
+    }
+    launch_jacobi(nm, P->nq, P->E3.p, P->W3.p, qinfo.p, dmass.p, s);
+    std::vector<int> qi(P->nq);
+    P->omega.resize(size_t(P->nq) * nm);
+    PHX_HIP(hipMemcpyAsync(qi.data(), qinfo.p, qinfo.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(P->omega.data(), P->W3.p, P->W3.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < P->nq; ++i) PHX_REQUIRE(qi[i] >= 0, "phonon eigensolve did not converge");
+    // phase rows e^{iq3.Rp} (explicit-q mode of the coupling kernel)
+    const int rows = coupling_rows_per_tile_f64(H->ldb);
+    P->rowsPad = ceil_div(P->nq, rows) * rows;
+    if (H->precision == PHX_FP64) {
+      P->A.alloc(size_t(P->rowsPad) * H->Kp);
+      launch_phase_rows<double>(P->nq, P->rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, P->A.p, s);
+    } else {
+      P->Af.alloc(size_t(P->rowsPad) * H->Kp);
+      launch_phase_rows<float>(P->nq, P->rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, P->Af.p, s);
+    }
+  }
+  // ---- per-k1 scratch and the fixed coupling row metadata
+  std::vector<int> u2off(P->nq), nb2(P->nq, nw), e3i(P->nq);
+  for (int q = 0; q < P->nq; ++q) {
+    u2off[q] = q * nw * nw;
+    e3i[q] = q;
+  }
+  h2d(P->rowU2off, u2off, s);
+  h2d(P->rowNb2, nb2, s);
+  h2d(P->rowE3, e3i, s);
+  P->U2.alloc(size_t(P->nq) * nw * nw);
+  P->E2.alloc(size_t(P->nq) * nw);
+  P->k2.alloc(size_t(P->nq) * 3);
+  P->info.alloc(P->nq);
+  int nb1max = 0;
+  for (int n : P->nb) nb1max = std::max(nb1max, n);
+  P->g2.alloc(size_t(P->nq) * nb1max * nw * nm);
+  P->F1.alloc(size_t(kCMax) * nw);
+  P->rowOutOff.alloc(P->nq);
+  std::vector<double> it(P->nCalc);
+  for (int c = 0; c < P->nCalc; ++c) it[c] = 1. / P->T[c];
+  h2d(P->invT, it, s);
+  P->lw.alloc(size_t(P->nCalc) * P->nq * nm);
+  launch_fill<double>(P->lw.p, P->lw.n, 0., s);
+  if (P->adaptive) P->V2.alloc(size_t(P->nq) * nw * 3);
+  PHX_HIP(hipStreamSynchronize(s));
+}
+
+// One outer point k1 (active index a) against every q3.
+static void phel_point(PhelHandle* P, int a) {
+  ElphHandle* H = P->H;
+  hipStream_t s = P->s;
+  const int nw = P->nw, nm = P->nm, nq = P->nq, nb1 = P->nb[a];
+  const double* k1 = &P->kcart[3 * size_t(P->active[a])];
+  // k2 = k1 + q3 (not folded), all bands (electronH0.populate, phel_scattering.cpp:225-243)
+  std::vector<double> k2(3 * size_t(nq));
+  for (int q = 0; q < nq; ++q)
+    for (int i = 0; i < 3; ++i) k2[3 * size_t(q) + i] = P->qcart[3 * size_t(q) + i] + k1[i];
+  PHX_HIP(hipMemcpyAsync(P->k2.p, k2.data(), k2.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  if (P->adaptive) {
+    // k2 velocities of the batched form (kokkosPopulate with velocities)
+    std::vector<double> E(size_t(nq) * nw), U(2 * size_t(nq) * nw * nw), V(2 * size_t(nq) * nw * nw * 3);
+    std::vector<double> hRh(P->Rh.n), hdeg(P->degh.n), hh(2 * P->h0R.n);
+    PHX_HIP(hipMemcpyAsync(hRh.data(), P->Rh.p, P->Rh.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(hdeg.data(), P->degh.p, P->degh.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(hh.data(), P->h0R.p, P->h0R.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    diag_velocities(false, nq, k2.data(), nw, int(P->degh.n), hRh.data(), hdeg.data(), hh.data(), nullptr,
+                    nullptr, PHX_GAMMA_BATCHED, E.data(), U.data(), V.data());
+    std::vector<double> gv(size_t(nq) * nw * 3);
+    for (int q = 0; q < nq; ++q)
+      for (int b = 0; b < nw; ++b)
+        for (int i = 0; i < 3; ++i)
+          gv[(size_t(q) * nw + b) * 3 + i] = V[2 * ((((size_t(q) * nw + b) * nw) + b) * 3 + i)];
+    PHX_HIP(hipMemcpyAsync(P->V2.p, gv.data(), gv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(P->E2.p, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(P->U2.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  } else {
+    launch_bloch(false, nq, nw, int(P->degh.n), P->k2.p, P->Rh.p, P->degh.p, P->h0R.p, P->U2.p, s);
+    launch_jacobi(nw, nq, P->U2.p, P->E2.p, P->info.p, nullptr, s);
+  }
+  // cacheElPh(k1) with its windowed eigenvectors
+  elph_cache(H, &P->U1[P->u1Off[a]], nb1, k1);
+  // calcCouplingSquared, store mode, every q3 in one launch
+  std::vector<long long> oo(nq);
+  for (int q = 0; q < nq; ++q) oo[q] = (long long)q * nb1 * nw * nm;
+  PHX_HIP(hipMemcpyAsync(P->rowOutOff.p, oo.data(), oo.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+  DevBuf<double> W0(nm);
+  CoupParams c{};
+  c.lda = H->Kp;
+  c.Kp = H->Kp;
+  c.ldb = H->ldb;
+  c.panelStride = size_t(H->Kp) * H->ldb;
+  c.M = nq;
+  c.nm = nm;
+  c.nw = nw;
+  c.nPanels = nb1;
+  c.nMtiles = P->rowsPad / coupling_rows_per_tile_f64(H->ldb);
+  c.panelKK = H->meta.p + 4;
+  c.panelB1 = H->meta.p + 4 + nb1;
+  c.U2pool = P->U2.p;
+  c.rowU2off = P->rowU2off.p;
+  c.rowNb2 = P->rowNb2.p;
+  c.E3pool = P->E3.p;
+  c.Wpool = W0.p;
+  c.qmode = 0;
+  c.rowE3 = P->rowE3.p;
+  c.out = P->g2.p;
+  c.rowOutOff = P->rowOutOff.p;
+  c.kkNb1 = H->meta.p + 1;
+  c.nb2max = nw;
+  if (H->precision == PHX_FP64) {
+    c.A = P->A.p;
+    c.B = H->Bd.p;
+    launch_coupling<double>(c, false, s);
+  } else {
+    c.A = P->Af.p;
+    c.B = H->Bf.p;
+    launch_coupling<float>(c, false, s);
+  }
+  // rates
+  const double* e1 = &P->stateE[P->stateOff[a]];
+  std::vector<double> F(size_t(P->nCalc) * nb1);
+  for (int cc = 0; cc < P->nCalc; ++cc)
+    for (int b = 0; b < nb1; ++b) {
+      // Particle::getPopPopPm1 (particle.cpp:117-128), fermions
+      const double x = 1. / std::cosh((e1[b] - P->mu[cc]) / 2. / P->T[cc]);
+      F[size_t(cc) * nb1 + b] = 0.25 * x * x;
+    }
+  DevBuf<double> de1, dF, dv1;
+  h2d(de1, e1, nb1, s);
+  h2d(dF, F, s);
+  PhelParams r{};
+  r.nq = nq;
+  r.nb1 = nb1;
+  r.nb2 = nw;
+  r.nm = nm;
+  r.nCalc = P->nCalc;
+  r.g2 = P->g2.p;
+  r.e1 = de1.p;
+  r.E2 = P->E2.p;
+  r.W3 = P->W3.p;
+  r.F1 = dF.p;
+  r.invT = P->invT.p;
+  r.norm = P->norm;
+  r.cutoff = phonon_cutoff();
+  r.invSigma = P->invSigma;
+  r.deltaPref = P->deltaPref;
+  r.adaptive = P->adaptive;
+  if (P->adaptive) {
+    h2d(dv1, &P->v1[3 * P->stateOff[a]], 3 * size_t(nb1), s);
+    r.v1 = dv1.p;
+    r.V2 = P->V2.p;
+    for (int i = 0; i < 9; ++i) r.qT[i] = P->qT[i];
+    r.minWidth = 0.0001 / units::energyRyToEv;
+  }
+  r.lw = P->lw.p;
+  launch_phel_reduce(r, s);
+  PHX_HIP(hipStreamSynchronize(s));
+  if (!P->adaptive) {
+    std::vector<int> inf(nq);
+    PHX_HIP(hipMemcpy(inf.data(), P->info.p, inf.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int q = 0; q < nq; ++q) PHX_REQUIRE(inf[q] >= 0, "electron eigensolve did not converge");
+  }
+  P->pairs += nq;
+}
+
+static void phel_get(PhelHandle* P, double* out, int average) {
+  const size_t ns = size_t(P->nq) * P->nm;
+  std::vector<double> lw(size_t(P->nCalc) * ns);
+  PHX_HIP(hipStreamSynchronize(P->s));
+  PHX_HIP(hipMemcpy(lw.data(), P->lw.p, lw.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (average) {
+    // degeneracyAveragingLinewidths over the phonon band structure, row 0
+    // (scattering.cpp:1168-1207)
+    for (int q = 0; q < P->nq; ++q) {
+      const double* w = &P->omega[size_t(q) * P->nm];
+      std::vector<double> tmp(P->nm);
+      for (int b1 = 0; b1 < P->nm; ++b1) {
+        double t2 = 0.;
+        int n = 0;
+        for (int b2 = 0; b2 < P->nm; ++b2)
+          if (std::abs(w[b2] - w[b1]) < 1.0e-6) {
+            n++;
+            t2 = t2 + lw[size_t(q) * P->nm + b2];
+          }
+        tmp[b1] = t2 / float(n);
+      }
+      for (int b1 = 0; b1 < P->nm; ++b1) lw[size_t(q) * P->nm + b1] = tmp[b1];
+    }
+  }
+  std::copy(lw.begin(), lw.end(), out);
+}
+
 static void lw_get(LwHandle* L, double* out, int average) {
   const int st = L->numStates;
   std::vector<double> lw(size_t(L->nCalc) * st);
@@ -1941,6 +2292,7 @@ struct phx_elph_file : phx::ElphFile {
 struct phx_ph_model {
   phx::PhononModel m;
 };
+struct phx_phel : phx::PhelHandle {};
 struct phx_el_model {
   phx::ElectronModel m;
 };
@@ -2413,6 +2765,69 @@ int phx_lw_setup_file(phx_lw** h, const phx_problem* prob, const char* elph_path
     L->prob.Re = L->prob.degE = L->prob.Rp = L->prob.degP = nullptr;
     *h = L;
   });
+}
+
+
+int phx_phel_setup(phx_phel** h, const phx_problem* prob, phx_elph* coupling, const int* q_mesh) {
+  return guard([&] {
+    PHX_REQUIRE(h && prob && coupling && q_mesh, "phx_phel_setup: null argument");
+    *h = nullptr;
+    auto* P = new phx_phel();
+    try {
+      phel_setup(P, *prob, coupling, q_mesh);
+    } catch (...) {
+      delete P;
+      throw;
+    }
+    *h = P;
+  });
+}
+
+int phx_phel_num_active(phx_phel* h) { return h ? h->na : -1; }
+
+long long phx_phel_run(phx_phel* h, int nk1, const int* k1_list) {
+  long long before = 0;
+  int rc = guard([&] {
+    PHX_REQUIRE(h, "phx_phel_run: null handle");
+    before = h->pairs;
+    if (nk1 < 0) {
+      for (int a = 0; a < h->na; ++a) phel_point(h, a);
+    } else {
+      PHX_REQUIRE(k1_list || nk1 == 0, "phx_phel_run: null k1 list");
+      for (int i = 0; i < nk1; ++i) {
+        PHX_REQUIRE(k1_list[i] >= 0 && k1_list[i] < h->na, "phx_phel_run: k1 index out of range");
+        phel_point(h, k1_list[i]);
+      }
+    }
+  });
+  return rc < 0 ? rc : h->pairs - before;
+}
+
+int phx_phel_get_linewidths(phx_phel* h, double* out, int average) {
+  return guard([&] {
+    PHX_REQUIRE(h && out, "phx_phel_get_linewidths: null argument");
+    phel_get(h, out, average);
+  });
+}
+
+int phx_phel_get_phonons(phx_phel* h, double* omega) {
+  return guard([&] {
+    PHX_REQUIRE(h && omega, "phx_phel_get_phonons: null argument");
+    std::copy(h->omega.begin(), h->omega.end(), omega);
+  });
+}
+
+int phx_phel_reset(phx_phel* h) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_phel_reset: null handle");
+    launch_fill<double>(h->lw.p, h->lw.n, 0., h->s);
+    PHX_HIP(hipStreamSynchronize(h->s));
+    h->pairs = 0;
+  });
+}
+
+int phx_phel_destroy(phx_phel* h) {
+  return guard([&] { delete h; });
 }
 
 int phx_lw_num_active(phx_lw* h) { return h ? h->na : -1; }

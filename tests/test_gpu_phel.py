@@ -1,0 +1,97 @@
+"""GPU: phonon linewidths from el-ph scattering (PhElScatteringMatrix::builder)
+against the toy closed form and the oracle, on the toy and on the reference's
+Si inputs."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import phoebe_amd as P
+from oracle import harmonic_oracle as HO
+from oracle import h5_oracle as H5
+from oracle import oracle as O
+from phoebe_amd import harmonic as Hm
+from tests.phel_cases import toy
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+FC4 = os.path.join(GOLD, "444_silicon.fc")
+TB = os.path.join(GOLD, "si_tb.dat")
+ELPH = os.path.join(GOLD, "silicon.phoebe.elph.hdf5")
+RY_PER_EV = 1.0 / HO.ENERGY_RY_TO_EV
+KELVIN_TO_RY = 1.38064852e-23 / HO.RYDBERG_SI
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("springs", [(8.0, 20.0, 45.0), (20.0, 20.0, 45.0)])
+def test_phel_toy_closed_form(springs):
+    say("ph-el toy", springs)
+    p, qm = toy(springs)
+    inter = P.InteractionElPhWan(p.gw_host(), p.Re, p.degE, p.Rp, p.degP, "fp64")
+    ph = P.PhononElectronLinewidths(p, inter, qm)
+    assert ph.num_active == 64
+    assert ph.run() == 64 * 27
+    got = ph.linewidths()
+    ref = p.expected_phel(qm)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say("  max err / max", err)
+    assert err < 1e-12
+
+
+def test_phel_partial_runs_accumulate():
+    p, qm = toy()
+    inter = P.InteractionElPhWan(p.gw_host(), p.Re, p.degE, p.Rp, p.degP, "fp64")
+    a = P.PhononElectronLinewidths(p, inter, qm)
+    a.run()
+    b = P.PhononElectronLinewidths(p, inter, qm)
+    b.run(np.arange(0, 20))
+    b.run(np.arange(20, 64))
+    assert np.array_equal(a.linewidths(), b.linewidths())
+    b.reset()
+    assert not np.any(b.linewidths())
+    with pytest.raises(P.PhxError, match="out of range"):
+        b.run([64])
+
+
+@pytest.mark.parametrize("smearing,precision,tol",
+                         [("gaussian", "fp64", 1e-10), ("adaptive", "fp64", 1e-7),
+                          ("gaussian", "fp32", 1e-4)])
+def test_phel_si_vs_oracle(smearing, precision, tol):
+    """Si from the reference's files: 444_silicon.fc (no sum rule: long-range
+    term on), si_tb.dat, silicon.phoebe.elph.hdf5; k mesh 4^3, q mesh 3^3.
+    Adaptive smearing inherits the 1e-8 relative noise of the finite-difference
+    velocities; FP32 couplings carry FP32 rounding."""
+    say("ph-el Si", smearing, precision)
+    ph = Hm.PhononH0(FC4, "")
+    el = Hm.ElectronH0Wannier(TB, ph.cell)
+    pts = np.array([[i / 4, j / 4, k / 4] for k in range(4) for j in range(4) for i in range(4)])
+    E, _ = el.diag((ph.recip @ pts.T).T)
+    vbm = E[:, 3].max()
+    prob = Hm.linewidth_problem(el, ph, ELPH, (4, 4, 4), temperatures=[300 * KELVIN_TO_RY, 900 * KELVIN_TO_RY],
+                                chem_pots=[vbm, vbm + 0.2 * RY_PER_EV], sigma=0.05 * RY_PER_EV,
+                                win_min=vbm - 1.5 * RY_PER_EV, win_max=vbm + 1.5 * RY_PER_EV,
+                                smearing=smearing)
+    gW = H5.read_elph(ELPH)["gW"]
+    orc = O.Oracle(prob, gW.reshape(-1))
+    qm = (3, 3, 3)
+    ref = orc.phel_linewidths(qm)
+    inter = P.InteractionElPhWan.parse(ELPH, precision=precision)
+    lw = P.PhononElectronLinewidths(prob, inter, qm)
+    om = lw.phonons()
+    assert lw.run() == orc.phel_pairs
+    got = lw.linewidths()
+    nz = np.abs(ref) > 1e-30 * np.abs(ref).max()
+    err = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+    say("  states", ref.shape, "nonzero", int(nz.sum()), "max rel err", err.max())
+    assert nz.sum() > 0.5 * ref.size
+    assert err.max() < tol
+    # modes below the 5 cm^-1 cut carry nothing
+    cut = om < 5 / (1.0e10 * (1.0 / (6.62607015e-34 / (2 * np.pi) / 4.359744650e-18 * 1.0e12) / 2.0 / (2 * np.pi)) / 299792458.0)
+    assert not np.any(got[:, cut])
