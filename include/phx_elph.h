@@ -309,30 +309,6 @@ int phx_lw_setup_file(phx_lw** h, const phx_problem* prob, const char* elph_path
 int phx_problem_set_models(phx_problem* prob, phx_el_model* el, phx_ph_model* ph);
 int phx_lw_num_active(phx_lw* h);
 
-/* ---------------------------------------------------------------- phonon linewidths
- * PhElScatteringMatrix::builder (phel_scattering.cpp:43-363): the phonon
- * linewidths from el-ph scattering, the second caller of the coupling.
- *   prob: recip, mesh (electron k mesh), the electron Hamiltonian, the force
- *         constants (+ long-range term), nCalc / temperatures / chemPots,
- *         sigma, winMin / winMax (electron window), smearing; its coupling
- *         fields are unused;
- *   coupling: the el-ph interaction (phx_elph_create*, phx_elph_create_from_file),
- *         which must outlive the handle;
- *   q_mesh: the Gamma-centred phonon mesh; every q point and mode is a state.
- * For every active k1 and every q3: k2 = k1 + q3 (all bands), |g|^2
- * symmetrized, Gamma(q3, nu) += |g|^2 f1 (1 - f1) delta(e1 - e2 + w) / (T N_k) pi. */
-typedef struct phx_phel phx_phel;
-int phx_phel_setup(phx_phel** h, const phx_problem* prob, phx_elph* coupling, const int* q_mesh);
-int phx_phel_num_active(phx_phel* h);
-/* outer points k1_list (active indices; nk1 < 0: all); returns the (k1, q3) pairs done */
-long long phx_phel_run(phx_phel* h, int nk1, const int* k1_list);
-/* out: nCalc x (nq x nm), (q, nu) row-major, Ry; average: degeneracy averaging
- * over the phonon modes of each q, row 0 only (scattering.cpp:1168-1207) */
-int phx_phel_get_linewidths(phx_phel* h, double* out, int average);
-/* phonon frequencies nq x nm of the q mesh */
-int phx_phel_get_phonons(phx_phel* h, double* omega);
-int phx_phel_reset(phx_phel* h);
-int phx_phel_destroy(phx_phel* h);
 int phx_lw_num_states(phx_lw* h);
 /* active points (full-mesh indices), band extrema, energies of the states */
 int phx_lw_get_window(phx_lw* h, int* active, int* band_lo, int* band_hi, double* energies);
@@ -444,6 +420,31 @@ void* phx_lw_stream(phx_lw* h);
 /* Copy the handle's gW to host memory (nRe*nRp*nm*nw*nw complex). */
 int phx_lw_copy_gw(phx_lw* h, double* out);
 int phx_lw_destroy(phx_lw* h);
+
+/* ---------------------------------------------------------------- phonon linewidths
+ * PhElScatteringMatrix::builder (phel_scattering.cpp:43-363): the phonon
+ * linewidths from el-ph scattering, the second caller of the coupling.
+ *   prob: recip, mesh (electron k mesh), the electron Hamiltonian, the force
+ *         constants (+ long-range term), nCalc / temperatures / chemPots,
+ *         sigma, winMin / winMax (electron window), smearing; its coupling
+ *         fields are unused;
+ *   coupling: the el-ph interaction (phx_elph_create*, phx_elph_create_from_file),
+ *         which must outlive the handle;
+ *   q_mesh: the Gamma-centred phonon mesh; every q point and mode is a state.
+ * For every active k1 and every q3: k2 = k1 + q3 (all bands), |g|^2
+ * symmetrized, Gamma(q3, nu) += |g|^2 f1 (1 - f1) delta(e1 - e2 + w) / (T N_k) pi. */
+typedef struct phx_phel phx_phel;
+int phx_phel_setup(phx_phel** h, const phx_problem* prob, phx_elph* coupling, const int* q_mesh);
+int phx_phel_num_active(phx_phel* h);
+/* outer points k1_list (active indices; nk1 < 0: all); returns the (k1, q3) pairs done */
+long long phx_phel_run(phx_phel* h, int nk1, const int* k1_list);
+/* out: nCalc x (nq x nm), (q, nu) row-major, Ry; average: degeneracy averaging
+ * over the phonon modes of each q, row 0 only (scattering.cpp:1168-1207) */
+int phx_phel_get_linewidths(phx_phel* h, double* out, int average);
+/* phonon frequencies nq x nm of the q mesh */
+int phx_phel_get_phonons(phx_phel* h, double* omega);
+int phx_phel_reset(phx_phel* h);
+int phx_phel_destroy(phx_phel* h);
 
 /* Host twin of the synthetic gW generator (bit-identical to the device). */
 int phx_synth_gw_host(double* gW, int nw, int nm, int nRp, int nRe, const double* decay,
