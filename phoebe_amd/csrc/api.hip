@@ -758,6 +758,7 @@ struct LwHandle {
   // energy-conservation screen of the mesh transform (units of sigma, 0 = off)
   double screen = 0.;
   DevBuf<double> qWlo, qWhi;
+  DevBuf<int> qModeGrp;        // per q and mode: first mode of its degenerate group | size << 8
   DevBuf<double2> rowE2range;  // per active point: lowest and highest windowed band energy
   DevBuf<int4> meshRow;        // per mesh point: (active row or -1, nb, U pool offset, state offset)
   DevBuf<unsigned long long> keepMask;  // last-stage row masks of one panel chunk
@@ -1034,6 +1035,22 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
       }
     h2d(L->qWlo, lo, s);
     h2d(L->qWhi, hi, s);
+    // degenerate mode groups of every q, as symmetrizeCoupling forms them
+    // (scattering.cpp:1286-1316): consecutive modes within 1e-6 of the first
+    std::vector<int> grp(size_t(nk) * nm, 0);
+    for (int q = 0; q < nk; ++q) {
+      const double* w = &L->omegaAll[size_t(q) * nm];
+      for (int s0 = 0; s0 < nm;) {
+        int deg = 0;
+        for (int i = s0; i < nm; ++i) {
+          if (std::abs(w[s0] - w[i]) < 1.0e-6) ++deg;
+          else break;
+        }
+        for (int i = s0; i < s0 + deg; ++i) grp[size_t(q) * nm + i] = s0 | (deg << 8);
+        s0 += deg;
+      }
+    }
+    h2d(L->qModeGrp, grp, s);
     PHX_HIP(hipStreamSynchronize(s));
   }
   // Bose (helper_el_scattering.cpp:261-267) and inner Fermi (el_scattering.cpp:87-109)
@@ -1701,6 +1718,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     cq.screen = L->screen;
     cq.qWlo = L->qWlo.p;
     cq.qWhi = L->qWhi.p;
+    cq.qModeGrp = L->qModeGrp.p;
     cq.rowE2range = L->rowE2range.p;
     cq.bf16x3 = L->precision == PHX_BF16X3;
     cq.meshRow = L->meshRow.p;

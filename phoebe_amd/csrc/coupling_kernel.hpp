@@ -80,6 +80,8 @@ struct CoupParams {
   const double* stateV;                   // numStates x 3 group velocities
   double qT[9];                           // reciprocal cell, row i divided by mesh(i), row-major
   double minWidth;                        // smallest width (0.1 meV)
+  // degenerate phonon modes: per q and mode, first mode of its group | group size << 8
+  const int* qModeGrp;
 };
 
 template <typename T> struct Vec2;
@@ -1014,28 +1016,68 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Row data of a 16-row tile into LDS: U2 (16 x nw x nb2max), e2 (16 x nb2max), omega (16 x nm).
+// Row data of a 16-row tile into LDS: U2 (row r at sU2 + r * u2s, nw x nb2max),
+// e2 (16 x nb2max), omega (16 x nm) and the electron occupations of the rows'
+// states (nCalc x 16 x nb2max).  Every global load of a chunk is issued before
+// the LDS stores, so a tile waits for one memory round trip, not one per
+// element.  Needs 16 nb2max <= 256, 16 nm <= 512 and nCalc 16 nb2max <= 512.
 __device__ inline void sep_xm_stage_rows(const CoupParams& p, const SepXmRow* rows, int m0r, int nr,
-                                         int nw, int nm, double2* sU2, double* se2, double* sw3) {
-  const int t = threadIdx.x;
-  for (int i = t; i < 16 * nw * p.nb2max; i += 256) {
-    const int r = i / (nw * p.nb2max), o = i % (nw * p.nb2max);
-    double2 v = make_double2(0., 0.);
-    if (r < nr) {
-      const SepXmRow& rr = rows[m0r + r];
-      if (o < nw * rr.nb2) v = p.U2pool[rr.u2off + o];
+                                         int nw, int nm, int u2s, double2* sU2, double* se2,
+                                         double* sw3, int* sg3, double* sf2) {
+  const int t = threadIdx.x, nb2max = p.nb2max;
+  const int ne = 16 * nb2max, nwv = 16 * nm, nf = p.nCalc * ne;
+  double ev = 0., wv[2] = {0., 0.}, fv[2] = {0., 0.};
+  int gv[2] = {0, 0};
+  if (t < ne) {
+    const int r = t / nb2max, b = t % nb2max;
+    if (r < nr && b < rows[m0r + r].nb2) ev = p.e2pool[rows[m0r + r].soff + b];
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int i = t + 256 * u;
+    if (i < nwv) {
+      const int r = i / nm, nu = i % nm;
+      if (r < nr) {
+        wv[u] = p.Wpool[size_t(rows[m0r + r].iq) * nm + nu];
+        gv[u] = p.qModeGrp[size_t(rows[m0r + r].iq) * nm + nu];
+      }
     }
-    sU2[i] = v;
+    if (i < nf) {
+      const int c = i / ne, o = i % ne, r = o / nb2max, b = o % nb2max;
+      if (r < nr && b < rows[m0r + r].nb2)
+        fv[u] = p.fermi[size_t(c) * p.numStates + rows[m0r + r].soff + b];
+    }
   }
-  for (int i = t; i < 16 * p.nb2max; i += 256) {
-    const int r = i / p.nb2max, b = i % p.nb2max;
-    double v = 0.;
-    if (r < nr && b < rows[m0r + r].nb2) v = p.e2pool[rows[m0r + r].soff + b];
-    se2[i] = v;
+  const int per = nw * nb2max, nU = 16 * per;
+  for (int i0 = t; i0 < nU; i0 += 4 * 256) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      v[u] = make_double2(0., 0.);
+      if (i < nU) {
+        const int r = i / per, o = i % per;
+        if (r < nr) {
+          const SepXmRow& rr = rows[m0r + r];
+          if (o < nw * rr.nb2) v[u] = p.U2pool[rr.u2off + o];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      if (i < nU) sU2[(i / per) * u2s + i % per] = v[u];
+    }
   }
-  for (int i = t; i < 16 * nm; i += 256) {
-    const int r = i / nm, nu = i % nm;
-    sw3[i] = r < nr ? p.Wpool[size_t(rows[m0r + r].iq) * nm + nu] : 0.;
+  if (t < ne) se2[t] = ev;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int i = t + 256 * u;
+    if (i < nwv) {
+      sw3[i] = wv[u];
+      sg3[i] = gv[u];
+    }
+    if (i < nf) sf2[i] = fv[u];
   }
 }
 
@@ -1051,14 +1093,17 @@ __device__ inline void bf16_split4(const float (&x)[4], bf16x4_t& hi, bf16x4_t& 
 
 // BF3: the x-stage GEMMs in split-BF16x3 (hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x16_bf16, FP32 accumulation); T must be float.
-template <typename T, bool REDUCE, int NMX, int CM, bool BF3 = false, int kXmNt = 3>
+// BREG: the B operands of all k-steps (at most 12, i.e. 24 x-groups) are loaded
+// once per work item, in the prologue, where their latency overlaps the row
+// compaction; otherwise they are loaded per tile in chunks of KSB k-steps.
+template <typename T, bool REDUCE, int NMX, int CM, bool BF3 = false, int kXmNt = 3, bool BREG = false>
 __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
   typedef typename Mfma4<T>::acc_t acc_t;
   constexpr int TPR = NMX <= 16 ? 16 : 32;  // epilogue threads per row
   constexpr int RPP = 256 / TPR;            // rows per epilogue pass
-  constexpr int KSB = (sizeof(T) == 4 && kXmNt <= 3) ? 6 : 3;   // k-steps of B held in registers
+  constexpr int KSB = (sizeof(T) == 4 && kXmNt <= 3) ? 12 : 3;  // k-steps of B held in registers
   const int nLinesYZ = p.m1 * p.m2;
   const int nTilesTot = p.nPanels * nLinesYZ;
   const int bid = blockIdx.x;
@@ -1089,11 +1134,17 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   T2v* Wk = stw + p.m0;                                                        // m0 x nG0p
   unsigned char* dyn = reinterpret_cast<unsigned char*>(Wk + size_t(p.m0) * nG0p);
   dyn = smem + (size_t(dyn - smem) + 15) / 16 * 16;
+  // Row strides of U2 and g' padded by one element: the 4 rows a wave serves
+  // then start in different LDS banks.
+  const int u2s = nw * p.nb2max + 1, gps = nm * p.nb2max + 1;
   T* g3 = reinterpret_cast<T*>(dyn);                                           // 16 x LDG
-  double2* sU2 = reinterpret_cast<double2*>(dyn + (size_t(16) * LDG * sizeof(T) + 15) / 16 * 16);  // 16 x nw x nb2max
-  double2* gp = sU2 + size_t(16) * nw * p.nb2max;                              // RPP x nm x nb2max
-  double* se2 = reinterpret_cast<double*>(gp + size_t(RPP) * nm * p.nb2max);   // 16 x nb2max
+  double2* sU2 = reinterpret_cast<double2*>(dyn + (size_t(16) * LDG * sizeof(T) + 15) / 16 * 16);  // 16 x u2s
+  double2* gp = sU2 + size_t(16) * u2s;                                        // RPP x gps
+  double* se2 = reinterpret_cast<double*>(gp + size_t(RPP) * gps);             // 16 x nb2max
   double* sw3 = se2 + size_t(16) * p.nb2max;                                   // 16 x nm
+  double* sf2 = sw3 + size_t(16) * nm;                                         // nCalc x 16 x nb2max
+  double* se1 = sf2 + size_t(p.nCalc) * 16 * p.nb2max;                         // kGMax
+  int* sg3 = reinterpret_cast<int*>(se1 + kGMax);                              // 16 x nm
 
   // twiddles and group coordinates
   for (int i = t; i < p.m0; i += 256) {
@@ -1101,20 +1152,19 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
     stw[i] = T2v{T(d.x), T(d.y)};
   }
   for (int i = t; i < nG0; i += 256) sg0[i] = p.g0val[i];
+  if (t < p.panelGSize[panel]) se1[t] = p.e1pool[p.kkStateOff[kk] + p.panelGStart[panel] + t];
 
   const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
                  (size_t(jy) * p.m2 + jz) * ncol;
   const size_t gStride = size_t(p.m1) * p.m2 * ncol;  // T2v elements between groups
   const int KS = (2 * nG0 + 3) / 4;                    // k-steps of 4 real K
-  // B operands of every k-step, loaded once for all tiles when they fit in registers
-  constexpr int KSR = (BF3 || sizeof(T) != 4 || kXmNt > 3) ? 1 : 12;  // FP32: up to 24 x-groups
-  const bool bReg = !BF3 && KS <= KSR;
+  constexpr int KSR = BREG ? 12 : 1;
   T bAll[KSR][kXmNt];
-  if (bReg) {
+  if constexpr (BREG) {
 #pragma unroll
     for (int q = 0; q < KSR; ++q) {
       const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
-      const bool gOk = g < nG0 && q < KS;
+      const bool gOk = g < nG0;
 #pragma unroll
       for (int j = 0; j < kXmNt; ++j) {
         const int c = (wave * kXmNt + j) * 16 + lr;
@@ -1242,7 +1292,6 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           if (c < p.nCalc) n3[c] = p.bose[(size_t(iq) * nm + sub) * p.nCalc + c];
       }
     };
-    load_e3(0);
     // Cr = A1 B, Ci = A2 B with B(2g + s, c) = (Xr, Xi)(g, c), A1(m, 2g + s) = (Wr, -Wi),
     // A2(m, 2g + s) = (Wi, Wr); wave owns complex column tiles wave*kXmNt .. +kXmNt
     acc_t accr[kXmNt], acci[kXmNt];
@@ -1251,7 +1300,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) accr[j][i] = acci[j][i] = T(0);
     if constexpr (BF3) {
-      sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, sU2, se2, sw3);
+      sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
       // 16 real K per step: lane (lr, lg) holds k = 16 kb + 4 lg + j, i.e. groups
       // g = 8 kb + 2 lg and g + 1, (Re, Im) each
       for (int kb = 0; kb < (2 * nG0 + 15) / 16; ++kb) {
@@ -1290,21 +1339,23 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2l, bh, acci[j], 0, 0, 0);
         }
       }
-    } else if (bReg) {
-      sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, sU2, se2, sw3);
+    } else if constexpr (BREG) {
+      sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
+      // fully unrolled with a predicate (no early exit), so that bAll stays in registers
 #pragma unroll
       for (int q = 0; q < KSR; ++q) {
-        if (q >= KS) break;
-        const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
-        const bool gOk = g < nG0;
-        const T2v w = wrow[gOk ? g : nG0];
-        const bool on = rowOk && gOk;
-        const T a1 = on ? (kc ? -w.y : w.x) : T(0);
-        const T a2 = on ? (kc ? w.x : w.y) : T(0);
+        if (q < KS) {
+          const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+          const bool gOk = g < nG0;
+          const T2v w = wrow[gOk ? g : nG0];
+          const bool on = rowOk && gOk;
+          const T a1 = on ? (kc ? -w.y : w.x) : T(0);
+          const T a2 = on ? (kc ? w.x : w.y) : T(0);
 #pragma unroll
-        for (int j = 0; j < kXmNt; ++j) {
-          accr[j] = Mfma4<T>::mac(a1, bAll[q][j], accr[j]);
-          acci[j] = Mfma4<T>::mac(a2, bAll[q][j], acci[j]);
+          for (int j = 0; j < kXmNt; ++j) {
+            accr[j] = Mfma4<T>::mac(a1, bAll[q][j], accr[j]);
+            acci[j] = Mfma4<T>::mac(a2, bAll[q][j], acci[j]);
+          }
         }
       }
     } else
@@ -1328,19 +1379,21 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
       }
       // the tile's row data into LDS while the B loads are in flight; the
       // previous tile's epilogue ended with a barrier
-      if (kb == 0) sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, sU2, se2, sw3);
+      if (kb == 0) sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
 #pragma unroll
       for (int q = 0; q < KSB; ++q) {
-        const int k = 4 * (kb + q) + lg, g = k >> 1, kc = k & 1;
-        const bool gOk = g < nG0 && kb + q < KS;
-        const T2v w = wrow[gOk ? g : nG0];
-        const bool on = rowOk && gOk;
-        const T a1 = on ? (kc ? -w.y : w.x) : T(0);
-        const T a2 = on ? (kc ? w.x : w.y) : T(0);
+        if (kb + q < KS) {
+          const int k = 4 * (kb + q) + lg, g = k >> 1, kc = k & 1;
+          const bool gOk = g < nG0;
+          const T2v w = wrow[gOk ? g : nG0];
+          const bool on = rowOk && gOk;
+          const T a1 = on ? (kc ? -w.y : w.x) : T(0);
+          const T a2 = on ? (kc ? w.x : w.y) : T(0);
 #pragma unroll
-        for (int j = 0; j < kXmNt; ++j) {
-          accr[j] = Mfma4<T>::mac(a1, bv[q][j], accr[j]);
-          acci[j] = Mfma4<T>::mac(a2, bv[q][j], acci[j]);
+          for (int j = 0; j < kXmNt; ++j) {
+            accr[j] = Mfma4<T>::mac(a1, bv[q][j], accr[j]);
+            acci[j] = Mfma4<T>::mac(a2, bv[q][j], acci[j]);
+          }
         }
       }
     }
@@ -1357,6 +1410,9 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
         }
       }
     }
+    // the first pass's phonon data: loads in flight across the barrier and the
+    // electron rotation, registers free during the staging and the MFMAs
+    load_e3(0);
     __syncthreads();
 
     // ---- epilogue, RPP rows per pass (the first pass's phonon data was prefetched)
@@ -1373,14 +1429,14 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
       if (modeOn) {
         const T* gr = g3 + r * LDG + sub * NWP;
         for (int b2 = 0; b2 < nb2; ++b2) {
-          const double2* u = sU2 + (size_t(r) * p.nb2max + b2) * nw;
+          const double2* u = sU2 + size_t(r) * u2s + b2 * nw;
           double sx = 0., sy = 0.;
           for (int w2 = 0; w2 < nw; ++w2) {
             const double xr = double(gr[2 * w2]), xi = double(gr[2 * w2 + 1]);
             sx += u[w2].x * xr + u[w2].y * xi;
             sy += u[w2].x * xi - u[w2].y * xr;
           }
-          gp[(rl * nm + sub) * p.nb2max + b2] = make_double2(sx, sy);
+          gp[rl * gps + sub * p.nb2max + b2] = make_double2(sx, sy);
         }
       }
       wave_sync();
@@ -1396,7 +1452,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 #pragma unroll
             for (int nu = 0; nu < NMX; ++nu)
               if (nu < nm) {
-                const double2 x = gp[(rl * nm + nu) * p.nb2max + b2];
+                const double2 x = gp[rl * gps + nu * p.nb2max + b2];
                 const double er = double(e3c[nu].x), ei = double(e3c[nu].y);
                 sx += x.x * er - x.y * ei;
                 sy += x.x * ei + x.y * er;
@@ -1433,33 +1489,25 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
           }
         }
         // ... and along nu (scattering.cpp:1286-1316), through LDS when modes are degenerate
+        // the mode's degenerate group (consecutive modes within 1e-6 of the
+        // group's first), tabulated per q at set-up
         int mydeg = 1, mystart = sub;
         if (modeOn) {
-          const double* wr = sw3 + r * nm;
-          for (int s0 = 0; s0 < nm;) {
-            int deg = 0;
-            for (int i = s0; i < nm; ++i) {
-              if (fabs(wr[s0] - wr[i]) < 1.0e-6) deg++;
-              else break;
-            }
-            if (sub >= s0 && sub < s0 + deg) {
-              mystart = s0;
-              mydeg = deg;
-            }
-            s0 += deg;
-          }
+          const int grp = sg3[r * nm + sub];
+          mystart = grp & 0xff;
+          mydeg = grp >> 8;
         }
         if (__any(mydeg > 1)) {  // g' of the row is dead here: its slots hold |g|^2
           double* gs = reinterpret_cast<double*>(gp);
           wave_sync();
           if (modeOn)
-            for (int b2 = 0; b2 < nb2; ++b2) gs[2 * ((rl * nm + sub) * p.nb2max + b2)] = v[b2];
+            for (int b2 = 0; b2 < nb2; ++b2) gs[2 * (rl * gps + sub * p.nb2max + b2)] = v[b2];
           wave_sync();
           if (modeOn && mydeg > 1)
             for (int b2 = 0; b2 < nb2; ++b2) {
               double s2 = 0.;
               for (int i = 0; i < mydeg; ++i)
-                s2 += gs[2 * ((rl * nm + mystart + i) * p.nb2max + b2)] / double(float(mydeg));
+                s2 += gs[2 * (rl * gps + (mystart + i) * p.nb2max + b2)] / double(float(mydeg));
 #pragma unroll
               for (int b = 0; b < kSepNb2; ++b)
                 if (b == b2) v[b] = s2;
@@ -1482,17 +1530,17 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
               double d1, d2;
               if (p.adaptive) {
                 const AdaptiveWidth aw = adaptive_width(p, int(e1 - p.e1pool) + gi, is2);
-                d1 = adaptive_delta(e1[gi] - en2 + w3, aw);
-                d2 = adaptive_delta(e1[gi] - en2 - w3, aw);
+                d1 = adaptive_delta(se1[gi] - en2 + w3, aw);
+                d2 = adaptive_delta(se1[gi] - en2 - w3, aw);
               } else {
-                d1 = gaussian_delta(e1[gi] - en2 + w3, p.invSigma, p.deltaPref);
-                d2 = gaussian_delta(e1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+                d1 = gaussian_delta(se1[gi] - en2 + w3, p.invSigma, p.deltaPref);
+                d2 = gaussian_delta(se1[gi] - en2 - w3, p.invSigma, p.deltaPref);
               }
               if (d1 <= 0. && d2 <= 0.) continue;
 #pragma unroll
               for (int c = 0; c < CM; ++c) {
                 if (c >= nCalc) break;
-                const double f2 = p.fermi[size_t(c) * p.numStates + is2];
+                const double f2 = sf2[(c * 16 + r) * p.nb2max + b2];
                 part[gi][c] += cpl * ((f2 + n3[c]) * d1 + (1. - f2 + n3[c]) * d2) * p.norm / w3 *
                                3.14159265358979323846;
               }
@@ -1545,14 +1593,16 @@ static __global__ void __launch_bounds__(256) k_sep_screen(CoupParams p) {
   p.keepMask[slot] = mask;
 }
 
-inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb2max, int rpp) {
+inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb2max, int rpp, int nCalc) {
   auto up = [](size_t b) { return (b + 15) / 16 * 16; };
   size_t b = up(size_t(m0) * sizeof(SepXmRow) + (8 + nG0) * sizeof(int));
   b = up(b + size_t(m0) * 2 * elem + size_t(m0) * (nG0 + 1) * 2 * elem);
   const size_t g3 = std::max(size_t(16) * nm * (2 * nw + 1) * elem, size_t(rpp) * nm * nb2max * sizeof(double));
   b += up(g3);
-  b += size_t(16) * nw * nb2max * sizeof(double2) + size_t(rpp) * nm * nb2max * sizeof(double2) +
-       size_t(16) * nb2max * sizeof(double) + size_t(16) * nm * sizeof(double);
+  b += size_t(16) * (nw * nb2max + 1) * sizeof(double2) + size_t(rpp) * (nm * nb2max + 1) * sizeof(double2) +
+       size_t(16) * nb2max * sizeof(double) + size_t(16) * nm * sizeof(double) +
+       size_t(nCalc) * 16 * nb2max * sizeof(double) + size_t(kGMax) * sizeof(double) +
+       size_t(16) * nm * sizeof(int);
   return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
 }
 
@@ -1584,12 +1634,16 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
   const bool one = p.nCalc == 1;
   constexpr bool bf3t = sizeof(T) == 4;  // split-BF16x3 exists for the FP32 data path
   const bool bf3 = p.bf16x3 != 0;
+  constexpr bool bregt = sizeof(T) == 4;  // B in registers for the whole item: FP32, <= 24 x-groups
+  const bool breg = bregt && (2 * p.nG0 + 3) / 4 <= 12;
   PHX_REQUIRE(!bf3 || (bf3t && reduce && sp.ncol <= 4 * 6 * 16),
               "split-BF16x3: FP32 data, the linewidth reduction and nm nw <= 384");
   if (reduce && sp.ncol <= 4 * 6 * 16) {
     const bool wide = sp.ncol > 4 * 3 * 16;
-    const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8);
+    const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8,
+                                             p.nCalc);
     PHX_REQUIRE(ldsXm <= 160 * 1024, "separable transform: LDS budget exceeded");
+    PHX_REQUIRE(p.qModeGrp, "separable transform: mode-group table missing");
     if (p.keepMask) {
       PHX_REQUIRE(p.m0 <= 64 && p.meshRow, "separable transform: row masks need m0 <= 64");
       hipLaunchKernelGGL(k_sep_screen, dim3(ceil_div(tiles, 256)), dim3(256), 0, s, p);
@@ -1610,6 +1664,8 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
       } else {                                                                      \
         if (bf3 && one) go(k_sep_xm<T, true, NMX, 1, bf3t>);                         \
         else if (bf3) go(k_sep_xm<T, true, NMX, kCMax, bf3t>);                       \
+        else if (breg && one) go(k_sep_xm<T, true, NMX, 1, false, 3, bregt>);        \
+        else if (breg) go(k_sep_xm<T, true, NMX, kCMax, false, 3, bregt>);           \
         else if (one) go(k_sep_xm<T, true, NMX, 1>);                                 \
         else go(k_sep_xm<T, true, NMX, kCMax>);                                      \
       }                                                                             \
