@@ -326,8 +326,10 @@ int phx_lw_reset(phx_lw* h);
  * skipped for a panel (k1, b1) when every band e2 of k2 and every mode omega
  * >= 5 cm^-1 at q = k2 - k1 give |e1 - e2 +- omega| >= nsigma * sigma, i.e.
  * when all its Gaussian weights (delta_function.cpp:40-45) are below
- * exp(-nsigma^2) of their peak.  0 turns it off (the default); otherwise
- * nsigma >= 6.  No counterpart in the reference, which evaluates every pair. */
+ * exp(-nsigma^2) of their peak; inside the rows it keeps, a (band, mode)
+ * pair whose two weights are both that small is skipped the same way.  0
+ * turns it off (the default); otherwise nsigma >= 6.  No counterpart in the
+ * reference, which evaluates every pair. */
 int phx_lw_set_screen(phx_lw* h, double nsigma);
 /* ---- Sweeps without host round trips.  Prepare every batch of a sweep over
  * k1_list (all active points when null; same batching as phx_lw_run) with

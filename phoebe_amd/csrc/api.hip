@@ -65,6 +65,7 @@ void launch_phase_rows(int nrows, int rowsPad, int nRp, int Kp, const double* kc
                        T* A, hipStream_t s);
 void launch_gather_eig(int na, int nw, const int* ik, const int* lo, const int* nb, const int* off,
                        const double2* U, double2* pool, hipStream_t s);
+void launch_transpose_e3(int nk, int nm, const double2* ev, void* out, bool fp32, hipStream_t s);
 void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const double2* ev,
                        double* wAll, double2* evAll, hipStream_t s);
 
@@ -704,6 +705,8 @@ struct LwHandle {
   DevBuf<double2> U2pool;
   DevBuf<double> e2pool, fermi, bose, Wall, lw;
   DevBuf<double2> E3all;
+  DevBuf<float2> E3Tf;         // separable transform: eigenvectors as (q, nu, nu2), FP32 ...
+  DevBuf<double2> E3Td;        // ... or FP64
   DevBuf<int> rowMesh, rowNb2, rowU2off, rowStateOff, gStartD, gSizeD;
   DevBuf<double> kcartActive;  // na x 3
   int Kp = 0, ldb = 0, rows = 32, rowsPad = 0;
@@ -1014,6 +1017,13 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
     }
     launch_jacobi(nm, nq, D.p, w.p, qinfo.p, dmass.p, s);
     launch_scatter_ph(nq, nm, dql.p, w.p, D.p, L->Wall.p, L->E3all.p, s);
+    if (L->transform == PHX_TRANSFORM_SEPARABLE) {
+      const bool f32 = L->precision != PHX_FP64;
+      if (f32) L->E3Tf.alloc(size_t(nk) * nm * nm);
+      else L->E3Td.alloc(size_t(nk) * nm * nm);
+      launch_transpose_e3(nk, nm, L->E3all.p, f32 ? static_cast<void*>(L->E3Tf.p) : static_cast<void*>(L->E3Td.p),
+                          f32, s);
+    }
     std::vector<int> qinf(nq);
     PHX_HIP(hipMemcpyAsync(qinf.data(), qinfo.p, qinfo.bytes(), hipMemcpyDeviceToHost, s));
     L->omegaAll.resize(size_t(nk) * nm);
@@ -1645,6 +1655,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
   cq.rowU2off = L->rowU2off.p + k2b;
   cq.rowNb2 = L->rowNb2.p + k2b;
   cq.E3pool = L->E3all.p;
+  cq.E3T = L->precision == PHX_FP64 ? static_cast<const void*>(L->E3Td.p) : static_cast<const void*>(L->E3Tf.p);
   cq.Wpool = L->Wall.p;
   cq.qmode = 1;
   cq.rowMesh = L->rowMesh.p + k2b;

@@ -82,6 +82,8 @@ struct CoupParams {
   double minWidth;                        // smallest width (0.1 meV)
   // degenerate phonon modes: per q and mode, first mode of its group | group size << 8
   const int* qModeGrp;
+  // separable transform: phonon eigenvectors (q, nu, nu2) in the transform's precision
+  const void* E3T;
 };
 
 template <typename T> struct Vec2;
@@ -1280,13 +1282,10 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
       for (int c = 0; c < CM; ++c) n3[c] = 0.;
       if (r < nr && sub < nm) {
         const int iq = rows[m0r + r].iq;
-        const double2* e3 = p.E3pool + size_t(iq) * nm * nm + size_t(nm) * sub;
+        const T2v* e3 = static_cast<const T2v*>(p.E3T) + size_t(iq) * nm * nm + sub;
 #pragma unroll
         for (int nu = 0; nu < NMX; ++nu)
-          if (nu < nm) {
-            const double2 d = e3[nu];
-            e3c[nu] = T2v{T(d.x), T(d.y)};
-          }
+          if (nu < nm) e3c[nu] = e3[nu * nm];
 #pragma unroll
         for (int c = 0; c < CM; ++c)
           if (c < p.nCalc) n3[c] = p.bose[(size_t(iq) * nm + sub) * p.nCalc + c];
@@ -1533,8 +1532,13 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
                 d1 = adaptive_delta(se1[gi] - en2 + w3, aw);
                 d2 = adaptive_delta(se1[gi] - en2 - w3, aw);
               } else {
-                d1 = gaussian_delta(se1[gi] - en2 + w3, p.invSigma, p.deltaPref);
-                d2 = gaussian_delta(se1[gi] - en2 - w3, p.invSigma, p.deltaPref);
+                const double x1 = se1[gi] - en2 + w3, x2 = se1[gi] - en2 - w3;
+                // with the energy screen on, a pair whose both weights are below
+                // exp(-screen^2) of the peak is skipped, as whole rows are
+                if (p.screen > 0. && fabs(x1) * p.invSigma >= p.screen && fabs(x2) * p.invSigma >= p.screen)
+                  continue;
+                d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
+                d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
               }
               if (d1 <= 0. && d2 <= 0.) continue;
 #pragma unroll
@@ -1643,7 +1647,7 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
     const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8,
                                              p.nCalc);
     PHX_REQUIRE(ldsXm <= 160 * 1024, "separable transform: LDS budget exceeded");
-    PHX_REQUIRE(p.qModeGrp, "separable transform: mode-group table missing");
+    PHX_REQUIRE(p.qModeGrp && p.E3T, "separable transform: mode-group or eigenvector table missing");
     if (p.keepMask) {
       PHX_REQUIRE(p.m0 <= 64 && p.meshRow, "separable transform: row masks need m0 <= 64");
       hipLaunchKernelGGL(k_sep_screen, dim3(ceil_div(tiles, 256)), dim3(256), 0, s, p);

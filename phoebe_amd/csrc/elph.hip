@@ -202,6 +202,32 @@ void launch_scatter_ph(int nq, int nm, const int* iq, const double* w, const dou
   PHX_HIP(hipGetLastError());
 }
 
+// Phonon eigenvectors of every q in the layout the separable transform's
+// epilogue reads, in its precision: out(q)(nu, nu2) at nu * nm + nu2, i.e.
+// the lanes that serve the modes nu2 of one row read consecutive elements.
+template <typename T2v>
+__global__ void k_transpose_e3(int nk, int nm, const double2* __restrict__ ev, T2v* __restrict__ out) {
+  const int q = blockIdx.x;
+  if (q >= nk) return;
+  const size_t base = size_t(q) * nm * nm;
+  for (int t = threadIdx.x; t < nm * nm; t += blockDim.x) {
+    const int nu = t / nm, nu2 = t % nm;
+    const double2 e = ev[base + nu + size_t(nm) * nu2];
+    out[base + t] = T2v{decltype(out->x)(e.x), decltype(out->x)(e.y)};
+  }
+}
+
+void launch_transpose_e3(int nk, int nm, const double2* ev, void* out, bool fp32, hipStream_t s) {
+  if (nk <= 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(k_transpose_e3<float2>, dim3(nk), dim3(256), 0, s, nk, nm, ev,
+                       static_cast<float2*>(out));
+  else
+    hipLaunchKernelGGL(k_transpose_e3<double2>, dim3(nk), dim3(256), 0, s, nk, nm, ev,
+                       static_cast<double2*>(out));
+  PHX_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ cacheElPh
 // g1(k1; Rp,nu,w1,w2) = sum_Re P(k1,Re) gW(Re,Rp,nu,w1,w2) as a real MFMA GEMM.
 // Output rows m < NKK hold Re(g1) of outer point m, rows NKK+m hold Im(g1);
