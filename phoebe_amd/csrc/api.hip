@@ -18,6 +18,7 @@
 #include "coupling_kernel.hpp"
 #include "elph_file.hpp"
 #include "harmonic_io.hpp"
+#include "phel.hpp"
 #include "phx_common.hpp"
 #include "screen.hpp"
 
@@ -35,22 +36,6 @@ void launch_vel_operator(bool phonon, int np, int n, double delta, int gammaRule
                          double2* vel, hipStream_t s);
 void launch_vel_degenerate(int np, int n, double threshold, int gammaRule, const double* E,
                            const double* kc, double2* vel, int* tooBig, hipStream_t s);
-struct PhelParams {
-  int nq, nb1, nb2, nm, nCalc;
-  double* g2;
-  const double* e1;
-  const double* E2;
-  const double* W3;
-  const double* F1;
-  const double* invT;
-  double norm, cutoff, invSigma, deltaPref;
-  int adaptive;
-  const double* v1;
-  const double* V2;
-  double qT[9], minWidth;
-  double* lw;
-};
-void launch_phel_reduce(const PhelParams& p, hipStream_t s);
 void launch_synth_gw(double2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
                      hipStream_t s);
 void launch_synth_gw_f32(float2* gW, size_t n, size_t blk, const double* decay, uint64_t seed,
@@ -1958,21 +1943,23 @@ static long long lw_plan_run(LwHandle* L, int graph) {
 struct PhelHandle {
   ElphHandle* H = nullptr;  // coupling (not owned)
   hipStream_t s = nullptr;
-  int nw = 0, nm = 0, nk = 0, nq = 0, nCalc = 0, na = 0, adaptive = 0;
-  int qmesh[3] = {0, 0, 0};
+  int nw = 0, nm = 0, nk = 0, nq = 0, nCalc = 0, na = 0, adaptive = 0, folded = 0, nb1max = 0;
+  int qmesh[3] = {0, 0, 0}, kmesh[3] = {0, 0, 0};
   double norm = 0., invSigma = 0., deltaPref = 0., qT[9] = {};
   std::vector<double> kcart, qcart, omega;   // nk x 3, nq x 3, nq x nm
   std::vector<int> active, lo, nb;
   std::vector<double> stateE, U1, v1;          // per active point: energies, (nw x nb) eigvecs, nb x 3
   std::vector<size_t> stateOff, u1Off;
   std::vector<double> T, mu;
-  DevBuf<double> Rh, degh, invT, W3, lw, E2, k2, v1d, V2, F1, g2;
-  DevBuf<double2> h0R, E3, U2, U1d;
-  DevBuf<double> A;   // FP64 phase rows
-  DevBuf<float> Af;   // FP32 phase rows
-  DevBuf<int> info, rowU2off, rowNb2, rowE3;
+  std::vector<double> hRh, hdegh, hh0R;        // electron Hamiltonian on the host (adaptive k2 velocities)
+  DevBuf<double> Rh, degh, invT, W3, lw, E2, k2, V2, F1, g2, stateEd, v1d, W0, kb, invE;
+  DevBuf<double2> h0R, E3, U2, U1d, phE, phP;
+  DevBuf<double> A, Bd;   // FP64 phase rows e^{iq3.Rp}, cached panels
+  DevBuf<float> Af, Bf;   // FP32 ones
+  DevBuf<int> info, fail, rowU2off, rowNb2, rowE3, e2off, cmeta, pmeta;
   DevBuf<long long> rowOutOff;
   int rowsPad = 0;
+  size_t pstride = 0, g2Stride = 0;
   long long pairs = 0;
 };
 
@@ -1984,6 +1971,7 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
               "phx_phel_setup: unknown smearing");
   PHX_REQUIRE(p.smearing == PHX_SMEAR_ADAPTIVE || p.sigma > 0., "phx_phel_setup: sigma must be > 0");
   for (int i = 0; i < 3; ++i) PHX_REQUIRE(p.mesh[i] >= 1 && qmesh[i] >= 1, "phx_phel_setup: bad meshes");
+  PHX_REQUIRE(H->nw <= 32, "phx_phel_setup: at most 32 Wannier functions");
   P->H = H;
   P->s = H->stream;
   hipStream_t s = P->s;
@@ -1993,7 +1981,10 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
   P->nCalc = p.nCalc;
   P->nk = p.mesh[0] * p.mesh[1] * p.mesh[2];
   P->nq = qmesh[0] * qmesh[1] * qmesh[2];
-  for (int i = 0; i < 3; ++i) P->qmesh[i] = qmesh[i];
+  for (int i = 0; i < 3; ++i) {
+    P->qmesh[i] = qmesh[i];
+    P->kmesh[i] = p.mesh[i];
+  }
   P->T.assign(p.temperatures, p.temperatures + p.nCalc);
   P->mu.assign(p.chemPots, p.chemPots + p.nCalc);
   // norm = 1 / prod(kMesh) (phel_scattering.cpp:61)
@@ -2003,6 +1994,12 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
     P->invSigma = 1. / p.sigma;
     P->deltaPref = 1. / p.sigma / std::sqrt(kPi);
   }
+  // k2 = k1 + q3 lies on the k mesh when the q mesh divides it: H(k2) = H(k2 - G)
+  // for the Wannier Hamiltonian, so the k-mesh eigenpairs serve every k2 (the
+  // phases of the two Fourier sums agree to rounding).  Adaptive smearing keeps
+  // the unfolded points, whose Gamma rule depends on |k2|.
+  P->folded = !P->adaptive && p.mesh[0] % qmesh[0] == 0 && p.mesh[1] % qmesh[1] == 0 &&
+              p.mesh[2] % qmesh[2] == 0;
   // ---- electrons on the k mesh and the window
   P->kcart.resize(3 * size_t(P->nk));
   for (int ik = 0; ik < P->nk; ++ik) mesh_cart(p.recip, p.mesh, ik, &P->kcart[3 * size_t(ik)]);
@@ -2024,6 +2021,10 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
     PHX_HIP(hipMemcpyAsync(hi.data(), inf.p, inf.bytes(), hipMemcpyDeviceToHost, s));
     PHX_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < P->nk; ++i) PHX_REQUIRE(hi[i] >= 0, "electron eigensolve did not converge");
+    if (P->folded) {  // the k-mesh eigenpairs become the k2 pools
+      P->U2 = std::move(U);
+      P->E2 = std::move(E);
+    }
   }
   for (int ik = 0; ik < P->nk; ++ik) {
     int l = -1, h = -1;
@@ -2044,13 +2045,16 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
   for (int a = 0; a < P->na; ++a) {
     const int ik = P->active[a], n = P->nb[a];
     P->stateOff.push_back(P->stateE.size());
-    P->u1Off.push_back(P->U1.size());
+    P->u1Off.push_back(P->U1.size() / 2);
     for (int b = 0; b < n; ++b) {
       P->stateE.push_back(Eall[size_t(ik) * nw + P->lo[a] + b]);
       const double* col = &Uall[2 * (size_t(ik) * nw * nw + size_t(nw) * (P->lo[a] + b))];
       P->U1.insert(P->U1.end(), col, col + 2 * nw);
     }
+    P->nb1max = std::max(P->nb1max, n);
   }
+  h2d(P->U1d, reinterpret_cast<const double2*>(P->U1.data()), P->U1.size() / 2, s);
+  h2d(P->stateEd, P->stateE, s);
   if (P->adaptive) {
     // k1 velocities as the electron band structure holds them (single-point
     // velocity operator, active_bandstructure.cpp:1032-1071)
@@ -2069,6 +2073,11 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
       }
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) P->qT[3 * i + j] = p.recip[i + 3 * j] / p.mesh[i];
+    h2d(P->v1d, P->v1, s);
+    P->hRh.assign(p.Rh, p.Rh + size_t(p.nRh) * 3);
+    P->hdegh.assign(p.degh, p.degh + p.nRh);
+    P->hh0R.assign(p.h0R, p.h0R + 2 * size_t(p.nRh) * nw * nw);
+    P->V2.alloc(size_t(P->nq) * nw * 3);
   }
   // ---- phonons on the q mesh (PhononH0 with the long-range term)
   P->qcart.resize(3 * size_t(P->nq));
@@ -2109,150 +2118,261 @@ static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const
       launch_phase_rows<float>(P->nq, P->rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, P->Af.p, s);
     }
   }
-  // ---- per-k1 scratch and the fixed coupling row metadata
-  std::vector<int> u2off(P->nq), nb2(P->nq, nw), e3i(P->nq);
+  // ---- batched cacheElPh: e^{ik1.Re}/de per outer point, 1/dp (the e^{iq3.Rp}
+  // factor sits in the phase rows), panels of up to kCacheNKK outer points
+  {
+    std::vector<double> degE(H->nRe), degP(H->nRp);
+    PHX_HIP(hipMemcpyAsync(degE.data(), H->degE.p, H->degE.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipMemcpyAsync(degP.data(), H->degP.p, H->degP.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
+    std::vector<double> inv(H->nRe);
+    for (int i = 0; i < H->nRe; ++i) inv[i] = 1. / degE[i];
+    h2d(P->invE, inv, s);
+    std::vector<double2> pp(size_t(kCacheNKK) * H->nRp);
+    for (int k = 0; k < kCacheNKK; ++k)
+      for (int i = 0; i < H->nRp; ++i) pp[size_t(k) * H->nRp + i] = make_double2(1. / degP[i], 0.);
+    h2d(P->phP, pp, s);
+    P->phE.alloc(size_t(kCacheNKK) * H->nRe);
+    P->kb.alloc(3 * size_t(kCacheNKK));
+    P->cmeta.alloc(3 * size_t(kCacheNKK));
+    P->pmeta.alloc(size_t(kCacheNKK) * (1 + 2 * P->nb1max));
+    P->pstride = size_t(H->Kp) * H->ldb;
+    const size_t nPanels = size_t(kCacheNKK) * P->nb1max;
+    if (H->precision == PHX_FP64) {
+      P->Bd.alloc(P->pstride * nPanels);
+      launch_fill<double>(P->Bd.p, P->Bd.n, 0., s);  // padding stays zero
+    } else {
+      P->Bf.alloc(P->pstride * nPanels);
+      launch_fill<float>(P->Bf.p, P->Bf.n, 0.f, s);
+    }
+  }
+  // ---- coupling rows (one per q3) and the reduction's buffers
+  P->g2Stride = size_t(P->nb1max) * nw * nm;
+  std::vector<int> u2off(P->nq), nb2(P->nq, nw), e3i(P->nq), e2o(P->nq);
+  std::vector<long long> oo(P->nq);
   for (int q = 0; q < P->nq; ++q) {
-    u2off[q] = q * nw * nw;
+    u2off[q] = q * nw * nw;  // unfolded k2: per outer point eigenpairs of nq points
+    e2o[q] = q * nw;
     e3i[q] = q;
+    oo[q] = (long long)q * (long long)P->g2Stride;
   }
   h2d(P->rowU2off, u2off, s);
+  h2d(P->e2off, e2o, s);
   h2d(P->rowNb2, nb2, s);
   h2d(P->rowE3, e3i, s);
-  P->U2.alloc(size_t(P->nq) * nw * nw);
-  P->E2.alloc(size_t(P->nq) * nw);
-  P->k2.alloc(size_t(P->nq) * 3);
-  P->info.alloc(P->nq);
-  int nb1max = 0;
-  for (int n : P->nb) nb1max = std::max(nb1max, n);
-  P->g2.alloc(size_t(P->nq) * nb1max * nw * nm);
-  P->F1.alloc(size_t(kCMax) * nw);
-  P->rowOutOff.alloc(P->nq);
+  h2d(P->rowOutOff, oo, s);
+  if (!P->folded) {
+    P->U2.alloc(size_t(P->nq) * nw * nw);
+    P->E2.alloc(size_t(P->nq) * nw);
+    P->k2.alloc(size_t(P->nq) * 3);
+    P->info.alloc(P->nq);
+  }
+  P->g2.alloc(size_t(P->nq) * P->g2Stride);
+  P->F1.alloc(size_t(kCacheNKK) * kCMax * nw);
+  P->W0.alloc(size_t(nm));
+  launch_fill<double>(P->W0.p, P->W0.n, 0., s);
+  P->fail.alloc(1);
+  launch_fill<int>(P->fail.p, 1, 0, s);
   std::vector<double> it(P->nCalc);
   for (int c = 0; c < P->nCalc; ++c) it[c] = 1. / P->T[c];
   h2d(P->invT, it, s);
   P->lw.alloc(size_t(P->nCalc) * P->nq * nm);
   launch_fill<double>(P->lw.p, P->lw.n, 0., s);
-  if (P->adaptive) P->V2.alloc(size_t(P->nq) * nw * 3);
   PHX_HIP(hipStreamSynchronize(s));
 }
 
-// One outer point k1 (active index a) against every q3.
-static void phel_point(PhelHandle* P, int a) {
+// Up to kCacheNKK outer points (active indices as[0..n)): one batched
+// cacheElPh (one gW stream for all of them), then per outer point the
+// coupling of every q3 in one launch and the symmetrize + rate kernels.
+// Enqueued on the handle's stream; host-side buffers of pageable memory are
+// staged by the copies, so nothing waits for the device here.
+static void phel_batch(PhelHandle* P, const int* as, int n) {
   ElphHandle* H = P->H;
   hipStream_t s = P->s;
-  const int nw = P->nw, nm = P->nm, nq = P->nq, nb1 = P->nb[a];
-  const double* k1 = &P->kcart[3 * size_t(P->active[a])];
-  // k2 = k1 + q3 (not folded), all bands (electronH0.populate, phel_scattering.cpp:225-243)
+  const int nw = P->nw, nm = P->nm, nq = P->nq, K = kCacheNKK, blk = 1 + 2 * P->nb1max;
+  const bool f64 = H->precision == PHX_FP64;
+  // ---- batched cacheElPh
+  std::vector<double> kb(3 * size_t(n));
+  std::vector<int> cm(3 * size_t(K), 0), pm(size_t(K) * blk, 0);
+  std::vector<double> F(size_t(K) * kCMax * nw, 0.);
+  int pb = 0;
+  for (int i = 0; i < n; ++i) {
+    const int a = as[i], nb1 = P->nb[a];
+    for (int d = 0; d < 3; ++d) kb[3 * size_t(i) + d] = P->kcart[3 * size_t(P->active[a]) + d];
+    cm[i] = int(P->u1Off[a]);
+    cm[K + i] = nb1;
+    cm[2 * K + i] = pb;
+    pb += nb1;
+    int* m = &pm[size_t(i) * blk];  // kkNb1, panelKK (all 0), panelB1
+    m[0] = nb1;
+    for (int b = 0; b < nb1; ++b) m[1 + P->nb1max + b] = b;
+    const double* e1 = &P->stateE[P->stateOff[a]];
+    for (int c = 0; c < P->nCalc; ++c)
+      for (int b = 0; b < nb1; ++b) {
+        // Particle::getPopPopPm1 (particle.cpp:117-128), fermions
+        const double x = 1. / std::cosh((e1[b] - P->mu[c]) / 2. / P->T[c]);
+        F[size_t(i) * kCMax * nw + size_t(c) * nb1 + b] = 0.25 * x * x;  // (calc, b1) of point i
+      }
+  }
+  PHX_HIP(hipMemcpyAsync(P->kb.p, kb.data(), kb.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  PHX_HIP(hipMemcpyAsync(P->cmeta.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  PHX_HIP(hipMemcpyAsync(P->pmeta.p, pm.data(), pm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  PHX_HIP(hipMemcpyAsync(P->F1.p, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_phase_table(n, H->nRe, P->kb.p, H->Re.p, P->invE.p, 1.0, P->phE.p, s);
+  CacheParams cp{};
+  cp.nRe = H->nRe;
+  cp.nRp = H->nRp;
+  cp.nm = nm;
+  cp.nw = nw;
+  cp.nkk = n;
+  cp.gW = H->gw_ptr();
+  cp.phE = P->phE.p;
+  cp.phP = P->phP.p;
+  cp.U1pool = P->U1d.p;
+  cp.U1off = P->cmeta.p;
+  cp.nb1 = P->cmeta.p + K;
+  cp.panelBase = P->cmeta.p + 2 * K;
+  cp.panelStride = P->pstride;
+  cp.ldb = H->ldb;
+  if (f64) {
+    cp.B = P->Bd.p;
+    launch_cache<double, double>(cp, s);
+  } else {
+    cp.B = P->Bf.p;
+    launch_cache<float, float>(cp, s);
+  }
+  // ---- every outer point against all q3
   std::vector<double> k2(3 * size_t(nq));
-  for (int q = 0; q < nq; ++q)
-    for (int i = 0; i < 3; ++i) k2[3 * size_t(q) + i] = P->qcart[3 * size_t(q) + i] + k1[i];
-  PHX_HIP(hipMemcpyAsync(P->k2.p, k2.data(), k2.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  if (P->adaptive) {
-    // k2 velocities of the batched form (kokkosPopulate with velocities)
-    std::vector<double> E(size_t(nq) * nw), U(2 * size_t(nq) * nw * nw), V(2 * size_t(nq) * nw * nw * 3);
-    std::vector<double> hRh(P->Rh.n), hdeg(P->degh.n), hh(2 * P->h0R.n);
-    PHX_HIP(hipMemcpyAsync(hRh.data(), P->Rh.p, P->Rh.bytes(), hipMemcpyDeviceToHost, s));
-    PHX_HIP(hipMemcpyAsync(hdeg.data(), P->degh.p, P->degh.bytes(), hipMemcpyDeviceToHost, s));
-    PHX_HIP(hipMemcpyAsync(hh.data(), P->h0R.p, P->h0R.bytes(), hipMemcpyDeviceToHost, s));
-    PHX_HIP(hipStreamSynchronize(s));
-    diag_velocities(false, nq, k2.data(), nw, int(P->degh.n), hRh.data(), hdeg.data(), hh.data(), nullptr,
-                    nullptr, PHX_GAMMA_BATCHED, E.data(), U.data(), V.data());
-    std::vector<double> gv(size_t(nq) * nw * 3);
-    for (int q = 0; q < nq; ++q)
-      for (int b = 0; b < nw; ++b)
-        for (int i = 0; i < 3; ++i)
-          gv[(size_t(q) * nw + b) * 3 + i] = V[2 * ((((size_t(q) * nw + b) * nw) + b) * 3 + i)];
-    PHX_HIP(hipMemcpyAsync(P->V2.p, gv.data(), gv.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    PHX_HIP(hipMemcpyAsync(P->E2.p, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    PHX_HIP(hipMemcpyAsync(P->U2.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  } else {
-    launch_bloch(false, nq, nw, int(P->degh.n), P->k2.p, P->Rh.p, P->degh.p, P->h0R.p, P->U2.p, s);
-    launch_jacobi(nw, nq, P->U2.p, P->E2.p, P->info.p, nullptr, s);
-  }
-  // cacheElPh(k1) with its windowed eigenvectors
-  elph_cache(H, &P->U1[P->u1Off[a]], nb1, k1);
-  // calcCouplingSquared, store mode, every q3 in one launch
-  std::vector<long long> oo(nq);
-  for (int q = 0; q < nq; ++q) oo[q] = (long long)q * nb1 * nw * nm;
-  PHX_HIP(hipMemcpyAsync(P->rowOutOff.p, oo.data(), oo.size() * sizeof(long long), hipMemcpyHostToDevice, s));
-  DevBuf<double> W0(nm);
-  CoupParams c{};
-  c.lda = H->Kp;
-  c.Kp = H->Kp;
-  c.ldb = H->ldb;
-  c.panelStride = size_t(H->Kp) * H->ldb;
-  c.M = nq;
-  c.nm = nm;
-  c.nw = nw;
-  c.nPanels = nb1;
-  c.nMtiles = P->rowsPad / coupling_rows_per_tile_f64(H->ldb);
-  c.panelKK = H->meta.p + 4;
-  c.panelB1 = H->meta.p + 4 + nb1;
-  c.U2pool = P->U2.p;
-  c.rowU2off = P->rowU2off.p;
-  c.rowNb2 = P->rowNb2.p;
-  c.E3pool = P->E3.p;
-  c.Wpool = W0.p;
-  c.qmode = 0;
-  c.rowE3 = P->rowE3.p;
-  c.out = P->g2.p;
-  c.rowOutOff = P->rowOutOff.p;
-  c.kkNb1 = H->meta.p + 1;
-  c.nb2max = nw;
-  if (H->precision == PHX_FP64) {
-    c.A = P->A.p;
-    c.B = H->Bd.p;
-    launch_coupling<double>(c, false, s);
-  } else {
-    c.A = P->Af.p;
-    c.B = H->Bf.p;
-    launch_coupling<float>(c, false, s);
-  }
-  // rates
-  const double* e1 = &P->stateE[P->stateOff[a]];
-  std::vector<double> F(size_t(P->nCalc) * nb1);
-  for (int cc = 0; cc < P->nCalc; ++cc)
-    for (int b = 0; b < nb1; ++b) {
-      // Particle::getPopPopPm1 (particle.cpp:117-128), fermions
-      const double x = 1. / std::cosh((e1[b] - P->mu[cc]) / 2. / P->T[cc]);
-      F[size_t(cc) * nb1 + b] = 0.25 * x * x;
+  std::vector<int> u2off, e2o;
+  for (int i = 0; i < n; ++i) {
+    const int a = as[i], nb1 = P->nb[a], ik = P->active[a];
+    const double* k1 = &P->kcart[3 * size_t(ik)];
+    if (P->folded) {
+      // k2 = k1 + q3 on the k mesh (first index fastest)
+      u2off.resize(nq);
+      e2o.resize(nq);
+      const int* m = P->kmesh;
+      const int j0 = ik % m[0], j1 = (ik / m[0]) % m[1], j2 = ik / (m[0] * m[1]);
+      for (int q = 0; q < nq; ++q) {
+        const int* mq = P->qmesh;
+        const int q0 = q % mq[0], q1 = (q / mq[0]) % mq[1], q2 = q / (mq[0] * mq[1]);
+        const int x = (j0 + q0 * (m[0] / mq[0])) % m[0], y = (j1 + q1 * (m[1] / mq[1])) % m[1],
+                  z = (j2 + q2 * (m[2] / mq[2])) % m[2];
+        const int k = (z * m[1] + y) * m[0] + x;
+        u2off[q] = k * nw * nw;
+        e2o[q] = k * nw;
+      }
+      PHX_HIP(hipMemcpyAsync(P->rowU2off.p, u2off.data(), nq * sizeof(int), hipMemcpyHostToDevice, s));
+      PHX_HIP(hipMemcpyAsync(P->e2off.p, e2o.data(), nq * sizeof(int), hipMemcpyHostToDevice, s));
+    } else {
+      // k2 = k1 + q3, not folded, all bands (electronH0.populate, phel_scattering.cpp:225-243)
+      for (int q = 0; q < nq; ++q)
+        for (int d = 0; d < 3; ++d) k2[3 * size_t(q) + d] = P->qcart[3 * size_t(q) + d] + k1[d];
+      if (P->adaptive) {
+        // k2 velocities of the batched form (kokkosPopulate with velocities)
+        std::vector<double> E(size_t(nq) * nw), U(2 * size_t(nq) * nw * nw), V(2 * size_t(nq) * nw * nw * 3);
+        diag_velocities(false, nq, k2.data(), nw, int(P->hdegh.size()), P->hRh.data(), P->hdegh.data(),
+                        P->hh0R.data(), nullptr, nullptr, PHX_GAMMA_BATCHED, E.data(), U.data(), V.data());
+        std::vector<double> gv(size_t(nq) * nw * 3);
+        for (int q = 0; q < nq; ++q)
+          for (int b = 0; b < nw; ++b)
+            for (int d = 0; d < 3; ++d)
+              gv[(size_t(q) * nw + b) * 3 + d] = V[2 * ((((size_t(q) * nw + b) * nw) + b) * 3 + d)];
+        PHX_HIP(hipMemcpyAsync(P->V2.p, gv.data(), gv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        PHX_HIP(hipMemcpyAsync(P->E2.p, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        PHX_HIP(hipMemcpyAsync(P->U2.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      } else {
+        PHX_HIP(hipMemcpyAsync(P->k2.p, k2.data(), k2.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_bloch(false, nq, nw, int(P->degh.n), P->k2.p, P->Rh.p, P->degh.p, P->h0R.p, P->U2.p, s);
+        launch_jacobi(nw, nq, P->U2.p, P->E2.p, P->info.p, nullptr, s);
+        launch_phel_flag_failures(nq, P->info.p, P->fail.p, s);
+      }
     }
-  DevBuf<double> de1, dF, dv1;
-  h2d(de1, e1, nb1, s);
-  h2d(dF, F, s);
-  PhelParams r{};
-  r.nq = nq;
-  r.nb1 = nb1;
-  r.nb2 = nw;
-  r.nm = nm;
-  r.nCalc = P->nCalc;
-  r.g2 = P->g2.p;
-  r.e1 = de1.p;
-  r.E2 = P->E2.p;
-  r.W3 = P->W3.p;
-  r.F1 = dF.p;
-  r.invT = P->invT.p;
-  r.norm = P->norm;
-  r.cutoff = phonon_cutoff();
-  r.invSigma = P->invSigma;
-  r.deltaPref = P->deltaPref;
-  r.adaptive = P->adaptive;
-  if (P->adaptive) {
-    h2d(dv1, &P->v1[3 * P->stateOff[a]], 3 * size_t(nb1), s);
-    r.v1 = dv1.p;
-    r.V2 = P->V2.p;
-    for (int i = 0; i < 9; ++i) r.qT[i] = P->qT[i];
-    r.minWidth = 0.0001 / units::energyRyToEv;
+    // calcCouplingSquared, store mode, every q3 in one launch
+    const int* pmk = P->pmeta.p + size_t(i) * blk;
+    CoupParams c{};
+    c.lda = H->Kp;
+    c.Kp = H->Kp;
+    c.ldb = H->ldb;
+    c.panelStride = P->pstride;
+    c.M = nq;
+    c.nm = nm;
+    c.nw = nw;
+    c.nPanels = nb1;
+    c.nMtiles = P->rowsPad / coupling_rows_per_tile_f64(H->ldb);
+    c.panelKK = pmk + 1;
+    c.panelB1 = pmk + 1 + P->nb1max;
+    c.kkNb1 = pmk;
+    c.U2pool = P->U2.p;
+    c.rowU2off = P->rowU2off.p;
+    c.rowNb2 = P->rowNb2.p;
+    c.E3pool = P->E3.p;
+    c.Wpool = P->W0.p;
+    c.qmode = 0;
+    c.rowE3 = P->rowE3.p;
+    c.out = P->g2.p;
+    c.rowOutOff = P->rowOutOff.p;
+    c.nb2max = nw;
+    const size_t panel0 = size_t(cm[2 * K + i]) * P->pstride;
+    if (f64) {
+      c.A = P->A.p;
+      c.B = P->Bd.p + panel0;
+      launch_coupling<double>(c, false, s);
+    } else {
+      c.A = P->Af.p;
+      c.B = P->Bf.p + panel0;
+      launch_coupling<float>(c, false, s);
+    }
+    PhelParams r{};
+    r.nq = nq;
+    r.nb1 = nb1;
+    r.nb2 = nw;
+    r.nm = nm;
+    r.nCalc = P->nCalc;
+    r.g2 = P->g2.p;
+    r.g2Stride = P->g2Stride;
+    r.e1 = P->stateEd.p + P->stateOff[a];
+    r.E2 = P->E2.p;
+    r.e2off = P->e2off.p;
+    r.W3 = P->W3.p;
+    r.F1 = P->F1.p + size_t(i) * kCMax * nw;
+    r.invT = P->invT.p;
+    r.norm = P->norm;
+    r.cutoff = phonon_cutoff();
+    r.invSigma = P->invSigma;
+    r.deltaPref = P->deltaPref;
+    r.adaptive = P->adaptive;
+    if (P->adaptive) {
+      r.v1 = P->v1d.p + 3 * P->stateOff[a];
+      r.V2 = P->V2.p;
+      for (int d = 0; d < 9; ++d) r.qT[d] = P->qT[d];
+      r.minWidth = 0.0001 / units::energyRyToEv;
+    }
+    r.lw = P->lw.p;
+    launch_phel_reduce(r, s);
+    P->pairs += nq;
   }
-  r.lw = P->lw.p;
-  launch_phel_reduce(r, s);
-  PHX_HIP(hipStreamSynchronize(s));
-  if (!P->adaptive) {
-    std::vector<int> inf(nq);
-    PHX_HIP(hipMemcpy(inf.data(), P->info.p, inf.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (int q = 0; q < nq; ++q) PHX_REQUIRE(inf[q] >= 0, "electron eigensolve did not converge");
+}
+
+// Outer points in batches of kCacheNKK; the eigensolver status of unfolded k2
+// points is checked once at the end.
+static void phel_run(PhelHandle* P, int nk1, const int* k1_list) {
+  std::vector<int> list;
+  if (nk1 < 0) {
+    list.resize(P->na);
+    std::iota(list.begin(), list.end(), 0);
+  } else {
+    PHX_REQUIRE(k1_list || nk1 == 0, "phx_phel_run: null k1 list");
+    for (int i = 0; i < nk1; ++i)
+      PHX_REQUIRE(k1_list[i] >= 0 && k1_list[i] < P->na, "phx_phel_run: k1 index out of range");
+    list.assign(k1_list, k1_list + nk1);
   }
-  P->pairs += nq;
+  for (size_t i = 0; i < list.size(); i += kCacheNKK)
+    phel_batch(P, list.data() + i, int(std::min<size_t>(kCacheNKK, list.size() - i)));
+  int bad = 0;
+  PHX_HIP(hipMemcpyAsync(&bad, P->fail.p, sizeof(int), hipMemcpyDeviceToHost, P->s));
+  PHX_HIP(hipStreamSynchronize(P->s));
+  PHX_REQUIRE(bad == 0, "electron eigensolve did not converge");
 }
 
 static void phel_get(PhelHandle* P, double* out, int average) {
@@ -2819,15 +2939,7 @@ long long phx_phel_run(phx_phel* h, int nk1, const int* k1_list) {
   int rc = guard([&] {
     PHX_REQUIRE(h, "phx_phel_run: null handle");
     before = h->pairs;
-    if (nk1 < 0) {
-      for (int a = 0; a < h->na; ++a) phel_point(h, a);
-    } else {
-      PHX_REQUIRE(k1_list || nk1 == 0, "phx_phel_run: null k1 list");
-      for (int i = 0; i < nk1; ++i) {
-        PHX_REQUIRE(k1_list[i] >= 0 && k1_list[i] < h->na, "phx_phel_run: k1 index out of range");
-        phel_point(h, k1_list[i]);
-      }
-    }
+    phel_run(h, nk1, k1_list);
   });
   return rc < 0 ? rc : h->pairs - before;
 }

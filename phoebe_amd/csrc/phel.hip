@@ -11,33 +11,19 @@
 // Every (q3, nu, c) entry is owned by one thread and gains one term per k1
 // launch, so the sums are deterministic (launch order = k1 order).
 #include "coupling_kernel.hpp"
+#include "phel.hpp"
 #include "phx_common.hpp"
 
 namespace phx {
 
-struct PhelParams {
-  int nq, nb1, nb2, nm, nCalc;
-  double* g2;             // nq blocks of nb1 * nb2 * nm, (b1, b2, nu) column-major
-  const double* e1;       // nb1 (the k1 bands)
-  const double* E2;       // nq x nb2 (all bands at k1 + q3)
-  const double* W3;       // nq x nm
-  const double* F1;       // nCalc x nb1: f (1 - f) of the k1 states
-  const double* invT;     // nCalc
-  double norm, cutoff, invSigma, deltaPref;
-  int adaptive;           // width from v1 - v2 (AdaptiveGaussianDeltaFunction)
-  const double* v1;       // nb1 x 3
-  const double* V2;       // nq x nb2 x 3
-  double qT[9], minWidth;
-  double* lw;             // nCalc x nq x nm
-};
 
 __global__ void __launch_bounds__(64) k_phel_sym(PhelParams p) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= p.nq) return;
   const int nb1 = p.nb1, nb2 = p.nb2, nm = p.nm;
-  double* c = p.g2 + size_t(q) * nb1 * nb2 * nm;
+  double* c = p.g2 + size_t(q) * p.g2Stride;
   auto C = [&](int a, int b, int v) -> double& { return c[a + size_t(nb1) * (b + size_t(nb2) * v)]; };
-  const double* e2 = p.E2 + size_t(q) * nb2;
+  const double* e2 = p.E2 + p.e2off[q];
   const double* w3 = p.W3 + size_t(q) * nm;
   for (int a = 0; a < nb1;) {
     int deg = 0;
@@ -113,8 +99,8 @@ __global__ void __launch_bounds__(256) k_phel_rates(PhelParams p) {
   const double w = p.W3[size_t(q) * p.nm + v];
   if (w < p.cutoff) return;  // phel_scattering.cpp:266-270
   const int nb1 = p.nb1, nb2 = p.nb2;
-  const double* c = p.g2 + size_t(q) * nb1 * nb2 * p.nm;
-  const double* e2 = p.E2 + size_t(q) * nb2;
+  const double* c = p.g2 + size_t(q) * p.g2Stride;
+  const double* e2 = p.E2 + p.e2off[q];
   double acc[kCMax];
   for (int k = 0; k < kCMax; ++k) acc[k] = 0.;
   for (int a = 0; a < nb1; ++a)
@@ -127,6 +113,18 @@ __global__ void __launch_bounds__(256) k_phel_rates(PhelParams p) {
     }
   for (int k = 0; k < kCMax; ++k)
     if (k < p.nCalc) p.lw[(size_t(k) * p.nq + q) * p.nm + v] += acc[k];
+}
+
+// flag[0] = 1 when any eigensolve status in info[0..n) is negative
+__global__ void __launch_bounds__(256) k_phel_flag_failures(int n, const int* info, int* flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && info[i] < 0) flag[0] = 1;
+}
+
+void launch_phel_flag_failures(int n, const int* info, int* flag, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_phel_flag_failures, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, info, flag);
+  PHX_HIP(hipGetLastError());
 }
 
 void launch_phel_reduce(const PhelParams& p, hipStream_t s) {

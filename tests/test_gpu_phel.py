@@ -30,14 +30,17 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("springs", [(8.0, 20.0, 45.0), (20.0, 20.0, 45.0)])
-def test_phel_toy_closed_form(springs):
-    say("ph-el toy", springs)
-    p, qm = toy(springs)
+@pytest.mark.parametrize("springs,q_mesh", [((8.0, 20.0, 45.0), (3, 3, 3)), ((20.0, 20.0, 45.0), (3, 3, 3)),
+                                            ((8.0, 20.0, 45.0), (2, 2, 2)), ((20.0, 20.0, 45.0), (4, 4, 4))])
+def test_phel_toy_closed_form(springs, q_mesh):
+    """q meshes that divide the 4^3 k mesh take k2 = k1 + q3 from the k-mesh
+    eigenpairs; 3^3 diagonalizes every k2 = k1 + q3."""
+    say("ph-el toy", springs, q_mesh)
+    p, qm = toy(springs, q_mesh=q_mesh)
     inter = P.InteractionElPhWan(p.gw_host(), p.Re, p.degE, p.Rp, p.degP, "fp64")
     ph = P.PhononElectronLinewidths(p, inter, qm)
     assert ph.num_active == 64
-    assert ph.run() == 64 * 27
+    assert ph.run() == 64 * int(np.prod(qm))
     got = ph.linewidths()
     ref = p.expected_phel(qm)
     err = np.abs(got - ref).max() / np.abs(ref).max()
@@ -60,15 +63,18 @@ def test_phel_partial_runs_accumulate():
         b.run([64])
 
 
-@pytest.mark.parametrize("smearing,precision,tol",
-                         [("gaussian", "fp64", 1e-10), ("adaptive", "fp64", 1e-7),
-                          ("gaussian", "fp32", 1e-4)])
-def test_phel_si_vs_oracle(smearing, precision, tol):
+@pytest.mark.parametrize("smearing,precision,tol,qm",
+                         [("gaussian", "fp64", 1e-10, (3, 3, 3)), ("adaptive", "fp64", 1e-7, (3, 3, 3)),
+                          ("gaussian", "fp32", 1e-4, (3, 3, 3)), ("gaussian", "fp64", 1e-10, (2, 2, 2)),
+                          ("gaussian", "fp64", 1e-10, (4, 4, 4)), ("adaptive", "fp64", 1e-7, (2, 2, 2))])
+def test_phel_si_vs_oracle(smearing, precision, tol, qm):
     """Si from the reference's files: 444_silicon.fc (no sum rule: long-range
-    term on), si_tb.dat, silicon.phoebe.elph.hdf5; k mesh 4^3, q mesh 3^3.
-    Adaptive smearing inherits the 1e-8 relative noise of the finite-difference
-    velocities; FP32 couplings carry FP32 rounding."""
-    say("ph-el Si", smearing, precision)
+    term on), si_tb.dat, silicon.phoebe.elph.hdf5; k mesh 4^3.  The oracle
+    diagonalizes every k2 = k1 + q3; the GPU takes them from the k mesh when
+    the q mesh divides it (Gaussian smearing).  Adaptive smearing inherits the
+    1e-8 relative noise of the finite-difference velocities; FP32 couplings
+    carry FP32 rounding."""
+    say("ph-el Si", smearing, precision, qm)
     ph = Hm.PhononH0(FC4, "")
     el = Hm.ElectronH0Wannier(TB, ph.cell)
     pts = np.array([[i / 4, j / 4, k / 4] for k in range(4) for j in range(4) for i in range(4)])
@@ -80,7 +86,6 @@ def test_phel_si_vs_oracle(smearing, precision, tol):
                                 smearing=smearing)
     gW = H5.read_elph(ELPH)["gW"]
     orc = O.Oracle(prob, gW.reshape(-1))
-    qm = (3, 3, 3)
     ref = orc.phel_linewidths(qm)
     inter = P.InteractionElPhWan.parse(ELPH, precision=precision)
     lw = P.PhononElectronLinewidths(prob, inter, qm)
