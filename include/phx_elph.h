@@ -447,6 +447,13 @@ int phx_phel_get_linewidths(phx_phel* h, double* out, int average);
 int phx_phel_get_phonons(phx_phel* h, double* omega);
 int phx_phel_reset(phx_phel* h);
 int phx_phel_destroy(phx_phel* h);
+/* Kernel timing of phx_phel_run with HIP events on the coupling's stream
+ * (one host synchronization per batch of 32 outer points while on); switching
+ * it on or off clears the totals.  No counterpart in the reference. */
+int phx_phel_set_timing(phx_phel* h, int on);
+/* out6 = {cacheElPh ms, coupling ms, symmetrize + rates ms, coupling launches,
+ * cacheElPh launches, R_ph-transform FLOPs of the coupling launches}. */
+int phx_phel_kernel_times(phx_phel* h, double* out6);
 
 /* Host twin of the synthetic gW generator (bit-identical to the device). */
 int phx_synth_gw_host(double* gW, int nw, int nm, int nRp, int nRe, const double* decay,

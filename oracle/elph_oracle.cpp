@@ -25,6 +25,7 @@
 // gauge invariance, k -> k+G invariance); Gaussian/Fermi/Bose closed forms;
 // linewidth reduction against hand-evaluable single-band/single-mode cases.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -1040,7 +1041,8 @@ int orc_num_threads() {
 // norm = 1 / prod(kMesh), then degeneracy averaging over the phonon modes
 // (row iCalc = 0, scattering.cpp:1168-1207).  out: nCalc x nq x nm.
 // Returns the number of (k1, q3) pairs, or -1.
-long long orc_phel(const orc_problem* prob, const int* qmesh, int average, double* out) {
+long long orc_phel(const orc_problem* prob, const int* qmesh, int nk1, const int* k1_list, int average,
+                   double* out, double* loop_seconds) {
   const orc_problem& p = *prob;
   const int nw = p.nw, nm = p.nm;
   const int nk = p.mesh[0] * p.mesh[1] * p.mesh[2];
@@ -1050,6 +1052,7 @@ long long orc_phel(const orc_problem* prob, const int* qmesh, int average, doubl
   // electrons on the k mesh, window
   std::vector<double> kcart(3 * size_t(nk)), allE(size_t(nk) * nw), allU(2 * size_t(nk) * nw * nw);
   int fail = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : fail)
   for (int ik = 0; ik < nk; ++ik) {
     double x[3];
     orc_mesh_point(ik, p.mesh, x);
@@ -1077,6 +1080,7 @@ long long orc_phel(const orc_problem* prob, const int* qmesh, int average, doubl
   std::vector<double> qcart(3 * size_t(nq)), w(size_t(nq) * nm), ev(2 * size_t(nq) * nm * nm);
   const orc_long_range lr{p.natoms, p.nG, p.atomPos, p.bornCharges, p.dielectric, p.gVectors,
                           p.lrConstant, p.lrNorm, p.gMax};
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : fail)
   for (int iq = 0; iq < nq; ++iq) {
     double x[3];
     orc_mesh_point(iq, qmesh, x);
@@ -1088,7 +1092,18 @@ long long orc_phel(const orc_problem* prob, const int* qmesh, int average, doubl
   std::vector<double> lw(size_t(p.nCalc) * nq * nm, 0.);
   long long pairs = 0;
   std::vector<double> cached(2 * size_t(p.nRp) * nm * nw * nw);
-  for (size_t a = 0; a < active.size(); ++a) {
+  // outer points: k1_list (active indices) or all of them
+  std::vector<int> list;
+  if (nk1 < 0) {
+    for (size_t a = 0; a < active.size(); ++a) list.push_back(int(a));
+  } else {
+    for (int i = 0; i < nk1; ++i) {
+      if (k1_list[i] < 0 || size_t(k1_list[i]) >= active.size()) return -1;
+      list.push_back(k1_list[i]);
+    }
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int a : list) {
     const int ik1 = active[a], nb1 = hi[a] - lo[a] + 1;
     const double* k1 = &kcart[3 * size_t(ik1)];
     std::vector<double> e1(nb1), U1(2 * size_t(nw) * nb1), v1(3 * size_t(nb1), 0.);
@@ -1164,6 +1179,8 @@ long long orc_phel(const orc_problem* prob, const int* qmesh, int average, doubl
     }
     pairs += nq;
   }
+  if (loop_seconds)
+    *loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (average) {
     for (int q = 0; q < nq; ++q) {
       const double* w3 = &w[size_t(q) * nm];

@@ -73,7 +73,7 @@ def lib():
         L.orc_adaptive_delta.argtypes = [C.c_double, _dp, _dp, _ip]
         L.orc_el_group_velocities.restype = C.c_int
         L.orc_phel.restype = C.c_longlong
-        L.orc_phel.argtypes = [C.POINTER(OrcProblem), _ip, C.c_int, _dp]
+        L.orc_phel.argtypes = [C.POINTER(OrcProblem), _ip, C.c_int, _ip, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -260,16 +260,26 @@ class Oracle:
             lib().orc_free(self.h)
             self.h = None
 
-    def phel_linewidths(self, q_mesh, average=True):
+    def phel_linewidths(self, q_mesh, average=True, k1_list=None):
         """Phonon linewidths (nCalc, nq, nm) of PhElScatteringMatrix::builder
-        on the q mesh, from this problem's electrons, phonons and coupling."""
+        on the q mesh, from this problem's electrons, phonons and coupling;
+        k1_list: outer points (active indices), all when None.  The outer
+        loop's wall time is left in self.phel_seconds."""
         nq = int(np.prod(q_mesh))
         out = np.zeros(len(self.prob.temperatures) * nq * self.prob.nm)
         qm = np.ascontiguousarray(q_mesh, dtype=np.int32)
-        pairs = lib().orc_phel(C.byref(self._s), qm.ctypes.data_as(_ip), int(bool(average)), _p(out))
+        sec = np.zeros(1)
+        if k1_list is None:
+            n, kl = -1, None
+        else:
+            k = np.ascontiguousarray(k1_list, dtype=np.int32)
+            n, kl = len(k), k.ctypes.data_as(_ip)
+        pairs = lib().orc_phel(C.byref(self._s), qm.ctypes.data_as(_ip), n, kl, int(bool(average)), _p(out),
+                               _p(sec))
         if pairs < 0:
             raise RuntimeError("oracle ph-el failed")
         self.phel_pairs = pairs
+        self.phel_seconds = float(sec[0])
         return out.reshape(len(self.prob.temperatures), nq, self.prob.nm)
 
     def eigvecs(self, a):

@@ -146,6 +146,8 @@ EXPORTS = {
     "phx_phel_get_phonons": (C.c_int, [C.c_void_p, _dp]),
     "phx_phel_reset": (C.c_int, [C.c_void_p]),
     "phx_phel_destroy": (C.c_int, [C.c_void_p]),
+    "phx_phel_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_phel_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_el_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
 
@@ -692,3 +694,14 @@ class PhononElectronLinewidths:
 
     def reset(self):
         _check(lib().phx_phel_reset(self._h))
+
+    def set_timing(self, on=True):
+        _check(lib().phx_phel_set_timing(self._h, int(bool(on))))
+
+    def kernel_times(self):
+        """Totals since timing was switched on: cacheElPh / coupling /
+        symmetrize + rates ms, launch counts, coupling transform FLOPs."""
+        o = np.zeros(6)
+        _check(lib().phx_phel_kernel_times(self._h, _p(o)))
+        return dict(cache_ms=o[0], coupling_ms=o[1], reduce_ms=o[2], coupling_launches=int(o[3]),
+                    cache_launches=int(o[4]), coupling_flops=o[5])

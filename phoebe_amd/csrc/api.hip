@@ -1961,6 +1961,14 @@ struct PhelHandle {
   int rowsPad = 0;
   size_t pstride = 0, g2Stride = 0;
   long long pairs = 0;
+  // kernel timing: 2 events for the batch's cacheElPh + 3 per outer point
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  double cacheMs = 0., coupMs = 0., redMs = 0., coupFlops = 0.;
+  long long coupLaunches = 0, cacheLaunches = 0;
+  ~PhelHandle() {
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+  }
 };
 
 static void phel_setup(PhelHandle* P, const phx_problem& p, ElphHandle* H, const int* qmesh) {
@@ -2191,6 +2199,13 @@ static void phel_batch(PhelHandle* P, const int* as, int n) {
   hipStream_t s = P->s;
   const int nw = P->nw, nm = P->nm, nq = P->nq, K = kCacheNKK, blk = 1 + 2 * P->nb1max;
   const bool f64 = H->precision == PHX_FP64;
+  if (P->timing && P->tev.empty()) {
+    P->tev.resize(2 + 3 * size_t(K));
+    for (hipEvent_t& e : P->tev) PHX_HIP(hipEventCreate(&e));
+  }
+  auto rec = [&](int j) {
+    if (P->timing) PHX_HIP(hipEventRecord(P->tev[j], s));
+  };
   // ---- batched cacheElPh
   std::vector<double> kb(3 * size_t(n));
   std::vector<int> cm(3 * size_t(K), 0), pm(size_t(K) * blk, 0);
@@ -2218,6 +2233,7 @@ static void phel_batch(PhelHandle* P, const int* as, int n) {
   PHX_HIP(hipMemcpyAsync(P->cmeta.p, cm.data(), cm.size() * sizeof(int), hipMemcpyHostToDevice, s));
   PHX_HIP(hipMemcpyAsync(P->pmeta.p, pm.data(), pm.size() * sizeof(int), hipMemcpyHostToDevice, s));
   PHX_HIP(hipMemcpyAsync(P->F1.p, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  rec(0);
   launch_phase_table(n, H->nRe, P->kb.p, H->Re.p, P->invE.p, 1.0, P->phE.p, s);
   CacheParams cp{};
   cp.nRe = H->nRe;
@@ -2241,6 +2257,7 @@ static void phel_batch(PhelHandle* P, const int* as, int n) {
     cp.B = P->Bf.p;
     launch_cache<float, float>(cp, s);
   }
+  rec(1);
   // ---- every outer point against all q3
   std::vector<double> k2(3 * size_t(nq));
   std::vector<int> u2off, e2o;
@@ -2314,6 +2331,7 @@ static void phel_batch(PhelHandle* P, const int* as, int n) {
     c.rowOutOff = P->rowOutOff.p;
     c.nb2max = nw;
     const size_t panel0 = size_t(cm[2 * K + i]) * P->pstride;
+    rec(2 + 3 * i);
     if (f64) {
       c.A = P->A.p;
       c.B = P->Bd.p + panel0;
@@ -2323,6 +2341,7 @@ static void phel_batch(PhelHandle* P, const int* as, int n) {
       c.B = P->Bf.p + panel0;
       launch_coupling<float>(c, false, s);
     }
+    rec(3 + 3 * i);
     PhelParams r{};
     r.nq = nq;
     r.nb1 = nb1;
@@ -2350,7 +2369,25 @@ static void phel_batch(PhelHandle* P, const int* as, int n) {
     }
     r.lw = P->lw.p;
     launch_phel_reduce(r, s);
+    rec(4 + 3 * i);
     P->pairs += nq;
+  }
+  if (P->timing) {
+    PHX_HIP(hipEventSynchronize(P->tev[1 + 3 * size_t(n)]));
+    auto ms = [&](int a, int b) {
+      float t = 0.f;
+      PHX_HIP(hipEventElapsedTime(&t, P->tev[a], P->tev[b]));
+      return double(t);
+    };
+    P->cacheMs += ms(0, 1);
+    P->cacheLaunches += 1;
+    for (int i = 0; i < n; ++i) {
+      P->coupMs += ms(2 + 3 * i, 3 + 3 * i);
+      P->redMs += ms(3 + 3 * i, 4 + 3 * i);
+      // R_ph transform of the coupling: real GEMM nq x 2 nRp x 2 nm nw per panel
+      P->coupFlops += 8. * nq * H->nRp * nm * nw * P->nb[as[i]];
+    }
+    P->coupLaunches += n;
   }
 }
 
@@ -2933,6 +2970,27 @@ int phx_phel_setup(phx_phel** h, const phx_problem* prob, phx_elph* coupling, co
 }
 
 int phx_phel_num_active(phx_phel* h) { return h ? h->na : -1; }
+
+int phx_phel_set_timing(phx_phel* h, int on) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_phel_set_timing: null handle");
+    h->timing = on != 0;
+    h->cacheMs = h->coupMs = h->redMs = h->coupFlops = 0.;
+    h->coupLaunches = h->cacheLaunches = 0;
+  });
+}
+
+int phx_phel_kernel_times(phx_phel* h, double* out6) {
+  return guard([&] {
+    PHX_REQUIRE(h && out6, "phx_phel_kernel_times: null argument");
+    out6[0] = h->cacheMs;
+    out6[1] = h->coupMs;
+    out6[2] = h->redMs;
+    out6[3] = double(h->coupLaunches);
+    out6[4] = double(h->cacheLaunches);
+    out6[5] = h->coupFlops;
+  });
+}
 
 long long phx_phel_run(phx_phel* h, int nk1, const int* k1_list) {
   long long before = 0;
