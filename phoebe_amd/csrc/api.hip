@@ -121,6 +121,16 @@ template <typename T>
 static void h2d(DevBuf<T>& d, const std::vector<T>& v, hipStream_t s = 0) {
   h2d(d, v.data(), v.size(), s);
 }
+// Same into a grow-only buffer: reallocates only when n exceeds its capacity.
+template <typename T>
+static void h2d_grow(DevBuf<T>& d, const T* h, size_t n, hipStream_t s) {
+  d.reserve(n);
+  if (n) PHX_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+static void h2d_grow(DevBuf<T>& d, const std::vector<T>& v, hipStream_t s) {
+  h2d_grow(d, v.data(), v.size(), s);
+}
 
 // Device copy of the long-range term's data (LongRange holds its pointers).
 struct DevLongRange {
@@ -469,6 +479,15 @@ struct ElphHandle {
   DevBuf<float> Bf;
   DevBuf<double2> U1;
   DevBuf<int> meta;  // U1off(1), nb1(1), panelBase(1), panelKK(nb1), panelB1(nb1), kkNb1(1)
+  // per-call API (cacheElPh / calcCouplingSquared): constant tables made on the
+  // first call, grow-only scratch reused by later calls
+  DevBuf<double> invE, W0;          // 1/deg(R_e); zero phonon energies (store mode)
+  DevBuf<double2> phP1;             // 1/deg(R_p) as complex
+  DevBuf<double> sK, sQ, sOut, sA;  // k1, q3, |g|^2, FP64 phase rows
+  DevBuf<float> sAf;                // FP32 phase rows
+  DevBuf<double2> sPhE, sU2, sE3;
+  DevBuf<int> sU2off, sNb2, sE3idx;
+  DevBuf<long long> sOutOff;
   ~ElphHandle() {
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -511,7 +530,7 @@ static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k
     }
   }
   H->nb1 = nb1;
-  h2d(H->U1, reinterpret_cast<const double2*>(U1), size_t(H->nw) * nb1, s);
+  h2d_grow(H->U1, reinterpret_cast<const double2*>(U1), size_t(H->nw) * nb1, s);
   std::vector<int> meta(4 + 2 * nb1, 0);
   meta[0] = 0;    // U1off
   meta[1] = nb1;  // nb1 / kkNb1
@@ -521,26 +540,24 @@ static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k
     meta[4 + b] = 0;          // panelKK
     meta[4 + nb1 + b] = b;    // panelB1
   }
-  h2d(H->meta, meta, s);
-  DevBuf<double> dk;
-  h2d(dk, k1, 3, s);
-  DevBuf<double2> phE(size_t(H->nRe)), phP(size_t(H->nRp));
-  std::vector<double> invdP(H->nRp);
-  DevBuf<double> dInv;
+  h2d_grow(H->meta, meta, s);
+  h2d_grow(H->sK, k1, 3, s);
   // phE = e^{ik1.Re}/de ; phP = 1/dp (explicit-q mode: e^{iq.Rp} sits in the A rows)
   {
-    std::vector<double> invE(H->nRe);
-    std::vector<double> hDegE(H->nRe), hDegP(H->nRp);
-    PHX_HIP(hipMemcpyAsync(hDegE.data(), H->degE.p, H->degE.bytes(), hipMemcpyDeviceToHost, s));
-    PHX_HIP(hipMemcpyAsync(hDegP.data(), H->degP.p, H->degP.bytes(), hipMemcpyDeviceToHost, s));
-    PHX_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < H->nRe; ++i) invE[i] = 1. / hDegE[i];
-    DevBuf<double> dInvE;
-    h2d(dInvE, invE, s);
-    launch_phase_table(1, H->nRe, dk.p, H->Re.p, dInvE.p, 1.0, phE.p, s);
-    std::vector<double2> pp(H->nRp);
-    for (int i = 0; i < H->nRp; ++i) pp[i] = make_double2(1. / hDegP[i], 0.);
-    PHX_HIP(hipMemcpyAsync(phP.p, pp.data(), pp.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    if (!H->invE.p) {
+      std::vector<double> hDegE(H->nRe), hDegP(H->nRp);
+      PHX_HIP(hipMemcpyAsync(hDegE.data(), H->degE.p, H->degE.bytes(), hipMemcpyDeviceToHost, s));
+      PHX_HIP(hipMemcpyAsync(hDegP.data(), H->degP.p, H->degP.bytes(), hipMemcpyDeviceToHost, s));
+      PHX_HIP(hipStreamSynchronize(s));
+      std::vector<double> invE(H->nRe);
+      for (int i = 0; i < H->nRe; ++i) invE[i] = 1. / hDegE[i];
+      h2d(H->invE, invE, s);
+      std::vector<double2> pp(H->nRp);
+      for (int i = 0; i < H->nRp; ++i) pp[i] = make_double2(1. / hDegP[i], 0.);
+      h2d(H->phP1, pp, s);
+    }
+    H->sPhE.reserve(size_t(H->nRe));
+    launch_phase_table(1, H->nRe, H->sK.p, H->Re.p, H->invE.p, 1.0, H->sPhE.p, s);
     CacheParams cp{};
     cp.nRe = H->nRe;
     cp.nRp = H->nRp;
@@ -548,8 +565,8 @@ static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k
     cp.nw = H->nw;
     cp.nkk = 1;
     cp.gW = H->gw_ptr();
-    cp.phE = phE.p;
-    cp.phP = phP.p;
+    cp.phE = H->sPhE.p;
+    cp.phP = H->phP1.p;
     cp.U1pool = H->U1.p;
     cp.U1off = H->meta.p + 0;
     cp.nb1 = H->meta.p + 1;
@@ -590,19 +607,18 @@ static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int
     totOut += (long long)nb1 * nb2s[i] * nm;
     nb2max = std::max(nb2max, nb2s[i]);
   }
-  DevBuf<double> dq;
-  h2d(dq, q3, size_t(nq) * 3, s);
-  DevBuf<double2> dU2, dE3;
-  h2d(dU2, reinterpret_cast<const double2*>(U2s), size_t(totU2), s);
-  h2d(dE3, reinterpret_cast<const double2*>(e3s), size_t(nq) * nm * nm, s);
-  DevBuf<int> dU2off, dNb2, dE3idx;
-  h2d(dU2off, u2off, s);
-  h2d(dNb2, nb2v, s);
-  h2d(dE3idx, e3idx, s);
-  DevBuf<long long> dOutOff;
-  h2d(dOutOff, outoff, s);
-  DevBuf<double> dOut{size_t(totOut)};
-  DevBuf<double> W0(nm);
+  h2d_grow(H->sQ, q3, size_t(nq) * 3, s);
+  h2d_grow(H->sU2, reinterpret_cast<const double2*>(U2s), size_t(totU2), s);
+  h2d_grow(H->sE3, reinterpret_cast<const double2*>(e3s), size_t(nq) * nm * nm, s);
+  h2d_grow(H->sU2off, u2off, s);
+  h2d_grow(H->sNb2, nb2v, s);
+  h2d_grow(H->sE3idx, e3idx, s);
+  h2d_grow(H->sOutOff, outoff, s);
+  H->sOut.reserve(size_t(totOut));
+  if (!H->W0.p) {
+    H->W0.alloc(size_t(nm));
+    launch_fill<double>(H->W0.p, H->W0.n, 0., s);
+  }
   CoupParams p{};
   p.lda = H->Kp;
   p.Kp = H->Kp;
@@ -615,33 +631,32 @@ static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int
   p.nMtiles = nMtiles;
   p.panelKK = H->meta.p + 4;
   p.panelB1 = H->meta.p + 4 + nb1;
-  p.U2pool = dU2.p;
-  p.rowU2off = dU2off.p;
-  p.rowNb2 = dNb2.p;
-  p.E3pool = dE3.p;
-  p.Wpool = W0.p;
+  p.U2pool = H->sU2.p;
+  p.rowU2off = H->sU2off.p;
+  p.rowNb2 = H->sNb2.p;
+  p.E3pool = H->sE3.p;
+  p.Wpool = H->W0.p;
   p.qmode = 0;
-  p.rowE3 = dE3idx.p;
-  p.out = dOut.p;
-  p.rowOutOff = dOutOff.p;
+  p.rowE3 = H->sE3idx.p;
+  p.out = H->sOut.p;
+  p.rowOutOff = H->sOutOff.p;
   p.kkNb1 = H->meta.p + 1;
   p.nb2max = nb2max;
   if (H->precision == PHX_FP64) {
-    DevBuf<double> A(size_t(rowsPad) * H->Kp);
-    launch_phase_rows<double>(nq, rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, A.p, s);
-    p.A = A.p;
+    H->sA.reserve(size_t(rowsPad) * H->Kp);
+    launch_phase_rows<double>(nq, rowsPad, H->nRp, H->Kp, H->sQ.p, H->Rp.p, H->sA.p, s);
+    p.A = H->sA.p;
     p.B = H->Bd.p;
     launch_coupling<double>(p, false, s);
-    PHX_HIP(hipStreamSynchronize(s));
   } else {
-    DevBuf<float> A(size_t(rowsPad) * H->Kp);
-    launch_phase_rows<float>(nq, rowsPad, H->nRp, H->Kp, dq.p, H->Rp.p, A.p, s);
-    p.A = A.p;
+    H->sAf.reserve(size_t(rowsPad) * H->Kp);
+    launch_phase_rows<float>(nq, rowsPad, H->nRp, H->Kp, H->sQ.p, H->Rp.p, H->sAf.p, s);
+    p.A = H->sAf.p;
     p.B = H->Bf.p;
     launch_coupling<float>(p, false, s);
-    PHX_HIP(hipStreamSynchronize(s));
   }
-  PHX_HIP(hipMemcpy(out, dOut.p, dOut.bytes(), hipMemcpyDeviceToHost));
+  PHX_HIP(hipMemcpyAsync(out, H->sOut.p, size_t(totOut) * sizeof(double), hipMemcpyDeviceToHost, s));
+  PHX_HIP(hipStreamSynchronize(s));
 }
 
 // ============================================================ linewidth builder

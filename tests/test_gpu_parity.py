@@ -106,6 +106,36 @@ def test_coupling_squared_vs_oracle(precision, tol):
     assert worst < tol
 
 
+def test_coupling_api_repeated_calls_of_changing_size():
+    """The per-call API keeps its device buffers between calls: outer points
+    with 3, 1 and 2 bands, against 40, 5 and 61 inner points, each checked
+    against the oracle."""
+    say("calcCouplingSquared repeated calls")
+    p = tiny_problem(natoms=2, nw=4, rg_e=3, rg_p=3, mesh=4, seed=9)
+    gW = p.gw_host()
+    rng = np.random.default_rng(18)
+    inter = P.InteractionElPhWan(gW, p.Re, p.degE, p.Rp, p.degP, "fp64")
+    for nb1, nq in ((3, 40), (1, 5), (2, 61)):
+        k1 = p.recip @ rng.uniform(0, 1, 3)
+        _, U1 = O.el_diag(p, k1)
+        U1 = U1[:, :nb1]
+        inter.cacheElPh(U1, k1)
+        q3s, U2s, e3s = [], [], []
+        for j in range(nq):
+            q = p.recip @ rng.uniform(-1, 1, 3)
+            _, U2 = O.el_diag(p, k1 + q)
+            U2s.append(U2[:, :1 + j % p.nw])
+            e3s.append(O.ph_diag(p, q)[1])
+            q3s.append(q)
+        inter.calcCouplingSquared(U1, U2s, e3s, q3s)
+        cached = O.cache_elph(p.nw, p.nm, p.Rp, p.Re, p.degE, gW, U1, k1)
+        ref = O.coupling_sq(p.nw, p.nm, p.Rp, p.degP, cached, U2s, e3s, q3s)
+        worst = max(np.abs(inter.getCouplingSquared(j) - ref[j]).max() / np.abs(ref[j]).max()
+                    for j in range(nq))
+        say("  nb1", nb1, "nq", nq, "max err", worst)
+        assert worst < 1e-10
+
+
 @pytest.mark.parametrize("transform", ["direct", "separable"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
 def test_linewidths_vs_oracle(precision, tol, transform):
