@@ -1121,6 +1121,11 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
     if (REDUCE && p.slotZero && t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
     return;
   }
+  // no row of the line passes the energy screen: zero partials, before any slab load
+  if (p.keepMask && p.keepMask[slot] == 0ull) {
+    if (REDUCE && t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
+    return;
+  }
   const int nm = p.nm, nw = p.nw, ncol = nm * nw;
   const int NWP = 2 * nw + 1, LDG = nm * NWP;  // g3 row layout: (nu, 2 w2 + ri), padded per nu
   const int kk = p.panelKK[panel];
