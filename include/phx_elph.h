@@ -364,6 +364,16 @@ int phx_lw_set_line_batching(phx_lw* h, int min_points);
  * Its two stage buffers hold one chunk: (lines * mz + groups * my * mz) *
  * nm * nw complex values per panel.  No counterpart in the reference. */
 int phx_lw_set_panel_chunk(phx_lw* h, int panels);
+/* Last stage of the separable transform as two kernels (on, the default) or
+ * one fused kernel (off).  Split: an x-stage kernel writes the kept rows' g3
+ * to device memory, (panel chunk) x (y, z lines) x m0 x nm nw complex FP32,
+ * and an epilogue kernel reads them; both use far fewer registers and LDS
+ * than the fused kernel, so more workgroups hide the memory latency.  Applies
+ * to PHX_FP32 with nm, nw <= 16, nm nw <= 192, an x mesh <= 64, at most 24
+ * x-groups and no learned screen; elsewhere the fused kernel runs.  The g3
+ * values are the same either way and the linewidths agree to FP64 rounding.
+ * No counterpart in the reference. */
+int phx_lw_set_split_last_stage(phx_lw* h, int on);
 /* ---- Learned q-block screen (separable transform with the energy screen on).
  * No counterpart in the reference, which evaluates every (k1, k2) pair
  * (el_scattering.cpp:161-386).  A q-block is one panel (k1, b1) times one

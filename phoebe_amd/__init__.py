@@ -147,6 +147,7 @@ EXPORTS = {
     "phx_phel_reset": (C.c_int, [C.c_void_p]),
     "phx_phel_destroy": (C.c_int, [C.c_void_p]),
     "phx_phel_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_set_split_last_stage": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_phel_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_el_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
@@ -560,6 +561,10 @@ class ElectronLinewidths:
     def set_panel_chunk(self, panels):
         """Panels per separable-transform launch (bounds its stage buffers)."""
         _check(lib().phx_lw_set_panel_chunk(self._h, int(panels)))
+
+    def set_split_last_stage(self, on=True):
+        """Separable last stage as two kernels (default) or the fused one."""
+        _check(lib().phx_lw_set_split_last_stage(self._h, int(bool(on))))
 
     # ---- learned q-block screen
     def screen_collect(self, k1_list=None, sample_frac=0.02, seed=1):

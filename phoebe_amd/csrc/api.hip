@@ -733,6 +733,8 @@ struct LwHandle {
   // batched together when the line holds at least lineMin of them (0 = off)
   int lineMin = 0;
   int panelChunk = 48;     // panels per separable-transform launch (bounds T1 / T2)
+  int splitLast = 1;       // last stage as k_sep_xg + k_sep_ep where they apply
+  DevBuf<float2> G3f;      // their g3 rows of one panel chunk, (slot, jx, col)
   int linesPerBatch = kLinesPerBatch;  // mesh lines per line batch (1 .. kLinesPerBatch)
   bool reLattice = false;  // every R_e is a lattice vector (required by the line form)
   int nGe = 0;
@@ -786,6 +788,15 @@ struct LwHandle {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+// Whether the separable transform's last stage runs split (x-stage and
+// epilogue as two kernels, g3 rows through HBM): FP32 data, nm, nw <= 16,
+// nm nw <= 192, row masks (x mesh <= 64); the launcher also needs <= 24
+// x-groups and no learned screen, and falls back to the fused kernel otherwise.
+static bool lw_split_last(const LwHandle* L, const phx_problem& p) {
+  return L->splitLast && L->transform == PHX_TRANSFORM_SEPARABLE && L->precision == PHX_FP32 && p.nm <= 16 &&
+         p.nw <= 16 && p.nm * p.nw <= 192 && p.mesh[0] <= 64;
+}
 
 // Fold the events of the last timed batch into kernelMs (stream must be idle).
 static void lw_harvest_timing(LwHandle* L) {
@@ -1528,6 +1539,7 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
     }
     const size_t slots = size_t(chunk) * nMtiles;
     if (p.mesh[0] <= 64) L->keepMask.reserve(slots);
+    if (lw_split_last(L, p)) L->G3f.reserve(slots * p.mesh[0] * ncol);
     if (L->lsMode && L->mlpTrained) {
       L->lsFeat.reserve(slots * kMlpIn);
       L->lsNscr.reserve(slots);
@@ -1734,6 +1746,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     cq.bf16x3 = L->precision == PHX_BF16X3;
     cq.meshRow = L->meshRow.p;
     if (p.mesh[0] <= 64) cq.keepMask = L->keepMask.p;
+    if (lw_split_last(L, p)) cq.G3 = L->G3f.p;
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
     if (L->precision == PHX_FP64) {
@@ -3186,6 +3199,13 @@ int phx_lw_set_line_batching(phx_lw* h, int min_points) {
     PHX_REQUIRE(min_points == 0 || h->reLattice,
                 "phx_lw_set_line_batching: the electronic R vectors are not lattice vectors");
     h->lineMin = min_points;
+  });
+}
+
+int phx_lw_set_split_last_stage(phx_lw* h, int on) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_lw_set_split_last_stage: null handle");
+    h->splitLast = on != 0;
   });
 }
 

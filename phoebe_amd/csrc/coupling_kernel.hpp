@@ -84,6 +84,8 @@ struct CoupParams {
   const int* qModeGrp;
   // separable transform: phonon eigenvectors (q, nu, nu2) in the transform's precision
   const void* E3T;
+  // split last stage (k_sep_xg + k_sep_ep): g3 rows of the chunk, (slot, jx, col) complex T
+  void* G3;
 };
 
 template <typename T> struct Vec2;
@@ -1093,6 +1095,137 @@ __device__ inline void bf16_split4(const float (&x)[4], bf16x4_t& hi, bf16x4_t& 
   }
 }
 
+// Second half of an epilogue pass of k_sep_ep (k_sep_xm keeps the same steps
+// inline, where the compiler allocates them without spills).  Once the
+// wave's g'(nu, b2) rows are in gp (row rl at gp + rl * gps, (nu, b2) at
+// nu * nb2max + b2): the phonon rotation, |g|^2, symmetrizeCoupling along b2
+// and nu, and the Gaussian-delta weights into part (or the store of |g|^2).
+// The thread serves row r (rl within the pass) and mode sub; the row data
+// (e2, mode groups, occupations) of the 16-row tile and the b1 group's
+// energies se1 are in LDS.  Wave-level ordering only.
+template <typename T, bool REDUCE, int NMX, int CM>
+__device__ __forceinline__ void sep_epilogue_tail(const CoupParams& p, const SepXmRow& ri, int r, int rl, int sub,
+                                                  int nb2, bool modeOn, double w3, double2* gp, int gps,
+                                                  const double* se2, const int* sg3, const double* sf2,
+                                                  const double* se1, const typename Vec2<T>::type (&e3c)[NMX],
+                                                  const double (&n3)[CM], double (&part)[kGMax][CM],
+                                                  const double* e1, int kk, int b1p, int gsize, double inv_g) {
+  const int nm = p.nm, nCalc = p.nCalc;
+  // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub; |g|^2 in registers
+  double v[kSepNb2];
+#pragma unroll
+  for (int b = 0; b < kSepNb2; ++b) v[b] = 0.;
+  if (modeOn) {
+#pragma unroll
+    for (int b2 = 0; b2 < kSepNb2; ++b2) {
+      if (b2 < nb2) {
+        double sx = 0., sy = 0.;
+#pragma unroll
+        for (int nu = 0; nu < NMX; ++nu)
+          if (nu < nm) {
+            const double2 x = gp[rl * gps + nu * p.nb2max + b2];
+            const double er = double(e3c[nu].x), ei = double(e3c[nu].y);
+            sx += x.x * er - x.y * ei;
+            sy += x.x * ei + x.y * er;
+          }
+        v[b2] = sx * sx + sy * sy;
+      }
+    }
+    if (!REDUCE) {
+      const int nb1 = p.kkNb1[kk];
+      for (int b2 = 0; b2 < nb2; ++b2)
+        p.out[p.rowOutOff[ri.row] + b1p + size_t(nb1) * (b2 + size_t(nb2) * sub)] = v[b2];
+    }
+  }
+  if (REDUCE) {
+    // symmetrizeCoupling along b2 (scattering.cpp:1250-1284), in registers
+    if (modeOn) {
+      const double* e2r = se2 + r * p.nb2max;
+      for (int b = 0; b < nb2;) {
+        int deg = 0;
+        for (int i = b; i < nb2; ++i) {
+          if (fabs(e2r[b] - e2r[i]) < 1.0e-6) deg++;
+          else break;
+        }
+        if (deg > 1) {
+          double s2 = 0.;
+#pragma unroll
+          for (int i = 0; i < kSepNb2; ++i)
+            if (i >= b && i < b + deg) s2 += v[i] / double(float(deg));
+#pragma unroll
+          for (int i = 0; i < kSepNb2; ++i)
+            if (i >= b && i < b + deg) v[i] = s2;
+        }
+        b += deg;
+      }
+    }
+    // ... and along nu (scattering.cpp:1286-1316), through LDS when modes are degenerate
+    // the mode's degenerate group (consecutive modes within 1e-6 of the
+    // group's first), tabulated per q at set-up
+    int mydeg = 1, mystart = sub;
+    if (modeOn) {
+      const int grp = sg3[r * nm + sub];
+      mystart = grp & 0xff;
+      mydeg = grp >> 8;
+    }
+    if (__any(mydeg > 1)) {  // g' of the row is dead here: its slots hold |g|^2
+      double* gs = reinterpret_cast<double*>(gp);
+      wave_sync();
+      if (modeOn)
+        for (int b2 = 0; b2 < nb2; ++b2) gs[2 * (rl * gps + sub * p.nb2max + b2)] = v[b2];
+      wave_sync();
+      if (modeOn && mydeg > 1)
+        for (int b2 = 0; b2 < nb2; ++b2) {
+          double s2 = 0.;
+          for (int i = 0; i < mydeg; ++i)
+            s2 += gs[2 * (rl * gps + (mystart + i) * p.nb2max + b2)] / double(float(mydeg));
+#pragma unroll
+          for (int b = 0; b < kSepNb2; ++b)
+            if (b == b2) v[b] = s2;
+        }
+    }
+    // Gaussian-delta weights (el_scattering.cpp:277-380); b2 loop kept rolled
+    if (modeOn && w3 >= p.cutoff) {
+#pragma unroll 1
+      for (int b2 = 0; b2 < nb2; ++b2) {
+        double vb = 0.;
+#pragma unroll
+        for (int b = 0; b < kSepNb2; ++b)
+          if (b == b2) vb = v[b];
+        const double en2 = se2[r * p.nb2max + b2];
+        const double cpl = vb * inv_g;
+        const int is2 = ri.soff + b2;
+#pragma unroll
+        for (int gi = 0; gi < kGMax; ++gi) {
+          if (gi >= gsize) break;
+          double d1, d2;
+          if (p.adaptive) {
+            const AdaptiveWidth aw = adaptive_width(p, int(e1 - p.e1pool) + gi, is2);
+            d1 = adaptive_delta(se1[gi] - en2 + w3, aw);
+            d2 = adaptive_delta(se1[gi] - en2 - w3, aw);
+          } else {
+            const double x1 = se1[gi] - en2 + w3, x2 = se1[gi] - en2 - w3;
+            // with the energy screen on, a pair whose both weights are below
+            // exp(-screen^2) of the peak is skipped, as whole rows are
+            if (p.screen > 0. && fabs(x1) * p.invSigma >= p.screen && fabs(x2) * p.invSigma >= p.screen)
+              continue;
+            d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
+            d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
+          }
+          if (d1 <= 0. && d2 <= 0.) continue;
+#pragma unroll
+          for (int c = 0; c < CM; ++c) {
+            if (c >= nCalc) break;
+            const double f2 = sf2[(c * 16 + r) * p.nb2max + b2];
+            part[gi][c] += cpl * ((f2 + n3[c]) * d1 + (1. - f2 + n3[c]) * d2) * p.norm / w3 *
+                           3.14159265358979323846;
+          }
+        }
+      }
+    }
+  }
+}
+
 // BF3: the x-stage GEMMs in split-BF16x3 (hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x16_bf16, FP32 accumulation); T must be float.
 // BREG: the B operands of all k-steps (at most 12, i.e. 24 x-groups) are loaded
@@ -1564,6 +1697,237 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   if (REDUCE) reduce_partials<CM>(p, slot, smem, part);
 }
 
+
+// ------------------------------------------------------------------ split last stage
+// The last stage as two kernels with small footprints, for FP32 data, nm, nw
+// <= 16, nm nw <= 192 and precomputed row masks: many more workgroups share a
+// CU than with the fused k_sep_xm (about 65 KB of LDS and 255 VGPRs), which
+// is bound by memory latency.  g3 rows go through HBM, (slot, jx, col), and
+// the epilogue is the same sequence of operations on the same values, so the
+// results equal the fused kernel's.
+
+// x-stage: g3(row, (nu, w2)) = sum_g w_x^{jx n0(g)} T2(g; nu, w2) of every
+// row kept by the slot's mask, stored at G3 (slot, jx).
+template <typename T, int kXmNt>
+__global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Vec2<T>::type T2v;
+  typedef typename Mfma4<T>::acc_t acc_t;
+  const int nLinesYZ = p.m1 * p.m2;
+  const int nTilesTot = p.nPanels * nLinesYZ;
+  const int bid = blockIdx.x;
+  const int perXcd = gridDim.x / 8;
+  const int Lx = (bid % 8) * perXcd + bid / 8;
+  if (Lx >= nTilesTot) return;
+  const int yz = Lx / p.nPanels, panel = Lx % p.nPanels;
+  const int slot = panel * nLinesYZ + yz;
+  const int jy = yz % p.m1, jz = yz / p.m1;
+  const unsigned long long mask = p.keepMask[slot];
+  if (mask == 0ull) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
+  const int nm = p.nm, nw = p.nw, ncol = nm * nw;
+  const int nG0 = p.nG0, nG0p = nG0 + 1;
+  int* jxs = reinterpret_cast<int*>(smem);                                     // 64: kept rows, ascending
+  int* sg0 = jxs + 64;                                                         // nG0
+  T2v* stw = reinterpret_cast<T2v*>(smem + (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16);  // m0
+  T2v* Wk = stw + p.m0;                                                        // nkeep x nG0p
+  const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
+                 (size_t(jy) * p.m2 + jz) * ncol;
+  const size_t gStride = size_t(p.m1) * p.m2 * ncol;
+  const int KS = (2 * nG0 + 3) / 4;  // <= 12, checked by the launcher
+  T bAll[12][kXmNt];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+    const bool gOk = g < nG0;
+#pragma unroll
+    for (int j = 0; j < kXmNt; ++j) {
+      const int c = (wave * kXmNt + j) * 16 + lr;
+      T v = T(0);
+      if (gOk && c < ncol) {
+        const T2v x = X[size_t(g) * gStride + c];
+        v = kc ? x.y : x.x;
+      }
+      bAll[q][j] = v;
+    }
+  }
+  for (int i = t; i < p.m0; i += 256) {
+    const double2 d = p.tw0[i];
+    stw[i] = T2v{T(d.x), T(d.y)};
+  }
+  for (int i = t; i < nG0; i += 256) sg0[i] = p.g0val[i];
+  if (t < p.m0 && ((mask >> t) & 1ull)) jxs[__popcll(mask & ((1ull << t) - 1ull))] = t;
+  const int nkeep = __popcll(mask);
+  __syncthreads();
+  for (int i = t; i < nkeep * nG0p; i += 256) {
+    const int k = i / nG0p, g = i % nG0p;
+    Wk[i] = g < nG0 ? stw[(jxs[k] * sg0[g]) % p.m0] : T2v{T(0), T(0)};
+  }
+  __syncthreads();
+  T2v* out = static_cast<T2v*>(p.G3) + size_t(slot) * p.m0 * ncol;
+  for (int m0r = 0; m0r < nkeep; m0r += 16) {
+    const int nr = min(16, nkeep - m0r);
+    const bool rowOk = lr < nr;
+    const T2v* wrow = Wk + size_t(m0r + (rowOk ? lr : 0)) * nG0p;
+    acc_t accr[kXmNt], acci[kXmNt];
+#pragma unroll
+    for (int j = 0; j < kXmNt; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accr[j][i] = acci[j][i] = T(0);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      if (q < KS) {
+        const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+        const bool gOk = g < nG0;
+        const T2v w = wrow[gOk ? g : nG0];
+        const bool on = rowOk && gOk;
+        const T a1 = on ? (kc ? -w.y : w.x) : T(0);
+        const T a2 = on ? (kc ? w.x : w.y) : T(0);
+#pragma unroll
+        for (int j = 0; j < kXmNt; ++j) {
+          accr[j] = Mfma4<T>::mac(a1, bAll[q][j], accr[j]);
+          acci[j] = Mfma4<T>::mac(a2, bAll[q][j], acci[j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kXmNt; ++j) {
+      const int c = (wave * kXmNt + j) * 16 + lr;
+      if (c < ncol) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = Mfma4<T>::row(lg, i);
+          if (row < nr) out[size_t(jxs[m0r + row]) * ncol + c] = T2v{accr[j][i], acci[j][i]};
+        }
+      }
+    }
+  }
+}
+
+inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0) {
+  return (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem + 64;
+}
+
+// epilogue of the rows k_sep_xg wrote: the electron rotation from each
+// thread's g3 slice (registers), then the phonon data and sep_epilogue_tail;
+// 16 threads per row, one pass of 16 rows per tile (nm <= NMX <= 16).  The
+// slice and the eigenvector column are never live together.
+template <typename T, int CM, int NMX>
+__global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Vec2<T>::type T2v;
+  constexpr int NWX = 16;
+  const int nLinesYZ = p.m1 * p.m2;
+  const int nTilesTot = p.nPanels * nLinesYZ;
+  const int bid = blockIdx.x;
+  const int perXcd = gridDim.x / 8;
+  const int Lx = (bid % 8) * perXcd + bid / 8;
+  if (Lx >= nTilesTot) return;
+  const int yz = Lx / p.nPanels, panel = Lx % p.nPanels;
+  const int slot = panel * nLinesYZ + yz;
+  const int jy = yz % p.m1, jz = yz / p.m1;
+  const int t = threadIdx.x;
+  const unsigned long long mask = p.keepMask[slot];
+  if (mask == 0ull) {
+    if (t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
+    return;
+  }
+  const int nm = p.nm, nw = p.nw, ncol = nm * nw;
+  const int kk = p.panelKK[panel];
+  const int b1p = p.panelB1[panel];
+  const int kkMesh = p.kkMesh[kk];
+  const int u2s = nw * p.nb2max + 1, gps = nm * p.nb2max + 1;
+  SepXmRow* rows = reinterpret_cast<SepXmRow*>(smem);                          // 64
+  double2* sU2 = reinterpret_cast<double2*>(smem + (64 * sizeof(SepXmRow) + 15) / 16 * 16);  // 16 x u2s
+  double2* gp = sU2 + size_t(16) * u2s;                                        // 16 x gps
+  double* se2 = reinterpret_cast<double*>(gp + size_t(16) * gps);              // 16 x nb2max
+  double* sw3 = se2 + size_t(16) * p.nb2max;                                   // 16 x nm
+  double* sf2 = sw3 + size_t(16) * nm;                                         // nCalc x 16 x nb2max
+  double* se1 = sf2 + size_t(p.nCalc) * 16 * p.nb2max;                         // kGMax
+  int* sg3 = reinterpret_cast<int*>(se1 + kGMax);                              // 16 x nm
+  if (t < p.panelGSize[panel]) se1[t] = p.e1pool[p.kkStateOff[kk] + p.panelGStart[panel] + t];
+  const int nkeep = __popcll(mask);
+  if (t < p.m0 && ((mask >> t) & 1ull)) {
+    const int mesh = (jz * p.m1 + jy) * p.m0 + t;
+    const int4 mr = p.meshRow[mesh];
+    rows[__popcll(mask & ((1ull << t) - 1ull))] =
+        SepXmRow{t, mr.x, mesh_diff_index(mesh, kkMesh, p.m0, p.m1, p.m2), mr.y, mr.z, mr.w};
+  }
+  if (t == 0 && p.keptRows) atomicAdd(p.keptRows, (unsigned long long)nkeep);
+  __syncthreads();
+  double part[kGMax][CM];
+#pragma unroll
+  for (int gi = 0; gi < kGMax; ++gi)
+#pragma unroll
+    for (int c = 0; c < CM; ++c) part[gi][c] = 0.;
+  const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
+  const double* e1 = p.e1pool + p.kkStateOff[kk] + gstart;
+  const double inv_g = 1. / double(float(gsize));
+  const T2v* G3 = static_cast<const T2v*>(p.G3) + size_t(slot) * p.m0 * ncol;
+  const int r = t / 16, sub = t % 16, rl = r;
+  for (int m0r = 0; m0r < nkeep; m0r += 16) {
+    const int nr = min(16, nkeep - m0r);
+    const bool rowOn = r < nr;
+    const bool modeOn = rowOn && sub < nm;
+    // the thread's g3 slice (mode sub of its row), loads issued before the
+    // row staging's
+    T2v g[NWX];
+    if (modeOn) {
+      const T2v* src = G3 + size_t(rows[m0r + r].jx) * ncol + sub * nw;
+#pragma unroll
+      for (int w2 = 0; w2 < NWX; ++w2)
+        if (w2 < nw) g[w2] = src[w2];
+    }
+    sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
+    __syncthreads();
+    const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
+    const int nb2 = rowOn ? ri.nb2 : 0;
+    const double w3 = modeOn ? sw3[r * nm + sub] : 0.;
+    // g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2), thread nu = sub
+    if (modeOn) {
+      for (int b2 = 0; b2 < nb2; ++b2) {
+        const double2* u = sU2 + size_t(r) * u2s + b2 * nw;
+        double sx = 0., sy = 0.;
+#pragma unroll
+        for (int w2 = 0; w2 < NWX; ++w2)
+          if (w2 < nw) {
+            const double xr = double(g[w2].x), xi = double(g[w2].y);
+            sx += u[w2].x * xr + u[w2].y * xi;
+            sy += u[w2].x * xi - u[w2].y * xr;
+          }
+        gp[rl * gps + sub * p.nb2max + b2] = make_double2(sx, sy);
+      }
+    }
+    // phonon eigenvector column and Bose factors of (row, mode sub)
+    T2v e3c[NMX];
+    double n3[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) n3[c] = 0.;
+    if (modeOn) {
+      const T2v* e3 = static_cast<const T2v*>(p.E3T) + size_t(ri.iq) * nm * nm + sub;
+#pragma unroll
+      for (int nu = 0; nu < NMX; ++nu)
+        if (nu < nm) e3c[nu] = e3[nu * nm];
+#pragma unroll
+      for (int c = 0; c < CM; ++c)
+        if (c < p.nCalc) n3[c] = p.bose[(size_t(ri.iq) * nm + sub) * p.nCalc + c];
+    }
+    wave_sync();
+    sep_epilogue_tail<T, true, NMX, CM>(p, ri, r, rl, sub, nb2, modeOn, w3, gp, gps, se2, sg3, sf2, se1, e3c, n3,
+                                       part, e1, kk, b1p, gsize, inv_g);
+    __syncthreads();  // the next tile rewrites the row data
+  }
+  reduce_partials<CM>(p, slot, smem, part);
+}
+
+inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc) {
+  size_t b = (64 * sizeof(SepXmRow) + 15) / 16 * 16;
+  b += size_t(16) * (nw * nb2max + 1) * sizeof(double2) + size_t(16) * (nm * nb2max + 1) * sizeof(double2) +
+       size_t(16) * nb2max * sizeof(double) + size_t(16) * nm * sizeof(double) +
+       size_t(nCalc) * 16 * nb2max * sizeof(double) + size_t(kGMax) * sizeof(double) + size_t(16) * nm * sizeof(int);
+  return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
+}
+
 // Row lists of the last stage: for every (panel, yz) slot, bit jx of
 // keepMask[slot] is set when mesh point (jx, jy, jz) is an inner point of the
 // range and passes the energy screen (same interval test as k_sep_x; every
@@ -1657,6 +2021,33 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
       PHX_REQUIRE(p.m0 <= 64 && p.meshRow, "separable transform: row masks need m0 <= 64");
       hipLaunchKernelGGL(k_sep_screen, dim3(ceil_div(tiles, 256)), dim3(256), 0, s, p);
       PHX_HIP(hipGetLastError());
+    }
+    if constexpr (sizeof(T) == 4) {
+      const bool split = p.G3 && !bf3 && !wide && p.keepMask && !p.slotOn && !p.gate && p.nm <= 16 &&
+                         p.nw <= 16 && (2 * p.nG0 + 3) / 4 <= 12;
+      if (split) {
+        const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0);
+        const size_t ldsB = sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc);
+        PHX_REQUIRE(ldsA <= 160 * 1024 && ldsB <= 160 * 1024, "separable transform: LDS budget exceeded");
+        PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_xg<T, 3>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
+        hipLaunchKernelGGL((k_sep_xg<T, 3>), dim3(grid), dim3(256), ldsA, s, p);
+        PHX_HIP(hipGetLastError());
+        auto ep = [&](auto kern) {
+          PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsB)));
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsB, s, p);
+        };
+        if (p.nm <= 12) {
+          if (one) ep(k_sep_ep<T, 1, 12>);
+          else ep(k_sep_ep<T, kCMax, 12>);
+        } else {
+          if (one) ep(k_sep_ep<T, 1, 16>);
+          else ep(k_sep_ep<T, kCMax, 16>);
+        }
+        PHX_HIP(hipGetLastError());
+        return;
+      }
     }
     auto go = [&](auto kern) {
       PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),

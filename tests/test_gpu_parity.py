@@ -106,6 +106,34 @@ def test_coupling_squared_vs_oracle(precision, tol):
     assert worst < tol
 
 
+@pytest.mark.parametrize("mesh,screen,nt", [((6, 6, 6), 6.0, 1), ((5, 4, 6), 0.0, 2), ((8, 8, 8), 7.0, 3)])
+def test_split_last_stage_equals_fused(mesh, screen, nt):
+    """FP32 separable transform: the split last stage (x-stage and epilogue
+    kernels, g3 through device memory) gives the fused kernel's linewidths,
+    with and without the energy screen, for 1 to 3 (T, mu).  The g3 values
+    are the same; the FP64 epilogue arithmetic may fuse multiply-adds
+    differently in the two kernels, so agreement is to FP64 rounding."""
+    say("split vs fused last stage", mesh, screen, nt)
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=mesh, seed=23)
+    p.temperatures = p.temperatures[:1].repeat(nt) * np.arange(1, nt + 1)
+    p.chem_pots = p.chem_pots[:1].repeat(nt)
+    gW = p.gw_host()
+    out = []
+    for split in (True, False):
+        lw = P.ElectronLinewidths(p, gW=gW, precision="fp32", transform="separable")
+        lw.set_split_last_stage(split)
+        if screen:
+            lw.set_screen(screen)
+        lw.run()
+        out.append(lw.linewidths(average=False))
+    assert np.any(out[0])
+    nz = out[1] != 0
+    assert np.array_equal(out[0] != 0, nz)
+    err = np.abs(out[0][nz] - out[1][nz]) / np.abs(out[1][nz])
+    say("  max rel diff", err.max())
+    assert err.max() < 1e-12
+
+
 def test_coupling_api_repeated_calls_of_changing_size():
     """The per-call API keeps its device buffers between calls: outer points
     with 3, 1 and 2 bands, against 40, 5 and 61 inner points, each checked
