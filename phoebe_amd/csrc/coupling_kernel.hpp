@@ -1226,6 +1226,97 @@ __device__ __forceinline__ void sep_epilogue_tail(const CoupParams& p, const Sep
   }
 }
 
+// Second half of an epilogue pass of k_sep_ep, with |g|^2 kept in LDS
+// rather than in registers: gsq holds the 16 rows' |g(nu, b2)|^2, (row, nu,
+// b2) at (r nm + nu) nb2max + b2.  The phonon rotation, symmetrizeCoupling
+// along b2 (each thread its own entries) and then along nu (read on the fly
+// from the group's entries), and the Gaussian-delta weights into part.  The
+// sums run in the same order as sep_epilogue_tail's.  Wave-level ordering.
+template <typename T, int NMX, int CM>
+__device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const SepXmRow& ri, int r, int rl,
+                                                      int sub, int nb2, bool modeOn, double w3, const double2* gp,
+                                                      int gps, double* gsq, const double* se2, const int* sg3,
+                                                      const double* sf2, const double* se1,
+                                                      const typename Vec2<T>::type (&e3c)[NMX],
+                                                      const double (&n3)[CM], double (&part)[kGMax][CM],
+                                                      const double* e1, int gsize, double inv_g) {
+  const int nm = p.nm, nCalc = p.nCalc, nb2max = p.nb2max;
+  double* own = gsq + (size_t(r) * nm + sub) * nb2max;
+  // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub
+  if (modeOn)
+    for (int b2 = 0; b2 < nb2; ++b2) {
+      double sx = 0., sy = 0.;
+#pragma unroll
+      for (int nu = 0; nu < NMX; ++nu)
+        if (nu < nm) {
+          const double2 x = gp[rl * gps + nu * nb2max + b2];
+          const double er = double(e3c[nu].x), ei = double(e3c[nu].y);
+          sx += x.x * er - x.y * ei;
+          sy += x.x * ei + x.y * er;
+        }
+      own[b2] = sx * sx + sy * sy;
+    }
+  // symmetrizeCoupling along b2 (scattering.cpp:1250-1284), own entries
+  if (modeOn) {
+    const double* e2r = se2 + r * nb2max;
+    for (int b = 0; b < nb2;) {
+      int deg = 0;
+      for (int i = b; i < nb2; ++i) {
+        if (fabs(e2r[b] - e2r[i]) < 1.0e-6) deg++;
+        else break;
+      }
+      if (deg > 1) {
+        double s2 = 0.;
+        for (int i = 0; i < deg; ++i) s2 += own[b + i] / double(float(deg));
+        for (int i = 0; i < deg; ++i) own[b + i] = s2;
+      }
+      b += deg;
+    }
+  }
+  wave_sync();
+  // ... along nu (scattering.cpp:1286-1316) and the Gaussian-delta weights
+  // (el_scattering.cpp:277-380)
+  if (!modeOn || w3 < p.cutoff) return;
+  const int grp = sg3[r * nm + sub];
+  const int mystart = grp & 0xff, mydeg = grp >> 8;
+  const double* g0 = gsq + (size_t(r) * nm + mystart) * nb2max;
+#pragma unroll 1
+  for (int b2 = 0; b2 < nb2; ++b2) {
+    double vb;
+    if (mydeg > 1) {
+      vb = 0.;
+      for (int i = 0; i < mydeg; ++i) vb += g0[size_t(i) * nb2max + b2] / double(float(mydeg));
+    } else {
+      vb = own[b2];
+    }
+    const double en2 = se2[r * nb2max + b2];
+    const double cpl = vb * inv_g;
+    const int is2 = ri.soff + b2;
+#pragma unroll
+    for (int gi = 0; gi < kGMax; ++gi) {
+      if (gi >= gsize) break;
+      double d1, d2;
+      if (p.adaptive) {
+        const AdaptiveWidth aw = adaptive_width(p, int(e1 - p.e1pool) + gi, is2);
+        d1 = adaptive_delta(se1[gi] - en2 + w3, aw);
+        d2 = adaptive_delta(se1[gi] - en2 - w3, aw);
+      } else {
+        const double x1 = se1[gi] - en2 + w3, x2 = se1[gi] - en2 - w3;
+        if (p.screen > 0. && fabs(x1) * p.invSigma >= p.screen && fabs(x2) * p.invSigma >= p.screen) continue;
+        d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
+        d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
+      }
+      if (d1 <= 0. && d2 <= 0.) continue;
+#pragma unroll
+      for (int c = 0; c < CM; ++c) {
+        if (c >= nCalc) break;
+        const double f2 = sf2[(c * 16 + r) * nb2max + b2];
+        part[gi][c] += cpl * ((f2 + n3[c]) * d1 + (1. - f2 + n3[c]) * d2) * p.norm / w3 * 3.14159265358979323846;
+      }
+    }
+  }
+}
+
 // BF3: the x-stage GEMMs in split-BF16x3 (hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x16_bf16, FP32 accumulation); T must be float.
 // BREG: the B operands of all k-steps (at most 12, i.e. 24 x-groups) are loaded
@@ -1844,7 +1935,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
   double* sw3 = se2 + size_t(16) * p.nb2max;                                   // 16 x nm
   double* sf2 = sw3 + size_t(16) * nm;                                         // nCalc x 16 x nb2max
   double* se1 = sf2 + size_t(p.nCalc) * 16 * p.nb2max;                         // kGMax
-  int* sg3 = reinterpret_cast<int*>(se1 + kGMax);                              // 16 x nm
+  double* gsq = se1 + kGMax;                                                   // 16 x nm x nb2max
+  int* sg3 = reinterpret_cast<int*>(gsq + size_t(16) * nm * p.nb2max);         // 16 x nm
   if (t < p.panelGSize[panel]) se1[t] = p.e1pool[p.kkStateOff[kk] + p.panelGStart[panel] + t];
   const int nkeep = __popcll(mask);
   if (t < p.m0 && ((mask >> t) & 1ull)) {
@@ -1913,8 +2005,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
         if (c < p.nCalc) n3[c] = p.bose[(size_t(ri.iq) * nm + sub) * p.nCalc + c];
     }
     wave_sync();
-    sep_epilogue_tail<T, true, NMX, CM>(p, ri, r, rl, sub, nb2, modeOn, w3, gp, gps, se2, sg3, sf2, se1, e3c, n3,
-                                       part, e1, kk, b1p, gsize, inv_g);
+    sep_epilogue_tail_lds<T, NMX, CM>(p, ri, r, rl, sub, nb2, modeOn, w3, gp, gps, gsq, se2, sg3, sf2, se1, e3c,
+                                      n3, part, e1, gsize, inv_g);
     __syncthreads();  // the next tile rewrites the row data
   }
   reduce_partials<CM>(p, slot, smem, part);
@@ -1924,7 +2016,8 @@ inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc) {
   size_t b = (64 * sizeof(SepXmRow) + 15) / 16 * 16;
   b += size_t(16) * (nw * nb2max + 1) * sizeof(double2) + size_t(16) * (nm * nb2max + 1) * sizeof(double2) +
        size_t(16) * nb2max * sizeof(double) + size_t(16) * nm * sizeof(double) +
-       size_t(nCalc) * 16 * nb2max * sizeof(double) + size_t(kGMax) * sizeof(double) + size_t(16) * nm * sizeof(int);
+       size_t(nCalc) * 16 * nb2max * sizeof(double) + size_t(kGMax) * sizeof(double) +
+       size_t(16) * nm * nb2max * sizeof(double) + size_t(16) * nm * sizeof(int);
   return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
 }
 
