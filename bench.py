@@ -75,11 +75,13 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
     elapsed = time.perf_counter() - t0
     kt = lw.kernel_times()
     st = lw.stage_times()
+    sst = lw.split_times() if transform == "separable" else None
     gam = lw.linewidths(average=False)
     log(f"[{form}] {len(k1_batches)} run calls in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
-    return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, stage=st, linewidths=gam,
-                    k1=int(sum(len(b) for b in k1_batches)), active=lw.num_active)
+    return lw, dict(pairs=pairs, seconds=elapsed, kernels=kt, stage=st, split=sst, linewidths=gam,
+                    k1=int(sum(len(b) for b in k1_batches)), active=lw.num_active,
+                    ncol=lw.prob.nm * lw.prob.nw)
 
 
 def traffic_per_dispatch(kernel, workload):
@@ -105,9 +107,24 @@ def roofline(res, form, workload):
         ms, launches, flops = kt["coupling_ms"], max(kt["batches"], 1), kt["coupling_flops"]
         kern = "k_coupling"
     else:
-        st = res["stage"]
-        ms, launches, flops = st["ms"], max(st["launches"], 1), st["flops"]
-        kern = "k_sep_xm"
+        st, sp = res["stage"], res.get("split")
+        if sp and sp["launches"] > 0:
+            # split last stage: x-stage (MFMA) and epilogue (streams the kept rows' g3)
+            n = sp["launches"]
+            extra = {"k_sep_xg_ms": round(sp["xg_ms"] / n, 4), "k_sep_ep_ms": round(sp["ep_ms"] / n, 4)}
+            if sp["ep_ms"] >= sp["xg_ms"]:
+                byts = st["rows"] * res["ncol"] * 8.0 / n  # g3 of every computed row, complex FP32
+                ach = byts / (sp["ep_ms"] / n * 1e-3) / 1e9
+                return {"bound": "hbm", "kernel": "k_sep_ep", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": traffic_per_dispatch("k_sep_ep", workload),
+                        "avg_launch_ms": round(sp["ep_ms"] / n, 4), "launches": n,
+                        "bytes_per_launch": byts, **extra}
+            ms, launches, flops = sp["xg_ms"], n, st["flops"]
+            kern = "k_sep_xg"
+        else:
+            ms, launches, flops = st["ms"], max(st["launches"], 1), st["flops"]
+            kern = "k_sep_xm"
     ach = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     return {"bound": "mfma", "kernel": kern, "achieved": round(ach, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(ach / peak, 4),

@@ -374,6 +374,10 @@ int phx_lw_set_panel_chunk(phx_lw* h, int panels);
  * values are the same either way and the linewidths agree to FP64 rounding.
  * No counterpart in the reference. */
 int phx_lw_set_split_last_stage(phx_lw* h, int on);
+/* With timing on, the launches of phx_lw_stage_times that ran split:
+ * out4 = {x-stage kernel ms, epilogue kernel ms, launches, sum over the rows
+ * the epilogue computed of their band counts}.  No counterpart in the reference. */
+int phx_lw_last_stage_split_times(phx_lw* h, double* out4);
 /* ---- Learned q-block screen (separable transform with the energy screen on).
  * No counterpart in the reference, which evaluates every (k1, k2) pair
  * (el_scattering.cpp:161-386).  A q-block is one panel (k1, b1) times one

@@ -148,6 +148,7 @@ EXPORTS = {
     "phx_phel_destroy": (C.c_int, [C.c_void_p]),
     "phx_phel_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_split_last_stage": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_last_stage_split_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_phel_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_el_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
@@ -565,6 +566,13 @@ class ElectronLinewidths:
     def set_split_last_stage(self, on=True):
         """Separable last stage as two kernels (default) or the fused one."""
         _check(lib().phx_lw_set_split_last_stage(self._h, int(bool(on))))
+
+    def split_times(self):
+        """Split last stage with timing on: x-stage ms, epilogue ms, launches,
+        summed band counts of the rows the epilogue computed."""
+        o = np.zeros(4)
+        _check(lib().phx_lw_last_stage_split_times(self._h, _p(o)))
+        return dict(xg_ms=o[0], ep_ms=o[1], launches=int(o[2]), bands=o[3])
 
     # ---- learned q-block screen
     def screen_collect(self, k1_list=None, sample_frac=0.02, seed=1):

@@ -65,8 +65,8 @@ extern template void launch_separable<float>(const SepParams&, const CoupParams&
                                              hipStream_t);
 extern template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t);
 extern template void launch_separable_zy<float>(const SepParams&, int, int, hipStream_t);
-extern template void launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t);
-extern template void launch_separable_x<float>(const SepParams&, const CoupParams&, bool, hipStream_t);
+extern template bool launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t);
+extern template bool launch_separable_x<float>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t);
 int coupling_rows_per_tile_f64(int ldb);
 
 
@@ -775,11 +775,14 @@ struct LwHandle {
   long long kernelLaunches = 0;
   double couplingFlops = 0.;  // algorithmic FLOPs of the coupling launches timed
   // last-stage launches of the separable transform: event pairs, summed at the next sync
+  // (start, mid, end) per launch; mid is recorded between the split stage's kernels
   std::vector<hipEvent_t> xev;
+  std::vector<char> xevSplit;        // per launch: whether it ran split
   size_t xevUsed = 0;
-  double xMs = 0.;
-  long long xLaunches = 0;
+  double xMs = 0., xgMs = 0., epMs = 0.;
+  long long xLaunches = 0, splitLaunches = 0;
   DevBuf<unsigned long long> xKept;  // rows computed by the last stage (timed launches)
+  DevBuf<unsigned long long> xBands; // sum of their band counts (split epilogue)
   ~LwHandle() {
     if (graphExec) (void)hipGraphExecDestroy(graphExec);
     for (auto& e : ev)
@@ -810,13 +813,22 @@ static void lw_harvest_timing(LwHandle* L) {
   L->kernelMs[2] += c;
   L->kernelLaunches += 1;
   L->evPending = false;
-  for (size_t i = 0; i + 1 < L->xevUsed; i += 2) {
+  for (size_t i = 0; i + 2 < L->xevUsed + 1; i += 3) {
     float x = 0.f;
-    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 1]));
+    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 2]));
     L->xMs += x;
     L->xLaunches += 1;
+    if (L->xevSplit[i / 3]) {
+      float a = 0.f, b = 0.f;
+      PHX_HIP(hipEventElapsedTime(&a, L->xev[i], L->xev[i + 1]));
+      PHX_HIP(hipEventElapsedTime(&b, L->xev[i + 1], L->xev[i + 2]));
+      L->xgMs += a;
+      L->epMs += b;
+      L->splitLaunches += 1;
+    }
   }
   L->xevUsed = 0;
+  L->xevSplit.clear();
 }
 
 static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Batch,
@@ -1773,20 +1785,24 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
       else sp.B = L->Bf.p + size_t(p0) * pstride;
       auto xstage = [&](const CoupParams& c0) {
         CoupParams c = c0;
+        hipEvent_t mid = nullptr;
         if (L->timing) {
-          while (L->xev.size() < L->xevUsed + 2) {
+          while (L->xev.size() < L->xevUsed + 3) {
             hipEvent_t e;
             PHX_HIP(hipEventCreate(&e));
             L->xev.push_back(e);
           }
           c.keptRows = L->xKept.p;
+          c.keptBands = L->xBands.p;
+          mid = L->xev[L->xevUsed + 1];
           PHX_HIP(hipEventRecord(L->xev[L->xevUsed], s));
         }
-        if (f64) launch_separable_x<double>(sp, c, true, s);
-        else launch_separable_x<float>(sp, c, true, s);
+        const bool split = f64 ? launch_separable_x<double>(sp, c, true, s, mid)
+                               : launch_separable_x<float>(sp, c, true, s, mid);
         if (L->timing) {
-          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 1], s));
-          L->xevUsed += 2;
+          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 2], s));
+          L->xevUsed += 3;
+          L->xevSplit.push_back(split ? 1 : 0);
         }
       };
       if (f64) launch_separable_zy<double>(sp, L->maxLineE, L->maxGrpE, s);
@@ -3445,11 +3461,14 @@ int phx_lw_set_timing(phx_lw* h, int on) {
     h->kernelMs[0] = h->kernelMs[1] = h->kernelMs[2] = 0.;
     h->kernelLaunches = 0;
     h->couplingFlops = 0.;
-    h->xMs = 0.;
-    h->xLaunches = 0;
+    h->xMs = h->xgMs = h->epMs = 0.;
+    h->xLaunches = h->splitLaunches = 0;
     h->xevUsed = 0;
+    h->xevSplit.clear();
     if (!h->xKept.p) h->xKept.alloc(1);
+    if (!h->xBands.p) h->xBands.alloc(1);
     launch_fill<int>(reinterpret_cast<int*>(h->xKept.p), 2, 0, h->stream);
+    launch_fill<int>(reinterpret_cast<int*>(h->xBands.p), 2, 0, h->stream);
     PHX_HIP(hipStreamSynchronize(h->stream));
   });
 }
@@ -3479,6 +3498,20 @@ int phx_lw_stage_times(phx_lw* h, double* out4) {
     out4[2] = double(kept);
     // complex multiply-adds of the last-stage transform for the rows computed
     out4[3] = 8. * double(kept) * h->nG0 * double(h->nm) * h->nw;
+  });
+}
+
+int phx_lw_last_stage_split_times(phx_lw* h, double* out4) {
+  return guard([&] {
+    PHX_REQUIRE(h && out4, "null argument");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    lw_harvest_timing(h);
+    unsigned long long bands = 0;
+    if (h->xBands.p) PHX_HIP(hipMemcpy(&bands, h->xBands.p, sizeof(bands), hipMemcpyDeviceToHost));
+    out4[0] = h->xgMs;
+    out4[1] = h->epMs;
+    out4[2] = double(h->splitLaunches);
+    out4[3] = double(bands);
   });
 }
 

@@ -86,6 +86,7 @@ struct CoupParams {
   const void* E3T;
   // split last stage (k_sep_xg + k_sep_ep): g3 rows of the chunk, (slot, jx, col) complex T
   void* G3;
+  unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
 };
 
 template <typename T> struct Vec2;
@@ -1947,6 +1948,11 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
   }
   if (t == 0 && p.keptRows) atomicAdd(p.keptRows, (unsigned long long)nkeep);
   __syncthreads();
+  if (t == 0 && p.keptBands) {
+    unsigned long long nb = 0;
+    for (int i = 0; i < nkeep; ++i) nb += rows[i].nb2;
+    atomicAdd(p.keptBands, nb);
+  }
   double part[kGMax][CM];
 #pragma unroll
   for (int gi = 0; gi < kGMax; ++gi)
@@ -2089,11 +2095,14 @@ void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStre
   PHX_HIP(hipGetLastError());
 }
 
+// mid (optional): recorded between the x-stage and the epilogue kernel when
+// the last stage runs split.  Returns whether it did.
 template <typename T>
-void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, hipStream_t s) {
+bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, hipStream_t s,
+                        hipEvent_t mid) {
   PHX_REQUIRE(p.nm <= kSepNm && p.nb2max <= kSepNb2,
               "separable transform supports nm <= 32 and at most 8 bands per inner point");
-  if (sp.nPanels <= 0) return;
+  if (sp.nPanels <= 0) return false;
   const int tiles = sp.nPanels * sp.my * sp.mz;
   const int grid = ceil_div(tiles, 8) * 8;
   const int nmb = p.nm <= 12 ? 12 : p.nm <= 16 ? 16 : p.nm <= 24 ? 24 : kSepNm;
@@ -2126,6 +2135,7 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
         hipLaunchKernelGGL((k_sep_xg<T, 3>), dim3(grid), dim3(256), ldsA, s, p);
         PHX_HIP(hipGetLastError());
+        if (mid) PHX_HIP(hipEventRecord(mid, s));
         auto ep = [&](auto kern) {
           PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsB)));
@@ -2139,7 +2149,7 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           else ep(k_sep_ep<T, kCMax, 16>);
         }
         PHX_HIP(hipGetLastError());
-        return;
+        return true;
       }
     }
     auto go = [&](auto kern) {
@@ -2166,7 +2176,7 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
     PHX_XM(12) PHX_XM(16) PHX_XM(24) PHX_XM(32)
 #undef PHX_XM
     PHX_HIP(hipGetLastError());
-    return;
+    return false;
   }
   PHX_REQUIRE(!p.slotOn && !p.gate, "separable transform: slot masks need nm nw <= 384");
   const size_t ldsX = std::max(sep_x_lds_bytes(sizeof(T), p.nG0, sp.ncol, p.m0, p.nw, p.nm, p.nb2max),
@@ -2188,13 +2198,14 @@ void launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
     else go(k_sep_x<T, false, kSepNm, 1>);
   }
   PHX_HIP(hipGetLastError());
+  return false;
 }
 
 template <typename T>
 void launch_separable(const SepParams& sp, const CoupParams& p, bool reduce, int maxLineE,
                       int maxGrpE, hipStream_t s) {
   launch_separable_zy<T>(sp, maxLineE, maxGrpE, s);
-  launch_separable_x<T>(sp, p, reduce, s);
+  launch_separable_x<T>(sp, p, reduce, s, nullptr);
 }
 
 }  // namespace phx
