@@ -106,8 +106,9 @@ def test_coupling_squared_vs_oracle(precision, tol):
     assert worst < tol
 
 
-@pytest.mark.parametrize("mesh,screen,nt", [((6, 6, 6), 6.0, 1), ((5, 4, 6), 0.0, 2), ((8, 8, 8), 7.0, 3)])
-def test_split_last_stage_equals_fused(mesh, screen, nt):
+@pytest.mark.parametrize("mesh,screen,nt,smearing", [((6, 6, 6), 6.0, 1, 0), ((5, 4, 6), 0.0, 2, 0),
+                                                    ((8, 8, 8), 7.0, 3, 0), ((6, 5, 4), 0.0, 2, 1)])
+def test_split_last_stage_equals_fused(mesh, screen, nt, smearing):
     """FP32 separable transform: the split last stage (x-stage and epilogue
     kernels, g3 through device memory) gives the fused kernel's linewidths,
     with and without the energy screen, for 1 to 3 (T, mu).  The g3 values
@@ -117,6 +118,7 @@ def test_split_last_stage_equals_fused(mesh, screen, nt):
     p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=mesh, seed=23)
     p.temperatures = p.temperatures[:1].repeat(nt) * np.arange(1, nt + 1)
     p.chem_pots = p.chem_pots[:1].repeat(nt)
+    p.smearing = smearing  # 1: adaptive Gaussian
     gW = p.gw_host()
     out = []
     for split in (True, False):
