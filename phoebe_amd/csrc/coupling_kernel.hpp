@@ -643,60 +643,73 @@ struct SepParams {
   void* T2; size_t t2PanelStride;               // (panel, group, jy, jz, col)
 };
 
-// T1(line, jz, c) = sum_{(f2, Rp) in line} w_z^{jz f2} C(Rp, c)
+// T1(line, jz, c) = sum_{(f2, Rp) in line} w_z^{jz f2} C(Rp, c).  The line's
+// entries and their twiddles w_z^{jz f2} (an entries x mz table) sit in LDS,
+// so the inner sum is two LDS reads and four FMAs per term.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sep_z(SepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
+  typedef typename Vec2<T>::type T2v;
   const int line = blockIdx.x, panel = blockIdx.y;
   const int e0 = p.lineStart[line], ne = p.lineStart[line + 1] - e0;
+  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
+  double2* W = X + size_t(ne) * p.ncol;           // (entries, mz)
   const T* B = static_cast<const T*>(p.B) + size_t(panel) * p.panelStride;
   for (int i = threadIdx.x; i < ne * p.ncol; i += 256) {
     const int e = i / p.ncol, c = i % p.ncol;
     const T* row = B + size_t(2 * p.lineRp[e0 + e]) * p.ldb;
     X[i] = make_double2(double(row[2 * c]), double(row[2 * c + 1]));
   }
+  for (int i = threadIdx.x; i < ne * p.mz; i += 256) {
+    const int e = i / p.mz, jz = i % p.mz;
+    W[i] = p.twz[(jz * p.lineF2[e0 + e]) % p.mz];
+  }
   __syncthreads();
-  T* out = static_cast<T*>(p.T1) + size_t(panel) * p.t1PanelStride + size_t(line) * p.mz * p.ncol * 2;
+  T2v* out = reinterpret_cast<T2v*>(static_cast<T*>(p.T1) + size_t(panel) * p.t1PanelStride) +
+             size_t(line) * p.mz * p.ncol;
   for (int i = threadIdx.x; i < p.mz * p.ncol; i += 256) {
     const int jz = i / p.ncol, c = i % p.ncol;
     double2 s = make_double2(0., 0.);
     for (int e = 0; e < ne; ++e) {
-      const double2 w = p.twz[(jz * p.lineF2[e0 + e]) % p.mz], x = X[e * p.ncol + c];
+      const double2 w = W[e * p.mz + jz], x = X[e * p.ncol + c];
       s.x += w.x * x.x - w.y * x.y;
       s.y += w.x * x.y + w.y * x.x;
     }
-    out[2 * size_t(i)] = T(s.x);
-    out[2 * size_t(i) + 1] = T(s.y);
+    out[i] = T2v{T(s.x), T(s.y)};
   }
 }
 
 // T2(group, jy, jz, c) = sum_{(f1, line) in group} w_y^{jy f1} T1(line, jz, c)
+// (entries and twiddles in LDS as in k_sep_z)
 template <typename T>
 __global__ void __launch_bounds__(256) k_sep_y(SepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
+  typedef typename Vec2<T>::type T2v;
   const int grp = blockIdx.x, jz = blockIdx.y, panel = blockIdx.z;
   const int e0 = p.grpStart[grp], ne = p.grpStart[grp + 1] - e0;
+  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
+  double2* W = X + size_t(ne) * p.ncol;           // (entries, my)
   const T* T1 = static_cast<const T*>(p.T1) + size_t(panel) * p.t1PanelStride;
   for (int i = threadIdx.x; i < ne * p.ncol; i += 256) {
     const int e = i / p.ncol, c = i % p.ncol;
     const T* src = T1 + (size_t(p.grpLine[e0 + e]) * p.mz + jz) * p.ncol * 2;
     X[i] = make_double2(double(src[2 * c]), double(src[2 * c + 1]));
   }
+  for (int i = threadIdx.x; i < ne * p.my; i += 256) {
+    const int e = i / p.my, jy = i % p.my;
+    W[i] = p.twy[(jy * p.grpF1[e0 + e]) % p.my];
+  }
   __syncthreads();
-  T* out = static_cast<T*>(p.T2) + size_t(panel) * p.t2PanelStride;
+  T2v* out = reinterpret_cast<T2v*>(static_cast<T*>(p.T2) + size_t(panel) * p.t2PanelStride);
   for (int i = threadIdx.x; i < p.my * p.ncol; i += 256) {
     const int jy = i / p.ncol, c = i % p.ncol;
     double2 s = make_double2(0., 0.);
     for (int e = 0; e < ne; ++e) {
-      const double2 w = p.twy[(jy * p.grpF1[e0 + e]) % p.my], x = X[e * p.ncol + c];
+      const double2 w = W[e * p.my + jy], x = X[e * p.ncol + c];
       s.x += w.x * x.x - w.y * x.y;
       s.y += w.x * x.y + w.y * x.x;
     }
-    const size_t o = ((size_t(grp) * p.my + jy) * p.mz + jz) * p.ncol + c;
-    out[2 * o] = T(s.x);
-    out[2 * o + 1] = T(s.y);
+    out[((size_t(grp) * p.my + jy) * p.mz + jz) * p.ncol + c] = T2v{T(s.x), T(s.y)};
   }
 }
 
@@ -2082,8 +2095,8 @@ inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb
 template <typename T>
 void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStream_t s) {
   if (sp.nPanels <= 0) return;
-  const size_t ldsZ = size_t(maxLineE) * sp.ncol * sizeof(double2);
-  const size_t ldsY = size_t(maxGrpE) * sp.ncol * sizeof(double2);
+  const size_t ldsZ = size_t(maxLineE) * (sp.ncol + sp.mz) * sizeof(double2);
+  const size_t ldsY = size_t(maxGrpE) * (sp.ncol + sp.my) * sizeof(double2);
   PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024, "separable transform: LDS budget exceeded");
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_z<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(ldsZ, 16))));
