@@ -491,34 +491,68 @@ __global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
 }
 
 // One workgroup per (Rp, nu) block of nw^2 columns, in passes of 256 columns
-// and, per line, of 16 outer points.
+// and, per line, of 16 outer points.  With at most kLineGe R_e groups, a
+// thread's partial sums of the line (one per group) are loaded once, all at
+// once, into registers and serve every batch of 16 outer points; the batch's
+// factors e^{2 pi i j0 n0 / m0} are staged in LDS.
+constexpr int kLineGe = 32;
+
 template <typename T, typename OutT>
 __global__ void __launch_bounds__(256) k_cache_line_b(CacheParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2;
   const int rp = blockIdx.x / p.nm, nu = blockIdx.x % p.nm;
   const int E = p.nw * p.nw, t = threadIdx.x;
-  T* Cb = reinterpret_cast<T*>(smem);  // 2 x 16 x 256
+  T* Cb = reinterpret_cast<T*>(smem);                                        // 2 x 16 x 256
+  T2* sw = reinterpret_cast<T2*>(smem + size_t(2) * 16 * 256 * sizeof(T));  // 16 x kLineGe
   const T2* Tg = static_cast<const T2*>(p.Tg);
+  const bool inReg = p.nGe <= kLineGe;
   for (int c0 = 0; c0 < E; c0 += cache_pass_cols(p.nw)) {
     const int cl = c0 + t;
     const bool ok = cl < E;
     const size_t colg = (size_t(rp) * p.nm + nu) * E + cl;
     for (int l = 0; l < p.nLinesB; ++l) {
       const int kend = p.lineKK[l + 1];
+      T2 v[kLineGe];
+      if (inReg) {
+#pragma unroll
+        for (int g = 0; g < kLineGe; ++g)
+          v[g] = (ok && g < p.nGe) ? Tg[(size_t(l) * p.nGe + g) * p.N + colg] : T2{T(0), T(0)};
+      }
       for (int kbase = p.lineKK[l]; kbase < kend; kbase += 16) {
         T accr[16], acci[16];
 #pragma unroll
         for (int kl = 0; kl < 16; ++kl) accr[kl] = acci[kl] = T(0);
-        for (int g = 0; g < p.nGe; ++g) {
-          const T2 v = ok ? Tg[(size_t(l) * p.nGe + g) * p.N + colg] : T2{T(0), T(0)};
+        if (inReg) {
+          __syncthreads();  // the previous batch's readers of sw are done
+          for (int i = t; i < 16 * p.nGe; i += 256) {
+            const int kl = i / p.nGe, g = i % p.nGe;
+            const double2 w = p.wLine[size_t(min(kbase + kl, kend - 1)) * p.nGe + g];
+            sw[kl * kLineGe + g] = T2{T(w.x), T(w.y)};
+          }
+          __syncthreads();
 #pragma unroll
-          for (int kl = 0; kl < 16; ++kl) {
-            const int kk = min(kbase + kl, kend - 1);
-            const double2 w = p.wLine[size_t(kk) * p.nGe + g];
-            const T wr = T(w.x), wi = T(w.y);
-            accr[kl] += wr * v.x - wi * v.y;
-            acci[kl] += wr * v.y + wi * v.x;
+          for (int g = 0; g < kLineGe; ++g) {
+            if (g < p.nGe) {
+#pragma unroll
+              for (int kl = 0; kl < 16; ++kl) {
+                const T2 w = sw[kl * kLineGe + g];
+                accr[kl] += w.x * v[g].x - w.y * v[g].y;
+                acci[kl] += w.x * v[g].y + w.y * v[g].x;
+              }
+            }
+          }
+        } else {
+          for (int g = 0; g < p.nGe; ++g) {
+            const T2 vg = ok ? Tg[(size_t(l) * p.nGe + g) * p.N + colg] : T2{T(0), T(0)};
+#pragma unroll
+            for (int kl = 0; kl < 16; ++kl) {
+              const int kk = min(kbase + kl, kend - 1);
+              const double2 w = p.wLine[size_t(kk) * p.nGe + g];
+              const T wr = T(w.x), wi = T(w.y);
+              accr[kl] += wr * vg.x - wi * vg.y;
+              acci[kl] += wr * vg.y + wi * vg.x;
+            }
           }
         }
         __syncthreads();
@@ -562,7 +596,7 @@ void launch_cache_line(const CacheParams& p, hipStream_t s) {
     case 3: launch_line_a<T, 3>(p, s); break;
     default: launch_line_a<T, 4>(p, s); break;
   }
-  const size_t lds = size_t(2) * 16 * 256 * sizeof(T);
+  const size_t lds = size_t(2) * 16 * 256 * sizeof(T) + size_t(16) * kLineGe * 2 * sizeof(T);
   auto kb = k_cache_line_b<T, OutT>;
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kb),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
