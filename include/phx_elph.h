@@ -416,7 +416,8 @@ int phx_mlp_num_params(void);
  * row-major; t: n targets), computed by the GPU forward / backward kernels. */
 int phx_mlp_gradients(const float* params, const float* X, const float* t, int n, float* grads,
                       float* loss, float* y);
-/* Timing of the separable transform's last stage (k_sep_xm), with HIP events
+/* Timing of the separable transform's last stage (the fused kernel, or the
+ * x-stage and epilogue kernels when it runs split), with HIP events
  * around each of its launches while timing is on: out4 = {milliseconds,
  * launches, rows computed (after the screens), FLOPs of its mesh-transform
  * GEMMs for those rows = 8 * rows * groups * nm * nw}. */
