@@ -303,7 +303,8 @@ def phel_main(args):
     t0 = time.perf_counter()
     prob = synth.config("cfg2")
     qm = tuple(int(x) for x in prob.mesh)
-    inter = P.InteractionElPhWan(None, prob.Re, prob.degE, prob.Rp, prob.degP, "fp64",
+    prec = args.precision
+    inter = P.InteractionElPhWan(None, prob.Re, prob.degE, prob.Rp, prob.degP, prec,
                                  synthetic=(prob.nw, prob.nm, prob.gw_decay, prob.gw_seed))
     ph = P.PhononElectronLinewidths(prob, inter, qm)
     log(f"[phel] setup {time.perf_counter() - t0:.1f}s: active {ph.num_active}, q mesh {qm}")
@@ -322,7 +323,7 @@ def phel_main(args):
     ph.set_timing(False)
     log(f"[phel] {pairs} pairs in {elapsed:.3f}s: {pairs / elapsed:.4g} pairs/s; kernels {kt}")
     # dominant kernel: the batched cacheElPh (one gW stream per batch) or the coupling
-    gw_bytes = 16.0 * prob.nRe * prob.nRp * prob.nm * prob.nw * prob.nw
+    gw_bytes = (16.0 if prec == "fp64" else 8.0) * prob.nRe * prob.nRp * prob.nm * prob.nw * prob.nw
     if kt["cache_ms"] >= kt["coupling_ms"]:
         n = max(kt["cache_launches"], 1)
         ach = gw_bytes / (kt["cache_ms"] / n * 1e-3) / 1e9
@@ -332,8 +333,8 @@ def phel_main(args):
     else:
         n = max(kt["coupling_launches"], 1)
         ach = kt["coupling_flops"] / (kt["coupling_ms"] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "k_coupling", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["fp64"],
-                "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS["fp64"], 4), "traffic": None,
+        roof = {"bound": "mfma", "kernel": "k_coupling", "achieved": round(ach, 3), "peak": PEAK_TFLOPS[prec],
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS[prec], 4), "traffic": None,
                 "avg_launch_ms": round(kt["coupling_ms"] / n, 4), "launches": n,
                 "flops_per_launch": kt["coupling_flops"] / n}
     cpu = None
@@ -356,11 +357,12 @@ def phel_main(args):
         cpu = {"value": cpairs / csec, "unit": "(k,q) pairs/s", "cores": int(O.lib().orc_num_threads()),
                "kind": "port", "sample": f"1 outer k1 x all {cpairs} q3 of the 40^3 mesh, cfg2 inputs "
                                          "(FP64 C++ restatement of PhElScatteringMatrix::builder)",
-               "seconds": round(csec, 2), "gpu_fp64_vs_oracle_max_rel_err": err}
-    out = {"metric": "ph-el (k,q) pairs/sec, phonon linewidths (PhElScatteringMatrix builder), FP64 on 1 GPU",
+               "seconds": round(csec, 2), f"gpu_{prec}_vs_oracle_max_rel_err": err}
+    out = {"metric": f"ph-el (k,q) pairs/sec, phonon linewidths (PhElScatteringMatrix builder), {prec.upper()} "
+                     "coupling on 1 GPU",
            "value": pairs / elapsed, "unit": "(k,q) pairs/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": elapsed / max(len(timed), 1) * 1e3, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+           "scaling": "weak", "vs_baseline": None, "dtype": prec, "data": "synthetic",
            "config": {"workload": "phel on cfg2: Nw=16, 12 modes, R_e WS(10^3)=1163, R_ph WS(9^3)=857, "
                                   "40^3 k mesh, 40^3 q mesh, 0.5 eV window",
                       "k1_per_step": K, "q_per_k1": int(np.prod(qm))},
@@ -387,6 +389,7 @@ def main():
     ap.add_argument("--audit", type=float, default=0.01, help="cfg4: audited fraction of skipped blocks")
     ap.add_argument("--sample", type=float, default=0.02, help="cfg4: training sample of tail blocks")
     ap.add_argument("--train-lines", type=int, default=8, help="cfg4: mesh lines for training samples")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="phel: coupling precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k1 sharding "
