@@ -813,7 +813,7 @@ static void lw_harvest_timing(LwHandle* L) {
   L->kernelMs[2] += c;
   L->kernelLaunches += 1;
   L->evPending = false;
-  for (size_t i = 0; i + 2 < L->xevUsed + 1; i += 3) {
+  for (size_t i = 0; i + 3 <= L->xevUsed; i += 3) {
     float x = 0.f;
     PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 2]));
     L->xMs += x;

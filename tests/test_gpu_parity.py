@@ -106,16 +106,17 @@ def test_coupling_squared_vs_oracle(precision, tol):
     assert worst < tol
 
 
-@pytest.mark.parametrize("mesh,screen,nt,smearing", [((6, 6, 6), 6.0, 1, 0), ((5, 4, 6), 0.0, 2, 0),
-                                                    ((8, 8, 8), 7.0, 3, 0), ((6, 5, 4), 0.0, 2, 1)])
-def test_split_last_stage_equals_fused(mesh, screen, nt, smearing):
+@pytest.mark.parametrize("mesh,screen,nt,smearing,natoms,nw", [
+    ((6, 6, 6), 6.0, 1, 0, 2, 6), ((5, 4, 6), 0.0, 2, 0, 2, 6), ((8, 8, 8), 7.0, 3, 0, 2, 6),
+    ((6, 5, 4), 0.0, 2, 1, 2, 6), ((5, 5, 5), 6.0, 4, 0, 5, 12)])
+def test_split_last_stage_equals_fused(mesh, screen, nt, smearing, natoms, nw):
     """FP32 separable transform: the split last stage (x-stage and epilogue
     kernels, g3 through device memory) gives the fused kernel's linewidths,
-    with and without the energy screen, for 1 to 3 (T, mu).  The g3 values
+    with and without the energy screen, for 1 to 4 (T, mu), 6 and 15 modes.  The g3 values
     are the same; the FP64 epilogue arithmetic may fuse multiply-adds
     differently in the two kernels, so agreement is to FP64 rounding."""
     say("split vs fused last stage", mesh, screen, nt)
-    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=mesh, seed=23)
+    p = tiny_problem(natoms=natoms, nw=nw, rg_e=3, rg_p=3, mesh=mesh, seed=23)
     p.temperatures = p.temperatures[:1].repeat(nt) * np.arange(1, nt + 1)
     p.chem_pots = p.chem_pots[:1].repeat(nt)
     p.smearing = smearing  # 1: adaptive Gaussian
