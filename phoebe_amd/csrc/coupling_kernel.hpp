@@ -1413,11 +1413,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 #pragma unroll
       for (int j = 0; j < kXmNt; ++j) {
         const int c = (wave * kXmNt + j) * 16 + lr;
-        T v = T(0);
-        if (gOk && c < ncol) {
-          const T2v x = X[size_t(g) * gStride + c];
-          v = kc ? x.y : x.x;
-        }
+        T v = T(0);  // one component: 4-byte (FP32) loads, half the bytes of a complex load
+        if (gOk && c < ncol) v = reinterpret_cast<const T*>(X)[2 * (size_t(g) * gStride + c) + kc];
         bAll[q][j] = v;
       }
     }
@@ -1611,11 +1608,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 #pragma unroll
         for (int j = 0; j < kXmNt; ++j) {
           const int c = (wave * kXmNt + j) * 16 + lr;
-          T v = T(0);
-          if (gOk && c < ncol) {
-            const T2v x = X[size_t(g) * gStride + c];
-            v = kc ? x.y : x.x;
-          }
+          T v = T(0);  // one component of the complex element
+          if (gOk && c < ncol) v = reinterpret_cast<const T*>(X)[2 * (size_t(g) * gStride + c) + kc];
           bv[q][j] = v;
         }
       }
@@ -1848,11 +1842,8 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
 #pragma unroll
     for (int j = 0; j < kXmNt; ++j) {
       const int c = (wave * kXmNt + j) * 16 + lr;
-      T v = T(0);
-      if (gOk && c < ncol) {
-        const T2v x = X[size_t(g) * gStride + c];
-        v = kc ? x.y : x.x;
-      }
+      T v = T(0);  // one component: 4-byte (FP32) loads, half the bytes of a complex load
+      if (gOk && c < ncol) v = reinterpret_cast<const T*>(X)[2 * (size_t(g) * gStride + c) + kc];
       bAll[q][j] = v;
     }
   }
