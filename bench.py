@@ -389,6 +389,8 @@ def main():
     ap.add_argument("--audit", type=float, default=0.01, help="cfg4: audited fraction of skipped blocks")
     ap.add_argument("--sample", type=float, default=0.02, help="cfg4: training sample of tail blocks")
     ap.add_argument("--train-lines", type=int, default=8, help="cfg4: mesh lines for training samples")
+    ap.add_argument("--calcs", type=int, default=1,
+                    help="(T, mu) calculations: the workload's T repeated at T, 1.5 T, 2 T, ...")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="phel: coupling precision")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -410,6 +412,9 @@ def main():
 
     t0 = time.perf_counter()
     prob = synth.config(args.config)
+    if args.calcs > 1:
+        prob.temperatures = prob.temperatures[:1].repeat(args.calcs) * (1 + 0.5 * np.arange(args.calcs))
+        prob.chem_pots = prob.chem_pots[:1].repeat(args.calcs)
     log(f"problem {args.config}: nw {prob.nw} nm {prob.nm} nRe {prob.nRe} nRp {prob.nRp} "
         f"mesh {prob.mesh} gW {prob.gw_bytes() / 1e9:.1f} GB ({time.perf_counter() - t0:.1f}s)")
 
@@ -475,6 +480,7 @@ def main():
                                f"{prob.mesh[0]}^3 k/q mesh, 0.5 eV window",
                    "k1_per_step": int(rc["k1"] // args.steps) if FORMS[args.chosen][3] else B,
                    "k2_per_k1": int(rc["pairs"] // rc["k1"]),
+                   "calculations": int(len(prob.temperatures)),
                    "active_k": int(rc["active"]), "chosen_form": args.chosen,
                    "chosen_precision": FORMS[args.chosen][0],
                    "chosen_transform": FORMS[args.chosen][1],

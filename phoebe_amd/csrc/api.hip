@@ -1779,6 +1779,8 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
       cq.panelB1 = dPB1 + p0;
       cq.panelGStart = dPGS + p0;
       cq.panelGSize = dPGZ + p0;
+      cq.gsizeMax = *std::max_element(b.hmeta.begin() + 5 * nkk + 3 * nPanels + p0,
+                                      b.hmeta.begin() + 5 * nkk + 3 * nPanels + p0 + n);
       cq.partial = partial0 + size_t(p0) * nMtiles * kGMax * kCMax;
       const bool f64 = L->precision == PHX_FP64;
       if (f64) sp.B = L->Bd.p + size_t(p0) * pstride;

@@ -20,6 +20,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "elph_params.hpp"
 #include "phx_common.hpp"
@@ -54,6 +55,7 @@ struct CoupParams {
   const double* fermi; int numStates;
   const double* bose;                     // (q, nu, calc)
   int nCalc; double invSigma, deltaPref, norm, cutoff;
+  int gsizeMax;                           // largest panelGSize of the launch (0: unknown, kGMax)
   double* partial;                        // (L = panel*nMtiles + mtile, kGMax, kCMax)
   int nb2max;
   // separable mesh transform
@@ -306,15 +308,16 @@ __device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned c
 }
 
 // Block reduction of the per-thread partial linewidths into partial[L]
-// (CM calculations held per thread; slots c >= CM are written as zero).
-template <int CM>
+// (CM calculations and GM group members held per thread; the slots beyond
+// them are written as zero).
+template <int CM, int GM = kGMax>
 __device__ inline void reduce_partials(const CoupParams& p, int L, unsigned char* smem,
-                                       double (&part)[kGMax][CM]) {
+                                       double (&part)[GM][CM]) {
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   double* red = reinterpret_cast<double*>(smem);
   __syncthreads();
 #pragma unroll
-  for (int gi = 0; gi < kGMax; ++gi)
+  for (int gi = 0; gi < GM; ++gi)
 #pragma unroll
     for (int c = 0; c < CM; ++c) {
       double v = part[gi][c];
@@ -323,9 +326,9 @@ __device__ inline void reduce_partials(const CoupParams& p, int L, unsigned char
     }
   __syncthreads();
   if (t < kGMax * kCMax) {
-    const int c = t % kCMax;
+    const int c = t % kCMax, gi = t / kCMax;
     double v = 0.;
-    if (c < CM) v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
+    if (c < CM && gi < GM) v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
     p.partial[size_t(L) * kGMax * kCMax + t] = v;
   }
 }
@@ -1246,13 +1249,13 @@ __device__ __forceinline__ void sep_epilogue_tail(const CoupParams& p, const Sep
 // along b2 (each thread its own entries) and then along nu (read on the fly
 // from the group's entries), and the Gaussian-delta weights into part.  The
 // sums run in the same order as sep_epilogue_tail's.  Wave-level ordering.
-template <typename T, int NMX, int CM>
+template <typename T, int NMX, int CM, int GM>
 __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const SepXmRow& ri, int r, int rl,
                                                       int sub, int nb2, bool modeOn, double w3, const double2* gp,
                                                       int gps, double* gsq, const double* se2, const int* sg3,
                                                       const double* sf2, const double* se1,
                                                       const typename Vec2<T>::type (&e3c)[NMX],
-                                                      const double (&n3)[CM], double (&part)[kGMax][CM],
+                                                      const double (&n3)[CM], double (&part)[GM][CM],
                                                       const double* e1, int gsize, double inv_g) {
   const int nm = p.nm, nCalc = p.nCalc, nb2max = p.nb2max;
   double* own = gsq + (size_t(r) * nm + sub) * nb2max;
@@ -1307,7 +1310,7 @@ __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const
     const double cpl = vb * inv_g;
     const int is2 = ri.soff + b2;
 #pragma unroll
-    for (int gi = 0; gi < kGMax; ++gi) {
+    for (int gi = 0; gi < GM; ++gi) {
       if (gi >= gsize) break;
       double d1, d2;
       if (p.adaptive) {
@@ -1907,8 +1910,10 @@ inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0) {
 // epilogue of the rows k_sep_xg wrote: the electron rotation from each
 // thread's g3 slice (registers), then the phonon data and sep_epilogue_tail;
 // 16 threads per row, one pass of 16 rows per tile (nm <= NMX <= 16).  The
-// slice and the eigenvector column are never live together.
-template <typename T, int CM, int NMX>
+// slice and the eigenvector column are never live together.  CM calculations
+// and GM members of the b1 group are held per thread: every panel of the
+// launch has gsize <= GM (checked on the host).
+template <typename T, int CM, int NMX, int GM>
 __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
@@ -1957,9 +1962,9 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
     for (int i = 0; i < nkeep; ++i) nb += rows[i].nb2;
     atomicAdd(p.keptBands, nb);
   }
-  double part[kGMax][CM];
+  double part[GM][CM];
 #pragma unroll
-  for (int gi = 0; gi < kGMax; ++gi)
+  for (int gi = 0; gi < GM; ++gi)
 #pragma unroll
     for (int c = 0; c < CM; ++c) part[gi][c] = 0.;
   const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
@@ -2015,11 +2020,11 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
         if (c < p.nCalc) n3[c] = p.bose[(size_t(ri.iq) * nm + sub) * p.nCalc + c];
     }
     wave_sync();
-    sep_epilogue_tail_lds<T, NMX, CM>(p, ri, r, rl, sub, nb2, modeOn, w3, gp, gps, gsq, se2, sg3, sf2, se1, e3c,
+    sep_epilogue_tail_lds<T, NMX, CM, GM>(p, ri, r, rl, sub, nb2, modeOn, w3, gp, gps, gsq, se2, sg3, sf2, se1, e3c,
                                       n3, part, e1, gsize, inv_g);
     __syncthreads();  // the next tile rewrites the row data
   }
-  reduce_partials<CM>(p, slot, smem, part);
+  reduce_partials<CM, GM>(p, slot, smem, part);
 }
 
 inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc) {
@@ -2145,13 +2150,24 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsB)));
           hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsB, s, p);
         };
-        if (p.nm <= 12) {
-          if (one) ep(k_sep_ep<T, 1, 12>);
-          else ep(k_sep_ep<T, kCMax, 12>);
-        } else {
-          if (one) ep(k_sep_ep<T, 1, 16>);
-          else ep(k_sep_ep<T, kCMax, 16>);
-        }
+        // partial sums held per thread: calculations x b1-group members of
+        // the launch, no more than the launch needs
+        const int gm = p.gsizeMax > 0 ? p.gsizeMax : kGMax;
+        PHX_REQUIRE(gm <= kGMax, "separable transform: b1 group larger than kGMax");
+        auto epc = [&](auto nmx) {
+          constexpr int NM = decltype(nmx)::value;
+          if (gm <= 4) {
+            if (one) ep(k_sep_ep<T, 1, NM, 4>);
+            else if (p.nCalc <= 2) ep(k_sep_ep<T, 2, NM, 4>);
+            else ep(k_sep_ep<T, kCMax, NM, 4>);
+          } else {
+            if (one) ep(k_sep_ep<T, 1, NM, kGMax>);
+            else if (p.nCalc <= 2) ep(k_sep_ep<T, 2, NM, kGMax>);
+            else ep(k_sep_ep<T, kCMax, NM, kGMax>);
+          }
+        };
+        if (p.nm <= 12) epc(std::integral_constant<int, 12>{});
+        else epc(std::integral_constant<int, 16>{});
         PHX_HIP(hipGetLastError());
         return true;
       }

@@ -137,6 +137,33 @@ def test_split_last_stage_equals_fused(mesh, screen, nt, smearing, natoms, nw):
     assert err.max() < 1e-12
 
 
+@pytest.mark.parametrize("split", [True, False])
+def test_fully_degenerate_bands_match_oracle(split):
+    """H(k) = eps(k) * identity: the six bands form one degenerate group at
+    every k, so the b1 groups have six members (beyond the four-member form of
+    the split epilogue) and both symmetrizations run over the whole band set."""
+    say("fully degenerate bands, split", split)
+    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    nw = p.nw
+    half = 0.5 * (p.win_max - p.win_min)
+    p.h0R = np.einsum("rii->r", p.h0R)[:, None, None] / nw * np.eye(nw)
+    kc = p.recip @ np.stack(np.meshgrid(*[np.arange(4) / 4] * 3, indexing="ij")).reshape(3, -1)
+    eps = np.real(np.exp(1j * (p.Rh @ kc)).T / p.degh @ np.einsum("rii->r", p.h0R) / nw)
+    mu = float(np.median(eps))
+    p.chem_pots = np.array([mu])
+    p.win_min, p.win_max = mu - half, mu + half
+    gW = p.gw_host()
+    ref = O.Oracle(p, gW.reshape(-1)).linewidths()
+    assert np.count_nonzero(ref) == ref.size
+    lw = P.ElectronLinewidths(p, gW=gW, precision="fp32", transform="separable")
+    lw.set_split_last_stage(split)
+    lw.set_screen(6.0)
+    lw.run()
+    err = rel_err(lw.linewidths(), ref)
+    say("  max rel err", err.max())
+    assert err.max() < 1e-4
+
+
 def test_coupling_api_repeated_calls_of_changing_size():
     """The per-call API keeps its device buffers between calls: outer points
     with 3, 1 and 2 bands, against 40, 5 and 61 inner points, each checked
