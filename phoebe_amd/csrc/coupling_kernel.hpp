@@ -56,6 +56,7 @@ struct CoupParams {
   const double* bose;                     // (q, nu, calc)
   int nCalc; double invSigma, deltaPref, norm, cutoff;
   int gsizeMax;                           // largest panelGSize of the launch (0: unknown, kGMax)
+  int epRows;                             // k_sep_ep: rows per pass, 4 x (rows per wave), rows per wave <= 64 / nm
   double* partial;                        // (L = panel*nMtiles + mtile, kGMax, kCMax)
   int nb2max;
   // separable mesh transform
@@ -1244,8 +1245,8 @@ __device__ __forceinline__ void sep_epilogue_tail(const CoupParams& p, const Sep
 }
 
 // Second half of an epilogue pass of k_sep_ep, with |g|^2 kept in LDS
-// rather than in registers: gsq holds the 16 rows' |g(nu, b2)|^2, (row, nu,
-// b2) at (r nm + nu) nb2max + b2.  The phonon rotation, symmetrizeCoupling
+// rather than in registers: gsq holds the rpp rows' |g(nu, b2)|^2, (row, nu,
+// b2) at (r nm + nu) nb2max + b2; occupations at sf2[(c rpp + r) nb2max + b].  The phonon rotation, symmetrizeCoupling
 // along b2 (each thread its own entries) and then along nu (read on the fly
 // from the group's entries), and the Gaussian-delta weights into part.  The
 // sums run in the same order as sep_epilogue_tail's.  Wave-level ordering.
@@ -1256,7 +1257,7 @@ __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const
                                                       const double* sf2, const double* se1,
                                                       const typename Vec2<T>::type (&e3c)[NMX],
                                                       const double (&n3)[CM], double (&part)[GM][CM],
-                                                      const double* e1, int gsize, double inv_g) {
+                                                      const double* e1, int gsize, double inv_g, int rpp) {
   const int nm = p.nm, nCalc = p.nCalc, nb2max = p.nb2max;
   double* own = gsq + (size_t(r) * nm + sub) * nb2max;
   // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub
@@ -1327,7 +1328,7 @@ __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const
 #pragma unroll
       for (int c = 0; c < CM; ++c) {
         if (c >= nCalc) break;
-        const double f2 = sf2[(c * 16 + r) * nb2max + b2];
+        const double f2 = sf2[(c * rpp + r) * nb2max + b2];
         part[gi][c] += cpl * ((f2 + n3[c]) * d1 + (1. - f2 + n3[c]) * d2) * p.norm / w3 * 3.14159265358979323846;
       }
     }
@@ -1907,10 +1908,72 @@ inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0) {
   return (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem + 64;
 }
 
+// Row data of an rpp-row pass of k_sep_ep into LDS: U2 (row r at sU2 + r u2s,
+// nw x nb2max), e2 (rpp x nb2max), omega and mode groups (rpp x nm) and the
+// occupations (nCalc x rpp x nb2max).  The loads of the e2, omega and U2
+// entries are issued before their LDS stores; rpp nm <= 256, rpp nb2max <= 512.
+__device__ inline void sep_ep_stage_rows(const CoupParams& p, const SepXmRow* rows, int m0r, int nr, int rpp,
+                                         int nw, int nm, int u2s, double2* sU2, double* se2, double* sw3,
+                                         int* sg3, double* sf2) {
+  const int t = threadIdx.x, nb2max = p.nb2max;
+  const int ne = rpp * nb2max, nwv = rpp * nm, nf = p.nCalc * ne;
+  double ev[2] = {0., 0.}, wv = 0.;
+  int gv = 0;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int i = t + 256 * u;
+    if (i < ne) {
+      const int r = i / nb2max, b = i % nb2max;
+      if (r < nr && b < rows[m0r + r].nb2) ev[u] = p.e2pool[rows[m0r + r].soff + b];
+    }
+  }
+  if (t < nwv) {
+    const int r = t / nm, nu = t % nm;
+    if (r < nr) {
+      wv = p.Wpool[size_t(rows[m0r + r].iq) * nm + nu];
+      gv = p.qModeGrp[size_t(rows[m0r + r].iq) * nm + nu];
+    }
+  }
+  const int per = nw * nb2max, nU = rpp * per;
+  for (int i0 = t; i0 < nU; i0 += 4 * 256) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      v[u] = make_double2(0., 0.);
+      if (i < nU) {
+        const int r = i / per, o = i % per;
+        if (r < nr) {
+          const SepXmRow& rr = rows[m0r + r];
+          if (o < nw * rr.nb2) v[u] = p.U2pool[rr.u2off + o];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 256 * u;
+      if (i < nU) sU2[(i / per) * u2s + i % per] = v[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    if (t + 256 * u < ne) se2[t + 256 * u] = ev[u];
+  if (t < nwv) {
+    sw3[t] = wv;
+    sg3[t] = gv;
+  }
+  for (int i = t; i < nf; i += 256) {
+    const int c = i / ne, o = i % ne, r = o / nb2max, b = o % nb2max;
+    double f = 0.;
+    if (r < nr && b < rows[m0r + r].nb2) f = p.fermi[size_t(c) * p.numStates + rows[m0r + r].soff + b];
+    sf2[i] = f;
+  }
+}
+
 // epilogue of the rows k_sep_xg wrote: the electron rotation from each
 // thread's g3 slice (registers), then the phonon data and sep_epilogue_tail;
-// 16 threads per row, one pass of 16 rows per tile (nm <= NMX <= 16).  The
-// slice and the eigenvector column are never live together.  CM calculations
+// nm threads per row, passes of epRows rows (whole rows per wave, nm <= NMX
+// <= 16).  The slice and the eigenvector column are never live together.  CM calculations
 // and GM members of the b1 group are held per thread: every panel of the
 // launch has gsize <= GM (checked on the host).
 template <typename T, int CM, int NMX, int GM>
@@ -1938,15 +2001,16 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
   const int b1p = p.panelB1[panel];
   const int kkMesh = p.kkMesh[kk];
   const int u2s = nw * p.nb2max + 1, gps = nm * p.nb2max + 1;
+  const int rpp = p.epRows, rpw = rpp / 4;
   SepXmRow* rows = reinterpret_cast<SepXmRow*>(smem);                          // 64
-  double2* sU2 = reinterpret_cast<double2*>(smem + (64 * sizeof(SepXmRow) + 15) / 16 * 16);  // 16 x u2s
-  double2* gp = sU2 + size_t(16) * u2s;                                        // 16 x gps
-  double* se2 = reinterpret_cast<double*>(gp + size_t(16) * gps);              // 16 x nb2max
-  double* sw3 = se2 + size_t(16) * p.nb2max;                                   // 16 x nm
-  double* sf2 = sw3 + size_t(16) * nm;                                         // nCalc x 16 x nb2max
-  double* se1 = sf2 + size_t(p.nCalc) * 16 * p.nb2max;                         // kGMax
-  double* gsq = se1 + kGMax;                                                   // 16 x nm x nb2max
-  int* sg3 = reinterpret_cast<int*>(gsq + size_t(16) * nm * p.nb2max);         // 16 x nm
+  double2* sU2 = reinterpret_cast<double2*>(smem + (64 * sizeof(SepXmRow) + 15) / 16 * 16);  // rpp x u2s
+  double2* gp = sU2 + size_t(rpp) * u2s;                                       // rpp x gps
+  double* se2 = reinterpret_cast<double*>(gp + size_t(rpp) * gps);             // rpp x nb2max
+  double* sw3 = se2 + size_t(rpp) * p.nb2max;                                  // rpp x nm
+  double* sf2 = sw3 + size_t(rpp) * nm;                                        // nCalc x rpp x nb2max
+  double* se1 = sf2 + size_t(p.nCalc) * rpp * p.nb2max;                        // kGMax
+  double* gsq = se1 + kGMax;                                                   // rpp x nm x nb2max
+  int* sg3 = reinterpret_cast<int*>(gsq + size_t(rpp) * nm * p.nb2max);        // rpp x nm
   if (t < p.panelGSize[panel]) se1[t] = p.e1pool[p.kkStateOff[kk] + p.panelGStart[panel] + t];
   const int nkeep = __popcll(mask);
   if (t < p.m0 && ((mask >> t) & 1ull)) {
@@ -1971,9 +2035,12 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
   const double* e1 = p.e1pool + p.kkStateOff[kk] + gstart;
   const double inv_g = 1. / double(float(gsize));
   const T2v* G3 = static_cast<const T2v*>(p.G3) + size_t(slot) * p.m0 * ncol;
-  const int r = t / 16, sub = t % 16, rl = r;
-  for (int m0r = 0; m0r < nkeep; m0r += 16) {
-    const int nr = min(16, nkeep - m0r);
+  // rpw whole rows of nm threads in each wave; the lanes beyond them idle
+  const int wave = t >> 6, lane = t & 63;
+  const bool laneOn = lane < rpw * nm;
+  const int r = laneOn ? wave * rpw + lane / nm : rpp, sub = laneOn ? lane % nm : 0, rl = r;
+  for (int m0r = 0; m0r < nkeep; m0r += rpp) {
+    const int nr = min(rpp, nkeep - m0r);
     const bool rowOn = r < nr;
     const bool modeOn = rowOn && sub < nm;
     // the thread's g3 slice (mode sub of its row), loads issued before the
@@ -1985,7 +2052,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
       for (int w2 = 0; w2 < NWX; ++w2)
         if (w2 < nw) g[w2] = src[w2];
     }
-    sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
+    sep_ep_stage_rows(p, rows, m0r, nr, rpp, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
     __syncthreads();
     const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
     const int nb2 = rowOn ? ri.nb2 : 0;
@@ -2021,18 +2088,18 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
     }
     wave_sync();
     sep_epilogue_tail_lds<T, NMX, CM, GM>(p, ri, r, rl, sub, nb2, modeOn, w3, gp, gps, gsq, se2, sg3, sf2, se1, e3c,
-                                      n3, part, e1, gsize, inv_g);
+                                      n3, part, e1, gsize, inv_g, rpp);
     __syncthreads();  // the next tile rewrites the row data
   }
   reduce_partials<CM, GM>(p, slot, smem, part);
 }
 
-inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc) {
+inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc, int rpp) {
   size_t b = (64 * sizeof(SepXmRow) + 15) / 16 * 16;
-  b += size_t(16) * (nw * nb2max + 1) * sizeof(double2) + size_t(16) * (nm * nb2max + 1) * sizeof(double2) +
-       size_t(16) * nb2max * sizeof(double) + size_t(16) * nm * sizeof(double) +
-       size_t(nCalc) * 16 * nb2max * sizeof(double) + size_t(kGMax) * sizeof(double) +
-       size_t(16) * nm * nb2max * sizeof(double) + size_t(16) * nm * sizeof(int);
+  b += size_t(rpp) * (nw * nb2max + 1) * sizeof(double2) + size_t(rpp) * (nm * nb2max + 1) * sizeof(double2) +
+       size_t(rpp) * nb2max * sizeof(double) + size_t(rpp) * nm * sizeof(double) +
+       size_t(nCalc) * rpp * nb2max * sizeof(double) + size_t(kGMax) * sizeof(double) +
+       size_t(rpp) * nm * nb2max * sizeof(double) + size_t(rpp) * nm * sizeof(int);
   return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
 }
 
@@ -2138,7 +2205,13 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
                          p.nw <= 16 && (2 * p.nG0 + 3) / 4 <= 12;
       if (split) {
         const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0);
-        const size_t ldsB = sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc);
+        // whole rows of nm threads per wave, at most 8, fewer if the LDS
+        // budget needs it (4 rows per wave always fit what 16 x 16 threads did)
+        CoupParams pe = p;
+        int rpw = std::min(64 / p.nm, 8);
+        while (rpw > 4 && sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, 4 * rpw) > 160 * 1024) --rpw;
+        pe.epRows = 4 * rpw;
+        const size_t ldsB = sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, pe.epRows);
         PHX_REQUIRE(ldsA <= 160 * 1024 && ldsB <= 160 * 1024, "separable transform: LDS budget exceeded");
         PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_xg<T, 3>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
@@ -2148,7 +2221,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         auto ep = [&](auto kern) {
           PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsB)));
-          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsB, s, p);
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsB, s, pe);
         };
         // partial sums held per thread: calculations x b1-group members of
         // the launch, no more than the launch needs

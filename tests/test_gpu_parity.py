@@ -1,4 +1,5 @@
 """GPU parity: HIP path (through the C ABI) against the FP64 oracle."""
+import dataclasses
 import sys
 
 import numpy as np
@@ -116,7 +117,8 @@ def test_split_last_stage_equals_fused(mesh, screen, nt, smearing, natoms, nw):
     are the same; the FP64 epilogue arithmetic may fuse multiply-adds
     differently in the two kernels, so agreement is to FP64 rounding."""
     say("split vs fused last stage", mesh, screen, nt)
-    p = tiny_problem(natoms=natoms, nw=nw, rg_e=3, rg_p=3, mesh=mesh, seed=23)
+    # a copy: tiny_problem's instances are shared between tests
+    p = dataclasses.replace(tiny_problem(natoms=natoms, nw=nw, rg_e=3, rg_p=3, mesh=mesh, seed=23))
     p.temperatures = p.temperatures[:1].repeat(nt) * np.arange(1, nt + 1)
     p.chem_pots = p.chem_pots[:1].repeat(nt)
     p.smearing = smearing  # 1: adaptive Gaussian
@@ -143,7 +145,8 @@ def test_fully_degenerate_bands_match_oracle(split):
     every k, so the b1 groups have six members (beyond the four-member form of
     the split epilogue) and both symmetrizations run over the whole band set."""
     say("fully degenerate bands, split", split)
-    p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
+    # a copy: tiny_problem's instances are shared between tests
+    p = dataclasses.replace(tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21))
     nw = p.nw
     half = 0.5 * (p.win_max - p.win_min)
     p.h0R = np.einsum("rii->r", p.h0R)[:, None, None] / nw * np.eye(nw)
