@@ -2048,9 +2048,23 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
     T2v g[NWX];
     if (modeOn) {
       const T2v* src = G3 + size_t(rows[m0r + r].jx) * ncol + sub * nw;
+      if (sizeof(T) == 4 && nw == NWX) {
+        // full slice: 16-byte loads (src is a multiple of 16 elements from
+        // the table's start)
+        const float4* s4 = reinterpret_cast<const float4*>(src);
 #pragma unroll
-      for (int w2 = 0; w2 < NWX; ++w2)
-        if (w2 < nw) g[w2] = src[w2];
+        for (int h = 0; h < NWX / 2; ++h) {
+          const float4 v = s4[h];
+          g[2 * h].x = v.x;
+          g[2 * h].y = v.y;
+          g[2 * h + 1].x = v.z;
+          g[2 * h + 1].y = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int w2 = 0; w2 < NWX; ++w2)
+          if (w2 < nw) g[w2] = src[w2];
+      }
     }
     sep_ep_stage_rows(p, rows, m0r, nr, rpp, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
     __syncthreads();

@@ -109,11 +109,12 @@ def test_coupling_squared_vs_oracle(precision, tol):
 
 @pytest.mark.parametrize("mesh,screen,nt,smearing,natoms,nw", [
     ((6, 6, 6), 6.0, 1, 0, 2, 6), ((5, 4, 6), 0.0, 2, 0, 2, 6), ((8, 8, 8), 7.0, 3, 0, 2, 6),
-    ((6, 5, 4), 0.0, 2, 1, 2, 6), ((5, 5, 5), 6.0, 4, 0, 5, 12)])
+    ((6, 5, 4), 0.0, 2, 1, 2, 6), ((5, 5, 5), 6.0, 4, 0, 5, 12), ((5, 5, 5), 6.0, 1, 0, 2, 16)])
 def test_split_last_stage_equals_fused(mesh, screen, nt, smearing, natoms, nw):
     """FP32 separable transform: the split last stage (x-stage and epilogue
     kernels, g3 through device memory) gives the fused kernel's linewidths,
-    with and without the energy screen, for 1 to 4 (T, mu), 6 and 15 modes.  The g3 values
+    with and without the energy screen, for 1 to 4 (T, mu), 6 and 15 modes,
+    6, 12 and 16 Wannier functions (16: whole-slice g3 loads).  The g3 values
     are the same; the FP64 epilogue arithmetic may fuse multiply-adds
     differently in the two kernels, so agreement is to FP64 rounding."""
     say("split vs fused last stage", mesh, screen, nt)
