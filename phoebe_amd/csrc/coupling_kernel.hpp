@@ -56,7 +56,7 @@ struct CoupParams {
   const double* bose;                     // (q, nu, calc)
   int nCalc; double invSigma, deltaPref, norm, cutoff;
   int gsizeMax;                           // largest panelGSize of the launch (0: unknown, kGMax)
-  int epRows;                             // k_sep_ep: rows per pass, 4 x (rows per wave), rows per wave <= 64 / nm
+  int epRows;                             // k_sep_ep: rows per pass, waves x (rows per wave), rows per wave <= 64 / nm
   double* partial;                        // (L = panel*nMtiles + mtile, kGMax, kCMax)
   int nb2max;
   // separable mesh transform
@@ -311,7 +311,7 @@ __device__ void epilogue16(const CoupParams& p, const T* g3, int PWp, unsigned c
 // Block reduction of the per-thread partial linewidths into partial[L]
 // (CM calculations and GM group members held per thread; the slots beyond
 // them are written as zero).
-template <int CM, int GM = kGMax>
+template <int CM, int GM = kGMax, int NWV = 4>
 __device__ inline void reduce_partials(const CoupParams& p, int L, unsigned char* smem,
                                        double (&part)[GM][CM]) {
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -323,13 +323,15 @@ __device__ inline void reduce_partials(const CoupParams& p, int L, unsigned char
     for (int c = 0; c < CM; ++c) {
       double v = part[gi][c];
       for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0) red[(gi * kCMax + c) * 4 + wave] = v;
+      if (lane == 0) red[(gi * kCMax + c) * NWV + wave] = v;
     }
   __syncthreads();
   if (t < kGMax * kCMax) {
     const int c = t % kCMax, gi = t / kCMax;
     double v = 0.;
-    if (c < CM && gi < GM) v = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
+    if (c < CM && gi < GM)
+#pragma unroll
+      for (int w = 0; w < NWV; ++w) v += red[t * NWV + w];
     p.partial[size_t(L) * kGMax * kCMax + t] = v;
   }
 }
@@ -1908,20 +1910,23 @@ inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0) {
   return (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem + 64;
 }
 
+constexpr int kSepEpThreads = 128;  // k_sep_ep workgroup: two waves
+
 // Row data of an rpp-row pass of k_sep_ep into LDS: U2 (row r at sU2 + r u2s,
 // nw x nb2max), e2 (rpp x nb2max), omega and mode groups (rpp x nm) and the
 // occupations (nCalc x rpp x nb2max).  The loads of the e2, omega and U2
-// entries are issued before their LDS stores; rpp nm <= 256, rpp nb2max <= 512.
+// entries are issued before their LDS stores; rpp nm <= blockDim, rpp nb2max
+// <= 2 blockDim.
 __device__ inline void sep_ep_stage_rows(const CoupParams& p, const SepXmRow* rows, int m0r, int nr, int rpp,
                                          int nw, int nm, int u2s, double2* sU2, double* se2, double* sw3,
                                          int* sg3, double* sf2) {
-  const int t = threadIdx.x, nb2max = p.nb2max;
+  const int t = threadIdx.x, nb2max = p.nb2max, bs = blockDim.x;
   const int ne = rpp * nb2max, nwv = rpp * nm, nf = p.nCalc * ne;
   double ev[2] = {0., 0.}, wv = 0.;
   int gv = 0;
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int i = t + 256 * u;
+    const int i = t + bs * u;
     if (i < ne) {
       const int r = i / nb2max, b = i % nb2max;
       if (r < nr && b < rows[m0r + r].nb2) ev[u] = p.e2pool[rows[m0r + r].soff + b];
@@ -1935,11 +1940,11 @@ __device__ inline void sep_ep_stage_rows(const CoupParams& p, const SepXmRow* ro
     }
   }
   const int per = nw * nb2max, nU = rpp * per;
-  for (int i0 = t; i0 < nU; i0 += 4 * 256) {
+  for (int i0 = t; i0 < nU; i0 += 4 * bs) {
     double2 v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int i = i0 + 256 * u;
+      const int i = i0 + bs * u;
       v[u] = make_double2(0., 0.);
       if (i < nU) {
         const int r = i / per, o = i % per;
@@ -1951,18 +1956,18 @@ __device__ inline void sep_ep_stage_rows(const CoupParams& p, const SepXmRow* ro
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int i = i0 + 256 * u;
+      const int i = i0 + bs * u;
       if (i < nU) sU2[(i / per) * u2s + i % per] = v[u];
     }
   }
 #pragma unroll
   for (int u = 0; u < 2; ++u)
-    if (t + 256 * u < ne) se2[t + 256 * u] = ev[u];
+    if (t + bs * u < ne) se2[t + bs * u] = ev[u];
   if (t < nwv) {
     sw3[t] = wv;
     sg3[t] = gv;
   }
-  for (int i = t; i < nf; i += 256) {
+  for (int i = t; i < nf; i += bs) {
     const int c = i / ne, o = i % ne, r = o / nb2max, b = o % nb2max;
     double f = 0.;
     if (r < nr && b < rows[m0r + r].nb2) f = p.fermi[size_t(c) * p.numStates + rows[m0r + r].soff + b];
@@ -1977,7 +1982,7 @@ __device__ inline void sep_ep_stage_rows(const CoupParams& p, const SepXmRow* ro
 // and GM members of the b1 group are held per thread: every panel of the
 // launch has gsize <= GM (checked on the host).
 template <typename T, int CM, int NMX, int GM>
-__global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
+__global__ void __launch_bounds__(kSepEpThreads, 2) k_sep_ep(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
   constexpr int NWX = 16;
@@ -2001,7 +2006,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
   const int b1p = p.panelB1[panel];
   const int kkMesh = p.kkMesh[kk];
   const int u2s = nw * p.nb2max + 1, gps = nm * p.nb2max + 1;
-  const int rpp = p.epRows, rpw = rpp / 4;
+  const int rpp = p.epRows, rpw = rpp / (kSepEpThreads / 64);
   SepXmRow* rows = reinterpret_cast<SepXmRow*>(smem);                          // 64
   double2* sU2 = reinterpret_cast<double2*>(smem + (64 * sizeof(SepXmRow) + 15) / 16 * 16);  // rpp x u2s
   double2* gp = sU2 + size_t(rpp) * u2s;                                       // rpp x gps
@@ -2105,7 +2110,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_ep(CoupParams p) {
                                       n3, part, e1, gsize, inv_g, rpp);
     __syncthreads();  // the next tile rewrites the row data
   }
-  reduce_partials<CM, GM>(p, slot, smem, part);
+  reduce_partials<CM, GM, kSepEpThreads / 64>(p, slot, smem, part);
 }
 
 inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc, int rpp) {
@@ -2223,8 +2228,9 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         // budget needs it (4 rows per wave always fit what 16 x 16 threads did)
         CoupParams pe = p;
         int rpw = std::min(64 / p.nm, 8);
-        while (rpw > 4 && sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, 4 * rpw) > 160 * 1024) --rpw;
-        pe.epRows = 4 * rpw;
+        constexpr int nWaves = kSepEpThreads / 64;
+        while (rpw > 4 && sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, nWaves * rpw) > 160 * 1024) --rpw;
+        pe.epRows = nWaves * rpw;
         const size_t ldsB = sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, pe.epRows);
         PHX_REQUIRE(ldsA <= 160 * 1024 && ldsB <= 160 * 1024, "separable transform: LDS budget exceeded");
         PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_xg<T, 3>),
@@ -2235,7 +2241,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         auto ep = [&](auto kern) {
           PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsB)));
-          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsB, s, pe);
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(kSepEpThreads), ldsB, s, pe);
         };
         // partial sums held per thread: calculations x b1-group members of
         // the launch, no more than the launch needs
