@@ -1910,7 +1910,7 @@ inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0) {
   return (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem + 64;
 }
 
-constexpr int kSepEpThreads = 128;  // k_sep_ep workgroup: two waves
+constexpr int kSepEpThreads = 64;  // k_sep_ep workgroup: one wave
 
 // Row data of an rpp-row pass of k_sep_ep into LDS: U2 (row r at sU2 + r u2s,
 // nw x nb2max), e2 (rpp x nb2max), omega and mode groups (rpp x nm) and the
