@@ -26,6 +26,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from phoebe_amd import CHOSEN, FORMS  # noqa: E402  (the library itself loads lazily)
+
 METRIC = "el-ph (k,q) pairs/sec, chosen vs plain FP64 on 1 GPU, at ≤1e-6 linewidth rel err"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
 HBM_PEAK_GBS = 8000.0
@@ -35,29 +37,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-FORMS = {  # name: (precision, transform, energy screen in sigma, line batching min points)
-    "fp64": ("fp64", "direct", 0.0, 0),
-    "fp32": ("fp32", "direct", 0.0, 0),
-    "sep64": ("fp64", "separable", 0.0, 0),
-    "sep32": ("fp32", "separable", 0.0, 0),
-    "sep32s": ("fp32", "separable", 7.0, 0),
-    "sep64s": ("fp64", "separable", 7.0, 0),
-    "line32s": ("fp32", "separable", 7.0, 8),
-    "line64s": ("fp64", "separable", 7.0, 8),
-    "line32s6": ("fp32", "separable", 6.0, 8),
-    "line3s6": ("bf16x3", "separable", 6.0, 8),
-}
 
 
 def run_form(P, prob, form, make_batches, warmup, k1_batch):
     t0 = time.perf_counter()
-    precision, transform, screen, line_min = FORMS[form]
-    lw = P.ElectronLinewidths(prob, gW=None, precision=precision, k1_batch=k1_batch,
-                              transform=transform)
-    if screen:
-        lw.set_screen(screen)
-    if line_min:
-        lw.set_line_batching(line_min)
+    precision, transform, screen, line_min = P.FORMS[form]
+    lw = P.linewidths_form(prob, form, k1_batch=k1_batch)
     log(f"[{form}] setup {time.perf_counter() - t0:.1f}s: active {lw.num_active}, "
         f"states {lw.num_states}")
     k1_batches, warm_batches = make_batches(lw.active, line_min > 0, warmup)
@@ -177,17 +162,8 @@ def uneven_main(args):
     log(f"[cfg3] {len(mats)} materials generated ({time.perf_counter() - t0:.0f}s)")
     handles, seq_t, pairs = [], 0.0, 0
     for i, p in enumerate(mats):
-        sep = p.nm * p.nw <= 192
-        lw = None
-        if sep:
-            try:  # the separable transform needs nm <= 32 and <= 8 windowed bands per point
-                lw = P.ElectronLinewidths(p, gW=None, precision="fp32", transform="separable")
-                lw.set_screen(6.0)
-            except P.PhxError:
-                sep = False
-        if lw is None:
-            lw = P.ElectronLinewidths(p, gW=None, precision="fp32", transform="direct")
-        lw.set_line_batching(4)
+        lw = P.linewidths_auto(p)
+        sep = lw.transform == "separable"
         if args.autotune:
             lw.autotune(np.arange(min(lw.num_active, 2 * p.mesh[0]), dtype=np.int32))
         lw.run()  # warm
@@ -377,7 +353,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--k1-batch", type=int, default=32)
-    ap.add_argument("--chosen", default="line32s6", choices=sorted(k for k in FORMS if k != "fp64"))
+    ap.add_argument("--chosen", default=CHOSEN, choices=sorted(k for k in FORMS if k != "fp64"))
     ap.add_argument("--cpu-k1", type=int, default=1, help="outer points in the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-chosen", action="store_true",

@@ -51,6 +51,11 @@ const char* phx_last_error(void);
 /* Library version string. */
 const char* phx_version(void);
 
+/* Number of HIP devices this process sees (0 when no GPU or driver is
+ * present: *count = 0 and the call still returns 0).  No counterpart in the
+ * reference, which gets its device from Kokkos::initialize. */
+int phx_device_count(int* count);
+
 /* ---------------------------------------------------------------- eigensolver
  * Batched Hermitian eigen-decomposition, in place.
  * Replaces kokkosZHEEV(StridedComplexView3D& A, DoubleView2D& W)

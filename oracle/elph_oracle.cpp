@@ -730,6 +730,26 @@ static void crys_of(const orc_problem& p, const double* c, double* x) {
   for (int i = 0; i < 3; ++i) x[i] = inv[i][0] * c[0] + inv[i][1] * c[1] + inv[i][2] * c[2];
 }
 
+// Window::internalEnWindow (window.cpp:137-157) for one point: a band passes
+// when winMin < e < winMax (both strict); the kept range is [first passing,
+// last passing] (window.cpp:152-156), so bands in between are kept whether
+// they pass or not.  Returns the number of passing bands; lo/hi are -1 when
+// none passes.
+int orc_window(int nb, const double* energies, double winMin, double winMax, int* lo, int* hi) {
+  int count = 0;
+  *lo = -1;
+  *hi = -1;
+  for (int ib = 0; ib < nb; ++ib) {
+    double e = energies[ib];
+    if (e < winMax && e > winMin) {
+      if (*lo < 0) *lo = ib;
+      *hi = ib;
+      ++count;
+    }
+  }
+  return count;
+}
+
 // Build band structure, window, q-set and phonons.  Returns a handle or null.
 void* orc_setup(const orc_problem* prob) {
   auto* s = new orc_state();
@@ -755,15 +775,8 @@ void* orc_setup(const orc_problem* prob) {
   // ascending mesh order (active_bandstructure.cpp:826-890).
   s->fullToActive.assign(s->nk, -1);
   for (int ik = 0; ik < s->nk; ++ik) {
-    int lo = -1, hi = -1;
-    for (int ib = 0; ib < p.nw; ++ib) {
-      double e = allE[size_t(ik) * p.nw + ib];
-      if (e < p.winMax && e > p.winMin) {
-        if (lo < 0) lo = ib;
-        hi = ib;
-      }
-    }
-    if (lo < 0) continue;
+    int lo, hi;
+    if (orc_window(p.nw, &allE[size_t(ik) * p.nw], p.winMin, p.winMax, &lo, &hi) == 0) continue;
     s->fullToActive[ik] = int(s->active.size());
     s->active.push_back(ik);
     s->bandLo.push_back(lo);
@@ -1000,27 +1013,34 @@ long long orc_linewidth_pairs(void* h, int nk1, const int* k1List, int k2Begin, 
   return pairs;
 }
 
-// ScatteringMatrix::degeneracyAveragingLinewidths (scattering.cpp:1168-1207):
-// row iCalc = 0 only.
+// ScatteringMatrix::degeneracyAveragingLinewidths (scattering.cpp:1168-1207)
+// at one point: each band takes the mean of the bands within 1e-6 Ry of its
+// own energy (not a transitive group), all means computed from the values
+// before averaging, divided by float(n).  lw points at the point's first state
+// of row iCalc = 0; other rows are not touched (scattering.cpp:1192,1204).
+void orc_average_point(int nb, const double* energies, double* lw) {
+  std::vector<double> tmp(nb, 0.);
+  for (int ib1 = 0; ib1 < nb; ++ib1) {
+    double ekk = energies[ib1];
+    int n = 0;
+    double t2 = 0.;
+    for (int ib2 = 0; ib2 < nb; ++ib2) {
+      if (std::abs(energies[ib2] - ekk) < 1.0e-6) {
+        n++;
+        t2 = t2 + lw[ib2];
+      }
+    }
+    tmp[ib1] = t2 / float(n);
+  }
+  for (int ib1 = 0; ib1 < nb; ++ib1) lw[ib1] = tmp[ib1];
+}
+
+// Every active point of a set-up problem; linewidth is (nCalc, numStates).
 void orc_degeneracy_average(void* h, double* linewidth) {
   auto* s = static_cast<orc_state*>(h);
   for (size_t a = 0; a < s->active.size(); ++a) {
     int off = s->stateOffset[a];
-    int nb = s->stateOffset[a + 1] - off;
-    std::vector<double> tmp(nb, 0.);
-    for (int ib1 = 0; ib1 < nb; ++ib1) {
-      double ekk = s->energies[off + ib1];
-      int n = 0;
-      double t2 = 0.;
-      for (int ib2 = 0; ib2 < nb; ++ib2) {
-        if (std::abs(s->energies[off + ib2] - ekk) < 1.0e-6) {
-          n++;
-          t2 = t2 + linewidth[off + ib2];
-        }
-      }
-      tmp[ib1] = t2 / float(n);
-    }
-    for (int ib1 = 0; ib1 < nb; ++ib1) linewidth[off + ib1] = tmp[ib1];
+    orc_average_point(s->stateOffset[a + 1] - off, &s->energies[off], linewidth + off);
   }
 }
 
@@ -1063,15 +1083,8 @@ long long orc_phel(const orc_problem* prob, const int* qmesh, int nk1, const int
   if (fail) return -1;
   std::vector<int> active, lo, hi;
   for (int ik = 0; ik < nk; ++ik) {
-    int l = -1, h = -1;
-    for (int ib = 0; ib < nw; ++ib) {
-      double e = allE[size_t(ik) * nw + ib];
-      if (e < p.winMax && e > p.winMin) {
-        if (l < 0) l = ib;
-        h = ib;
-      }
-    }
-    if (l < 0) continue;
+    int l, h;
+    if (orc_window(nw, &allE[size_t(ik) * nw], p.winMin, p.winMax, &l, &h) == 0) continue;
     active.push_back(ik);
     lo.push_back(l);
     hi.push_back(h);
@@ -1181,23 +1194,8 @@ long long orc_phel(const orc_problem* prob, const int* qmesh, int nk1, const int
   }
   if (loop_seconds)
     *loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (average) {
-    for (int q = 0; q < nq; ++q) {
-      const double* w3 = &w[size_t(q) * nm];
-      std::vector<double> tmp(nm);
-      for (int b1 = 0; b1 < nm; ++b1) {
-        double t2 = 0.;
-        int n = 0;
-        for (int b2 = 0; b2 < nm; ++b2)
-          if (std::abs(w3[b2] - w3[b1]) < 1.0e-6) {
-            n++;
-            t2 += lw[size_t(q) * nm + b2];
-          }
-        tmp[b1] = t2 / float(n);
-      }
-      for (int b1 = 0; b1 < nm; ++b1) lw[size_t(q) * nm + b1] = tmp[b1];
-    }
-  }
+  if (average)  // the phonon modes of each q3, row iCalc = 0
+    for (int q = 0; q < nq; ++q) orc_average_point(nm, &w[size_t(q) * nm], &lw[size_t(q) * nm]);
   std::copy(lw.begin(), lw.end(), out);
   return pairs;
 }

@@ -68,6 +68,9 @@ def lib():
         L.orc_linewidth_pairs.restype = C.c_longlong
         L.orc_linewidth_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, C.c_int, _dp]
         L.orc_degeneracy_average.argtypes = [C.c_void_p, _dp]
+        L.orc_average_point.argtypes = [C.c_int, _dp, _dp]
+        L.orc_window.restype = C.c_int
+        L.orc_window.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _ip, _ip]
         L.orc_num_threads.restype = C.c_int
         L.orc_adaptive_delta.restype = C.c_double
         L.orc_adaptive_delta.argtypes = [C.c_double, _dp, _dp, _ip]
@@ -184,6 +187,21 @@ def symmetrize_coupling(c, e1, e2, e3):
     buf = a.reshape(-1, order="F")
     lib().orc_symmetrize_coupling(_p(buf), nb1, nb2, nm, _p(_c(e1)), _p(_c(e2)), _p(_c(e3)))
     return buf.reshape((nb1, nb2, nm), order="F")
+
+
+def window(energies, win_min, win_max):
+    """(passing count, lo, hi) of one point's bands (lo = hi = -1 when none passes)."""
+    e = _c(energies)
+    lo, hi = C.c_int(), C.c_int()
+    n = lib().orc_window(len(e), _p(e), float(win_min), float(win_max), C.byref(lo), C.byref(hi))
+    return n, lo.value, hi.value
+
+
+def average_point(energies, lw):
+    """Degeneracy averaging of one point's linewidths (one row)."""
+    out = _c(lw).copy()
+    lib().orc_average_point(len(out), _p(_c(energies)), _p(out))
+    return out
 
 
 def gaussian_delta(e, sigma):

@@ -36,6 +36,26 @@ TRANSFORMS = {"direct": TRANSFORM_DIRECT, "separable": TRANSFORM_SEPARABLE}
 GAMMA_BATCHED, GAMMA_POINT = 0, 1
 SMEAR_GAUSSIAN, SMEAR_ADAPTIVE = 0, 1
 
+# Named configurations of the linewidth builder: (precision, transform,
+# energy screen in units of sigma (0: off), line batching: least outer points
+# of a mesh line for the line form of cacheElPh (0: off)).  "fp64" is the
+# plain form (the reference algorithm in FP64); CHOSEN is the fastest form
+# whose linewidths all stay within 1e-6 relative of FP64.
+FORMS = {
+    "fp64": ("fp64", "direct", 0.0, 0),
+    "fp32": ("fp32", "direct", 0.0, 0),
+    "sep64": ("fp64", "separable", 0.0, 0),
+    "sep32": ("fp32", "separable", 0.0, 0),
+    "sep32s": ("fp32", "separable", 7.0, 0),
+    "sep64s": ("fp64", "separable", 7.0, 0),
+    "line32s": ("fp32", "separable", 7.0, 8),
+    "line64s": ("fp64", "separable", 7.0, 8),
+    "line64s6": ("fp64", "separable", 6.0, 8),
+    "line32s6": ("fp32", "separable", 6.0, 8),
+    "line3s6": ("bf16x3", "separable", 6.0, 8),
+}
+CHOSEN = "line32s6"
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _lib = None
@@ -65,6 +85,7 @@ class PhxProblem(C.Structure):
 EXPORTS = {
     "phx_last_error": (C.c_char_p, []),
     "phx_version": (C.c_char_p, []),
+    "phx_device_count": (C.c_int, [_ip]),
     "phx_zheev_batched": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _ip]),
     "phx_zheev_batched_dev": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
@@ -192,6 +213,13 @@ def _fortran_flat(m):
 
 
 # ------------------------------------------------------------------ eigensolver / Bloch
+def device_count() -> int:
+    """HIP devices visible to this process (0 without a GPU)."""
+    n = C.c_int(0)
+    _check(lib().phx_device_count(C.byref(n)))
+    return n.value
+
+
 def zheev_batched(A: np.ndarray):
     """Eigen-decomposition of a stack of Hermitian matrices A[M, N, N].
 
@@ -475,6 +503,8 @@ class ElectronLinewidths:
         self._h = C.c_void_p()
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         tr = TRANSFORMS[transform] if isinstance(transform, str) else int(transform)
+        self.precision = {v: k for k, v in PRECISIONS.items()}[prec]
+        self.transform = {v: k for k, v in TRANSFORMS.items()}[tr]
         if elph_file is not None:
             _check(lib().phx_lw_setup_file(C.byref(self._h), C.byref(self._s), _path(elph_file),
                                            prec, int(k1_batch), tr))
@@ -646,6 +676,38 @@ class ElectronLinewidths:
         out = np.empty(2 * p.nRe * p.nRp * p.nm * p.nw * p.nw)
         _check(lib().phx_lw_copy_gw(self._h, _p(out)))
         return out
+
+
+def linewidths_form(prob, form=CHOSEN, gW=None, k1_batch=32, elph_file=None):
+    """ElectronLinewidths set up in one of the named FORMS."""
+    precision, transform, screen, line_min = FORMS[form]
+    lw = ElectronLinewidths(prob, gW=gW, precision=precision, k1_batch=k1_batch, transform=transform,
+                            elph_file=elph_file)
+    if screen:
+        lw.set_screen(screen)
+    if line_min:
+        lw.set_line_batching(line_min)
+    return lw
+
+
+def linewidths_auto(prob, gW=None, line_batching=4):
+    """The reduced-precision builder for any system: the FP32 separable
+    transform with the 6-sigma screen where its kernels take the shape (at
+    most 32 modes, 8 windowed bands per point, nm nw <= 192), else the FP32
+    direct transform; mesh-line cacheElPh for lines of >= line_batching
+    points."""
+    lw = None
+    if prob.nm * prob.nw <= 192:
+        try:
+            lw = ElectronLinewidths(prob, gW=gW, precision="fp32", transform="separable")
+            lw.set_screen(6.0)
+        except PhxError:
+            lw = None
+    if lw is None:
+        lw = ElectronLinewidths(prob, gW=gW, precision="fp32", transform="direct")
+    if line_batching:
+        lw.set_line_batching(line_batching)
+    return lw
 
 
 def synth_gw_host(prob) -> np.ndarray:

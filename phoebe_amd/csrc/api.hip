@@ -767,6 +767,8 @@ struct LwHandle {
   DevBuf<double2> rowE2range;  // per active point: lowest and highest windowed band energy
   DevBuf<int4> meshRow;        // per mesh point: (active row or -1, nb, U pool offset, state offset)
   DevBuf<unsigned long long> keepMask;  // last-stage row masks of one panel chunk
+  DevBuf<double> screenOf;     // per state: screen width (sigma) of the panel whose b1 group starts there
+  DevBuf<double> xminPart;     // per (panel of a chunk, block): closest approach to resonance
   // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1551,6 +1553,10 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
     }
     const size_t slots = size_t(chunk) * nMtiles;
     if (p.mesh[0] <= 64) L->keepMask.reserve(slots);
+    if (L->screen > 0.) {
+      L->screenOf.reserve(size_t(L->numStates));
+      L->xminPart.reserve(size_t(chunk) * kXminBlocks);
+    }
     if (lw_split_last(L, p)) L->G3f.reserve(slots * p.mesh[0] * ncol);
     if (L->lsMode && L->mlpTrained) {
       L->lsFeat.reserve(slots * kMlpIn);
@@ -1751,6 +1757,10 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     cq.g0val = L->g0val.p;
     cq.tw0 = L->twx.p;
     cq.screen = L->screen;
+    if (L->screen > 0.) {
+      cq.screenOf = L->screenOf.p;
+      cq.xminPart = L->xminPart.p;
+    }
     cq.qWlo = L->qWlo.p;
     cq.qWhi = L->qWhi.p;
     cq.qModeGrp = L->qModeGrp.p;
@@ -2533,6 +2543,15 @@ extern "C" {
 
 const char* phx_last_error(void) { return phx::last_error(); }
 const char* phx_version(void) { return "phx_elph 0.1 (gfx950)"; }
+
+int phx_device_count(int* count) {
+  return guard([&] {
+    PHX_REQUIRE(count, "phx_device_count: null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
 
 int phx_zheev_batched(int M, int N, double* A, double* W, int* info) {
   return guard([&] {

@@ -66,6 +66,9 @@ struct CoupParams {
   const double2* tw0;                     // e^{2 pi i t / m0}, t < m0
   int rowLo, rowHi;                       // rows (active indices) taken by the mesh transform
   double screen;                          // energy screen in units of sigma (0: off)
+  double* screenOf;                       // per outer state: screen width (sigma) of the panel whose b1 group
+                                          // starts there (k_sep_xmin); null: `screen` for every panel
+  double* xminPart;                       // (panel, kXminBlocks) closest approach to resonance, energy units
   const double* qWlo; const double* qWhi; // per q: lowest / highest omega >= cutoff
   const double2* rowE2range;              // per row: (lowest, highest) band energy
   // learned screen (k_sep_xm): per (panel, yz) slot masks
@@ -95,6 +98,12 @@ struct CoupParams {
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };
+
+// Energy-screen width, in units of sigma, of the panel whose b1 group starts
+// at outer state is1 (see k_sep_xmin).
+__device__ inline double screen_width(const CoupParams& p, int is1) {
+  return p.screenOf ? p.screenOf[is1] : p.screen;
+}
 
 __device__ inline double gaussian_delta(double energy, double invW, double pref) {
   // GaussianDeltaFunction::getSmearing, delta_function.cpp:40-45
@@ -784,8 +793,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_x(CoupParams p) {
       if (p.screen > 0.) {
         const double wlo = p.qWlo[ri.iq], whi = p.qWhi[ri.iq];
         const double e2lo = p.e2pool[ri.soff], e2hi = p.e2pool[ri.soff + ri.nb2 - 1];
-        const double lim = p.screen / p.invSigma;
         const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
+        const double lim = screen_width(p, int(e1 - p.e1pool)) / p.invSigma;
         bool keep = false;
         if (whi >= wlo) {  // at least one mode above the cutoff
           for (int gi = 0; gi < p.panelGSize[panel]; ++gi) {
@@ -1131,6 +1140,7 @@ __device__ __forceinline__ void sep_epilogue_tail(const CoupParams& p, const Sep
                                                   const double (&n3)[CM], double (&part)[kGMax][CM],
                                                   const double* e1, int kk, int b1p, int gsize, double inv_g) {
   const int nm = p.nm, nCalc = p.nCalc;
+  const double scrW = p.screen > 0. ? screen_width(p, int(e1 - p.e1pool)) : 0.;
   // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub; |g|^2 in registers
   double v[kSepNb2];
 #pragma unroll
@@ -1227,7 +1237,7 @@ __device__ __forceinline__ void sep_epilogue_tail(const CoupParams& p, const Sep
             const double x1 = se1[gi] - en2 + w3, x2 = se1[gi] - en2 - w3;
             // with the energy screen on, a pair whose both weights are below
             // exp(-screen^2) of the peak is skipped, as whole rows are
-            if (p.screen > 0. && fabs(x1) * p.invSigma >= p.screen && fabs(x2) * p.invSigma >= p.screen)
+            if (p.screen > 0. && fmin(fabs(x1), fabs(x2)) * p.invSigma >= scrW)
               continue;
             d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
             d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
@@ -1261,6 +1271,7 @@ __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const
                                                       const double (&n3)[CM], double (&part)[GM][CM],
                                                       const double* e1, int gsize, double inv_g, int rpp) {
   const int nm = p.nm, nCalc = p.nCalc, nb2max = p.nb2max;
+  const double scrW = p.screen > 0. ? screen_width(p, int(e1 - p.e1pool)) : 0.;
   double* own = gsq + (size_t(r) * nm + sub) * nb2max;
   // g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), thread nu2 = sub
   if (modeOn)
@@ -1322,7 +1333,7 @@ __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const
         d2 = adaptive_delta(se1[gi] - en2 - w3, aw);
       } else {
         const double x1 = se1[gi] - en2 + w3, x2 = se1[gi] - en2 - w3;
-        if (p.screen > 0. && fabs(x1) * p.invSigma >= p.screen && fabs(x2) * p.invSigma >= p.screen) continue;
+        if (p.screen > 0. && fmin(fabs(x1), fabs(x2)) * p.invSigma >= scrW) continue;
         d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
         d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
       }
@@ -1461,8 +1472,8 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
         if (p.screen > 0.) {
           const double2 er = p.rowE2range[row];
           const double e2lo = er.x, e2hi = er.y;
-          const double lim = p.screen / p.invSigma;
           const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
+          const double lim = screen_width(p, int(e1 - p.e1pool)) / p.invSigma;
           keep = false;
           if (whi >= wlo) {  // at least one mode above the cutoff
             for (int gi = 0; gi < p.panelGSize[panel]; ++gi) {
@@ -1513,6 +1524,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
   const int nCalc = p.nCalc;
   const int gstart = p.panelGStart[panel], gsize = p.panelGSize[panel];
   const double* e1 = p.e1pool + p.kkStateOff[kk] + gstart;
+  const double scrW = p.screen > 0. ? screen_width(p, int(e1 - p.e1pool)) : 0.;
   const double inv_g = 1. / double(float(gsize));
 
   for (int m0r = 0; m0r < nkeep; m0r += 16) {
@@ -1778,7 +1790,7 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
                 const double x1 = se1[gi] - en2 + w3, x2 = se1[gi] - en2 - w3;
                 // with the energy screen on, a pair whose both weights are below
                 // exp(-screen^2) of the peak is skipped, as whole rows are
-                if (p.screen > 0. && fabs(x1) * p.invSigma >= p.screen && fabs(x2) * p.invSigma >= p.screen)
+                if (p.screen > 0. && fmin(fabs(x1), fabs(x2)) * p.invSigma >= scrW)
                   continue;
                 d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
                 d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
@@ -2122,6 +2134,86 @@ inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc, int rpp) {
   return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
 }
 
+constexpr int kXminBlocks = 16;
+
+// Closest approach to resonance of every panel of the chunk, for the energy
+// screen widths: the minimum over the panel's rows (inner points of the
+// range), their bands b2, the modes above the cutoff and the b1 group's
+// energies e1 of |e1 - e2 +- omega|.  Grid (kXminBlocks, panels): block b
+// takes mesh points b * 256 + t, strided by 256 * kXminBlocks.  A block first
+// evaluates only the rows that pass the interval test at the configured
+// width n: a row that fails it is at least n sigma from resonance in every
+// pair, so once the block has found a pair closer than n the skipped rows
+// cannot hold its minimum.  Otherwise it evaluates every row.  One partial
+// minimum (energy units, HUGE_VAL when there is no pair) per block.
+static __global__ void __launch_bounds__(256) k_sep_xmin(CoupParams p) {
+  const int panel = blockIdx.y, t = threadIdx.x;
+  const int kk = p.panelKK[panel];
+  const int kmesh = p.kkMesh[kk];
+  const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
+  const int gsz = p.panelGSize[panel];
+  const double lim = p.screen / p.invSigma;
+  const int nmesh = p.m0 * p.m1 * p.m2;
+  __shared__ double red[256];
+  double best = HUGE_VAL, bmin = HUGE_VAL;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int mesh = blockIdx.x * 256 + t; mesh < nmesh; mesh += 256 * kXminBlocks) {
+      const int row = p.meshToRow[mesh];
+      if (row < p.rowLo || row >= p.rowHi) continue;
+      const int iq = mesh_diff_index(mesh, kmesh, p.m0, p.m1, p.m2);
+      const double wlo = p.qWlo[iq], whi = p.qWhi[iq];
+      if (whi < wlo) continue;  // no mode above the cutoff
+      if (pass == 0) {
+        const double2 er = p.rowE2range[row];
+        bool keep = false;
+        for (int gi = 0; gi < gsz; ++gi) {
+          const double a_lo = e1[gi] - er.y + wlo, a_hi = e1[gi] - er.x + whi;
+          const double b_lo = e1[gi] - er.y - whi, b_hi = e1[gi] - er.x - wlo;
+          keep |= (a_hi > -lim && a_lo < lim) || (b_hi > -lim && b_lo < lim);
+        }
+        if (!keep) continue;
+      }
+      const double* w = p.Wpool + size_t(iq) * p.nm;
+      const int soff = p.rowStateOff[row], nb2 = p.rowNb2[row];
+      for (int b2 = 0; b2 < nb2; ++b2) {
+        const double e2 = p.e2pool[soff + b2];
+        for (int gi = 0; gi < gsz; ++gi) {
+          const double d = e1[gi] - e2;
+          for (int nu = 0; nu < p.nm; ++nu) {
+            const double om = w[nu];
+            if (om >= p.cutoff) best = fmin(best, fmin(fabs(d + om), fabs(d - om)));
+          }
+        }
+      }
+    }
+    red[t] = best;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) red[t] = fmin(red[t], red[t + s]);
+      __syncthreads();
+    }
+    bmin = red[0];
+    __syncthreads();
+    if (bmin < lim) break;  // block-uniform
+  }
+  if (t == 0) p.xminPart[size_t(panel) * kXminBlocks + blockIdx.x] = bmin;
+}
+
+// Screen width of each panel: sqrt(xmin^2 + n^2) sigma, with xmin its closest
+// approach to resonance in units of sigma.  A pair that the screen drops then
+// weighs at most exp(-n^2) of the panel's most resonant pair, whatever the
+// panel's distance to resonance (a panel whose pairs all lie in the Gaussian
+// tails keeps its closest rows).  One thread per panel.
+static __global__ void __launch_bounds__(64) k_sep_xmin_fin(CoupParams p) {
+  const int panel = blockIdx.x * 64 + threadIdx.x;
+  if (panel >= p.nPanels) return;
+  double m = HUGE_VAL;
+  for (int b = 0; b < kXminBlocks; ++b) m = fmin(m, p.xminPart[size_t(panel) * kXminBlocks + b]);
+  const double x = m * p.invSigma;
+  const int is1 = p.kkStateOff[p.panelKK[panel]] + p.panelGStart[panel];
+  p.screenOf[is1] = x < HUGE_VAL ? sqrt(x * x + p.screen * p.screen) : p.screen;
+}
+
 // Row lists of the last stage: for every (panel, yz) slot, bit jx of
 // keepMask[slot] is set when mesh point (jx, jy, jz) is an inner point of the
 // range and passes the energy screen (same interval test as k_sep_x; every
@@ -2136,7 +2228,7 @@ static __global__ void __launch_bounds__(256) k_sep_screen(CoupParams p) {
   const int kmesh = p.kkMesh[kk];
   const double* e1 = p.e1pool + p.kkStateOff[kk] + p.panelGStart[panel];
   const int gsz = p.panelGSize[panel];
-  const double lim = p.screen / p.invSigma;
+  const double lim = screen_width(p, int(e1 - p.e1pool)) / p.invSigma;
   unsigned long long mask = 0ull;
   for (int jx = 0; jx < p.m0; ++jx) {
     const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
@@ -2208,6 +2300,13 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
   const bool breg = bregt && (2 * p.nG0 + 3) / 4 <= 12;
   PHX_REQUIRE(!bf3 || (bf3t && reduce && sp.ncol <= 4 * 6 * 16),
               "split-BF16x3: FP32 data, the linewidth reduction and nm nw <= 384");
+  if (reduce && p.screen > 0. && p.screenOf) {
+    PHX_REQUIRE(p.xminPart && p.meshToRow, "energy screen: closest-approach buffers missing");
+    hipLaunchKernelGGL(k_sep_xmin, dim3(kXminBlocks, sp.nPanels), dim3(256), 0, s, p);
+    PHX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sep_xmin_fin, dim3(ceil_div(sp.nPanels, 64)), dim3(64), 0, s, p);
+    PHX_HIP(hipGetLastError());
+  }
   if (reduce && sp.ncol <= 4 * 6 * 16) {
     const bool wide = sp.ncol > 4 * 3 * 16;
     const size_t ldsXm = sep_xm_lds_bytes(sizeof(T), p.nG0, p.m0, p.nw, p.nm, p.nb2max, nmb <= 16 ? 16 : 8,

@@ -13,9 +13,14 @@ def pytest_configure(config):
 
 
 def gpu_available():
+    """A GPU is there when the ROCm driver's device node exists, or when the
+    HIP library reports a device.  With the device node present the GPU tests
+    are never skipped: a library that fails to load makes them fail."""
+    if os.path.exists("/dev/kfd"):
+        return True
     try:
-        import torch
-        return torch.cuda.is_available()
+        import phoebe_amd
+        return phoebe_amd.device_count() > 0
     except Exception:  # pragma: no cover
         return False
 

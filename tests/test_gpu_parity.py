@@ -72,7 +72,7 @@ def test_ph_bloch_diag_vs_oracle():
         assert projector_diff(sm * ev[i], sm * evo, wo) < 1e-9
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
 def test_coupling_squared_vs_oracle(precision, tol):
     say("calcCouplingSquared", precision)
     p = tiny_problem(natoms=2, nw=4, rg_e=3, rg_p=3, mesh=4, seed=9)
@@ -165,7 +165,7 @@ def test_fully_degenerate_bands_match_oracle(split):
     lw.run()
     err = rel_err(lw.linewidths(), ref)
     say("  max rel err", err.max())
-    assert err.max() < 1e-4
+    assert err.max() < 1e-6
 
 
 def test_coupling_api_repeated_calls_of_changing_size():
@@ -199,7 +199,7 @@ def test_coupling_api_repeated_calls_of_changing_size():
 
 
 @pytest.mark.parametrize("transform", ["direct", "separable"])
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
 def test_linewidths_vs_oracle(precision, tol, transform):
     say("linewidths", precision, transform)
     p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
@@ -266,7 +266,7 @@ def test_separable_k2_range_and_subsets():
 
 
 @pytest.mark.parametrize("nsigma", [6.0, 7.0])
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
 def test_energy_screen_keeps_parity(precision, tol, nsigma):
     """Rows skipped by the 7-sigma energy screen carry only Gaussian tails below
     exp(-49) of their peak: the linewidths still match the oracle."""
@@ -282,6 +282,59 @@ def test_energy_screen_keeps_parity(precision, tol, nsigma):
     assert err.max() < tol
 
 
+def _closest_approach(orc, p):
+    """Per outer state: min over every inner state and mode above the cutoff
+    of |e1 - e2 +- omega| (Ry), by brute force over the oracle's energies and
+    phonons (q = k2 - k1 folded by integer mesh arithmetic)."""
+    m = np.array(p.mesh)
+    om, _, _ = orc.phonons()
+    cut = O.phonon_cutoff()
+    nb = orc.band_hi - orc.band_lo + 1
+    start = np.concatenate([[0], np.cumsum(nb)])
+    coords = np.stack([orc.active % m[0], (orc.active // m[0]) % m[1], orc.active // (m[0] * m[1])], 1)
+    out = np.full(orc.num_states, np.inf)
+    for a1 in range(orc.num_active):
+        e1 = orc.energies[start[a1]:start[a1 + 1]]
+        d = (coords - coords[a1]) % m
+        iq = d[:, 2] * m[0] * m[1] + d[:, 1] * m[0] + d[:, 0]
+        for a2 in range(orc.num_active):
+            w = om[iq[a2]]
+            w = w[w >= cut]
+            e2 = orc.energies[start[a2]:start[a2 + 1]]
+            x = e1[:, None, None] - e2[None, :, None] + np.concatenate([w, -w])[None, None, :]
+            out[start[a1]:start[a1 + 1]] = np.minimum(out[start[a1]:start[a1 + 1]], np.abs(x).min(axis=(1, 2)))
+    return out
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
+def test_energy_screen_keeps_tail_only_linewidths(precision, tol):
+    """States whose every partner lies in the Gaussian tails (closest approach
+    to resonance between 6 and 15 sigma) have small but nonzero linewidths in
+    the reference (its Gaussian is cut only at x > 6, delta_function.cpp:40-45).
+    The screen widens per panel to sqrt(xmin^2 + n^2) sigma, so those
+    linewidths still match the oracle instead of dropping to zero."""
+    say("energy screen, tail-only states", precision)
+    p = dataclasses.replace(tiny_problem(natoms=2, nw=4, rg_e=3, rg_p=3, mesh=4, seed=9))
+    gW = p.gw_host()
+    xmin = _closest_approach(O.Oracle(p, gW.reshape(-1)), p)
+    # sigma such that the state farthest from resonance sits 10 sigma away
+    p.sigma = float(np.max(xmin[np.isfinite(xmin)])) / 10.0
+    x = xmin / p.sigma
+    tail = (x > 6.5) & (x < 15.0)
+    say("  sigma", p.sigma, "tail-only states", int(tail.sum()), "of", len(x))
+    assert tail.sum() >= 1
+    ref = O.Oracle(p, gW.reshape(-1)).linewidths(average=False)
+    assert np.all(ref[0, tail] > 0)
+    for nsigma in (6.0, 7.0):
+        s = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="separable")
+        s.set_screen(nsigma)
+        s.run()
+        got = s.linewidths(average=False)
+        err = rel_err(got, ref)
+        say("  screen", nsigma, "max rel err", err.max(), "on tail-only states", err[:, tail].max())
+        assert err.max() < tol
+
+
 def test_screen_requires_separable_and_valid_width():
     p = tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=5, seed=21)
     d = P.ElectronLinewidths(p, gW=p.gw_host(), precision="fp64", transform="direct")
@@ -293,7 +346,7 @@ def test_screen_requires_separable_and_valid_width():
 
 
 @pytest.mark.parametrize("transform", ["direct", "separable"])
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
 def test_line_batching_vs_oracle(precision, tol, transform):
     """Outer points batched by mesh line (cacheElPh streams gW once per line)
     give the oracle's linewidths; lines below the threshold take the GEMM path."""
@@ -347,7 +400,13 @@ def test_line_batching_argument_validation():
 
 @pytest.mark.parametrize("mesh", [(5, 5, 5), (6, 4, 5)])
 def test_split_bf16x3_separable_tracks_fp64(mesh):
-    """Split-BF16x3 x-stage: linewidths within the split-BF16 error level of FP64."""
+    """Split-BF16x3 x-stage (hi + lo BF16 halves of every FP32 operand, the
+    lo*lo product dropped, FP32 accumulation): about 16 significant bits per
+    operand instead of FP32's 24.  Measured max linewidth errors against the
+    oracle are 2.6e-6 and 3.2e-6 on these meshes, outside the 1e-6 gate that
+    the chosen FP32 form meets (cfg2 measures 3.7e-7), so split-BF16x3 is a
+    precision variant, not a gated form.  The bound asserts its own error
+    level, 1e-5."""
     say("bf16x3", mesh)
     p = synth.make_problem(2, 6, 3, 3, mesh, seed=21, name="bf3")
     gW = p.gw_host()
@@ -357,7 +416,7 @@ def test_split_bf16x3_separable_tracks_fp64(mesh):
     s.run()
     err = rel_err(s.linewidths(), ref)
     say("  max rel err", err.max())
-    assert err.max() < 1e-3
+    assert err.max() < 1e-5
     with pytest.raises(P.PhxError):
         P.ElectronLinewidths(p, gW=gW, precision="bf16x3", transform="direct")
 
@@ -370,7 +429,7 @@ def test_wide_panels_direct_vs_oracle(natoms, nw):
     gW = p.gw_host()
     orc = O.Oracle(p, gW.reshape(-1))
     ref = orc.linewidths()
-    for precision, tol in (("fp64", 1e-10), ("fp32", 1e-4)):
+    for precision, tol in (("fp64", 1e-10), ("fp32", 1e-6)):
         for lines in (0, 1):  # GEMM and mesh-line forms of cacheElPh (nw need not divide 256)
             lw = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="direct")
             lw.set_line_batching(lines)
@@ -378,3 +437,31 @@ def test_wide_panels_direct_vs_oracle(natoms, nw):
             err = rel_err(lw.linewidths(), ref)
             say("  ", precision, "lines", lines, "states", lw.num_states, "max rel err", err.max())
             assert err.max() < tol
+
+
+def test_chosen_form_vs_oracle():
+    """The chosen form exactly as measured (FP32 mesh-line cacheElPh, FP32
+    separable transform, 6-sigma screen, split last stage) against the FP64
+    oracle at the 1e-6 gate, on a mesh whose x lines hold 10 points so that
+    most outer points take the line form."""
+    say("chosen form", P.CHOSEN, P.FORMS[P.CHOSEN])
+    p = synth.make_problem(2, 6, 3, 3, (10, 5, 6), seed=31, name="chosen")
+    gW = p.gw_host()
+    orc = O.Oracle(p, gW.reshape(-1))
+    ref = orc.linewidths()
+    lw = P.linewidths_form(p, P.CHOSEN, gW=gW)
+    assert np.array_equal(lw.active, orc.active)
+    assert np.array_equal(lw.band_lo, orc.band_lo) and np.array_equal(lw.band_hi, orc.band_hi)
+    counts = np.bincount(lw.active // p.mesh[0])
+    say("  lines with >= 8 points:", int((counts >= 8).sum()), "of", int((counts > 0).sum()))
+    assert (counts >= 8).sum() >= 10
+    lw.set_timing(True)
+    assert lw.run() == orc.last_pairs
+    lw.sync()
+    st = lw.split_times()
+    say("  split last-stage launches", st["launches"])
+    assert st["launches"] > 0
+    err = rel_err(lw.linewidths(), ref)
+    say("  states", lw.num_states, "max rel err", err.max())
+    assert np.all(ref > 0)
+    assert err.max() < 1e-6

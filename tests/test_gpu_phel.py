@@ -65,15 +65,17 @@ def test_phel_partial_runs_accumulate():
 
 @pytest.mark.parametrize("smearing,precision,tol,qm",
                          [("gaussian", "fp64", 1e-10, (3, 3, 3)), ("adaptive", "fp64", 1e-7, (3, 3, 3)),
-                          ("gaussian", "fp32", 1e-4, (3, 3, 3)), ("gaussian", "fp64", 1e-10, (2, 2, 2)),
+                          ("gaussian", "fp32", 3e-6, (3, 3, 3)), ("gaussian", "fp64", 1e-10, (2, 2, 2)),
                           ("gaussian", "fp64", 1e-10, (4, 4, 4)), ("adaptive", "fp64", 1e-7, (2, 2, 2))])
 def test_phel_si_vs_oracle(smearing, precision, tol, qm):
     """Si from the reference's files: 444_silicon.fc (no sum rule: long-range
     term on), si_tb.dat, silicon.phoebe.elph.hdf5; k mesh 4^3.  The oracle
     diagonalizes every k2 = k1 + q3; the GPU takes them from the k mesh when
     the q mesh divides it (Gaussian smearing).  Adaptive smearing inherits the
-    1e-8 relative noise of the finite-difference velocities; FP32 couplings
-    carry FP32 rounding."""
+    1e-8 relative noise of the finite-difference velocities.  FP32 couplings
+    measure 1.1e-6 here: the ph-el builder's measured form is FP64, and its
+    FP32 option is outside the 1e-6 gate of the electron builder's chosen
+    form; the bound states that error level (3e-6)."""
     say("ph-el Si", smearing, precision, qm)
     ph = Hm.PhononH0(FC4, "")
     el = Hm.ElectronH0Wannier(TB, ph.cell)
