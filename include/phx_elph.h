@@ -124,22 +124,69 @@ int phx_elph_cache(phx_elph* h, const double* U1, int nb1, const double* k1);
  * (interaction_elph.h:149-154, interaction_elph.cpp:229-448).
  *   U2s  : concatenation of the nq matrices (nw x nb2s[i]) column-major;
  *   e3s  : nq x (nm x nm) column-major;  q3: nq x 3 Cartesian;
- *   polar: must be NULL (polar term unsupported);
+ *   polar: NULL (the reference's usePolarCorrection = false), or nq x nm
+ *          complex: polarCorrectionPart1 of each q (phx_elph_polar_part1);
+ *          gFinal(nu, b1, b2) += polar(q, nu) sum_w conj(U2(w, b2)) U1(w, b1)
+ *          before |g|^2 for every q with |q3| > 1e-8 (:285-312, :406-413,
+ *          polarCorrectionPart2 :205-227);
  *   out  : concatenation of the nq tensors (nb1, nb2s[i], nm) column-major,
  *          the getCouplingSquared(ik) layout (interaction_elph.cpp:434-446). */
 int phx_elph_coupling_sq(phx_elph* h, int nq, const double* U2s, const int* nb2s,
                          const double* e3s, const double* q3, const double* polar,
                          double* out);
 
+/* ---------------------------------------------------------------- polar term
+ * The Froehlich long-range part of the coupling (Verdi and Giustino, PRL 115,
+ * 176401, Eq. 4), as InteractionElPhWan's polarCorrectionPart1Static
+ * (interaction_elph.cpp:152-203) computes it. */
+typedef struct {
+  int natoms;
+  const double* atomPos;     /* natoms x 3, Cartesian Bohr */
+  const double* bornCharges; /* natoms x 3 x 3: Z(atom, i, pol), gqDotZ = sum_i (q+G)_i Z(atom, i, pol) */
+  const double* dielectric;  /* 3 x 3 */
+  double recip[9];           /* reciprocal cell, column-major, columns b_i */
+  double volume;             /* unit-cell volume, Bohr^3 */
+  int coarseMesh[3];         /* the force constants' q grid: G = sum_i m_i b_i, |m_i| <= coarseMesh[i] */
+} phx_polar;
+
+/* Whether the reference turns the polar term on (interaction_elph.cpp:24-32):
+ * a dielectric tensor with squared norm > 1e-10 and more than one species.
+ * Returns 1 or 0. */
+int phx_polar_in_use(const double* dielectric, int num_species);
+
+/* Stores the polar parameters on the interaction (the Crystal and PhononH0
+ * fields polarCorrectionPart1 reads, :140-151); NULL clears them. */
+int phx_elph_set_polar(phx_elph* h, const phx_polar* polar);
+
+/* InteractionElPhWan::polarCorrectionPart1(q3, ev3) for nq wavevectors on the
+ * GPU: x(nu) = sum over q+G with 0 < (q+G).eps.(q+G) < 4 gMax (gMax = 14) of
+ * i 2 fourPi / V e^{-(q+G).eps.(q+G)/4} / ((q+G).eps.(q+G))
+ * e^{-i (q+G).tau_a} ((q+G).Z(a, :, pol)) e3(3 a + pol, nu), with
+ * fourPi = 2 pi as constants.h:12 defines it.
+ *   q3 : nq x 3 Cartesian;  e3s: nq x (nm x nm) column-major;
+ *   out: nq x nm complex. */
+int phx_elph_polar_part1(phx_elph* h, int nq, const double* q3, const double* e3s, double* out);
+
 /* InteractionElPhWan::getCouplingDimensions() (interaction_elph.cpp:450-456):
  * the device extents (nRe, nRp, nm, nw, nw). */
 int phx_elph_get_coupling_dims(phx_elph* h, int* dims5);
 
 /* InteractionElPhWan::estimateNumBatches(nk2, nb1) (interaction_elph.cpp:458-490)
- * with the reference's memory model and a budget of max_mem_bytes (<= 0:
- * the MAXMEM environment variable in GB, default 16, common_kokkos.cpp:107-112).
- * Returns the number of batches (>= 1) or a negative status. */
-int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double max_mem_bytes);
+ * with the reference's memory model: ceil(nk2 * bytes per q / available).
+ *   available_bytes > 0: the caller's DeviceManager::getAvailableMemory()
+ *     (common_kokkos.cpp:131-133), i.e. MAXMEM minus what every object
+ *     registered, this one included (phx_elph_device_memory_usage);
+ *   available_bytes <= 0: a DeviceManager holding only this object, MAXMEM
+ *     from the environment in GB, default 16 (common_kokkos.cpp:104-112).
+ * Returns the number of batches (0 when nk2 = 0), or a negative status with
+ * "Insufficient memory!" when one q does not fit (:477-488). */
+int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double available_bytes);
+
+/* InteractionElPhWan::getDeviceMemoryUsage() (interaction_elph.cpp:703-707):
+ * the bytes the reference registers with its DeviceManager for this object
+ * (16 per element of the coupling tensor and of elPhCached after a
+ * cacheElPh, 8 per R-vector component and degeneracy). */
+int phx_elph_device_memory_usage(phx_elph* h, double* bytes);
 
 int phx_elph_destroy(phx_elph* h);
 
@@ -336,6 +383,15 @@ int phx_lw_reset(phx_lw* h);
  * turns it off (the default); otherwise nsigma >= 6.  No counterpart in the
  * reference, which evaluates every pair. */
 int phx_lw_set_screen(phx_lw* h, double nsigma);
+
+/* The polar term of the linewidth builder (el_scattering.cpp through
+ * helper_el_scattering.cpp:170-195 and calcCouplingSquared :285-312,
+ * :406-413): polarCorrectionPart1 of every q of the q-set (folded mesh
+ * wavevector, its mass-scaled eigenvectors) is computed on the GPU, and every
+ * pair with k2 != k1 adds polar(q, nu) (U2^H U1)(b2, b1) to gFinal.  Runs with
+ * the direct transform (the separable transform raises an error with it on);
+ * NULL turns it off.  Call after phx_lw_setup. */
+int phx_lw_set_polar(phx_lw* h, const phx_polar* polar);
 /* ---- Sweeps without host round trips.  Prepare every batch of a sweep over
  * k1_list (all active points when null; same batching as phx_lw_run) with
  * its own device metadata and all scratch sized; returns the batch count.

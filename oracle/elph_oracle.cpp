@@ -415,9 +415,53 @@ void orc_cache_elph(int nw, int nm, int nRp, int nRe, const double* gW, const do
 // column-major (rows = 3*atom+pol, columns = modes).  q3: nq x 3 Cartesian.
 // out: packed per q, each (nb1, nb2_q, nm) column-major.
 // ---------------------------------------------------------------------------
-void orc_coupling_sq(int nw, int nm, int nRp, const double* Rp, const double* degP,
-                     const double* cached, int nb1, int nq, const double* U2s,
-                     const int* nb2s, const double* e3s, const double* q3, double* out) {
+// polarCorrectionPart1Static (interaction_elph.cpp:152-203): x(nu) summed
+// over q+G, G = m1 b1 + m2 b2 + m3 b3 with |m_i| <= coarse[i], for
+// 0 < qEq = (q+G).eps.(q+G) and qEq / 4 < gMax = 14, of
+// i chargeSquare fourPi / V e^{-qEq/4} / qEq e^{-i (q+G).tau} ((q+G).Z) ev3(k, nu),
+// chargeSquare = 2, fourPi = 2 pi (constants.h:12), k = 3 iAt + iPol
+// (compress2Indices, utilities.cpp:29-33).  born: (natoms, 3, 3) Z(iAt, i, iPol);
+// eps (3, 3) row-major; recip column-major; e3 (nm, nm) column-major; x nm complex.
+void orc_polar_part1(int natoms, const double* pos, const double* born, const double* eps,
+                     const double* recip, double volume, const int* coarse, int nm, const double* q,
+                     const double* e3, double* x) {
+  const double gMax = 14.;
+  const double chargeSquare = 2.;
+  const double fourPi = 2. * kPi;
+  const cd factor = chargeSquare * fourPi / volume * cd(0., 1.);
+  std::vector<cd> xs(nm, cd(0.));
+  for (int m1 = -coarse[0]; m1 <= coarse[0]; m1++)
+    for (int m2 = -coarse[1]; m2 <= coarse[1]; m2++)
+      for (int m3 = -coarse[2]; m3 <= coarse[2]; m3++) {
+        double G[3];
+        for (int i = 0; i < 3; ++i) G[i] = recip[i] * m1 + recip[i + 3] * m2 + recip[i + 6] * m3 + q[i];
+        double qEq = 0.;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) qEq += G[i] * eps[3 * i + j] * G[j];
+        if (!(qEq > 0. && qEq / 4. < gMax)) continue;
+        const cd factor2 = factor * std::exp(-qEq / 4.) / qEq;
+        for (int iAt = 0; iAt < natoms; iAt++) {
+          const double arg = -(G[0] * pos[3 * iAt] + G[1] * pos[3 * iAt + 1] + G[2] * pos[3 * iAt + 2]);
+          const cd factor3 = factor2 * cd(std::cos(arg), std::sin(arg));
+          for (int iPol = 0; iPol < 3; ++iPol) {
+            const double* Z = born + 9 * iAt;
+            const double gqDotZ = G[0] * Z[iPol] + G[1] * Z[3 + iPol] + G[2] * Z[6 + iPol];
+            const int k = 3 * iAt + iPol;
+            for (int ib3 = 0; ib3 < nm; ib3++) xs[ib3] += factor3 * gqDotZ * ld(e3, k + size_t(nm) * ib3);
+          }
+        }
+      }
+  for (int ib3 = 0; ib3 < nm; ++ib3) st(x, ib3, xs[ib3]);
+}
+
+// calcCouplingSquared with the polar term: polar (nq x nm complex, or null)
+// is polarCorrectionPart1 of each q; where usePolarCorrection and |q3| > 1e-8
+// (:285-312) gFinal(nu, b1, b2) += x(nu) overlap(b1, b2), overlap = (U2^H U1)^T
+// (polarCorrectionPart2, :205-227), before |g|^2 (:406-413).
+void orc_coupling_sq_polar(int nw, int nm, int nRp, const double* Rp, const double* degP,
+                           const double* cached, int nb1, int nq, const double* U2s,
+                           const int* nb2s, const double* e3s, const double* q3, const double* U1,
+                           const double* polar, double* out) {
   std::vector<size_t> u2off(nq + 1, 0), outoff(nq + 1, 0);
   int nb2max = 0;
   for (int iq = 0; iq < nq; ++iq) {
@@ -455,15 +499,36 @@ void orc_coupling_sq(int nw, int nm, int nRp, const double* Rp, const double* de
     const int nb2 = nb2s[ik];
     const double* U2 = U2s + 2 * u2off[ik];
     double* o = out + outoff[ik];
+    const double qn = std::sqrt(q3[3 * ik] * q3[3 * ik] + q3[3 * ik + 1] * q3[3 * ik + 1] +
+                                q3[3 * ik + 2] * q3[3 * ik + 2]);
+    const bool usePolar = polar != nullptr && qn > 1.0e-8;
+    std::vector<cd> overlap;  // (nb1, nb2)
+    if (usePolar) {
+      overlap.assign(size_t(nb1) * nb2, cd(0.));
+      for (int ib1 = 0; ib1 < nb1; ++ib1)
+        for (int ib2 = 0; ib2 < nb2; ++ib2) {
+          cd t = 0.;
+          for (int w = 0; w < nw; ++w)
+            t += std::conj(ld(U2, w + size_t(nw) * ib2)) * ld(U1, w + size_t(nw) * ib1);
+          overlap[ib1 + size_t(nb1) * ib2] = t;
+        }
+    }
     for (int nu = 0; nu < nm; ++nu)
       for (int ib1 = 0; ib1 < nb1; ++ib1)
         for (int ib2 = 0; ib2 < nb2; ++ib2) {
           cd tmp = 0.;
           for (int iw2 = 0; iw2 < nw; ++iw2)
             tmp += std::conj(ld(U2, iw2 + size_t(nw) * ib2)) * g4[(size_t(nu) * nb1 + ib1) * nw + iw2];
+          if (usePolar) tmp += ld(polar, size_t(ik) * nm + nu) * overlap[ib1 + size_t(nb1) * ib2];
           o[ib1 + size_t(nb1) * (ib2 + size_t(nb2) * nu)] = std::norm(tmp);
         }
   }
+}
+
+void orc_coupling_sq(int nw, int nm, int nRp, const double* Rp, const double* degP,
+                     const double* cached, int nb1, int nq, const double* U2s,
+                     const int* nb2s, const double* e3s, const double* q3, double* out) {
+  orc_coupling_sq_polar(nw, nm, nRp, Rp, degP, cached, nb1, nq, U2s, nb2s, e3s, q3, nullptr, nullptr, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -681,6 +746,12 @@ struct orc_problem {
   const double *atomPos, *bornCharges, *dielectric, *gVectors, *lrConstant;
   double lrNorm, gMax;
   int smearing;    // 0 Gaussian, 1 adaptive Gaussian
+  // polar term of the coupling (0: off); the long-range fields' atoms, with
+  // their own Born charges and dielectric tensor
+  int polarOn;
+  const double *polarPos, *polarBorn, *polarEps;
+  double polarVolume;
+  int polarCoarse[3];
 };
 
 struct orc_state {
@@ -700,6 +771,7 @@ struct orc_state {
   std::vector<double> omega;      // nk x nm
   std::vector<double> phev;       // nk x (nm x nm) col-major interleaved
   std::vector<char> qInSet;
+  std::vector<double> polarX;     // nk x nm complex: polarCorrectionPart1 of each q of the q-set
 };
 
 static void cart_of(const orc_problem& p, const double* x, double* c) {
@@ -861,6 +933,17 @@ void* orc_setup(const orc_problem* prob) {
     delete s;
     return nullptr;
   }
+  if (p.polarOn) {  // helper_el_scattering.cpp:170-195: Part1 at every q3 of the q-set
+    s->polarX.assign(2 * size_t(s->nk) * p.nm, 0.);
+    const int natoms = p.nm / 3;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int iq = 0; iq < s->nk; ++iq) {
+      if (!s->qInSet[iq]) continue;
+      orc_polar_part1(natoms, p.polarPos, p.polarBorn, p.polarEps, p.recip, p.polarVolume, p.polarCoarse, p.nm,
+                      &s->kcart[3 * size_t(iq)], &s->phev[2 * size_t(iq) * p.nm * p.nm],
+                      &s->polarX[2 * size_t(iq) * p.nm]);
+    }
+  }
   return s;
 }
 
@@ -955,8 +1038,15 @@ long long orc_linewidth_pairs(void* h, int nk1, const int* k1List, int k2Begin, 
       outoff[j + 1] = outoff[j] + size_t(nb1) * nb2s[j] * p.nm;
     }
     std::vector<double> coupling(outoff[nq]);
-    orc_coupling_sq(p.nw, p.nm, p.nRp, p.Rp, p.degP, cached.data(), nb1, nq, U2s.data(),
-                    nb2s.data(), e3s.data(), q3.data(), coupling.data());
+    std::vector<double> polar;
+    if (p.polarOn) {
+      polar.resize(2 * size_t(nq) * p.nm);
+      for (int j = 0; j < nq; ++j)
+        std::copy(s->polarX.begin() + 2 * size_t(iq3s[j]) * p.nm, s->polarX.begin() + 2 * size_t(iq3s[j] + 1) * p.nm,
+                  polar.begin() + 2 * size_t(j) * p.nm);
+    }
+    orc_coupling_sq_polar(p.nw, p.nm, p.nRp, p.Rp, p.degP, cached.data(), nb1, nq, U2s.data(), nb2s.data(),
+                          e3s.data(), q3.data(), U1, p.polarOn ? polar.data() : nullptr, coupling.data());
     // symmetrize coupling (el_scattering.cpp:216-224)
 #pragma omp parallel for schedule(dynamic)
     for (int j = 0; j < nq; ++j) {

@@ -33,6 +33,8 @@ class OrcProblem(C.Structure):
         ("dielectric", _dp), ("gVectors", _dp), ("lrConstant", _dp),
         ("lrNorm", C.c_double), ("gMax", C.c_double),
         ("smearing", C.c_int),
+        ("polarOn", C.c_int), ("polarPos", _dp), ("polarBorn", _dp), ("polarEps", _dp),
+        ("polarVolume", C.c_double), ("polarCoarse", C.c_int * 3),
     ]
 
 
@@ -75,6 +77,7 @@ def lib():
         L.orc_adaptive_delta.restype = C.c_double
         L.orc_adaptive_delta.argtypes = [C.c_double, _dp, _dp, _ip]
         L.orc_el_group_velocities.restype = C.c_int
+        L.orc_polar_part1.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, _ip, C.c_int, _dp, _dp, _dp]
         L.orc_phel.restype = C.c_longlong
         L.orc_phel.argtypes = [C.POINTER(OrcProblem), _ip, C.c_int, _ip, C.c_int, _dp, _dp]
         _lib = L
@@ -181,6 +184,41 @@ def coupling_sq(nw, nm, Rp, degP, cached, U2s, e3s, q3s):
     return res
 
 
+def polar_part1(pol, recip, q, e3):
+    """polarCorrectionPart1Static at q with eigenvectors e3 (nm, nm); pol: dict of
+    positions, born, dielectric, volume, coarse_mesh."""
+    nm = e3.shape[0]
+    x = np.zeros(2 * nm)
+    rc = np.asfortranarray(recip, dtype=np.float64).reshape(-1, order="F").copy()
+    cm = np.ascontiguousarray(pol["coarse_mesh"], dtype=np.int32)
+    lib().orc_polar_part1(len(pol["positions"]), _p(_c(pol["positions"])), _p(_c(pol["born"])),
+                          _p(_c(pol["dielectric"])), _p(rc), C.c_double(pol["volume"]), cm.ctypes.data_as(_ip), nm,
+                          _p(_c(q)), _p(cplx_view(np.asfortranarray(e3).reshape(-1, order="F"))), _p(x))
+    return x.view(np.complex128)
+
+
+def coupling_sq_polar(nw, nm, Rp, degP, cached, U2s, e3s, q3s, U1, polar):
+    """coupling_sq with the polar term (polar: nq x nm complex, or None)."""
+    nRp = len(Rp)
+    nb1 = cached.shape[2]
+    nq = len(U2s)
+    nb2s = np.array([u.shape[1] for u in U2s], dtype=np.int32)
+    U2p = np.concatenate([np.asfortranarray(u, dtype=np.complex128).reshape(-1, order="F") for u in U2s])
+    e3p = np.concatenate([np.asfortranarray(e, dtype=np.complex128).reshape(-1, order="F") for e in e3s])
+    out = np.zeros(int(nb1 * nm * nb2s.sum()))
+    pp = None if polar is None else _p(cplx_view(np.asarray(polar).reshape(nq, nm)))
+    lib().orc_coupling_sq_polar(nw, nm, nRp, _p(_c(Rp)), _p(_c(degP)), _p(cplx_view(cached)), nb1, nq,
+                                _p(cplx_view(U2p)), nb2s.ctypes.data_as(_ip), _p(cplx_view(e3p)),
+                                _p(_c(np.asarray(q3s).reshape(-1))),
+                                _p(cplx_view(np.asfortranarray(U1).reshape(-1, order="F"))), pp, _p(out))
+    res, off = [], 0
+    for nb2 in nb2s:
+        n = nb1 * nb2 * nm
+        res.append(out[off:off + n].reshape((nb1, nb2, nm), order="F"))
+        off += n
+    return res
+
+
 def symmetrize_coupling(c, e1, e2, e3):
     a = np.asfortranarray(c, dtype=np.float64).copy()
     nb1, nb2, nm = a.shape
@@ -259,6 +297,14 @@ class Oracle:
             s.lrConstant = _p(keep(lr["constant"]))
             s.lrNorm, s.gMax = float(lr["norm"]), float(lr["gMax"])
         s.smearing = int(getattr(prob, "smearing", 0))
+        pol = getattr(prob, "polar", None)
+        if pol is not None:
+            s.polarOn = 1
+            s.polarPos, s.polarBorn = _p(keep(pol["positions"])), _p(keep(pol["born"]))
+            s.polarEps = _p(keep(pol["dielectric"]))
+            s.polarVolume = float(pol["volume"])
+            for i in range(3):
+                s.polarCoarse[i] = int(pol["coarse_mesh"][i])
         self._s = s
         self.h = L.orc_setup(C.byref(s))
         if not self.h:
@@ -350,3 +396,28 @@ def el_group_velocities(prob, k):
                                        _p(cplx_view(prob.h0R)), _p(_c(k)), _p(out))
     assert rc == 0
     return out.reshape(prob.nw, 3)
+
+
+def accounted_bytes(nw, nm, nRp, nRe, nb1_cached=0):
+    """InteractionElPhWan::getDeviceMemoryUsage (interaction_elph.cpp:703-707):
+    16 bytes per element of the coupling tensor and of elPhCached, 8 per
+    element of the R vectors and degeneracies."""
+    return 16.0 * (nRe * nRp * nm * nw * nw + nRp * nm * nb1_cached * nw) + 8.0 * (nRp + 3 * nRp + 3 * nRe + nRe)
+
+
+def estimate_num_batches(nw, nm, nRp, nk2, nb1, available):
+    """InteractionElPhWan::estimateNumBatches (interaction_elph.cpp:458-490):
+    ceil(nk2 * per-q bytes / available); None for the "Insufficient memory!"
+    error (one q does not fit)."""
+    import math
+    evs = 16 * (nw * nw + nm * nm)
+    phase = 16 * nRp
+    g3 = 2 * 16 * nm * nb1 * nw
+    g4 = 2 * 16 * nm * nb1 * nw
+    gfinal = 2 * 16 * nm * nb1 * nw
+    coupling = 16 * nb1 * nw * nm
+    polar = 16 * nm * nb1 * nw
+    usage = nk2 * (evs + polar + max(phase + g3, g3 + g4, g4 + gfinal, gfinal + coupling))
+    if nk2 > 0 and available < usage / nk2:
+        return None
+    return int(math.ceil(usage / available))

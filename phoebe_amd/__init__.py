@@ -101,6 +101,11 @@ EXPORTS = {
     "phx_elph_coupling_sq": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp]),
     "phx_elph_get_coupling_dims": (C.c_int, [C.c_void_p, _ip]),
     "phx_elph_estimate_num_batches": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+    "phx_elph_device_memory_usage": (C.c_int, [C.c_void_p, _dp]),
+    "phx_polar_in_use": (C.c_int, [_dp, C.c_int]),
+    "phx_elph_set_polar": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phx_elph_polar_part1": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "phx_lw_set_polar": (C.c_int, [C.c_void_p, C.c_void_p]),
     "phx_elph_destroy": (C.c_int, [C.c_void_p]),
     "phx_lw_setup": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PhxProblem), C.c_int, C.c_int,
                                C.c_int]),
@@ -350,6 +355,38 @@ def write_elph_file(path, gW, elR, elDeg, phR, phDeg, numElectrons, numSpin=1,
                                      _p(eD), _p(pR), _p(pD), _p(g.view(np.float64))))
 
 
+# ------------------------------------------------------------------ polar term
+class PhxPolar(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("atomPos", _dp), ("bornCharges", _dp), ("dielectric", _dp),
+                ("recip", C.c_double * 9), ("volume", C.c_double), ("coarseMesh", C.c_int * 3)]
+
+
+def polar_struct(positions, born, dielectric, recip, volume, coarse_mesh):
+    """phx_polar from atomic positions (natoms, 3) Bohr, Born charges
+    Z(atom, i, pol) (natoms, 3, 3), the dielectric tensor (3, 3), the
+    reciprocal cell (columns b_i), the cell volume (Bohr^3) and the force
+    constants' q grid.  Returns (struct, arrays it points into)."""
+    pos, z, eps = _f64(positions), _f64(born), _f64(dielectric)
+    if pos.ndim != 2 or pos.shape[1] != 3 or z.shape != (len(pos), 3, 3) or eps.shape != (3, 3):
+        raise PhxError("polar_struct: inconsistent shapes")
+    st = PhxPolar()
+    st.natoms = len(pos)
+    st.atomPos, st.bornCharges, st.dielectric = _p(pos), _p(z), _p(eps)
+    rc = np.asarray(recip, dtype=np.float64)
+    for j in range(3):
+        for i in range(3):
+            st.recip[i + 3 * j] = rc[i, j]
+    st.volume = float(volume)
+    for i in range(3):
+        st.coarseMesh[i] = int(coarse_mesh[i])
+    return st, (pos, z, eps)
+
+
+def polar_in_use(dielectric, num_species):
+    """The reference's usePolarCorrection test (interaction_elph.cpp:24-32)."""
+    return bool(lib().phx_polar_in_use(_p(_f64(dielectric)), int(num_species)))
+
+
 # ------------------------------------------------------------------ InteractionElPhWan
 class InteractionElPhWan:
     """Wannier electron-phonon coupling on the GPU (interaction_elph.h:30-219)."""
@@ -403,9 +440,31 @@ class InteractionElPhWan:
         self._nb1 = int(eigvec1.shape[1])
         _check(lib().phx_elph_cache(self._h, _p(U1.view(np.float64)), self._nb1, _p(_f64(k1C))))
 
+    usePolarCorrection = False
+
+    def setPolar(self, positions, born, dielectric, recip, volume, coarse_mesh, num_species):
+        """The Crystal / PhononH0 fields of the polar term (interaction_elph.cpp:
+        24-32, 140-151); the term is used when the reference would use it."""
+        st, self._polar_keep = polar_struct(positions, born, dielectric, recip, volume, coarse_mesh)
+        _check(lib().phx_elph_set_polar(self._h, C.byref(st)))
+        self.usePolarCorrection = polar_in_use(dielectric, num_species)
+
+    def polarCorrectionPart1(self, q3, ev3):
+        """x(nu) of the Froehlich term at q3 (interaction_elph.cpp:138-203)."""
+        return self.polarCorrectionPart1Batch([q3], [ev3])[0]
+
+    def polarCorrectionPart1Batch(self, q3s, ev3s):
+        nq = len(q3s)
+        nm = self.numPhBands
+        q = _f64(np.asarray(q3s).reshape(nq, 3))
+        e3 = np.concatenate([_fortran_flat(e) for e in ev3s])
+        out = np.zeros(2 * nq * nm)
+        _check(lib().phx_elph_polar_part1(self._h, nq, _p(q), _p(e3.view(np.float64)), _p(out)))
+        return out.view(np.complex128).reshape(nq, nm)
+
     def calcCouplingSquared(self, eigvec1, eigvecs2, eigvecs3, q3Cs, polarData=None):
-        if polarData is not None and any(np.any(p != 0) for p in polarData):
-            raise PhxError("polar correction is not supported")
+        """polarData: per q the polarCorrectionPart1 vector (nm), used when
+        usePolarCorrection (interaction_elph.cpp:285-312, 406-413)."""
         nb1 = int(eigvec1.shape[1])
         if nb1 != self._nb1:
             raise PhxError("eigvec1 does not match the cached k1")
@@ -419,9 +478,14 @@ class InteractionElPhWan:
         e3 = np.concatenate([_fortran_flat(e) for e in eigvecs3])
         q3 = _f64(np.asarray(q3Cs).reshape(nq, 3))
         out = np.zeros(int(nb1 * nm * nb2s.sum()))
+        pol = None
+        if self.usePolarCorrection:
+            if polarData is None or len(polarData) != nq:
+                raise PhxError("calcCouplingSquared: polarData needed for every q3")
+            pol = np.ascontiguousarray(np.asarray(polarData, dtype=np.complex128).reshape(nq, nm)).view(np.float64)
         _check(lib().phx_elph_coupling_sq(self._h, nq, _p(U2.view(np.float64)),
                                           nb2s.ctypes.data_as(_ip), _p(e3.view(np.float64)),
-                                          _p(q3), None, _p(out)))
+                                          _p(q3), None if pol is None else _p(pol), _p(out)))
         self._cache, off = [], 0
         for nb2 in nb2s:
             n = nb1 * nb2 * nm
@@ -436,9 +500,18 @@ class InteractionElPhWan:
         _check(lib().phx_elph_get_coupling_dims(self._h, d.ctypes.data_as(_ip)))
         return d
 
-    def estimateNumBatches(self, nk2, nb1, max_mem_bytes=0.0):
+    def estimateNumBatches(self, nk2, nb1, available_bytes=0.0):
+        """Batches of inner points (interaction_elph.cpp:458-490); available_bytes:
+        the caller's available device memory, or <= 0 for MAXMEM (env, GB,
+        default 16) minus this object's accounted usage."""
         return _check(lib().phx_elph_estimate_num_batches(self._h, int(nk2), int(nb1),
-                                                          float(max_mem_bytes)))
+                                                          float(available_bytes)))
+
+    def getDeviceMemoryUsage(self):
+        """Bytes the reference accounts for this object (interaction_elph.cpp:703-707)."""
+        out = C.c_double(0.0)
+        _check(lib().phx_elph_device_memory_usage(self._h, C.byref(out)))
+        return out.value
 
 
 # ------------------------------------------------------------------ linewidth builder
@@ -579,6 +652,14 @@ class ElectronLinewidths:
 
     def reset(self):
         _check(lib().phx_lw_reset(self._h))
+
+    def set_polar(self, positions=None, born=None, dielectric=None, volume=None, coarse_mesh=None):
+        """Polar term of the coupling (direct transform); no arguments: off."""
+        if positions is None:
+            _check(lib().phx_lw_set_polar(self._h, None))
+            return
+        st, self._polar_keep = polar_struct(positions, born, dielectric, self.prob.recip, volume, coarse_mesh)
+        _check(lib().phx_lw_set_polar(self._h, C.byref(st)))
 
     def set_screen(self, nsigma):
         """Energy-conservation screen (separable transform), in units of sigma; 0 = off."""

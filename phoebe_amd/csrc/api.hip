@@ -19,6 +19,7 @@
 #include "elph_file.hpp"
 #include "harmonic_io.hpp"
 #include "phel.hpp"
+#include "polar.hpp"
 #include "phx_common.hpp"
 #include "screen.hpp"
 
@@ -488,6 +489,9 @@ struct ElphHandle {
   DevBuf<double2> sPhE, sU2, sE3;
   DevBuf<int> sU2off, sNb2, sE3idx;
   DevBuf<long long> sOutOff;
+  // polar term: parameters (phx_elph_set_polar) and per-call buffers
+  PolarHost polar;
+  DevBuf<double2> sPolX;
   ~ElphHandle() {
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -585,7 +589,7 @@ static void elph_cache(ElphHandle* H, const double* U1, int nb1, const double* k
 }
 
 static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int* nb2s,
-                             const double* e3s, const double* q3, double* out) {
+                             const double* e3s, const double* q3, const double* polar, double* out) {
   PHX_REQUIRE(H->nb1 >= 1, "calcCouplingSquared called before cacheElPh");
   if (nq <= 0) return;
   hipStream_t s = H->stream;
@@ -642,6 +646,21 @@ static void elph_coupling_sq(ElphHandle* H, int nq, const double* U2s, const int
   p.rowOutOff = H->sOutOff.p;
   p.kkNb1 = H->meta.p + 1;
   p.nb2max = nb2max;
+  if (polar) {
+    // the reference adds the term only where |q3| > 1e-8 (interaction_elph.cpp:289)
+    std::vector<double2> x(size_t(nq) * nm);
+    for (int i = 0; i < nq; ++i) {
+      const double qn = std::sqrt(q3[3 * i] * q3[3 * i] + q3[3 * i + 1] * q3[3 * i + 1] + q3[3 * i + 2] * q3[3 * i + 2]);
+      for (int nu = 0; nu < nm; ++nu)
+        x[size_t(i) * nm + nu] = qn > 1.0e-8 ? make_double2(polar[2 * (size_t(i) * nm + nu)],
+                                                             polar[2 * (size_t(i) * nm + nu) + 1])
+                                             : make_double2(0., 0.);
+    }
+    h2d_grow(H->sPolX, x, s);
+    p.polarX = H->sPolX.p;
+    p.U1pool = H->U1.p;
+    p.U1off = H->meta.p + 0;
+  }
   if (H->precision == PHX_FP64) {
     H->sA.reserve(size_t(rowsPad) * H->Kp);
     launch_phase_rows<double>(nq, rowsPad, H->nRp, H->Kp, H->sQ.p, H->Rp.p, H->sA.p, s);
@@ -768,6 +787,9 @@ struct LwHandle {
   DevBuf<int4> meshRow;        // per mesh point: (active row or -1, nb, U pool offset, state offset)
   DevBuf<unsigned long long> keepMask;  // last-stage row masks of one panel chunk
   DevBuf<double> screenOf;     // per state: screen width (sigma) of the panel whose b1 group starts there
+  // polar term of the direct transform: parameters and polarCorrectionPart1 per mesh q (nk x nm)
+  PolarHost polar;
+  DevBuf<double2> polX;
   DevBuf<double> xminPart;     // per (panel of a chunk, block): closest approach to resonance
   // per-kernel timing with HIP events on `stream` (cache, coupling, collect)
   bool timing = false;
@@ -1713,6 +1735,11 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
   cq.cutoff = phonon_cutoff();
   cq.partial = L->partial.p;
   cq.nb2max = b.nb2max;
+  if (L->polar.on) {
+    cq.polarX = L->polX.p;
+    cq.U1pool = L->U2pool.p;
+    cq.U1off = dU1off;
+  }
   if (L->transform == PHX_TRANSFORM_DIRECT) {
     if (L->precision == PHX_FP64) {
       cq.A = L->Ad.p + size_t(k2b) * L->Kp;
@@ -2920,9 +2947,41 @@ int phx_elph_coupling_sq(phx_elph* h, int nq, const double* U2s, const int* nb2s
                          const double* q3, const double* polar, double* out) {
   return guard([&] {
     PHX_REQUIRE(h, "phx_elph_coupling_sq: null handle");
-    PHX_REQUIRE(polar == nullptr, "phx_elph_coupling_sq: polar correction not supported");
     PHX_REQUIRE(nq == 0 || (U2s && nb2s && e3s && q3 && out), "phx_elph_coupling_sq: null argument");
-    elph_coupling_sq(h, nq, U2s, nb2s, e3s, q3, out);
+    elph_coupling_sq(h, nq, U2s, nb2s, e3s, q3, polar, out);
+  });
+}
+
+int phx_polar_in_use(const double* dielectric, int num_species) {
+  if (!dielectric) return 0;
+  double n2 = 0.;
+  for (int i = 0; i < 9; ++i) n2 += dielectric[i] * dielectric[i];
+  return (n2 > 1.0e-10 && num_species > 1) ? 1 : 0;
+}
+
+int phx_elph_set_polar(phx_elph* h, const phx_polar* polar) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_elph_set_polar: null handle");
+    PHX_REQUIRE(!polar || 3 * polar->natoms == h->nm, "phx_elph_set_polar: nm must be 3 natoms");
+    h->polar.set(polar, h->stream);
+  });
+}
+
+int phx_elph_polar_part1(phx_elph* h, int nq, const double* q3, const double* e3s, double* out) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_elph_polar_part1: null handle");
+    PHX_REQUIRE(h->polar.on, "phx_elph_polar_part1: no polar parameters (phx_elph_set_polar)");
+    PHX_REQUIRE(nq >= 0 && (nq == 0 || (q3 && e3s && out)), "phx_elph_polar_part1: bad argument");
+    if (nq == 0) return;
+    hipStream_t s = h->stream;
+    const int nm = h->nm;
+    DevBuf<double> dq;
+    DevBuf<double2> de3, dx(size_t(nq) * nm);
+    h2d_grow(dq, q3, size_t(nq) * 3, s);
+    h2d_grow(de3, reinterpret_cast<const double2*>(e3s), size_t(nq) * nm * nm, s);
+    launch_polar_part1(h->polar.d, nq, nm, dq.p, de3.p, dx.p, s);
+    PHX_HIP(hipMemcpyAsync(out, dx.p, dx.bytes(), hipMemcpyDeviceToHost, s));
+    PHX_HIP(hipStreamSynchronize(s));
   });
 }
 
@@ -2937,15 +2996,34 @@ int phx_elph_get_coupling_dims(phx_elph* h, int* d) {
   });
 }
 
-int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double maxMem) {
-  int result = 1;
+// Device memory the reference accounts for an InteractionElPhWan
+// (getDeviceMemoryUsage, interaction_elph.cpp:703-707): the coupling tensor
+// and elPhCached (nRp nm nb1 nw complex once cacheElPh has run) at 16 bytes
+// per element, the R vectors and their degeneracies at 8.
+static double elph_accounted_bytes(const phx_elph* h) {
+  const double cached = h->nb1 > 0 ? double(h->nRp) * h->nm * h->nb1 * h->nw : 0.;
+  return 16. * (double(h->nRe) * h->nRp * h->nm * h->nw * h->nw + cached) +
+         8. * (4. * h->nRp + 4. * h->nRe);
+}
+
+int phx_elph_device_memory_usage(phx_elph* h, double* bytes) {
+  return guard([&] {
+    PHX_REQUIRE(h && bytes, "phx_elph_device_memory_usage: null argument");
+    *bytes = elph_accounted_bytes(h);
+  });
+}
+
+int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double available) {
+  int result = 0;
   int rc = guard([&] {
     PHX_REQUIRE(h && nk2 >= 0 && nb1 >= 1, "phx_elph_estimate_num_batches: bad argument");
-    if (maxMem <= 0) {
-      maxMem = 16.e9;  // DeviceManager default (common_kokkos.cpp:107-112)
-      if (const char* e = std::getenv("MAXMEM")) maxMem = std::atof(e) * 1.e9;
+    if (available <= 0) {
+      // a DeviceManager (common_kokkos.cpp:104-141) holding only this object
+      double total = 16.e9;
+      if (const char* e = std::getenv("MAXMEM")) total = std::atof(e) * 1.e9;
+      available = total - elph_accounted_bytes(h);
     }
-    // memory model of interaction_elph.cpp:462-474 (numElBands = nw, numPhBands = nm)
+    // memory model of interaction_elph.cpp:459-474 (numElBands = nw, numPhBands = nm)
     const double nw = h->nw, nm = h->nm, nRp = h->nRp;
     const double evs = 16 * (nw * nw + nm * nm);
     const double phase = 16 * nRp;
@@ -2955,11 +3033,9 @@ int phx_elph_estimate_num_batches(phx_elph* h, int nk2, int nb1, double maxMem) 
     const double coupling = 16 * nb1 * nw * nm;
     const double polar = 16 * nm * nb1 * nw;
     const double maxUsage = nk2 * (evs + polar + std::max({phase + g3, g3 + g4, g4 + gFinal, gFinal + coupling}));
-    // the device arrays held by the handle count as used memory
-    const double used = double(h->gW.bytes() + h->gWf.bytes() + h->Bd.bytes() + h->Bf.bytes());
-    const double available = maxMem - used;
-    PHX_REQUIRE(available >= maxUsage / std::max(nk2, 1), "Insufficient memory!");
-    result = std::max(1, int(std::ceil(maxUsage / available)));
+    // :477 and :482-488 (the reference divides by nk2 also when it is 0: no error then)
+    PHX_REQUIRE(nk2 == 0 || available >= maxUsage / nk2, "Insufficient memory!");
+    result = int(std::ceil(maxUsage / available));
   });
   return rc ? rc : result;
 }
@@ -3214,6 +3290,32 @@ int phx_lw_reset(phx_lw* h) {
     PHX_REQUIRE(h, "null handle");
     launch_fill<double>(h->lw.p, h->lw.n, 0., h->stream);
     PHX_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int phx_lw_set_polar(phx_lw* h, const phx_polar* polar) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_lw_set_polar: null handle");
+    if (!polar) {
+      h->polar.set(nullptr, h->stream);
+      h->polX.release();
+      return;
+    }
+    PHX_REQUIRE(h->transform == PHX_TRANSFORM_DIRECT,
+                "phx_lw_set_polar: the polar term runs with the direct transform");
+    PHX_REQUIRE(3 * polar->natoms == h->nm, "phx_lw_set_polar: nm must be 3 natoms");
+    hipStream_t s = h->stream;
+    h->polar.set(polar, s);
+    // polarCorrectionPart1 at every mesh q (folded wavevector and its mass-scaled
+    // eigenvectors, helper_el_scattering.cpp:170-195); points outside the
+    // q-set have zero eigenvectors and get x = 0
+    DevBuf<double> dq;
+    h2d(dq, h->kcart, s);
+    h->polX.alloc(size_t(h->nk) * h->nm);
+    launch_polar_part1(h->polar.d, h->nk, h->nm, dq.p, h->E3all.p, h->polX.p, s);
+    // q = 0 only for k2 = k1, where the reference skips the term (:289)
+    launch_fill<double>(reinterpret_cast<double*>(h->polX.p), 2 * size_t(h->nm), 0., s);
+    PHX_HIP(hipStreamSynchronize(s));
   });
 }
 

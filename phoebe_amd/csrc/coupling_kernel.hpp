@@ -90,6 +90,13 @@ struct CoupParams {
   const int* qModeGrp;
   // separable transform: phonon eigenvectors (q, nu, nu2) in the transform's precision
   const void* E3T;
+  // polar term (direct transform epilogue): per e3 index (row of the per-call
+  // API, folded q of the builder) nm complex polarCorrectionPart1 values, zero
+  // where the reference skips the term; U1 of the outer point: U1pool +
+  // U1off[kk], (nw, nb1) column-major.  polarX null: no polar term.
+  const double2* polarX;
+  const double2* U1pool;
+  const int* U1off;
   // split last stage (k_sep_xg + k_sep_ep): g3 rows of the chunk, (slot, jx, col) complex T
   void* G3;
   unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
@@ -213,6 +220,7 @@ __device__ void epilogue16_tail(const CoupParams& p, unsigned char* scratch, int
   const bool valid = myRow >= 0;
   const int nb2 = valid ? p.rowNb2[myRow] : 0;
   const double2* e3 = p.E3pool + size_t(myIq) * nm * nm;
+  const bool polar = p.polarX != nullptr && valid;
   for (int o = sub; o < nm * nb2; o += 16) {
     const int nu2 = o / nb2, b2 = o % nb2;
     double2 s = make_double2(0., 0.);
@@ -220,6 +228,21 @@ __device__ void epilogue16_tail(const CoupParams& p, unsigned char* scratch, int
       const double2 e = e3[nu + nm * nu2], x = gp[nu * nb2 + b2];
       s.x += x.x * e.x - x.y * e.y;
       s.y += x.x * e.y + x.y * e.x;
+    }
+    if (polar) {
+      // gFinal += x(nu2) <U2(b2)|U1(b1)> (interaction_elph.cpp:406-413 with
+      // polarCorrectionPart2, :205-227)
+      const double2* u1 = p.U1pool + p.U1off[kk] + size_t(p.nw) * b1p;
+      const double2* u2 = p.U2pool + p.rowU2off[myRow] + size_t(p.nw) * b2;
+      double2 ov = make_double2(0., 0.);
+      for (int w = 0; w < p.nw; ++w) {
+        const double2 a = u2[w], b = u1[w];
+        ov.x += a.x * b.x + a.y * b.y;
+        ov.y += a.x * b.y - a.y * b.x;
+      }
+      const double2 x = p.polarX[size_t(myIq) * nm + nu2];
+      s.x += x.x * ov.x - x.y * ov.y;
+      s.y += x.x * ov.y + x.y * ov.x;
     }
     const double v = s.x * s.x + s.y * s.y;
     if (REDUCE) {

@@ -1,0 +1,123 @@
+"""Pins the oracle's polar (Froehlich) term: polarCorrectionPart1Static
+(interaction_elph.cpp:152-203) and its addition to gFinal in
+calcCouplingSquared (:285-312, :406-413, polarCorrectionPart2 :205-227).
+
+* A cell small enough that only G = 0 lies inside the cutoff sphere
+  (qEq / 4 < 14) reduces Part1 to one closed-form term, evaluated here with
+  matrix products: a dropped factor (fourPi = 2 pi of constants.h:12), a
+  wrong phase sign (atoms off the origin), a transposed Born tensor (Z not
+  symmetric) or the eigenvector row index 3 atom + pol (two atoms) each fail it.
+* Periodicity in q: with every q+G of the cutoff inside the box,
+  x(q + b) = x(q).
+* The coupling with the term: gW = 0 leaves |x(nu)|^2 |(U2^H U1)(b2, b1)|^2;
+  with gW != 0 the interference with the brute-force g fixes the conjugation
+  and the b1/b2 order of the overlap; q = 0 gets no term.
+"""
+import numpy as np
+
+from oracle import oracle as O
+from phoebe_amd import synth
+from tests.helpers import tiny_problem
+from tests.test_oracle_pinning import _brute_force_g
+
+
+def _polar_params(rng, A, natoms, eps_scale):
+    tau = (A @ rng.uniform(0, 1, (3, natoms))).T
+    Z = rng.standard_normal((natoms, 3, 3))
+    Z -= Z.mean(axis=0)  # charge neutrality (sum over atoms of Z vanishes)
+    E = rng.standard_normal((3, 3)) * 0.3
+    eps = eps_scale * np.eye(3) + E @ E.T
+    return dict(positions=tau, born=Z, dielectric=eps, volume=abs(np.linalg.det(A)), coarse_mesh=(2, 2, 2))
+
+
+def _box(recip, q, coarse):
+    m = np.stack(np.meshgrid(*[np.arange(-c, c + 1) for c in coarse], indexing="ij"), -1).reshape(-1, 3)
+    return q[None, :] + m @ recip.T
+
+
+def test_polar_part1_single_term_closed_form():
+    rng = np.random.default_rng(1)
+    A = synth._fcc(2.0)           # small cell: |G| large
+    recip = synth.reciprocal_cell(A)
+    natoms = 2
+    pol = _polar_params(rng, A, natoms, eps_scale=12.0)
+    q = recip @ np.array([0.013, -0.021, 0.008])
+    G = _box(recip, q, pol["coarse_mesh"])
+    qe = np.einsum("gi,ij,gj->g", G, pol["dielectric"], G)
+    inside = (qe > 0) & (qe / 4 < 14)
+    assert inside.sum() == 1 and np.allclose(G[inside][0], q)  # only G = 0 survives
+    nm = 3 * natoms
+    e3 = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+    got = O.polar_part1(pol, recip, q, e3)
+    qeq = q @ pol["dielectric"] @ q
+    pref = 1j * 4.0 * np.pi / pol["volume"] * np.exp(-qeq / 4) / qeq  # chargeSquare 2 x fourPi 2 pi
+    phase = np.exp(-1j * pol["positions"] @ q)                         # (natoms,)
+    qz = np.stack([q @ pol["born"][a] for a in range(natoms)])         # (natoms, pol)
+    want = pref * ((phase[:, None] * qz).reshape(-1) @ e3)             # row index 3 atom + pol
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    # the planted slips each change the result
+    bad = pref * ((np.conj(phase)[:, None] * qz).reshape(-1) @ e3)
+    assert not np.allclose(got, bad, rtol=1e-6)
+    qzt = np.stack([pol["born"][a] @ q for a in range(natoms)])
+    assert not np.allclose(got, pref * ((phase[:, None] * qzt).reshape(-1) @ e3), rtol=1e-6)
+    assert not np.allclose(got, pref * ((phase[:, None] * qz).T.reshape(-1) @ e3), rtol=1e-6)
+
+
+def test_polar_part1_periodic_in_q():
+    rng = np.random.default_rng(2)
+    A = synth._fcc(10.26)
+    recip = synth.reciprocal_cell(A)
+    pol = _polar_params(rng, A, 2, eps_scale=10.0)
+    pol["coarse_mesh"] = (6, 6, 6)
+    q = recip @ np.array([0.11, 0.27, -0.05])
+    shifted = q + recip[:, 0] - 2 * recip[:, 2]
+
+    def inside(qq, c):
+        G = _box(recip, qq, (c, c, c))
+        qe = np.einsum("gi,ij,gj->g", G, pol["dielectric"], G)
+        return int(((qe > 0) & (qe / 4 < 14)).sum())
+    for qq in (q, shifted):  # the whole cutoff sphere lies inside the box of Part1
+        assert inside(qq, 6) == inside(qq, 9) > 1
+    e3 = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    x0 = O.polar_part1(pol, recip, q, e3)
+    x1 = O.polar_part1(pol, recip, shifted, e3)
+    np.testing.assert_allclose(x1, x0, rtol=1e-12, atol=1e-14 * np.abs(x0).max())
+    assert np.abs(x0).max() > 0
+
+
+def test_coupling_with_polar_term():
+    p = tiny_problem(natoms=1, nw=3, rg_e=2, rg_p=2, mesh=4, seed=5)
+    rng = np.random.default_rng(7)
+    gW = p.gw_host()
+    k1 = p.recip @ rng.uniform(0, 1, 3)
+    _, U1 = O.el_diag(p, k1)
+    U1 = U1[:, :2]
+    qs = [p.recip @ rng.uniform(-1, 1, 3) for _ in range(4)] + [np.zeros(3)]
+    U2s = [O.el_diag(p, k1 + q)[1][:, :2] for q in qs]
+    e3s = [O.ph_diag(p, q)[1] for q in qs]
+    x = rng.standard_normal((len(qs), p.nm)) + 1j * rng.standard_normal((len(qs), p.nm))
+    # gW = 0: only the polar term
+    zero = O.cache_elph(p.nw, p.nm, p.Rp, p.Re, p.degE, np.zeros_like(gW), U1, k1)
+    got = O.coupling_sq_polar(p.nw, p.nm, p.Rp, p.degP, zero, U2s, e3s, qs, U1, x)
+    for j in range(4):
+        ov = U2s[j].conj().T @ U1                                  # (b2, b1)
+        want = np.abs(x[j])[None, None, :] ** 2 * np.abs(ov.T)[:, :, None] ** 2
+        np.testing.assert_allclose(got[j], want, rtol=1e-12, atol=0)
+    assert not np.any(got[4])                                      # q = 0: no term
+    # with the coupling: |g + x (U2^H U1)^T|^2 against the brute-force g
+    cached = O.cache_elph(p.nw, p.nm, p.Rp, p.Re, p.degE, gW, U1, k1)
+    got = O.coupling_sq_polar(p.nw, p.nm, p.Rp, p.degP, cached, U2s, e3s, qs, U1, x)
+    for j in range(5):
+        g = _brute_force_g(p, gW, U1, k1, U2s[j], e3s[j], qs[j])  # (b1, b2, nu)
+        ov = (U2s[j].conj().T @ U1).T                               # (b1, b2)
+        add = x[j][None, None, :] * ov[:, :, None] if j < 4 else 0.0
+        want = np.abs(g + add) ** 2
+        np.testing.assert_allclose(got[j], want, rtol=1e-10, atol=1e-13 * want.max())
+        if j < 4:  # the conjugate overlap (U1^H U2) interferes differently
+            bad = np.abs(g + x[j][None, None, :] * ov.conj()[:, :, None]) ** 2
+            assert not np.allclose(got[j], bad, rtol=1e-6)
+    # without polar data the plain coupling is unchanged
+    plain = O.coupling_sq(p.nw, p.nm, p.Rp, p.degP, cached, U2s, e3s, qs)
+    none = O.coupling_sq_polar(p.nw, p.nm, p.Rp, p.degP, cached, U2s, e3s, qs, U1, None)
+    for a, b in zip(plain, none):
+        assert np.array_equal(a, b)
