@@ -2162,7 +2162,8 @@ constexpr int kXminBlocks = 16;
 // Closest approach to resonance of every panel of the chunk, for the energy
 // screen widths: the minimum over the panel's rows (inner points of the
 // range), their bands b2, the modes above the cutoff and the b1 group's
-// energies e1 of |e1 - e2 +- omega|.  Grid (kXminBlocks, panels): block b
+// energies e1 of |x| sigma, x = (e1 - e2 +- omega) / sigma, over the pairs
+// with x <= 6 (the others have zero weight under the one-sided cut).  Grid (kXminBlocks, panels): block b
 // takes mesh points b * 256 + t, strided by 256 * kXminBlocks.  A block first
 // evaluates only the rows that pass the interval test at the configured
 // width n: a row that fails it is at least n sigma from resonance in every
@@ -2204,7 +2205,11 @@ static __global__ void __launch_bounds__(256) k_sep_xmin(CoupParams p) {
           const double d = e1[gi] - e2;
           for (int nu = 0; nu < p.nm; ++nu) {
             const double om = w[nu];
-            if (om >= p.cutoff) best = fmin(best, fmin(fabs(d + om), fabs(d - om)));
+            if (om < p.cutoff) continue;
+            // only pairs the one-sided Gaussian can weigh (x <= 6, delta_function.cpp:43)
+            const double a = d + om, b = d - om;
+            if (a * p.invSigma <= 6.) best = fmin(best, fabs(a));
+            if (b * p.invSigma <= 6.) best = fmin(best, fabs(b));
           }
         }
       }
@@ -2224,9 +2229,9 @@ static __global__ void __launch_bounds__(256) k_sep_xmin(CoupParams p) {
 
 // Screen width of each panel: sqrt(xmin^2 + n^2) sigma, with xmin its closest
 // approach to resonance in units of sigma.  A pair that the screen drops then
-// weighs at most exp(-n^2) of the panel's most resonant pair, whatever the
-// panel's distance to resonance (a panel whose pairs all lie in the Gaussian
-// tails keeps its closest rows).  One thread per panel.
+// weighs at most exp(-n^2) of the panel's most resonant weighed pair, whatever
+// the panel's distance to resonance (a panel whose pairs all lie in the
+// Gaussian tails keeps its closest rows).  One thread per panel.
 static __global__ void __launch_bounds__(64) k_sep_xmin_fin(CoupParams p) {
   const int panel = blockIdx.x * 64 + threadIdx.x;
   if (panel >= p.nPanels) return;
