@@ -92,6 +92,17 @@ int phx_ph_bloch_diag(int nq, const double* q, int nm, int nR, const double* R,
                       const double* w, const double* mat2R, const double* masses,
                       int mass_scaling, double* omega, double* eigvecs);
 
+/* Timing form of phx_el_bloch_diag (phonon = 0) and phx_ph_bloch_diag
+ * (phonon != 0, masses given, mass scaling on): inputs uploaded once, then
+ * reps repetitions of the Bloch build and the batched eigensolve on device
+ * buffers, each kernel bracketed by HIP events.  ms[0], ms[1]: average
+ * milliseconds per launch of the Bloch build and of the eigensolve; the
+ * results of the last repetition are returned as the untimed calls do.  No
+ * counterpart in the reference (a measurement entry point). */
+int phx_bloch_diag_timed(int phonon, int nk, const double* k, int n, int nR, const double* R,
+                         const double* w, const double* coef, const double* masses, int reps, double* ms,
+                         double* energies, double* eigvecs);
+
 /* ---------------------------------------------------------------- InteractionElPhWan */
 typedef struct phx_elph phx_elph;
 

@@ -86,6 +86,8 @@ EXPORTS = {
     "phx_last_error": (C.c_char_p, []),
     "phx_version": (C.c_char_p, []),
     "phx_device_count": (C.c_int, [_ip]),
+    "phx_bloch_diag_timed": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                       C.c_int, _dp, _dp, _dp]),
     "phx_zheev_batched": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _ip]),
     "phx_zheev_batched_dev": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
@@ -259,6 +261,21 @@ def ph_bloch_diag(qc, R, w, mat2R, masses, mass_scaling=True):
                                    _p(_f64(mat2R)), _p(_f64(masses)), int(bool(mass_scaling)),
                                    _p(om), _p(ev)))
     return om, ev.view(np.complex128).reshape(nq, nm, nm).transpose(0, 2, 1)
+
+
+def bloch_diag_timed(kind, kc, R, weight, coef, masses=None, reps=3):
+    """Bloch build + batched eigensolve timed on device-resident inputs:
+    returns (W, V, ms_bloch, ms_eig) with ms per launch."""
+    kc = _f64(kc).reshape(-1, 3)
+    nk, n = len(kc), coef.shape[1]
+    W = np.zeros((nk, n))
+    U = np.zeros(2 * nk * n * n)
+    ms = np.zeros(2)
+    ph = kind == "ph"
+    cf = _f64(coef) if ph else _cview(coef)
+    _check(lib().phx_bloch_diag_timed(int(ph), nk, _p(kc), n, len(R), _p(_f64(R)), _p(_f64(weight)), _p(cf),
+                                      _p(_f64(masses)) if ph else None, int(reps), _p(ms), _p(W), _p(U)))
+    return W, U.view(np.complex128).reshape(nk, n, n).transpose(0, 2, 1), float(ms[0]), float(ms[1])
 
 
 def stream_copy_ms(nbytes, reps=5):

@@ -317,7 +317,8 @@ static void zheev_batched(int M, int N, double* A, double* W, int* info) {
 
 static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, const double* R,
                        const double* rw, const void* coef, const double* masses, int massScaling,
-                       double* energies, double* eigvecs, const PhononModel* lrModel = nullptr) {
+                       double* energies, double* eigvecs, const PhononModel* lrModel = nullptr,
+                       int reps = 0, double* msOut = nullptr) {
   hipStream_t s = 0;
   DevBuf<double> dk, dR, drw, dm;
   h2d(dk, k, size_t(nk) * 3);
@@ -353,6 +354,30 @@ static void bloch_diag(bool phonon, int nk, const double* k, int n, int nR, cons
     mptr = dm.p;
   }
   launch_jacobi(n, nk, H.p, W.p, info.p, mptr, s);
+  if (reps > 0 && msOut) {
+    // timed repetitions on device-resident inputs: the Bloch build and the
+    // eigensolve, each bracketed by its own events
+    hipEvent_t ev[3];
+    for (auto& e : ev) PHX_HIP(hipEventCreate(&e));
+    double tb = 0., tj = 0.;
+    for (int r = 0; r < reps; ++r) {
+      PHX_HIP(hipEventRecord(ev[0], s));
+      launch_bloch(phonon, nk, n, nR, dk.p, dR.p, drw.p,
+                   phonon ? static_cast<const void*>(dcoefr.p) : static_cast<const void*>(dcoefc.p), H.p, s);
+      PHX_HIP(hipEventRecord(ev[1], s));
+      launch_jacobi(n, nk, H.p, W.p, info.p, mptr, s);
+      PHX_HIP(hipEventRecord(ev[2], s));
+      PHX_HIP(hipEventSynchronize(ev[2]));
+      float a = 0.f, b = 0.f;
+      PHX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+      PHX_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+      tb += a;
+      tj += b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    msOut[0] = tb / reps;
+    msOut[1] = tj / reps;
+  }
   PHX_HIP(hipMemcpy(energies, W.p, W.bytes(), hipMemcpyDeviceToHost));
   PHX_HIP(hipMemcpy(eigvecs, H.p, H.bytes(), hipMemcpyDeviceToHost));
   std::vector<int> inf(nk);
@@ -2609,6 +2634,16 @@ int phx_ph_bloch_diag(int nq, const double* q, int nm, int nR, const double* R, 
   return guard([&] {
     PHX_REQUIRE(nq >= 0 && nm >= 1 && nm <= 44 && nR >= 1, "phx_ph_bloch_diag: bad sizes");
     if (nq) bloch_diag(true, nq, q, nm, nR, R, w, mat2R, masses, mass_scaling, omega, eigvecs);
+  });
+}
+
+int phx_bloch_diag_timed(int phonon, int nk, const double* k, int n, int nR, const double* R, const double* w,
+                         const double* coef, const double* masses, int reps, double* ms, double* energies,
+                         double* eigvecs) {
+  return guard([&] {
+    PHX_REQUIRE(nk >= 1 && n >= 1 && n <= 44 && nR >= 1 && reps >= 1 && ms && (!phonon || masses),
+                "phx_bloch_diag_timed: bad arguments");
+    bloch_diag(phonon != 0, nk, k, n, nR, R, w, coef, masses, 1, energies, eigvecs, nullptr, reps, ms);
   });
 }
 
