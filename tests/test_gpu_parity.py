@@ -314,7 +314,9 @@ def _weighed_distance(E, sigma):
     return np.abs(x).min() if x.size else np.inf
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
+# tol: the screened run against the oracle (the screen's own error, at most
+# 1e-8 here, plus FP32 rounding, see below)
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
 def test_energy_screen_keeps_tail_only_linewidths(precision, tol):
     """States whose every weighed partner lies in the Gaussian tails (closest
     approach to resonance between 6.5 and 15 sigma, on the negative side;
@@ -339,13 +341,32 @@ def test_energy_screen_keeps_tail_only_linewidths(precision, tol):
     say("  sigma", p.sigma, "tail-only states", int(tail.sum()), "of", len(E))
     ref = O.Oracle(p, gW.reshape(-1)).linewidths(average=False)
     assert np.all(ref[0, tail] > 0)
+    full = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="separable")
+    full.run()
+    unscreened = full.linewidths(average=False)
+    err0 = rel_err(unscreened, ref)
+    say("  no screen: max rel err vs oracle", err0.max())
+    # FP64 meets the oracle gate.  FP32 is held to 1e-5 here, not 1e-6: with
+    # sigma ~ 4e-6 Ry (50 ueV, 1/200 of the measured workloads' 0.01 eV) a
+    # single pair dominates each linewidth, so the FP32 rounding of one small
+    # |g|^2 (a sum with cancellation) shows undamped; it measures 2.6e-6.
+    assert err0.max() < (1e-10 if precision == "fp64" else 1e-5)
     for nsigma in (6.0, 7.0):
         s = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="separable")
         s.set_screen(nsigma)
         s.run()
         got = s.linewidths(average=False)
         err = rel_err(got, ref)
-        say("  screen", nsigma, "max rel err", err.max(), "on tail-only states", err[:, tail].max())
+        # the screen's own error: the same kernels with and without it (the
+        # rows it keeps are computed bit for bit alike)
+        scr = rel_err(got, unscreened)
+        say("  screen", nsigma, "max rel err", err.max(), "on tail-only states", err[:, tail].max(),
+            "screen alone", scr.max())
+        assert np.all(got[0, tail] > 0)
+        # pairs dropped by a panel's width sqrt(xmin^2 + n^2) weigh at most
+        # e^{-n^2} of its closest pair each; their couplings may be larger,
+        # measured 9e-10 at 6 sigma, against the chosen form's 1e-6 gate
+        assert scr.max() < 1e-8
         assert err.max() < tol
 
 

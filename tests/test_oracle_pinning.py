@@ -123,9 +123,9 @@ def test_coupling_gauge_and_reciprocal_lattice_invariance():
 
     base = g2(U1, U2, e3, k1, q)
     ph = lambda n: np.exp(1j * rng.uniform(0, 2 * np.pi, n))
-    assert np.allclose(g2(U1 * ph(p.nw), U2 * ph(p.nw), e3 * ph(p.nm), k1, q), base, rtol=1e-11)
+    assert np.allclose(g2(U1 * ph(p.nw), U2 * ph(p.nw), e3 * ph(p.nm), k1, q), base, rtol=1e-11, atol=0.0)
     G = p.recip @ np.array([1, -2, 1])
-    assert np.allclose(g2(U1, U2, e3, k1 + G, q - G), base, rtol=1e-9)
+    assert np.allclose(g2(U1, U2, e3, k1 + G, q - G), base, rtol=1e-9, atol=0.0)
 
 
 # ------------------------------------------------------------------ delta, populations, reduction
@@ -160,7 +160,7 @@ def test_symmetrize_coupling_groups():
     ref = c.copy()
     ref[0:2] = ref[0:2].mean(axis=0)
     ref[:, :, 0:3] = ref[:, :, 0:3].mean(axis=2, keepdims=True)
-    assert np.allclose(out, ref, rtol=1e-14)
+    assert np.allclose(out, ref, rtol=1e-14, atol=0.0)
 
 
 def test_single_band_linewidth_closed_form():

@@ -57,9 +57,9 @@ def test_polar_part1_single_term_closed_form():
     np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
     # the planted slips each change the result
     bad = pref * ((np.conj(phase)[:, None] * qz).reshape(-1) @ e3)
-    assert not np.allclose(got, bad, rtol=1e-6)
+    assert not np.allclose(got, bad, rtol=1e-6, atol=0.0)
     qzt = np.stack([pol["born"][a] @ q for a in range(natoms)])
-    assert not np.allclose(got, pref * ((phase[:, None] * qzt).reshape(-1) @ e3), rtol=1e-6)
+    assert not np.allclose(got, pref * ((phase[:, None] * qzt).reshape(-1) @ e3), rtol=1e-6, atol=0.0)
     assert not np.allclose(got, pref * ((phase[:, None] * qz).T.reshape(-1) @ e3), rtol=1e-6)
 
 
