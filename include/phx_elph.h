@@ -81,6 +81,21 @@ int phx_el_bloch_diag(int nk, const double* k, int nw, int nR, const double* R,
                       const double* deg, const double* h0R, double* energies,
                       double* eigvecs);
 
+/* phx_el_bloch_diag without any allocation: the same kernels and results,
+ * with every device array (the copies of k, R, deg and h0R, the Bloch
+ * matrices, eigenvalues and convergence flags) in one fixed 64 MiB device
+ * array per GPU, held until the process ends, instead of hipMalloc'd
+ * buffers; no hipMalloc, hipFree or host heap allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Needs
+ * 24 nk + 32 nR + 16 nw^2 (nk + nR) + 8 nk nw + 4 nk bytes, each array rounded
+ * up to 256 (nw = 16: about 15000 points per call with nR = 1000); larger
+ * calls return an error.  Calls that overlap on one GPU share the array and
+ * race. */
+int phx_el_bloch_diag_noalloc(int nk, const double* k, int nw, int nR, const double* R,
+                              const double* deg, const double* h0R, double* energies,
+                              double* eigvecs);
+
 /* PhononH0::kokkosBatchedDiagonalizeFromCoordinates
  * (src/harmonic/phonon_h0_kokkos.cpp:6-219), no long-range term:
  *   D(q) = sum_R e^{-iq.R} w(R) mat2R(R,:,:), hermitized, divided by

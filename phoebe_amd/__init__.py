@@ -92,6 +92,7 @@ EXPORTS = {
     "phx_zheev_batched_dev": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
     "phx_el_bloch_diag": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+    "phx_el_bloch_diag_noalloc": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
     "phx_ph_bloch_diag": (C.c_int, [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int,
                                     _dp, _dp]),
     "phx_elph_create": (C.c_int, [C.POINTER(C.c_void_p), _dp, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -242,13 +243,15 @@ def zheev_batched(A: np.ndarray):
     return W, V
 
 
-def el_bloch_diag(kc, R, deg, h0R):
+def el_bloch_diag(kc, R, deg, h0R, noalloc=False):
+    """Energies (nk, nw) and eigenvectors (nk, nw, nw) of H(k).  noalloc: the
+    entry point that works in the library's fixed device scratch."""
     kc = _f64(kc).reshape(-1, 3)
     nk, nw = len(kc), h0R.shape[1]
     E = np.zeros((nk, nw))
     U = np.zeros(2 * nk * nw * nw)
-    _check(lib().phx_el_bloch_diag(nk, _p(kc), nw, len(R), _p(_f64(R)), _p(_f64(deg)),
-                                   _p(_cview(h0R)), _p(E), _p(U)))
+    fn = lib().phx_el_bloch_diag_noalloc if noalloc else lib().phx_el_bloch_diag
+    _check(fn(nk, _p(kc), nw, len(R), _p(_f64(R)), _p(_f64(deg)), _p(_cview(h0R)), _p(E), _p(U)))
     return E, U.view(np.complex128).reshape(nk, nw, nw).transpose(0, 2, 1)
 
 

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <string>
@@ -2577,6 +2578,39 @@ static void lw_get(LwHandle* L, double* out, int average) {
 
 using namespace phx;
 
+// ------------------------------------------------------------ fixed device scratch
+// Scratch of phx_el_bloch_diag_noalloc: one fixed array on every GPU the
+// library runs on, held until the process ends, so that the entry point
+// allocates nothing.  In host code `scratch` names a host-side stand-in, not
+// device memory: kernels, copies and fills use the device address that
+// hipGetSymbolAddress returns, looked up once per device on the entry
+// point's first call there (each GPU has its own copy at its own address).
+// Every call on a GPU uses the same array: calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex gScratchMutex;
+unsigned char* gScratchDev[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device.
+unsigned char* scratch_on_current_device() {
+  int dev = 0;
+  PHX_HIP(hipGetDevice(&dev));
+  PHX_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "fixed scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratchDev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    PHX_REQUIRE(e == hipSuccess && p, std::string("fixed scratch: hipGetSymbolAddress failed: ") +
+                                          hipGetErrorString(e));
+    gScratchDev[dev] = static_cast<unsigned char*>(p);
+  }
+  return gScratchDev[dev];
+}
+}  // namespace
+
 // ============================================================ C ABI
 struct phx_elph : phx::ElphHandle {};
 struct phx_lw : phx::LwHandle {};
@@ -2625,6 +2659,59 @@ int phx_el_bloch_diag(int nk, const double* k, int nw, int nR, const double* R, 
   return guard([&] {
     PHX_REQUIRE(nk >= 0 && nw >= 1 && nw <= 44 && nR >= 1, "phx_el_bloch_diag: bad sizes");
     if (nk) bloch_diag(false, nk, k, nw, nR, R, deg, h0R, nullptr, 0, energies, eigvecs);
+  });
+}
+
+int phx_el_bloch_diag_noalloc(int nk, const double* k, int nw, int nR, const double* R,
+                              const double* deg, const double* h0R, double* energies, double* eigvecs) {
+  return guard([&] {
+    PHX_REQUIRE(nk >= 0 && nw >= 1 && nw <= 44 && nR >= 1, "phx_el_bloch_diag_noalloc: bad sizes");
+    // the same device arrays as phx_el_bloch_diag, placed in the fixed
+    // scratch at 256-byte boundaries
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+      const size_t at = used;
+      used += (bytes + 255) / 256 * 256;
+      return at;
+    };
+    const size_t nE = size_t(nw) * nw;
+    const size_t oK = take(size_t(nk) * 3 * sizeof(double)), oR = take(size_t(nR) * 3 * sizeof(double)),
+                 oDeg = take(size_t(nR) * sizeof(double)), oCoef = take(size_t(nR) * nE * sizeof(double2)),
+                 oH = take(size_t(nk) * nE * sizeof(double2)), oW = take(size_t(nk) * nw * sizeof(double)),
+                 oInfo = take(size_t(nk) * sizeof(int));
+    PHX_REQUIRE(used <= kNoAllocScratchBytes,
+                "phx_el_bloch_diag_noalloc: the call needs " + std::to_string(used) +
+                    " bytes of device scratch, more than its fixed " + std::to_string(kNoAllocScratchBytes) +
+                    " (use phx_el_bloch_diag, or fewer points per call)");
+    if (!nk) return;
+    PHX_REQUIRE(k && R && deg && h0R && energies && eigvecs, "phx_el_bloch_diag_noalloc: null argument");
+    unsigned char* base = scratch_on_current_device();
+    auto* dk = reinterpret_cast<double*>(base + oK);
+    auto* dR = reinterpret_cast<double*>(base + oR);
+    auto* dDeg = reinterpret_cast<double*>(base + oDeg);
+    auto* dCoef = reinterpret_cast<double2*>(base + oCoef);
+    auto* dH = reinterpret_cast<double2*>(base + oH);
+    auto* dW = reinterpret_cast<double*>(base + oW);
+    auto* dInfo = reinterpret_cast<int*>(base + oInfo);
+    const hipStream_t s = 0;
+    PHX_HIP(hipMemcpyAsync(dk, k, size_t(nk) * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(dR, R, size_t(nR) * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(dDeg, deg, size_t(nR) * sizeof(double), hipMemcpyHostToDevice, s));
+    PHX_HIP(hipMemcpyAsync(dCoef, h0R, size_t(nR) * nE * sizeof(double2), hipMemcpyHostToDevice, s));
+    // k_bloch writes every entry of H and k_jacobi every eigenvalue and every
+    // info flag, so nothing left from an earlier call is read
+    launch_bloch(false, nk, nw, nR, dk, dR, dDeg, dCoef, dH, s);
+    launch_jacobi(nw, nk, dH, dW, dInfo, nullptr, s);
+    PHX_HIP(hipMemcpy(energies, dW, size_t(nk) * nw * sizeof(double), hipMemcpyDeviceToHost));
+    PHX_HIP(hipMemcpy(eigvecs, dH, size_t(nk) * nE * sizeof(double2), hipMemcpyDeviceToHost));
+    // convergence flags through a stack buffer (no host allocation either)
+    int inf[1024];
+    for (int i0 = 0; i0 < nk; i0 += 1024) {
+      const int n = std::min(1024, nk - i0);
+      PHX_HIP(hipMemcpy(inf, dInfo + i0, size_t(n) * sizeof(int), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i)
+        if (inf[i] < 0) throw Error(-4, "Bloch eigensolve did not converge");
+    }
   });
 }
 
