@@ -126,8 +126,9 @@ typedef struct phx_elph phx_elph;
  * (interaction_elph.h:93-100, interaction_elph.cpp:10-62).
  *   gW : the Eigen tensor (nw(w2), nw(w1), nm, nRp, nRe) column-major, i.e.
  *        (nRe, nRp, nm, w1, w2) row-major;  elR: nRe x 3;  phR: nRp x 3.
- * The polar (Froehlich) term is not supported (the reference enables it only
- * with a dielectric tensor and more than one species, :24-32). */
+ * The polar (Froehlich) term is off until phx_elph_set_polar gives its
+ * parameters (the reference's constructor turns it on for a dielectric tensor
+ * and more than one species, :24-32; phx_polar_in_use states that rule). */
 int phx_elph_create(phx_elph** h, const double* gW, int nw, int nm, int nRp, int nRe,
                     const double* elR, const double* elDeg, const double* phR,
                     const double* phDeg, int precision);
@@ -459,6 +460,9 @@ int phx_lw_set_panel_chunk(phx_lw* h, int panels);
  * to PHX_FP32 with nm, nw <= 16, nm nw <= 192, an x mesh <= 64, at most 24
  * x-groups and no learned screen; elsewhere the fused kernel runs.  The g3
  * values are the same either way and the linewidths agree to FP64 rounding.
+ * on = 1 (default) runs the lean epilogue kernel where it applies (fixed
+ * Gaussian smearing, at most 8 bands per inner point), on = 2 the first
+ * epilogue kernel always; both give the same linewidths.
  * No counterpart in the reference. */
 int phx_lw_set_split_last_stage(phx_lw* h, int on);
 /* With timing on, the launches of phx_lw_stage_times that ran split:

@@ -27,7 +27,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libphx_elph.so")
+# PHX_LIBRARY: load the library from another path (a build of different options)
+LIB_PATH = os.environ.get("PHX_LIBRARY") or os.path.join(_HERE, "libphx_elph.so")
 
 PHX_FP64, PHX_FP32, PHX_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": PHX_FP64, "fp32": PHX_FP32, "bf16x3": PHX_BF16X3}
@@ -695,8 +696,9 @@ class ElectronLinewidths:
         _check(lib().phx_lw_set_panel_chunk(self._h, int(panels)))
 
     def set_split_last_stage(self, on=True):
-        """Separable last stage as two kernels (default) or the fused one."""
-        _check(lib().phx_lw_set_split_last_stage(self._h, int(bool(on))))
+        """Separable last stage as two kernels (default) or the fused one;
+        on=2 runs the split stage with the first epilogue kernel."""
+        _check(lib().phx_lw_set_split_last_stage(self._h, 2 if on == 2 else int(bool(on))))
 
     def split_times(self):
         """Split last stage with timing on: x-stage ms, epilogue ms, launches,

@@ -100,6 +100,7 @@ struct CoupParams {
   // split last stage (k_sep_xg + k_sep_ep): g3 rows of the chunk, (slot, jx, col) complex T
   void* G3;
   unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
+  int epFirst;                            // split epilogue: 1 = k_sep_ep, 0 = k_sep_ep2 where it applies
 };
 
 template <typename T> struct Vec2;
@@ -2157,6 +2158,362 @@ inline size_t sep_ep_lds_bytes(int nw, int nm, int nb2max, int nCalc, int rpp) {
   return std::max(b, size_t(kGMax) * kCMax * 4 * sizeof(double)) + 64;
 }
 
+// ------------------------------------------------------------------ split epilogue, lean form
+// Only what the epilogue of the split last stage reads (fixed Gaussian
+// smearing).  A small block: the kernel keeps its scalars in SGPRs instead of
+// spilling the full CoupParams' into VGPR lanes.
+struct EpParams {
+  const unsigned long long* keepMask;
+  double* partial;  // (slot, kGMax, kCMax)
+  const int* panelKK;
+  const int* panelGStart;
+  const int* panelGSize;
+  const int* kkMesh;
+  const int* kkStateOff;
+  const int4* meshRow;     // per mesh point: (row, nb2, u2off, stateOff)
+  const double* e1pool;
+  const double* e2pool;
+  const double* Wpool;     // (q, nu)
+  const int* qModeGrp;     // (q, nu): first mode of the group | size << 8
+  const double2* U2pool;
+  const double* fermi;     // (calc, state)
+  const double* bose;      // (q, nu, calc)
+  const void* E3T;         // (q, nu, nu2) in the transform's precision
+  const void* G3;          // (slot, jx, (nu, w2)) in the transform's precision
+  const double* screenOf;  // per outer state: screen width of the panel its b1 group starts (null: screen)
+  unsigned long long* keptRows;
+  unsigned long long* keptBands;
+  double invSigma, deltaPref, norm, cutoff, screen;
+  int m0, m1, m2, nPanels, nm, nw, nCalc, numStates;
+  int rpw;                 // rows per pass: whole rows of nm lanes, rpw nm <= 64
+};
+
+// Epilogue of the rows k_sep_xg wrote, one wave per slot (panel, yz line),
+// passes of rpw rows with lane (row r, mode sub).  The same operations in the
+// same order as k_sep_ep (so the same linewidths), arranged so that little is
+// live at once:
+//   A  electron rotation g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2),
+//      nu = sub, each g3 value converted once (outer loop over w2, NB2
+//      accumulators), U2 of the pass in LDS;
+//   B  phonon rotation g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2), nu2 = sub,
+//      the eigenvector column in registers, then |g|^2 and symmetrizeCoupling
+//      along b2 (scattering.cpp:1250-1284) on the lane's own entries;
+//   C  symmetrizeCoupling along nu (:1286-1316) from the group's entries and
+//      the Gaussian-delta weights of el_scattering.cpp:277-380 into per-lane
+//      partial sums (CM calculations, GM members of the b1 group).
+// Phases A and B run the pass's largest band count for every row (uniform
+// branches; a row's U2 columns beyond its own count are zero).  NB2 >= nb2max
+// is the LDS stride of a row's bands, NMX >= nm the eigenvector column length.
+// Phase A of k_sep_ep2 for NB bands: g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2)
+// for the lane's mode, w2 in order.  No branches inside: entries beyond nw
+// are zero in g and in U2, so every term adds an exact zero.
+template <typename T2v, int NWX, int NB2, int NB>
+__device__ __forceinline__ void ep2_rot_el(const T2v (&g)[NWX], const double2* u, double2* out) {
+  double ar[NB], ai[NB];
+#pragma unroll
+  for (int b2 = 0; b2 < NB; ++b2) ar[b2] = ai[b2] = 0.;
+#pragma unroll
+  for (int w2 = 0; w2 < NWX; ++w2) {
+    const double xr = double(g[w2].x), xi = double(g[w2].y);
+#pragma unroll
+    for (int b2 = 0; b2 < NB; ++b2) {
+      const double2 c = u[b2 * NWX + w2];
+      ar[b2] += c.x * xr + c.y * xi;
+      ai[b2] += c.x * xi - c.y * xr;
+    }
+  }
+#pragma unroll
+  for (int b2 = 0; b2 < NB; ++b2) out[b2] = make_double2(ar[b2], ai[b2]);
+}
+
+// Phase B of k_sep_ep2 for NB bands: g(nu2, b2) = sum_nu e3(nu, nu2) g'(nu, b2)
+// for the lane's mode nu2, nu in order, and |g|^2 into own.  Modes beyond nm
+// have zero eigenvector entries and read the last mode's (finite) g'.
+template <typename T2v, int NMX, int NB2, int NB>
+__device__ __forceinline__ void ep2_rot_ph(const T2v (&e3c)[NMX], const double2* x, int nm, double* own) {
+  double gr[NB], gi[NB];
+#pragma unroll
+  for (int b2 = 0; b2 < NB; ++b2) gr[b2] = gi[b2] = 0.;
+#pragma unroll
+  for (int nu = 0; nu < NMX; ++nu) {
+    const double er = double(e3c[nu].x), ei = double(e3c[nu].y);
+    const double2* xn = x + min(nu, nm - 1) * NB2;
+#pragma unroll
+    for (int b2 = 0; b2 < NB; ++b2) {
+      const double2 v = xn[b2];
+      gr[b2] += v.x * er - v.y * ei;
+      gi[b2] += v.x * ei + v.y * er;
+    }
+  }
+#pragma unroll
+  for (int b2 = 0; b2 < NB; ++b2) own[b2] = gr[b2] * gr[b2] + gi[b2] * gi[b2];
+}
+
+template <typename T, int CM, int NMX, int GM, int NB2>
+__global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Vec2<T>::type T2v;
+  constexpr int NWX = 16;
+  constexpr int U2S = NWX * NB2 + 1;  // double2 per row of the U2 table (odd: rows in different banks)
+  constexpr int RMX = NB2 <= 4 ? 5 : 4;           // rows per pass at most (the launcher's cap)
+  constexpr int UPL = (NWX * NB2 * RMX + 63) / 64;  // U2 entries per lane
+  const int nLinesYZ = p.m1 * p.m2;
+  const int nTilesTot = p.nPanels * nLinesYZ;
+  const int bid = blockIdx.x;
+  const int perXcd = gridDim.x / 8;
+  const int Lx = (bid % 8) * perXcd + bid / 8;
+  if (Lx >= nTilesTot) return;
+  const int yz = Lx / p.nPanels, panel = Lx % p.nPanels;
+  const int slot = panel * nLinesYZ + yz;
+  const int jy = yz % p.m1, jz = yz / p.m1;
+  const int t = threadIdx.x;
+  const unsigned long long mask = p.keepMask[slot];
+  if (mask == 0ull) {
+    if (t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
+    return;
+  }
+  const int nm = p.nm, nw = p.nw, ncol = nm * nw, R = p.rpw;
+  const int kk = p.panelKK[panel];
+  const int gstart = p.panelGStart[panel], gsize = __builtin_amdgcn_readfirstlane(p.panelGSize[panel]);
+  const int is1 = p.kkStateOff[kk] + gstart;
+  const double scrW = p.screen > 0. ? (p.screenOf ? p.screenOf[is1] : p.screen) : 0.;
+  const double inv_g = 1. / double(float(gsize));
+  // LDS: rows (64) | U2 (R x U2S double2), later |g|^2 (R x nm x NB2) | g' (R x nm x NB2 + 1 per row)
+  //      | e2 (R x NB2) | occupations (CM x R x NB2) | e1 (kGMax)
+  SepXmRow* rows = reinterpret_cast<SepXmRow*>(smem);
+  double2* sU2 = reinterpret_cast<double2*>(smem + (64 * sizeof(SepXmRow) + 15) / 16 * 16);
+  double* gsq = reinterpret_cast<double*>(sU2);
+  const int gps = nm * NB2 + 1;
+  double2* gp = sU2 + size_t(R) * U2S;
+  double* se2 = reinterpret_cast<double*>(gp + size_t(R) * gps);
+  double* sf2 = se2 + R * NB2;
+  double* se1 = sf2 + CM * R * NB2;
+  if (t < gsize) se1[t] = p.e1pool[is1 + t];
+  const int nkeep = __popcll(mask);
+  {
+    const int kkMesh = p.kkMesh[kk];
+    if (t < p.m0 && ((mask >> t) & 1ull)) {
+      const int mesh = (jz * p.m1 + jy) * p.m0 + t;
+      const int4 mr = p.meshRow[mesh];
+      rows[__popcll(mask & ((1ull << t) - 1ull))] =
+          SepXmRow{t, mr.x, mesh_diff_index(mesh, kkMesh, p.m0, p.m1, p.m2), mr.y, mr.z, mr.w};
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (p.keptRows) atomicAdd(p.keptRows, (unsigned long long)nkeep);
+    if (p.keptBands) {
+      unsigned long long nb = 0;
+      for (int i = 0; i < nkeep; ++i) nb += rows[i].nb2;
+      atomicAdd(p.keptBands, nb);
+    }
+  }
+  // lane (r, sub): r < R rows of a pass, whole rows of nm lanes; the rest idle
+  const bool laneOn = t < R * nm;
+  const int r = laneOn ? t / nm : 0, sub = laneOn ? t % nm : 0;
+  const T2v* G3 = static_cast<const T2v*>(p.G3) + size_t(slot) * p.m0 * ncol;
+  const T2v* E3T = static_cast<const T2v*>(p.E3T);
+  double part[GM][CM];
+#pragma unroll
+  for (int gi = 0; gi < GM; ++gi)
+#pragma unroll
+    for (int c = 0; c < CM; ++c) part[gi][c] = 0.;
+
+  // registers of one pass, loaded at its start
+  T2v g[NWX];         // g3(nu = sub, w2) of the lane's row
+  T2v e3c[NMX];       // e3(nu, nu2 = sub) at the row's q
+  double2 u2v[UPL];   // U2 entries of the pass (lane-strided over R x NWX x NB2)
+  double e2v = 0., f2v[CM], w3 = 0., n3[CM];
+  int grp = 0;
+  auto load_pass = [&](int m0r) {
+    const int nr = min(R, nkeep - m0r);
+    const bool rowOn = laneOn && r < nr;
+    const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
+    if (rowOn) {
+      const T2v* src = G3 + size_t(ri.jx) * ncol + sub * nw;
+      if (sizeof(T) == 4 && nw == NWX) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+        for (int h = 0; h < NWX / 2; ++h) {
+          const float4 v = s4[h];
+          g[2 * h].x = v.x;
+          g[2 * h].y = v.y;
+          g[2 * h + 1].x = v.z;
+          g[2 * h + 1].y = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int w2 = 0; w2 < NWX; ++w2) g[w2] = w2 < nw ? src[w2] : T2v{T(0), T(0)};
+      }
+      const T2v* e3 = E3T + size_t(ri.iq) * nm * nm + sub;
+#pragma unroll
+      for (int nu = 0; nu < NMX; ++nu) e3c[nu] = nu < nm ? e3[nu * nm] : T2v{T(0), T(0)};
+      w3 = p.Wpool[size_t(ri.iq) * nm + sub];
+      grp = p.qModeGrp[size_t(ri.iq) * nm + sub];
+#pragma unroll
+      for (int c = 0; c < CM; ++c) n3[c] = c < p.nCalc ? p.bose[(size_t(ri.iq) * nm + sub) * p.nCalc + c] : 0.;
+    } else {
+      w3 = 0.;
+      grp = 0;
+    }
+    // U2 of the pass: entry i = (row, b2, w2), w2 fastest, NWX x NB2 per row
+#pragma unroll
+    for (int u = 0; u < UPL; ++u) {
+      const int i = t + 64 * u, rr = i / (NWX * NB2), o = i % (NWX * NB2), b2 = o / NWX, w2 = o % NWX;
+      double2 v = make_double2(0., 0.);
+      if (rr < nr) {
+        const SepXmRow& q = rows[m0r + rr];
+        if (b2 < q.nb2 && w2 < nw) v = p.U2pool[q.u2off + b2 * nw + w2];
+      }
+      u2v[u] = v;
+    }
+    // band energies and occupations: lane = (row, b2) for lanes < R NB2
+    e2v = 0.;
+#pragma unroll
+    for (int c = 0; c < CM; ++c) f2v[c] = 0.;
+    if (t < R * NB2) {
+      const int rr = t / NB2, b2 = t % NB2;
+      if (rr < nr && b2 < rows[m0r + rr].nb2) {
+        const int st = rows[m0r + rr].soff + b2;
+        e2v = p.e2pool[st];
+#pragma unroll
+        for (int c = 0; c < CM; ++c)
+          if (c < p.nCalc) f2v[c] = p.fermi[size_t(c) * p.numStates + st];
+      }
+    }
+  };
+  for (int m0r = 0; m0r < nkeep; m0r += R) {
+    load_pass(m0r);
+    const int nr = min(R, nkeep - m0r);
+    const bool rowOn = laneOn && r < nr;
+    const int nb2 = rowOn ? rows[m0r + r].nb2 : 0;
+    // largest band count of the pass's rows, uniform over the wave: phases A
+    // and B run that many bands for every row (a row's U2 columns beyond its
+    // own count are zero, so those entries come out zero and are never read)
+    int nbw = 0;
+    for (int i = 0; i < nr; ++i) nbw = max(nbw, rows[m0r + i].nb2);
+    nbw = __builtin_amdgcn_readfirstlane(nbw);
+    // row data of the pass into LDS
+#pragma unroll
+    for (int u = 0; u < UPL; ++u) {
+      const int i = t + 64 * u, rr = i / (NWX * NB2);
+      if (rr < R) sU2[rr * U2S + i % (NWX * NB2)] = u2v[u];  // R <= RMX
+    }
+    if (t < R * NB2) {
+      se2[t] = e2v;
+#pragma unroll
+      for (int c = 0; c < CM; ++c) sf2[c * R * NB2 + t] = f2v[c];
+    }
+    __syncthreads();
+    // A: electron rotation, thread nu = sub
+    if (rowOn) {
+      const double2* u = sU2 + r * U2S;
+      double2* out = gp + r * gps + sub * NB2;
+      switch (nbw) {
+        case 1: ep2_rot_el<T2v, NWX, NB2, 1>(g, u, out); break;
+        case 2: ep2_rot_el<T2v, NWX, NB2, 2>(g, u, out); break;
+        case 3: ep2_rot_el<T2v, NWX, NB2, 3>(g, u, out); break;
+        case 4: ep2_rot_el<T2v, NWX, NB2, 4>(g, u, out); break;
+        default:
+          if constexpr (NB2 > 4) ep2_rot_el<T2v, NWX, NB2, NB2>(g, u, out);
+      }
+    }
+    wave_sync();
+    // B: phonon rotation, thread nu2 = sub, then |g|^2 and the b2 groups
+    double* own = gsq + (size_t(r) * nm + sub) * NB2;
+    if (rowOn) {
+      const double2* x = gp + r * gps;
+      switch (nbw) {
+        case 1: ep2_rot_ph<T2v, NMX, NB2, 1>(e3c, x, nm, own); break;
+        case 2: ep2_rot_ph<T2v, NMX, NB2, 2>(e3c, x, nm, own); break;
+        case 3: ep2_rot_ph<T2v, NMX, NB2, 3>(e3c, x, nm, own); break;
+        case 4: ep2_rot_ph<T2v, NMX, NB2, 4>(e3c, x, nm, own); break;
+        default:
+          if constexpr (NB2 > 4) ep2_rot_ph<T2v, NMX, NB2, NB2>(e3c, x, nm, own);
+      }
+      const double* e2r = se2 + r * NB2;
+      for (int b = 0; b < nb2;) {
+        int deg = 0;
+        for (int i = b; i < nb2; ++i) {
+          if (fabs(e2r[b] - e2r[i]) < 1.0e-6) deg++;
+          else break;
+        }
+        if (deg > 1) {
+          double s2 = 0.;
+          for (int i = 0; i < deg; ++i) s2 += own[b + i] / double(float(deg));
+          for (int i = 0; i < deg; ++i) own[b + i] = s2;
+        }
+        b += deg;
+      }
+    }
+    wave_sync();
+    const double w3c = w3;
+    const int grpc = grp;
+    double n3c[CM];
+#pragma unroll
+    for (int c = 0; c < CM; ++c) n3c[c] = n3[c];
+    // C: the nu groups and the Gaussian weights
+    if (rowOn && w3c >= p.cutoff) {
+      const int mystart = grpc & 0xff, mydeg = grpc >> 8;
+      const double* g0 = gsq + (size_t(r) * nm + mystart) * NB2;
+#pragma unroll 1
+      for (int b2 = 0; b2 < nb2; ++b2) {
+        double vb;
+        if (mydeg > 1) {
+          vb = 0.;
+          for (int i = 0; i < mydeg; ++i) vb += g0[size_t(i) * NB2 + b2] / double(float(mydeg));
+        } else {
+          vb = own[b2];
+        }
+        const double en2 = se2[r * NB2 + b2];
+        const double cpl = vb * inv_g;
+#pragma unroll
+        for (int gi = 0; gi < GM; ++gi) {
+          if (gi >= gsize) break;
+          const double x1 = se1[gi] - en2 + w3c, x2 = se1[gi] - en2 - w3c;
+          if (p.screen > 0. && fmin(fabs(x1), fabs(x2)) * p.invSigma >= scrW) continue;
+          const double d1 = gaussian_delta(x1, p.invSigma, p.deltaPref);
+          const double d2 = gaussian_delta(x2, p.invSigma, p.deltaPref);
+          if (d1 <= 0. && d2 <= 0.) continue;
+#pragma unroll
+          for (int c = 0; c < CM; ++c) {
+            if (c >= p.nCalc) break;
+            const double f2 = sf2[(c * R + r) * NB2 + b2];
+            part[gi][c] += cpl * ((f2 + n3c[c]) * d1 + (1. - f2 + n3c[c]) * d2) * p.norm / w3c *
+                           3.14159265358979323846;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next pass rewrites the row data
+  }
+  // wave reduction of the partial linewidths into partial[slot]
+  double* red = reinterpret_cast<double*>(smem);
+  const int lane = t & 63;
+#pragma unroll
+  for (int gi = 0; gi < GM; ++gi)
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      double v = part[gi][c];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red[gi * kCMax + c] = v;
+    }
+  __syncthreads();
+  if (t < kGMax * kCMax) {
+    const int c = t % kCMax, gi = t / kCMax;
+    p.partial[size_t(slot) * kGMax * kCMax + t] = (c < CM && gi < GM) ? red[t] : 0.;
+  }
+}
+
+inline size_t sep_ep2_lds_bytes(int nm, int nb2, int cm, int rpw) {
+  size_t b = (64 * sizeof(SepXmRow) + 15) / 16 * 16;
+  b += size_t(rpw) * (16 * nb2 + 1) * sizeof(double2);              // U2, later |g|^2
+  b += size_t(rpw) * (nm * nb2 + 1) * sizeof(double2);              // g'
+  b += size_t(rpw) * nb2 * sizeof(double) * (1 + cm);               // e2, occupations
+  b += size_t(kGMax) * sizeof(double);                              // e1
+  return std::max(b, size_t(kGMax) * kCMax * sizeof(double)) + 64;
+}
+
 constexpr int kXminBlocks = 16;
 
 // Closest approach to resonance of every panel of the chunk, for the energy
@@ -2374,6 +2731,74 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         // the launch, no more than the launch needs
         const int gm = p.gsizeMax > 0 ? p.gsizeMax : kGMax;
         PHX_REQUIRE(gm <= kGMax, "separable transform: b1 group larger than kGMax");
+        if (!p.epFirst && !p.adaptive && p.nb2max <= 8) {
+          // lean epilogue: rows per pass capped by its U2 prefetch registers
+          const int nb2 = p.nb2max <= 4 ? 4 : 8;
+          const int rpw2 = std::min(64 / p.nm, nb2 <= 4 ? 5 : 4);
+          const int cm = one ? 1 : p.nCalc <= 2 ? 2 : kCMax;
+          const size_t lds2 = sep_ep2_lds_bytes(p.nm, nb2, cm, rpw2);
+          PHX_REQUIRE(lds2 <= 160 * 1024, "separable transform: LDS budget exceeded");
+          EpParams q{};
+          q.keepMask = p.keepMask;
+          q.partial = p.partial;
+          q.panelKK = p.panelKK;
+          q.panelGStart = p.panelGStart;
+          q.panelGSize = p.panelGSize;
+          q.kkMesh = p.kkMesh;
+          q.kkStateOff = p.kkStateOff;
+          q.meshRow = p.meshRow;
+          q.e1pool = p.e1pool;
+          q.e2pool = p.e2pool;
+          q.Wpool = p.Wpool;
+          q.qModeGrp = p.qModeGrp;
+          q.U2pool = p.U2pool;
+          q.fermi = p.fermi;
+          q.bose = p.bose;
+          q.E3T = p.E3T;
+          q.G3 = p.G3;
+          q.screenOf = p.screenOf;
+          q.keptRows = p.keptRows;
+          q.keptBands = p.keptBands;
+          q.invSigma = p.invSigma;
+          q.deltaPref = p.deltaPref;
+          q.norm = p.norm;
+          q.cutoff = p.cutoff;
+          q.screen = p.screen;
+          q.m0 = p.m0;
+          q.m1 = p.m1;
+          q.m2 = p.m2;
+          q.nPanels = p.nPanels;
+          q.nm = p.nm;
+          q.nw = p.nw;
+          q.nCalc = p.nCalc;
+          q.numStates = p.numStates;
+          q.rpw = rpw2;
+          auto ep2 = [&](auto kern) {
+            PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2)));
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds2, s, q);
+          };
+          auto ep2c = [&](auto nmx, auto nb2c) {
+            constexpr int NM = decltype(nmx)::value, NB = decltype(nb2c)::value;
+            if (gm <= 4) {
+              if (one) ep2(k_sep_ep2<T, 1, NM, 4, NB>);
+              else if (p.nCalc <= 2) ep2(k_sep_ep2<T, 2, NM, 4, NB>);
+              else ep2(k_sep_ep2<T, kCMax, NM, 4, NB>);
+            } else {
+              if (one) ep2(k_sep_ep2<T, 1, NM, kGMax, NB>);
+              else if (p.nCalc <= 2) ep2(k_sep_ep2<T, 2, NM, kGMax, NB>);
+              else ep2(k_sep_ep2<T, kCMax, NM, kGMax, NB>);
+            }
+          };
+          using I12 = std::integral_constant<int, 12>;
+          using I16 = std::integral_constant<int, 16>;
+          using B4 = std::integral_constant<int, 4>;
+          using B8 = std::integral_constant<int, 8>;
+          if (p.nm <= 12) nb2 <= 4 ? ep2c(I12{}, B4{}) : ep2c(I12{}, B8{});
+          else nb2 <= 4 ? ep2c(I16{}, B4{}) : ep2c(I16{}, B8{});
+          PHX_HIP(hipGetLastError());
+          return true;
+        }
         auto epc = [&](auto nmx) {
           constexpr int NM = decltype(nmx)::value;
           if (gm <= 4) {
