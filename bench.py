@@ -61,6 +61,8 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
     kt = lw.kernel_times()
     st = lw.stage_times()
     sst = lw.split_times() if transform == "separable" else None
+    if sst is not None:
+        sst.update(lw.split_bytes())
     gam = lw.linewidths(average=False)
     log(f"[{form}] {len(k1_batches)} run calls in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
@@ -85,9 +87,11 @@ def traffic_per_dispatch(kernel, workload):
 
 
 def roofline(res, form, workload):
-    """Dominant kernel: k_coupling (direct form) or k_sep_xm (separable form),
-    algorithmic FLOPs of its transform GEMMs over the rows it computed, per
-    launch, against the MFMA peak of the precision."""
+    """Dominant kernel of the form: k_coupling (direct), on the FLOPs of its
+    transform GEMMs against the MFMA peak; the slower of k_sep_xg and
+    k_sep_ep2 (split separable last stage), on the bytes it streams against
+    HBM peak; k_sep_xm (fused separable last stage), on the FLOPs of its
+    x-stage GEMMs over the rows it computed."""
     precision, transform = FORMS[form][:2]
     peak = PEAK_TFLOPS[precision]
     if transform == "direct":
@@ -97,34 +101,39 @@ def roofline(res, form, workload):
     else:
         st, sp = res["stage"], res.get("split")
         if sp and sp["launches"] > 0:
-            # split last stage: x-stage (MFMA) and epilogue (streams the kept rows' g3)
+            # split last stage: both kernels stream HBM.  The x-stage kernel reads
+            # the x-stage slabs of the slots it computes and writes the kept rows'
+            # g3 (complex FP32); the epilogue reads those g3 rows back (and the
+            # per-pair reduction bytes are reported beside)
             n = sp["launches"]
-            extra = {"k_sep_xg_ms": round(sp["xg_ms"] / n, 4), "k_sep_ep_ms": round(sp["ep_ms"] / n, 4)}
-            if sp["ep_ms"] >= sp["xg_ms"]:
-                t = sp["ep_ms"] / n * 1e-3
-                # per computed (k1, k2) pair the reduction reads |g|^2 (8 nb1 nb2 nm bytes), the
-                # inner point's occupations (16 nb2) and the phonon data (16 nm); a computed
-                # row is one (k1, b1, k2) triple, so pairs = rows / nb1
-                nb1, nb2, nm = res["nb1_mean"], res["nb2_mean"], res["nm"]
-                pair_bytes = 8.0 * nb1 * nb2 * nm + 16.0 * nb2 + 16.0 * nm
-                pairs = st["rows"] / nb1 / n
-                alg = pair_bytes * pairs
-                ach = alg / t / 1e9
-                g3 = st["rows"] * res["ncol"] * 8.0 / n  # g3 of every computed row, complex FP32
-                return {"bound": "hbm", "kernel": "k_sep_ep", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": traffic_per_dispatch("k_sep_ep", workload),
-                        "avg_launch_ms": round(sp["ep_ms"] / n, 4), "launches": n,
-                        "bytes_per_launch": alg, "pair_bytes": round(pair_bytes, 1),
-                        "pairs_per_launch": pairs,
-                        "g3_bytes_per_launch": g3, "g3_GBps": round(g3 / t / 1e9, 1),
-                        "frac_g3_bytes": round(g3 / t / 1e9 / HBM_PEAK_GBS, 4),
-                        "flop_model_note": "the per-pair FLOP model of the direct R_ph transform "
-                                           "(8 N_Rph nm nb1 Nw + rotations) does not apply: the separable "
-                                           "transform computes the same g3 with about 26x fewer operations",
-                        **extra}
-            ms, launches, flops = sp["xg_ms"], n, st["flops"]
-            kern = "k_sep_xg"
+            t_xg, t_ep = sp["xg_ms"] / n * 1e-3, sp["ep_ms"] / n * 1e-3
+            g3 = sp["g3_bytes"] / n
+            xg_bytes = (sp["slab_bytes"] + sp["g3_bytes"]) / n
+            # per computed (k1, k2) pair the reduction reads |g|^2 (8 nb1 nb2 nm bytes), the
+            # inner point's occupations (16 nb2) and the phonon data (16 nm); a computed
+            # row is one (k1, b1, k2) triple, so pairs = rows / nb1
+            nb1, nb2, nm = res["nb1_mean"], res["nb2_mean"], res["nm"]
+            pair_bytes = 8.0 * nb1 * nb2 * nm + 16.0 * nb2 + 16.0 * nm
+            pairs = st["rows"] / nb1 / n
+            xg = {"kernel": "k_sep_xg", "avg_launch_ms": round(t_xg * 1e3, 4), "bytes_per_launch": xg_bytes,
+                  "achieved_GBps": round(xg_bytes / t_xg / 1e9, 1),
+                  "frac": round(xg_bytes / t_xg / 1e9 / HBM_PEAK_GBS, 4),
+                  "traffic": traffic_per_dispatch("k_sep_xg", workload),
+                  "mfma_TFLOPs": round(st["flops"] / n / t_xg / 1e12, 3)}
+            ep = {"kernel": "k_sep_ep2", "avg_launch_ms": round(t_ep * 1e3, 4), "bytes_per_launch": g3,
+                  "achieved_GBps": round(g3 / t_ep / 1e9, 1), "frac": round(g3 / t_ep / 1e9 / HBM_PEAK_GBS, 4),
+                  "traffic": traffic_per_dispatch("k_sep_ep2", workload),
+                  "pair_bytes": round(pair_bytes, 1), "pairs_per_launch": pairs,
+                  "frac_pair_bytes": round(pair_bytes * pairs / t_ep / 1e9 / HBM_PEAK_GBS, 4)}
+            dom = xg if t_xg >= t_ep else ep
+            return {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["achieved_GBps"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["frac"], "traffic": dom["traffic"],
+                    "avg_launch_ms": dom["avg_launch_ms"], "launches": n,
+                    "bytes_per_launch": dom["bytes_per_launch"],
+                    "flop_model_note": "the per-pair FLOP model of the direct R_ph transform "
+                                       "(8 N_Rph nm nb1 Nw + rotations) does not apply: the separable "
+                                       "transform computes the same g3 with about 26x fewer operations",
+                    "k_sep_xg": xg, "k_sep_ep2": ep}
         else:
             ms, launches, flops = st["ms"], max(st["launches"], 1), st["flops"]
             kern = "k_sep_xm"

@@ -430,7 +430,7 @@ long long phx_lw_plan(phx_lw* h, int nk1, const int* k1_list, int k2_begin, int 
  * Returns the (k1, k2) pair count. */
 long long phx_lw_plan_run(phx_lw* h, int graph);
 /* Batch shapes: outer points per GEMM-form batch (1..32, the cacheElPh GEMM
- * M tile is 2 x k1_batch) and mesh lines per line batch (1..4). */
+ * M tile is 2 x k1_batch) and mesh lines per line batch (1..16). */
 int phx_lw_set_batching(phx_lw* h, int k1_batch, int lines_per_batch);
 /* Autotuner: times full runs over the sample k1_list for k1 batches {8, 16,
  * 32}, lines per batch {1, 2, 4} (line batching on) and panels per separable
@@ -469,12 +469,19 @@ int phx_lw_set_split_last_stage(phx_lw* h, int on);
  * out4 = {x-stage kernel ms, epilogue kernel ms, launches, sum over the rows
  * the epilogue computed of their band counts}.  No counterpart in the reference. */
 int phx_lw_last_stage_split_times(phx_lw* h, double* out4);
+/* With timing on and the last stage split: out2 = {bytes of the x-stage
+ * slabs the x-stage kernel read (slots computed x x-groups x nm nw complex
+ * FP32), bytes of g3 rows it wrote and the epilogue read (rows x nm nw
+ * complex FP32)}.  No counterpart in the reference (a measurement call). */
+int phx_lw_last_stage_split_bytes(phx_lw* h, double* out2);
 /* ---- Learned q-block screen (separable transform with the energy screen on).
  * No counterpart in the reference, which evaluates every (k1, k2) pair
  * (el_scattering.cpp:161-386).  A q-block is one panel (k1, b1) times one
  * (jy, jz) line of inner points.  A 16 -> 64 -> 64 -> 1 MLP with BF16 operands
  * and FP32 accumulation predicts ln of the block's contribution to the
- * linewidths of b1's degenerate group.  Blocks with a row within 3 sigma of
+ * linewidths of b1's degenerate group, as a residual on the block's Gaussian
+ * baseline ln(sum over its rows of e^{-x^2}), x a row's closest approach to
+ * resonance in units of sigma.  Blocks with a row within 3 sigma of
  * resonance are always computed; the others are skipped when their predicted
  * contribution, raised by the 99.9th-percentile under-prediction seen in
  * training, stays below threshold times the panel's predicted total.
@@ -482,10 +489,12 @@ int phx_lw_last_stage_split_times(phx_lw* h, double* out4);
  * above 10 x the threshold makes the batch recompute every skipped block.
  *
  * Collect training samples: runs the exact last stage on a random fraction
- * of the tail blocks (rows inside the energy screen, none within 3 sigma)
- * for the given outer points (all
- * when k1_list is null) and appends their features and log contributions.
- * The linewidths are not touched.  Returns the total sample count. */
+ * of the blocks inside the energy screen (resonant and tail blocks alike)
+ * for the given outer points (all when k1_list is null) and appends their
+ * features and log contributions.  The linewidths are not touched.  Returns
+ * the total sample count.  The training's under-prediction margin is taken
+ * over the samples without a row within 3 sigma (the blocks the screen may
+ * skip). */
 int phx_lw_screen_collect(phx_lw* h, int nk1, const int* k1_list, double sample_frac,
                           unsigned long long seed);
 /* Train the network on the collected samples on the GPU (Adam, MSE on the

@@ -179,6 +179,7 @@ EXPORTS = {
     "phx_phel_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_split_last_stage": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_last_stage_split_times": (C.c_int, [C.c_void_p, _dp]),
+    "phx_lw_last_stage_split_bytes": (C.c_int, [C.c_void_p, _dp]),
     "phx_phel_kernel_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_el_model_diag_velocities": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
 }
@@ -656,7 +657,7 @@ class ElectronLinewidths:
             raise PhxError(lib().phx_last_error().decode())
         return pairs
 
-    def set_batching(self, k1_batch=32, lines_per_batch=4):
+    def set_batching(self, k1_batch=32, lines_per_batch=16):
         _check(lib().phx_lw_set_batching(self._h, int(k1_batch), int(lines_per_batch)))
 
     def autotune(self, k1_list=None):
@@ -706,6 +707,13 @@ class ElectronLinewidths:
         o = np.zeros(4)
         _check(lib().phx_lw_last_stage_split_times(self._h, _p(o)))
         return dict(xg_ms=o[0], ep_ms=o[1], launches=int(o[2]), bands=o[3])
+
+    def split_bytes(self):
+        """Split last stage with timing on: bytes of x-stage slabs read by the
+        x-stage kernel, bytes of g3 rows it wrote (and the epilogue read)."""
+        o = np.zeros(2)
+        _check(lib().phx_lw_last_stage_split_bytes(self._h, _p(o)))
+        return dict(slab_bytes=o[0], g3_bytes=o[1])
 
     # ---- learned q-block screen
     def screen_collect(self, k1_list=None, sample_frac=0.02, seed=1):

@@ -833,6 +833,7 @@ struct LwHandle {
   long long xLaunches = 0, splitLaunches = 0;
   DevBuf<unsigned long long> xKept;  // rows computed by the last stage (timed launches)
   DevBuf<unsigned long long> xBands; // sum of their band counts (split epilogue)
+  DevBuf<unsigned long long> xSlots; // slots the split x-stage computed (timed launches)
   ~LwHandle() {
     if (graphExec) (void)hipGraphExecDestroy(graphExec);
     for (auto& e : ev)
@@ -845,7 +846,8 @@ struct LwHandle {
 // Whether the separable transform's last stage runs split (x-stage and
 // epilogue as two kernels, g3 rows through HBM): FP32 data, nm, nw <= 16,
 // nm nw <= 192, row masks (x mesh <= 64); the launcher also needs <= 24
-// x-groups and no learned screen, and falls back to the fused kernel otherwise.
+// x-groups (and, with the learned screen, the lean epilogue), and falls back
+// to the fused kernel otherwise.
 static bool lw_split_last(const LwHandle* L, const phx_problem& p) {
   return L->splitLast && L->transform == PHX_TRANSFORM_SEPARABLE && L->precision == PHX_FP32 && p.nm <= 16 &&
          p.nw <= 16 && p.nm * p.nw <= 192 && p.mesh[0] <= 64;
@@ -1373,10 +1375,10 @@ static void learned_screen_chunk(LwHandle* L, const CoupParams& cq, XStage&& xst
     PHX_HIP(hipStreamSynchronize(s));
     L->hMask.assign(slots, 0);
     L->hSel.clear();
-    // tail blocks only (rows inside the energy screen, none within 3 sigma): the
-    // blocks the network may skip
+    // a random fraction of every block inside the energy screen, resonant or
+    // not: the network predicts the panels' totals from all of them
     for (int i = 0; i < slots; ++i)
-      if ((L->hNscr[i] & 0xffff) > 0 && (L->hNscr[i] >> 16) == 0 &&
+      if ((L->hNscr[i] & 0xffff) > 0 &&
           screen_draw(L->lsSeed ^ 0xC011EC7ull, L->lsBatch, panelBase + i / nLinesYZ, nLinesYZ,
                       i % nLinesYZ) < L->lsSample) {
         L->hMask[i] = 1;
@@ -1422,6 +1424,7 @@ static void learned_screen_chunk(LwHandle* L, const CoupParams& cq, XStage&& xst
   dp.seed = L->lsSeed;
   dp.batchId = L->lsBatch;
   dp.margin = L->lsMargin;
+  dp.feat = L->lsFeat.p;
   dp.computeMask = L->lsMaskC.p;
   dp.auditMask = L->lsMaskA.p;
   dp.skipMask = L->lsMaskS.p;
@@ -1537,7 +1540,9 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
     nLinesB = int(lk.size());
     lk.push_back(nkk);
     PHX_REQUIRE(nLinesB <= kLinesPerBatch, "line batch: too many mesh lines");
-    b.hphLine.resize(size_t(nLinesB) * L->nRe);
+    // rows beyond nLinesB (the line count rounded up for stage a) stay zero
+    const int nLinesA = line_count_padded(nLinesB);
+    b.hphLine.assign(size_t(nLinesA) * L->nRe, make_double2(0., 0.));  // (R_e, line)
     for (int l = 0; l < nLinesB; ++l) {
       int j0, j1, j2;
       mesh_coords(L->active[k1s[lk[l]]], m0, m1, j0, j1, j2);
@@ -1545,7 +1550,7 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
         const double2 ty = twiddle((long long)j1 * L->reF1[i], m1);
         const double2 tz = twiddle((long long)j2 * L->reF2[i], m2);
         const double d = L->reInvDeg[i];
-        b.hphLine[size_t(l) * L->nRe + i] =
+        b.hphLine[size_t(i) * nLinesA + l] =
             make_double2((ty.x * tz.x - ty.y * tz.y) * d, (ty.x * tz.y + ty.y * tz.x) * d);
       }
     }
@@ -1575,8 +1580,9 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
                              hipMemcpyHostToDevice, s));
     }
     const size_t N = size_t(L->nRp) * nm * nw * nw;
-    if (L->precision == PHX_FP64) L->TgD.reserve(size_t(nLinesB) * L->nGe * N);
-    else L->TgF.reserve(size_t(nLinesB) * L->nGe * N);
+    const size_t nA = size_t(line_count_padded(nLinesB));
+    if (L->precision == PHX_FP64) L->TgD.reserve(nA * L->nGe * N);
+    else L->TgF.reserve(nA * L->nGe * N);
   } else {
     L->phE.reserve(size_t(nkk) * L->nRe);
   }
@@ -1684,6 +1690,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     cp.reOrder = L->reOrderD.p;
     cp.reGroupEnd = L->reGroupEndD.p;
     cp.nLinesB = nLinesB;
+    cp.nLinesA = line_count_padded(nLinesB);
     cp.lineKK = b.lineKK.p;
     cp.wLine = b.wLine.p;
   }
@@ -1862,6 +1869,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
           }
           c.keptRows = L->xKept.p;
           c.keptBands = L->xBands.p;
+          c.keptSlots = L->xSlots.p;
           mid = L->xev[L->xevUsed + 1];
           PHX_HIP(hipEventRecord(L->xev[L->xevUsed], s));
         }
@@ -3374,7 +3382,7 @@ static void lw_autotune(LwHandle* L, int nk1, const int* k1list, double* out4) {
     *knob = bestv;
   };
   tune(&L->k1Batch, {8, 16, 32});
-  if (L->lineMin > 0) tune(&L->linesPerBatch, {1, 2, 4});
+  if (L->lineMin > 0) tune(&L->linesPerBatch, {1, 2, 4, 8, 16});
   if (L->transform == PHX_TRANSFORM_SEPARABLE) tune(&L->panelChunk, {16, 48, 128});
   PHX_HIP(hipMemcpyAsync(L->lw.p, keep.p, L->lw.bytes(), hipMemcpyDeviceToDevice, s));
   PHX_HIP(hipStreamSynchronize(s));
@@ -3397,7 +3405,7 @@ int phx_lw_set_batching(phx_lw* h, int k1_batch, int lines_per_batch) {
     PHX_REQUIRE(h, "null handle");
     PHX_REQUIRE(k1_batch >= 1 && k1_batch <= kCacheNKK && lines_per_batch >= 1 &&
                     lines_per_batch <= kLinesPerBatch,
-                "phx_lw_set_batching: k1_batch in 1..32, lines_per_batch in 1..4");
+                "phx_lw_set_batching: k1_batch in 1..32, lines_per_batch in 1..16");
     h->k1Batch = k1_batch;
     h->linesPerBatch = lines_per_batch;
   });
@@ -3570,10 +3578,17 @@ static void ls_train(LwHandle* L, int epochs, int batch, float lr, uint64_t seed
     std::vector<float> yv(n);
     PHX_HIP(hipMemcpy(yv.data(), L->mlpScratch.p + size_t(n) * (4 * kMlpHid + 1), n * sizeof(float),
                       hipMemcpyDeviceToHost));
-    std::vector<double> r(n);
-    for (int i = 0; i < n; ++i) r[i] = (double(tn[i]) - yv[i]) * sd;
+    // over the samples the screen may skip (no row within 3 sigma: feature 11
+    // is zero), all samples when fewer than 100 of them
+    std::vector<double> r;
+    for (int pass = 0; pass < 2 && r.size() < 100; ++pass) {
+      r.clear();
+      for (int i = 0; i < n; ++i)
+        if (pass == 1 || L->trX[size_t(i) * kMlpIn + 11] == 0.f) r.push_back((double(tn[i]) - yv[i]) * sd);
+    }
     std::sort(r.begin(), r.end());
-    L->lsMargin = std::max(0., r[std::min(n - 1, int(0.999 * n))]);
+    const int nr = int(r.size());
+    L->lsMargin = std::max(0., r[std::min(nr - 1, int(0.999 * nr))]);
   }
   // fold the target normalization into the output layer: the network predicts ln c
   PHX_HIP(hipMemcpy(P.data(), L->mlpP.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
@@ -3716,8 +3731,10 @@ int phx_lw_set_timing(phx_lw* h, int on) {
     h->xevSplit.clear();
     if (!h->xKept.p) h->xKept.alloc(1);
     if (!h->xBands.p) h->xBands.alloc(1);
+    if (!h->xSlots.p) h->xSlots.alloc(1);
     launch_fill<int>(reinterpret_cast<int*>(h->xKept.p), 2, 0, h->stream);
     launch_fill<int>(reinterpret_cast<int*>(h->xBands.p), 2, 0, h->stream);
+    launch_fill<int>(reinterpret_cast<int*>(h->xSlots.p), 2, 0, h->stream);
     PHX_HIP(hipStreamSynchronize(h->stream));
   });
 }
@@ -3761,6 +3778,21 @@ int phx_lw_last_stage_split_times(phx_lw* h, double* out4) {
     out4[1] = h->epMs;
     out4[2] = double(h->splitLaunches);
     out4[3] = double(bands);
+  });
+}
+
+int phx_lw_last_stage_split_bytes(phx_lw* h, double* out2) {
+  return guard([&] {
+    PHX_REQUIRE(h && out2, "null argument");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    lw_harvest_timing(h);
+    unsigned long long slots = 0, rows = 0;
+    if (h->xSlots.p) PHX_HIP(hipMemcpy(&slots, h->xSlots.p, sizeof(slots), hipMemcpyDeviceToHost));
+    if (h->xKept.p) PHX_HIP(hipMemcpy(&rows, h->xKept.p, sizeof(rows), hipMemcpyDeviceToHost));
+    // complex FP32: the x-stage reads nG0 x (nm nw) per slot and writes nm nw per kept row
+    const double row = 8. * h->nm * h->nw;
+    out2[0] = double(slots) * h->nG0 * row;
+    out2[1] = h->splitLaunches > 0 ? double(rows) * row : 0.;
   });
 }
 

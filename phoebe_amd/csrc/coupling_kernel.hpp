@@ -101,6 +101,7 @@ struct CoupParams {
   void* G3;
   unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
   int epFirst;                            // split epilogue: 1 = k_sep_ep, 0 = k_sep_ep2 where it applies
+  unsigned long long* keptSlots;          // k_sep_xg: slots (panel, yz line) it computed (null: off)
 };
 
 template <typename T> struct Vec2;
@@ -1064,7 +1065,8 @@ struct SepXmRow {            // per kept row, staged in LDS
   int jx, row, iq, nb2, u2off, soff;
 };
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short s16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 // Orders LDS accesses between the lanes of one wavefront (no block barrier).
 __device__ __forceinline__ void wave_sync() {
@@ -1138,14 +1140,18 @@ __device__ inline void sep_xm_stage_rows(const CoupParams& p, const SepXmRow* ro
   }
 }
 
-// Split-BF16 operands: x ~ hi + lo with hi = bf16(x), lo = bf16(x - hi).
-__device__ inline void bf16_split4(const float (&x)[4], bf16x4_t& hi, bf16x4_t& lo) {
+// Split-BF16 operands of v_mfma_f32_16x16x32_bf16: x ~ hi + lo with
+// hi = bf16(x), lo = bf16(x - hi).
+__device__ inline void bf16_split8(const float (&x)[8], bf16x8_t& hi, bf16x8_t& lo) {
+  s16x8_t h8, l8;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < 8; ++j) {
     const short h = bf16_bits(x[j]);
-    hi[j] = h;
-    lo[j] = bf16_bits(x[j] - bf16_value(h));
+    h8[j] = h;
+    l8[j] = bf16_bits(x[j] - bf16_value(h));
   }
+  hi = __builtin_bit_cast(bf16x8_t, h8);
+  lo = __builtin_bit_cast(bf16x8_t, l8);
 }
 
 // Second half of an epilogue pass of k_sep_ep (k_sep_xm keeps the same steps
@@ -1373,7 +1379,7 @@ __device__ __forceinline__ void sep_epilogue_tail_lds(const CoupParams& p, const
 }
 
 // BF3: the x-stage GEMMs in split-BF16x3 (hi*hi + hi*lo + lo*hi on
-// v_mfma_f32_16x16x16_bf16, FP32 accumulation); T must be float.
+// v_mfma_f32_16x16x32_bf16, FP32 accumulation); T must be float.
 // BREG: the B operands of all k-steps (at most 12, i.e. 24 x-groups) are loaded
 // once per work item, in the prologue, where their latency overlaps the row
 // compaction; otherwise they are loaded per tile in chunks of KSB k-steps.
@@ -1582,42 +1588,43 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
       for (int i = 0; i < 4; ++i) accr[j][i] = acci[j][i] = T(0);
     if constexpr (BF3) {
       sep_xm_stage_rows(p, rows, m0r, nr, nw, nm, u2s, sU2, se2, sw3, sg3, sf2);
-      // 16 real K per step: lane (lr, lg) holds k = 16 kb + 4 lg + j, i.e. groups
-      // g = 8 kb + 2 lg and g + 1, (Re, Im) each
-      for (int kb = 0; kb < (2 * nG0 + 15) / 16; ++kb) {
-        const int gA = 8 * kb + 2 * lg;
-        const T2v w0 = wrow[gA < nG0 ? gA : nG0], w1 = wrow[gA + 1 < nG0 ? gA + 1 : nG0];
-        float a1[4] = {w0.x, -w0.y, w1.x, -w1.y}, a2[4] = {w0.y, w0.x, w1.y, w1.x};
-        if (!rowOk)
+      // gfx950's v_mfma_f32_16x16x32_bf16, 32 real K per step: lane (lr, lg)
+      // holds groups g = 16 kb + 4 lg + h, h < 4, (Re, Im) each, in the A and
+      // the B operand alike (the pairing of K slots is all the product needs)
+      for (int kb = 0; kb < (2 * nG0 + 31) / 32; ++kb) {
+        const int gA = 16 * kb + 4 * lg;
+        float a1[8], a2[8];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) a1[j] = a2[j] = 0.f;
-        bf16x4_t a1h, a1l, a2h, a2l;
-        bf16_split4(a1, a1h, a1l);
-        bf16_split4(a2, a2h, a2l);
+        for (int h = 0; h < 4; ++h) {
+          const T2v w = wrow[gA + h < nG0 ? gA + h : nG0];
+          const bool on = rowOk && gA + h < nG0;
+          a1[2 * h] = on ? w.x : 0.f;
+          a1[2 * h + 1] = on ? -w.y : 0.f;
+          a2[2 * h] = on ? w.y : 0.f;
+          a2[2 * h + 1] = on ? w.x : 0.f;
+        }
+        bf16x8_t a1h, a1l, a2h, a2l;
+        bf16_split8(a1, a1h, a1l);
+        bf16_split8(a2, a2h, a2l);
 #pragma unroll
         for (int j = 0; j < kXmNt; ++j) {
           const int c = (wave * kXmNt + j) * 16 + lr;
-          float b[4] = {0.f, 0.f, 0.f, 0.f};
-          if (c < ncol) {
-            if (gA < nG0) {
-              const T2v x = X[size_t(gA) * gStride + c];
-              b[0] = x.x;
-              b[1] = x.y;
-            }
-            if (gA + 1 < nG0) {
-              const T2v x = X[size_t(gA + 1) * gStride + c];
-              b[2] = x.x;
-              b[3] = x.y;
-            }
+          float b[8];
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            T2v x = T2v{0.f, 0.f};
+            if (c < ncol && gA + h < nG0) x = X[size_t(gA + h) * gStride + c];
+            b[2 * h] = x.x;
+            b[2 * h + 1] = x.y;
           }
-          bf16x4_t bh, bl;
-          bf16_split4(b, bh, bl);
-          accr[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1h, bh, accr[j], 0, 0, 0);
-          accr[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1h, bl, accr[j], 0, 0, 0);
-          accr[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1l, bh, accr[j], 0, 0, 0);
-          acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2h, bh, acci[j], 0, 0, 0);
-          acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2h, bl, acci[j], 0, 0, 0);
-          acci[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2l, bh, acci[j], 0, 0, 0);
+          bf16x8_t bh, bl;
+          bf16_split8(b, bh, bl);
+          accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, bh, accr[j], 0, 0, 0);
+          accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, bl, accr[j], 0, 0, 0);
+          accr[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1l, bh, accr[j], 0, 0, 0);
+          acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bh, acci[j], 0, 0, 0);
+          acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bl, acci[j], 0, 0, 0);
+          acci[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l, bh, acci[j], 0, 0, 0);
         }
       }
     } else if constexpr (BREG) {
@@ -1863,9 +1870,12 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
   const int yz = Lx / p.nPanels, panel = Lx % p.nPanels;
   const int slot = panel * nLinesYZ + yz;
   const int jy = yz % p.m1, jz = yz / p.m1;
+  if (p.gate && *p.gate == 0) return;
+  if (p.slotOn && !p.slotOn[slot]) return;
   const unsigned long long mask = p.keepMask[slot];
   if (mask == 0ull) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
+  if (t == 0 && p.keptSlots) atomicAdd(p.keptSlots, 1ull);
   const int nm = p.nm, nw = p.nw, ncol = nm * nw;
   const int nG0 = p.nG0, nG0p = nG0 + 1;
   int* jxs = reinterpret_cast<int*>(smem);                                     // 64: kept rows, ascending
@@ -2183,6 +2193,9 @@ struct EpParams {
   const double* screenOf;  // per outer state: screen width of the panel its b1 group starts (null: screen)
   unsigned long long* keptRows;
   unsigned long long* keptBands;
+  const unsigned char* slotOn;  // compute only slots with slotOn != 0 (null: all)
+  const int* gate;              // run only when *gate != 0 (null: always)
+  int slotZero;                 // slots off: 1 = write zero partials, 0 = leave them
   double invSigma, deltaPref, norm, cutoff, screen;
   int m0, m1, m2, nPanels, nm, nw, nCalc, numStates;
   int rpw;                 // rows per pass: whole rows of nm lanes, rpw nm <= 64
@@ -2267,6 +2280,11 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   const int slot = panel * nLinesYZ + yz;
   const int jy = yz % p.m1, jz = yz / p.m1;
   const int t = threadIdx.x;
+  if (p.gate && *p.gate == 0) return;
+  if (p.slotOn && !p.slotOn[slot]) {
+    if (p.slotZero && t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
+    return;
+  }
   const unsigned long long mask = p.keepMask[slot];
   if (mask == 0ull) {
     if (t < kGMax * kCMax) p.partial[size_t(slot) * kGMax * kCMax + t] = 0.;
@@ -2704,8 +2722,10 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
       PHX_HIP(hipGetLastError());
     }
     if constexpr (sizeof(T) == 4) {
-      const bool split = p.G3 && !bf3 && !wide && p.keepMask && !p.slotOn && !p.gate && p.nm <= 16 &&
-                         p.nw <= 16 && (2 * p.nG0 + 3) / 4 <= 12;
+      // the lean epilogue also takes slot masks and a gate (the learned screen's passes)
+      const bool ep2ok = !p.epFirst && !p.adaptive && p.nb2max <= 8;
+      const bool split = p.G3 && !bf3 && !wide && p.keepMask && (ep2ok || (!p.slotOn && !p.gate)) &&
+                         p.nm <= 16 && p.nw <= 16 && (2 * p.nG0 + 3) / 4 <= 12;
       if (split) {
         const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0);
         // whole rows of nm threads per wave, at most 8, fewer if the LDS
@@ -2731,7 +2751,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         // the launch, no more than the launch needs
         const int gm = p.gsizeMax > 0 ? p.gsizeMax : kGMax;
         PHX_REQUIRE(gm <= kGMax, "separable transform: b1 group larger than kGMax");
-        if (!p.epFirst && !p.adaptive && p.nb2max <= 8) {
+        if (ep2ok) {
           // lean epilogue: rows per pass capped by its U2 prefetch registers
           const int nb2 = p.nb2max <= 4 ? 4 : 8;
           const int rpw2 = std::min(64 / p.nm, nb2 <= 4 ? 5 : 4);
@@ -2759,6 +2779,9 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           q.screenOf = p.screenOf;
           q.keptRows = p.keptRows;
           q.keptBands = p.keptBands;
+          q.slotOn = p.slotOn;
+          q.gate = p.gate;
+          q.slotZero = p.slotZero;
           q.invSigma = p.invSigma;
           q.deltaPref = p.deltaPref;
           q.norm = p.norm;

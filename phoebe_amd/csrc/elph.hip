@@ -417,7 +417,7 @@ template void launch_cache<float, float>(const CacheParams&, hipStream_t);
 // Outer points that share the mesh line (j1, j2) and differ in j0 factor the
 // phase: with n_d = b_d.R_e / 2pi,
 //   e^{ik1.R_e} = e^{2 pi i j0 n0/m0} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}.
-// Stage a (one HBM stream over gW for up to kLinesPerBatch lines):
+// Stage a (one HBM stream over gW for up to kLinesPerBatch = 16 lines):
 //   Tg(l, g, c) = sum_{R_e in group g} e^{2 pi i (j1 n1/m1 + j2 n2/m2)}/de gW(R_e, c)
 // with groups of equal n0 mod m0 (about 2 * 10 + 1 of them for a 10^3 R_e grid).
 // Stage b (per outer point): g1(k1, c) = sum_g e^{2 pi i j0 n0(g)/m0} Tg(l(k1), g, c),
@@ -425,7 +425,8 @@ template void launch_cache<float, float>(const CacheParams&, hipStream_t);
 constexpr int kLineDepth = 4;  // gW rows in flight per thread (16 bytes each)
 
 // NL lines, CPT adjacent columns per thread (one 16-byte load per R_e for
-// CPT * sizeof(T2) == 16).
+// CPT * sizeof(T2) == 16).  The phase table is (R_e, line): the NL factors of
+// one R_e are contiguous, read by the whole workgroup alike.
 template <typename T, int NL, int CPT>
 __global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
   typedef typename Vec2<T>::type T2;
@@ -459,7 +460,7 @@ __global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
       if (i < p.nRe) {
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
-          const T2 f = ph[size_t(l) * p.nRe + i];
+          const T2 f = ph[size_t(i) * NL + l];
 #pragma unroll
           for (int c = 0; c < CPT; ++c) {
             ar[l][c] += f.x * cur[u][2 * c] - f.y * cur[u][2 * c + 1];
@@ -588,13 +589,18 @@ template <typename T, typename OutT>
 void launch_cache_line(const CacheParams& p, hipStream_t s) {
   PHX_REQUIRE(p.nw <= 32, "cacheElPh kernel supports nw <= 32");
   PHX_REQUIRE(p.nGe >= 1 && p.N == size_t(p.nRp) * p.nm * p.nw * p.nw, "cacheElPh line: bad sizes");
-  PHX_REQUIRE(p.nLinesB >= 1 && p.nLinesB <= kLinesPerBatch, "cacheElPh line: 1 to 4 lines per batch");
+  PHX_REQUIRE(p.nLinesB >= 1 && p.nLinesB <= kLinesPerBatch && p.nLinesA == line_count_padded(p.nLinesB),
+              "cacheElPh line: 1 to 16 lines per batch");
   if (p.nkk <= 0) return;
-  switch (p.nLinesB) {
+  switch (p.nLinesA) {
     case 1: launch_line_a<T, 1>(p, s); break;
     case 2: launch_line_a<T, 2>(p, s); break;
     case 3: launch_line_a<T, 3>(p, s); break;
-    default: launch_line_a<T, 4>(p, s); break;
+    case 4: launch_line_a<T, 4>(p, s); break;
+    case 6: launch_line_a<T, 6>(p, s); break;
+    case 8: launch_line_a<T, 8>(p, s); break;
+    case 12: launch_line_a<T, 12>(p, s); break;
+    default: launch_line_a<T, 16>(p, s); break;
   }
   const size_t lds = size_t(2) * 16 * 256 * sizeof(T) + size_t(16) * kLineGe * 2 * sizeof(T);
   auto kb = k_cache_line_b<T, OutT>;

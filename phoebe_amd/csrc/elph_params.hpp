@@ -11,8 +11,14 @@ namespace phx {
 constexpr int kGMax = 8;       // largest degenerate b1 group handled in one panel
 constexpr int kCMax = 4;       // largest number of (T, mu) calculations
 constexpr int kCacheNKK = 32;  // outer points per GEMM cacheElPh launch
-constexpr int kLinesPerBatch = 4;   // mesh lines per line-form cacheElPh launch
-constexpr int kLinePointsMax = 256; // outer points per line-form batch
+constexpr int kLinesPerBatch = 16;   // mesh lines per line-form cacheElPh launch
+constexpr int kLinePointsMax = 1024; // outer points per line-form batch
+
+// Line counts k_cache_line_a is instantiated for; a batch's count is rounded
+// up to the next one (the extra lines have zero phases and are never read).
+inline int line_count_padded(int n) {
+  return n <= 4 ? n : n <= 6 ? 6 : n <= 8 ? 8 : n <= 12 ? 12 : 16;
+}
 
 // g1(k1; Rp,nu,w1,w2) = sum_Re e^{ik1.Re}/de gW(Re,Rp,nu,w1,w2), rotated by U1
 // and scaled by s(k1, Rp) into one real panel per (k1, b1).
@@ -34,9 +40,10 @@ struct CacheParams {
   const int* reOrder;     // nRe: native index of the i-th R_e in group order
   const int* reGroupEnd;  // nRe: group index if the i-th R_e ends its group, else -1
   int nLinesB;            // lines of the batch (<= kLinesPerBatch)
+  int nLinesA;            // lines stage a computes: line_count_padded(nLinesB), rows of phLine and Tg
   const int* lineKK;      // nLinesB + 1: first outer point of each line (points grouped by line)
-  const void* phLine;     // (nLinesB, nRe) complex T: e^{2 pi i (j1 n1/m1 + j2 n2/m2)} / de
-  void* Tg;               // (nLinesB, nGe, N) complex T: partial sums of the lines
+  const void* phLine;     // (nRe, nLinesA) complex T: e^{2 pi i (j1 n1/m1 + j2 n2/m2)} / de
+  void* Tg;               // (nLinesA, nGe, N) complex T: partial sums of the lines
   const double2* wLine;   // (nkk, nGe): e^{2 pi i j0(kk) n0(g) / m0}
 };
 

@@ -1,5 +1,5 @@
 // Learned q-block screen: MLP forward / backward / Adam on
-// v_mfma_f32_16x16x16_bf16, block features, skip decisions and the audit.
+// v_mfma_f32_16x16x32_bf16, block features, skip decisions and the audit.
 // No counterpart in the reference, which evaluates every (k1, k2) pair
 // (el_scattering.cpp:161-386).
 #include <algorithm>
@@ -11,13 +11,15 @@
 
 namespace phx {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ GEMM
 // Workgroup tile 64 x 64: wave w owns rows 16w..16w+15 and four 16-column
-// tiles.  v_mfma_f32_16x16x16_bf16: A lane l holds A[l%16][4(l/16)+j], B lane l
-// holds B[4(l/16)+j][l%16], D lane l holds D[4(l/16)+i][l%16].
+// tiles.  gfx950's v_mfma_f32_16x16x32_bf16: A lane l holds row l%16, B lane l
+// column l%16, each with 8 K values (here k = 8(l/16) + j, the same slots in
+// A and B, which is all the product needs); D lane l holds D[4(l/16)+i][l%16].
 __global__ void __launch_bounds__(256) k_mlp_gemm(MlpGemm g) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   const int m0 = blockIdx.x * 64 + wave * 16, n0 = blockIdx.y * 64;
@@ -25,11 +27,11 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(MlpGemm g) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int m = m0 + lr;
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    bf16x4 a;
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    s16x8 a;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + 4 * lg + j;
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + 8 * lg + j;
       float v = 0.f;
       if (m < g.M && k < g.K) v = g.ta ? g.A[size_t(k) * g.lda + m] : g.A[size_t(m) * g.lda + k];
       a[j] = bf16_bits(v);
@@ -37,15 +39,16 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(MlpGemm g) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int n = n0 + 16 * t + lr;
-      bf16x4 b;
+      s16x8 b;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int k = k0 + 4 * lg + j;
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + 8 * lg + j;
         float v = 0.f;
         if (n < g.N && k < g.K) v = g.tb ? g.B[size_t(n) * g.ldb + k] : g.B[size_t(k) * g.ldb + n];
         b[j] = bf16_bits(v);
       }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), acc[t], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(256) k_block_features(FeatParams p) {
   const double* e1g = p.epool + p.kkStateOff[kk] + p.panelGStart[panel];
   const int gsz = p.panelGSize[panel];
   const double e1 = p.epool[p.kkStateOff[kk] + p.panelB1[panel]];
-  double dmin = 1e300, e2min = 1e300, e2max = -1e300, wmin = 1e300, wmax = -1e300;
+  double dmin = 1e300, e2min = 1e300, e2max = -1e300, wmin = 1e300, wmax = -1e300, wsum = 0.;
   int n3 = 0, nscr = 0, nval = 0;
   for (int jx = 0; jx < p.m0; ++jx) {
     const int mesh = (jz * p.m1 + jy) * p.m0 + jx;
@@ -225,7 +228,12 @@ __global__ void __launch_bounds__(256) k_block_features(FeatParams p) {
     }
     dmin = fmin(dmin, d);
     n3 += d < 3. * p.sigma;
-    nscr += d < p.screen * p.sigma;
+    if (d < p.screen * p.sigma) {
+      ++nscr;
+      // the row's largest Gaussian weight: its interval gap to resonance
+      const double x = d / p.sigma;
+      wsum += exp(-x * x);
+    }
   }
   if (nval == 0) e2min = e2max = p.mu;
   if (!(wmax >= wmin)) wmin = wmax = 0.;
@@ -245,7 +253,9 @@ __global__ void __launch_bounds__(256) k_block_features(FeatParams p) {
   f[12] = float(nscr) / p.m0;
   f[13] = float(nval) / p.m0;
   f[14] = float(gsz) / kGMax;
-  f[15] = 1.f;
+  // ln of the summed row weights: the baseline the network's prediction is
+  // added to (it learns ln contribution - baseline), stored scaled
+  f[15] = float((wsum > 0. ? log(wsum) : -kScreenBaseFloor) / kScreenBaseScale);
   // rows inside the energy screen (low 16 bits) and within 3 sigma of resonance (high bits)
   p.nscreen[slot] = nscr | (n3 << 16);
 }
@@ -258,6 +268,11 @@ void launch_block_features(const FeatParams& p, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ decisions
+// The block's baseline ln contribution (feature 15, scaled back).
+__device__ inline double screen_base(const float* feat, size_t slot) {
+  return double(feat[slot * kMlpIn + 15]) * kScreenBaseScale;
+}
+
 __global__ void __launch_bounds__(256) k_screen_decide(DecideParams p) {
   __shared__ double red[256];
   __shared__ long long cnt[3];
@@ -265,7 +280,7 @@ __global__ void __launch_bounds__(256) k_screen_decide(DecideParams p) {
   const size_t s0 = size_t(panel) * p.nLinesYZ;
   double sum = 0.;
   for (int yz = t; yz < p.nLinesYZ; yz += 256)
-    if ((p.nscreen[s0 + yz] & 0xffff) > 0) sum += exp(double(p.pred[s0 + yz]));
+    if ((p.nscreen[s0 + yz] & 0xffff) > 0) sum += exp(double(p.pred[s0 + yz]) + screen_base(p.feat, s0 + yz));
   red[t] = sum;
   if (t < 3) cnt[t] = 0;
   __syncthreads();
@@ -283,7 +298,7 @@ __global__ void __launch_bounds__(256) k_screen_decide(DecideParams p) {
     if (nscr > 0) {
       ++nin;
       // blocks with a row within 3 sigma of resonance are always exact
-      if (nnear > 0 || exp(double(p.pred[slot]) + p.margin) >= thrAbs) {
+      if (nnear > 0 || exp(double(p.pred[slot]) + screen_base(p.feat, slot) + p.margin) >= thrAbs) {
         c = 1;
       } else {
         ++nskip;
@@ -394,7 +409,8 @@ __global__ void k_gather_targets(int nSel, const int* slots, const double* parti
   const double* src = partial + slot * kGMax * kCMax;
   double c = 0.;
   for (int gi = 0; gi < kGMax; ++gi) c += src[gi * kCMax];
-  t[i] = float(log(fmax(c, 1e-40)));
+  // residual target: ln contribution minus the block's Gaussian baseline
+  t[i] = float(log(fmax(c, 1e-40)) - double(feat[slot * kMlpIn + 15]) * kScreenBaseScale);
   for (int f = 0; f < kMlpIn; ++f) X[size_t(i) * kMlpIn + f] = feat[slot * kMlpIn + f];
 }
 

@@ -29,6 +29,12 @@ constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHid * kMlpHid;
 constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHid;
 constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHid;
 constexpr int kMlpParams = kMlpOffB3 + 1;
+// Feature 15 of a block is ln(sum over its rows inside the energy screen of
+// e^{-x^2}), x the row's closest approach to resonance in units of sigma,
+// divided by kScreenBaseScale; -kScreenBaseFloor when no row is inside.  The
+// network predicts ln(contribution) minus this baseline.
+constexpr double kScreenBaseScale = 32.;
+constexpr double kScreenBaseFloor = 64.;
 
 // C(M x N) = opA(M x K) opB(K x N) with operands rounded to BF16 (round to
 // nearest even) and FP32 accumulation, then C += bias(n), relu, or
@@ -73,6 +79,7 @@ void launch_block_features(const FeatParams& p, hipStream_t s);
 // exp(pred + margin) >= thr * (sum of exp(pred) over the panel's blocks).
 struct DecideParams {
   int nPanels, nLinesYZ;
+  const float* feat;       // block features; feature 15 is the block's baseline / kScreenBaseScale
   int panelBase;           // batch-wide index of panel 0 (hash key of the audit draw)
   const float* pred; const int* nscreen;
   double thr, auditFrac; uint64_t seed, batchId;

@@ -43,10 +43,11 @@ def _problem():
     return synth.make_problem(2, 6, 3, 3, (12, 12, 12), seed=61, name="screen")
 
 
-def _handle(p, gW):
-    lw = P.ElectronLinewidths(p, gW=gW, precision="fp64", transform="separable")
+def _handle(p, gW, precision="fp64", split=True):
+    lw = P.ElectronLinewidths(p, gW=gW, precision=precision, transform="separable")
     lw.set_screen(7.0)
     lw.set_line_batching(1)
+    lw.set_split_last_stage(split)  # FP32: x-stage and lean epilogue kernels, or the fused kernel
     return lw
 
 
@@ -82,13 +83,18 @@ def test_trained_screen_skips_blocks_within_its_error_bound():
     assert np.array_equal(g1 == 0, g0 == 0)
 
 
-def test_failed_audit_falls_back_to_the_exact_blocks():
+@pytest.mark.parametrize("precision,split", [("fp64", True), ("fp32", True), ("fp32", False)])
+def test_failed_audit_falls_back_to_the_exact_blocks(precision, split):
+    """The three passes of a screened batch (computed blocks, audited blocks,
+    the gated fallback over the skipped ones) reproduce the exact screened
+    run, in the fused kernel and in the split x-stage + epilogue kernels."""
+    say("failed audit", precision, "split" if split else "fused")
     p = _problem()
     gW = p.gw_host()
-    exact = _handle(p, gW)
+    exact = _handle(p, gW, precision, split)
     exact.run()
     g0 = exact.linewidths(average=False)
-    lw = _handle(p, gW)
+    lw = _handle(p, gW, precision, split)
     # y = 100 relu(f11) - 300 with f11 > 0 only for blocks with a row within 3 sigma:
     # every prediction is absurdly small, so the skipped tail blocks' exact
     # contributions dwarf the threshold and the audit must fail
@@ -107,6 +113,32 @@ def test_failed_audit_falls_back_to_the_exact_blocks():
     g1 = lw.linewidths(average=False)
     m = g0 != 0
     assert rel_err(g1[m], g0[m]).max() < 1e-12
+
+
+def test_learned_screen_split_equals_fused():
+    """FP32 with the trained network: the split last stage (which takes the
+    screen's slot masks and gate) skips the same blocks and gives the fused
+    kernel's linewidths."""
+    p = _problem()
+    gW = p.gw_host()
+    lw = _handle(p, gW, "fp32", True)
+    ns = lw.screen_collect(sample_frac=0.2, seed=3)
+    lw.screen_train(epochs=200, batch=512, lr=2e-3, seed=5)
+    params = lw.screen_params()
+    out, stats = [], []
+    for split in (True, False):
+        h = _handle(p, gW, "fp32", split)
+        h.screen_params(params)
+        h.set_learned_screen(True, threshold=1e-6, audit_frac=0.05, seed=9)
+        h.run()
+        out.append(h.linewidths(average=False))
+        st = h.screen_stats()
+        stats.append((st["blocks_in_screen"], st["learned_skips"], st["audited"], st["fallbacks"]))
+    say(f"{ns} samples; split vs fused stats {stats}")
+    assert stats[0] == stats[1] and stats[0][1] > 0
+    nz = out[1] != 0
+    assert np.array_equal(out[0] != 0, nz)
+    assert rel_err(out[0][nz], out[1][nz]).max() < 1e-12
 
 
 def test_learned_screen_argument_checks():

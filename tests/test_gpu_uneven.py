@@ -82,7 +82,7 @@ def test_autotuner_keeps_results():
     k1 = np.arange(min(lw.num_active, 64), dtype=np.int32)
     best = lw.autotune(k1)
     say("autotuned", best)
-    assert best["k1_batch"] in (8, 16, 32) and best["lines_per_batch"] in (1, 2, 4)
+    assert best["k1_batch"] in (8, 16, 32) and best["lines_per_batch"] in (1, 2, 4, 8, 16)
     assert np.array_equal(lw.linewidths(average=False), np.zeros_like(ref))  # restored
     lw.run()
     got = lw.linewidths(average=False)
