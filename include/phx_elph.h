@@ -462,7 +462,10 @@ int phx_lw_set_panel_chunk(phx_lw* h, int panels);
  * values are the same either way and the linewidths agree to FP64 rounding.
  * on = 1 (default) runs the lean epilogue kernel where it applies (fixed
  * Gaussian smearing, at most 8 bands per inner point), on = 2 the first
- * epilogue kernel always; both give the same linewidths.
+ * epilogue kernel always; both give the same linewidths.  on = 3 runs the
+ * lean epilogue kernel alone, computing each kept row's x-stage from the
+ * x-stage slabs itself (FP32 FMAs instead of the x-stage kernel's MFMAs), so
+ * no g3 goes through device memory; the linewidths agree to FP32 rounding.
  * No counterpart in the reference. */
 int phx_lw_set_split_last_stage(phx_lw* h, int on);
 /* With timing on, the launches of phx_lw_stage_times that ran split:

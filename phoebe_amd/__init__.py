@@ -698,8 +698,9 @@ class ElectronLinewidths:
 
     def set_split_last_stage(self, on=True):
         """Separable last stage as two kernels (default) or the fused one;
-        on=2 runs the split stage with the first epilogue kernel."""
-        _check(lib().phx_lw_set_split_last_stage(self._h, 2 if on == 2 else int(bool(on))))
+        on=2 runs the split stage with the first epilogue kernel, on=3 the
+        lean epilogue kernel computing the x-stage itself."""
+        _check(lib().phx_lw_set_split_last_stage(self._h, on if on in (2, 3) else int(bool(on))))
 
     def split_times(self):
         """Split last stage with timing on: x-stage ms, epilogue ms, launches,

@@ -778,7 +778,8 @@ struct LwHandle {
   // batched together when the line holds at least lineMin of them (0 = off)
   int lineMin = 0;
   int panelChunk = 48;     // panels per separable-transform launch (bounds T1 / T2)
-  int splitLast = 1;       // last stage as k_sep_xg + epilogue where they apply (2: the first epilogue, k_sep_ep)
+  int splitLast = 1;       // last stage as k_sep_xg + epilogue where they apply (2: the first epilogue,
+                           // k_sep_ep; 3: the lean epilogue computing the x-stage itself)
   DevBuf<float2> G3f;      // their g3 rows of one panel chunk, (slot, jx, col)
   int linesPerBatch = kLinesPerBatch;  // mesh lines per line batch (1 .. kLinesPerBatch)
   bool reLattice = false;  // every R_e is a lattice vector (required by the line form)
@@ -1831,6 +1832,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     if (lw_split_last(L, p)) {
       cq.G3 = L->G3f.p;
       cq.epFirst = L->splitLast == 2;
+      cq.epFuse = L->splitLast == 3;
     }
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
@@ -3477,7 +3479,7 @@ int phx_lw_set_line_batching(phx_lw* h, int min_points) {
 int phx_lw_set_split_last_stage(phx_lw* h, int on) {
   return guard([&] {
     PHX_REQUIRE(h, "phx_lw_set_split_last_stage: null handle");
-    PHX_REQUIRE(on >= 0 && on <= 2, "phx_lw_set_split_last_stage: 0, 1 or 2");
+    PHX_REQUIRE(on >= 0 && on <= 3, "phx_lw_set_split_last_stage: 0 to 3");
     h->splitLast = on;
   });
 }

@@ -101,6 +101,7 @@ struct CoupParams {
   void* G3;
   unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
   int epFirst;                            // split epilogue: 1 = k_sep_ep, 0 = k_sep_ep2 where it applies
+  int epFuse;                             // k_sep_ep2 computes the x-stage itself (no k_sep_xg, no g3 in HBM)
   unsigned long long* keptSlots;          // k_sep_xg: slots (panel, yz line) it computed (null: off)
 };
 
@@ -683,40 +684,67 @@ struct SepParams {
   void* T2; size_t t2PanelStride;               // (panel, group, jy, jz, col)
 };
 
+// Register-blocked short transform shared by k_sep_z and k_sep_y:
+// out(j, c) = sum_e W(e, j) X(e, c) for j < m, c < ncol, X and W in LDS.
+// Thread (wave w, lane l) takes the columns c = l, l + 64, ... and the rows
+// j = w + 4 i, kSepJB of them at once, so one LDS read of X(e, c) serves
+// kSepJB rows and W(e, j) is the same for the whole wave (a broadcast); no
+// integer division.  Every output sums its terms in ascending e, as a plain
+// loop over the outputs would.
+constexpr int kSepJB = 10;
+template <typename T, typename Store>
+__device__ __forceinline__ void sep_short_transform(const double2* X, const double2* W, int ne, int m, int ncol,
+                                                    Store store) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  for (int c = l; c < ncol; c += 64) {
+    for (int j0 = w; j0 < m; j0 += 4 * kSepJB) {
+      double2 acc[kSepJB];
+#pragma unroll
+      for (int i = 0; i < kSepJB; ++i) acc[i] = make_double2(0., 0.);
+      for (int e = 0; e < ne; ++e) {
+        const double2 x = X[e * ncol + c];
+        const double2* we = W + e * m;
+#pragma unroll
+        for (int i = 0; i < kSepJB; ++i) {
+          const int j = j0 + 4 * i;
+          if (j < m) {
+            const double2 ww = we[j];
+            acc[i].x += ww.x * x.x - ww.y * x.y;
+            acc[i].y += ww.x * x.y + ww.y * x.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kSepJB; ++i)
+        if (j0 + 4 * i < m) store(j0 + 4 * i, c, acc[i]);
+    }
+  }
+}
+
 // T1(line, jz, c) = sum_{(f2, Rp) in line} w_z^{jz f2} C(Rp, c).  The line's
-// entries and their twiddles w_z^{jz f2} (an entries x mz table) sit in LDS,
-// so the inner sum is two LDS reads and four FMAs per term.
+// entries and their twiddles w_z^{jz f2} (an entries x mz table) sit in LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sep_z(SepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
   const int line = blockIdx.x, panel = blockIdx.y;
   const int e0 = p.lineStart[line], ne = p.lineStart[line + 1] - e0;
+  const int ncol = p.ncol, mz = p.mz;
   double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
-  double2* W = X + size_t(ne) * p.ncol;           // (entries, mz)
+  double2* W = X + size_t(ne) * ncol;             // (entries, mz)
   const T* B = static_cast<const T*>(p.B) + size_t(panel) * p.panelStride;
-  for (int i = threadIdx.x; i < ne * p.ncol; i += 256) {
-    const int e = i / p.ncol, c = i % p.ncol;
+  for (int e = 0; e < ne; ++e) {
     const T* row = B + size_t(2 * p.lineRp[e0 + e]) * p.ldb;
-    X[i] = make_double2(double(row[2 * c]), double(row[2 * c + 1]));
-  }
-  for (int i = threadIdx.x; i < ne * p.mz; i += 256) {
-    const int e = i / p.mz, jz = i % p.mz;
-    W[i] = p.twz[(jz * p.lineF2[e0 + e]) % p.mz];
+    for (int c = threadIdx.x; c < ncol; c += 256) X[e * ncol + c] = make_double2(double(row[2 * c]), double(row[2 * c + 1]));
+    const int f = p.lineF2[e0 + e];
+    for (int jz = threadIdx.x; jz < mz; jz += 256) W[e * mz + jz] = p.twz[(jz * f) % mz];
   }
   __syncthreads();
   T2v* out = reinterpret_cast<T2v*>(static_cast<T*>(p.T1) + size_t(panel) * p.t1PanelStride) +
-             size_t(line) * p.mz * p.ncol;
-  for (int i = threadIdx.x; i < p.mz * p.ncol; i += 256) {
-    const int jz = i / p.ncol, c = i % p.ncol;
-    double2 s = make_double2(0., 0.);
-    for (int e = 0; e < ne; ++e) {
-      const double2 w = W[e * p.mz + jz], x = X[e * p.ncol + c];
-      s.x += w.x * x.x - w.y * x.y;
-      s.y += w.x * x.y + w.y * x.x;
-    }
-    out[i] = T2v{T(s.x), T(s.y)};
-  }
+             size_t(line) * mz * ncol;
+  sep_short_transform<T>(X, W, ne, mz, ncol, [&](int jz, int c, double2 v) {
+    out[size_t(jz) * ncol + c] = T2v{T(v.x), T(v.y)};
+  });
 }
 
 // T2(group, jy, jz, c) = sum_{(f1, line) in group} w_y^{jy f1} T1(line, jz, c)
@@ -727,30 +755,21 @@ __global__ void __launch_bounds__(256) k_sep_y(SepParams p) {
   typedef typename Vec2<T>::type T2v;
   const int grp = blockIdx.x, jz = blockIdx.y, panel = blockIdx.z;
   const int e0 = p.grpStart[grp], ne = p.grpStart[grp + 1] - e0;
+  const int ncol = p.ncol, my = p.my;
   double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
-  double2* W = X + size_t(ne) * p.ncol;           // (entries, my)
+  double2* W = X + size_t(ne) * ncol;             // (entries, my)
   const T* T1 = static_cast<const T*>(p.T1) + size_t(panel) * p.t1PanelStride;
-  for (int i = threadIdx.x; i < ne * p.ncol; i += 256) {
-    const int e = i / p.ncol, c = i % p.ncol;
-    const T* src = T1 + (size_t(p.grpLine[e0 + e]) * p.mz + jz) * p.ncol * 2;
-    X[i] = make_double2(double(src[2 * c]), double(src[2 * c + 1]));
-  }
-  for (int i = threadIdx.x; i < ne * p.my; i += 256) {
-    const int e = i / p.my, jy = i % p.my;
-    W[i] = p.twy[(jy * p.grpF1[e0 + e]) % p.my];
+  for (int e = 0; e < ne; ++e) {
+    const T* src = T1 + (size_t(p.grpLine[e0 + e]) * p.mz + jz) * ncol * 2;
+    for (int c = threadIdx.x; c < ncol; c += 256) X[e * ncol + c] = make_double2(double(src[2 * c]), double(src[2 * c + 1]));
+    const int f = p.grpF1[e0 + e];
+    for (int jy = threadIdx.x; jy < my; jy += 256) W[e * my + jy] = p.twy[(jy * f) % my];
   }
   __syncthreads();
   T2v* out = reinterpret_cast<T2v*>(static_cast<T*>(p.T2) + size_t(panel) * p.t2PanelStride);
-  for (int i = threadIdx.x; i < p.my * p.ncol; i += 256) {
-    const int jy = i / p.ncol, c = i % p.ncol;
-    double2 s = make_double2(0., 0.);
-    for (int e = 0; e < ne; ++e) {
-      const double2 w = W[e * p.my + jy], x = X[e * p.ncol + c];
-      s.x += w.x * x.x - w.y * x.y;
-      s.y += w.x * x.y + w.y * x.x;
-    }
-    out[((size_t(grp) * p.my + jy) * p.mz + jz) * p.ncol + c] = T2v{T(s.x), T(s.y)};
-  }
+  sep_short_transform<T>(X, W, ne, my, ncol, [&](int jy, int c, double2 v) {
+    out[((size_t(grp) * my + jy) * p.mz + jz) * ncol + c] = T2v{T(v.x), T(v.y)};
+  });
 }
 
 // Last stage + epilogue for one (jy, jz) line of mesh points and one panel.
@@ -2190,6 +2209,12 @@ struct EpParams {
   const double* bose;      // (q, nu, calc)
   const void* E3T;         // (q, nu, nu2) in the transform's precision
   const void* G3;          // (slot, jx, (nu, w2)) in the transform's precision
+  // FUSE: the x-stage in the epilogue kernel, from the x-stage slabs
+  const void* T2;          // (panel, x-group, jy, jz, (nu, w2)) complex T
+  size_t t2PanelStride;    // T elements per panel
+  int nG0;                 // x-groups
+  const int* g0val;        // folded x coordinate of each group
+  const double2* tw0;      // e^{2 pi i t / m0}, t < m0
   const double* screenOf;  // per outer state: screen width of the panel its b1 group starts (null: screen)
   unsigned long long* keptRows;
   unsigned long long* keptBands;
@@ -2262,7 +2287,7 @@ __device__ __forceinline__ void ep2_rot_ph(const T2v (&e3c)[NMX], const double2*
   for (int b2 = 0; b2 < NB; ++b2) own[b2] = gr[b2] * gr[b2] + gi[b2] * gi[b2];
 }
 
-template <typename T, int CM, int NMX, int GM, int NB2>
+template <typename T, int CM, int NMX, int GM, int NB2, bool FUSE = false>
 __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
@@ -2306,6 +2331,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   double* se2 = reinterpret_cast<double*>(gp + size_t(R) * gps);
   double* sf2 = se2 + R * NB2;
   double* se1 = sf2 + CM * R * NB2;
+  T2v* sWx = reinterpret_cast<T2v*>(se1 + kGMax);  // FUSE: (kept row, x-group) twiddles w_x^{jx n0(g)}
   if (t < gsize) se1[t] = p.e1pool[is1 + t];
   const int nkeep = __popcll(mask);
   {
@@ -2318,6 +2344,14 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     }
   }
   __syncthreads();
+  if constexpr (FUSE) {
+    for (int i = t; i < nkeep * p.nG0; i += 64) {
+      const int k = i / p.nG0, gg = i % p.nG0;
+      const double2 d = p.tw0[(rows[k].jx * p.g0val[gg]) % p.m0];
+      sWx[i] = T2v{T(d.x), T(d.y)};
+    }
+    __syncthreads();
+  }
   if (t == 0) {
     if (p.keptRows) atomicAdd(p.keptRows, (unsigned long long)nkeep);
     if (p.keptBands) {
@@ -2347,10 +2381,42 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     const int nr = min(R, nkeep - m0r);
     const bool rowOn = laneOn && r < nr;
     const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
+    if (rowOn && FUSE) {
+      // x-stage of the lane's (row, mode): g3(nu, w2) = sum_g w_x^{jx n0(g)} T2(g; nu, w2)
+      const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
+                     (size_t(jy) * p.m2 + jz) * ncol + sub * nw;
+      const size_t gStride = size_t(p.m1) * p.m2 * ncol;
+      const T2v* wk = sWx + (m0r + r) * p.nG0;
+#pragma unroll
+      for (int w2 = 0; w2 < NWX; ++w2) g[w2] = T2v{T(0), T(0)};
+      for (int gg = 0; gg < p.nG0; ++gg) {
+        const T2v w = wk[gg];
+        const T2v* xg = X + size_t(gg) * gStride;
+        if (sizeof(T) == 4 && nw == NWX) {
+          const float4* s4 = reinterpret_cast<const float4*>(xg);
+#pragma unroll
+          for (int h = 0; h < NWX / 2; ++h) {
+            const float4 v = s4[h];
+            g[2 * h].x += w.x * v.x - w.y * v.y;
+            g[2 * h].y += w.x * v.y + w.y * v.x;
+            g[2 * h + 1].x += w.x * v.z - w.y * v.w;
+            g[2 * h + 1].y += w.x * v.w + w.y * v.z;
+          }
+        } else {
+#pragma unroll
+          for (int w2 = 0; w2 < NWX; ++w2)
+            if (w2 < nw) {
+              const T2v v = xg[w2];
+              g[w2].x += w.x * v.x - w.y * v.y;
+              g[w2].y += w.x * v.y + w.y * v.x;
+            }
+        }
+      }
+    }
     if (rowOn) {
-      const T2v* src = G3 + size_t(ri.jx) * ncol + sub * nw;
-      if (sizeof(T) == 4 && nw == NWX) {
-        const float4* s4 = reinterpret_cast<const float4*>(src);
+      if (FUSE) {
+      } else if (sizeof(T) == 4 && nw == NWX) {
+        const float4* s4 = reinterpret_cast<const float4*>(G3 + size_t(ri.jx) * ncol + sub * nw);
 #pragma unroll
         for (int h = 0; h < NWX / 2; ++h) {
           const float4 v = s4[h];
@@ -2360,6 +2426,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
           g[2 * h + 1].y = v.w;
         }
       } else {
+        const T2v* src = G3 + size_t(ri.jx) * ncol + sub * nw;
 #pragma unroll
         for (int w2 = 0; w2 < NWX; ++w2) g[w2] = w2 < nw ? src[w2] : T2v{T(0), T(0)};
       }
@@ -2523,12 +2590,14 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   }
 }
 
-inline size_t sep_ep2_lds_bytes(int nm, int nb2, int cm, int rpw) {
+// xtw: entries of the fused x-stage's twiddle table (kept rows x x-groups), 0 without it
+inline size_t sep_ep2_lds_bytes(int nm, int nb2, int cm, int rpw, int xtw = 0) {
   size_t b = (64 * sizeof(SepXmRow) + 15) / 16 * 16;
   b += size_t(rpw) * (16 * nb2 + 1) * sizeof(double2);              // U2, later |g|^2
   b += size_t(rpw) * (nm * nb2 + 1) * sizeof(double2);              // g'
   b += size_t(rpw) * nb2 * sizeof(double) * (1 + cm);               // e2, occupations
   b += size_t(kGMax) * sizeof(double);                              // e1
+  b += size_t(xtw) * sizeof(float2);                                // x-stage twiddles (FP32)
   return std::max(b, size_t(kGMax) * kCMax * sizeof(double)) + 64;
 }
 
@@ -2685,6 +2754,9 @@ void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStre
   PHX_HIP(hipGetLastError());
 }
 
+// The fused x-stage of k_sep_ep2: a whole slot's twiddle table (m0 x x-groups) in LDS.
+inline bool ep2okFuse(const CoupParams& p) { return p.T2 && p.m0 <= 64 && p.nG0 <= 24; }
+
 // mid (optional): recorded between the x-stage and the epilogue kernel when
 // the last stage runs split.  Returns whether it did.
 template <typename T>
@@ -2737,10 +2809,13 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         pe.epRows = nWaves * rpw;
         const size_t ldsB = sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, pe.epRows);
         PHX_REQUIRE(ldsA <= 160 * 1024 && ldsB <= 160 * 1024, "separable transform: LDS budget exceeded");
-        PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_xg<T, 3>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
-        hipLaunchKernelGGL((k_sep_xg<T, 3>), dim3(grid), dim3(256), ldsA, s, p);
-        PHX_HIP(hipGetLastError());
+        const bool fuse = p.epFuse && ep2ok && ep2okFuse(p);
+        if (!fuse) {
+          PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_xg<T, 3>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
+          hipLaunchKernelGGL((k_sep_xg<T, 3>), dim3(grid), dim3(256), ldsA, s, p);
+          PHX_HIP(hipGetLastError());
+        }
         if (mid) PHX_HIP(hipEventRecord(mid, s));
         auto ep = [&](auto kern) {
           PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -2756,7 +2831,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           const int nb2 = p.nb2max <= 4 ? 4 : 8;
           const int rpw2 = std::min(64 / p.nm, nb2 <= 4 ? 5 : 4);
           const int cm = one ? 1 : p.nCalc <= 2 ? 2 : kCMax;
-          const size_t lds2 = sep_ep2_lds_bytes(p.nm, nb2, cm, rpw2);
+          const size_t lds2 = sep_ep2_lds_bytes(p.nm, nb2, cm, rpw2, fuse ? p.m0 * p.nG0 : 0);
           PHX_REQUIRE(lds2 <= 160 * 1024, "separable transform: LDS budget exceeded");
           EpParams q{};
           q.keepMask = p.keepMask;
@@ -2796,22 +2871,32 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           q.nCalc = p.nCalc;
           q.numStates = p.numStates;
           q.rpw = rpw2;
+          q.T2 = p.T2;
+          q.t2PanelStride = p.t2PanelStride;
+          q.nG0 = p.nG0;
+          q.g0val = p.g0val;
+          q.tw0 = p.tw0;
           auto ep2 = [&](auto kern) {
             PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2)));
             hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds2, s, q);
           };
-          auto ep2c = [&](auto nmx, auto nb2c) {
+          auto ep2f = [&](auto nmx, auto nb2c, auto fz) {
             constexpr int NM = decltype(nmx)::value, NB = decltype(nb2c)::value;
+            constexpr bool F = decltype(fz)::value;
             if (gm <= 4) {
-              if (one) ep2(k_sep_ep2<T, 1, NM, 4, NB>);
-              else if (p.nCalc <= 2) ep2(k_sep_ep2<T, 2, NM, 4, NB>);
-              else ep2(k_sep_ep2<T, kCMax, NM, 4, NB>);
+              if (one) ep2(k_sep_ep2<T, 1, NM, 4, NB, F>);
+              else if (p.nCalc <= 2) ep2(k_sep_ep2<T, 2, NM, 4, NB, F>);
+              else ep2(k_sep_ep2<T, kCMax, NM, 4, NB, F>);
             } else {
-              if (one) ep2(k_sep_ep2<T, 1, NM, kGMax, NB>);
-              else if (p.nCalc <= 2) ep2(k_sep_ep2<T, 2, NM, kGMax, NB>);
-              else ep2(k_sep_ep2<T, kCMax, NM, kGMax, NB>);
+              if (one) ep2(k_sep_ep2<T, 1, NM, kGMax, NB, F>);
+              else if (p.nCalc <= 2) ep2(k_sep_ep2<T, 2, NM, kGMax, NB, F>);
+              else ep2(k_sep_ep2<T, kCMax, NM, kGMax, NB, F>);
             }
+          };
+          auto ep2c = [&](auto nmx, auto nb2c) {
+            if (fuse) ep2f(nmx, nb2c, std::true_type{});
+            else ep2f(nmx, nb2c, std::false_type{});
           };
           using I12 = std::integral_constant<int, 12>;
           using I16 = std::integral_constant<int, 16>;

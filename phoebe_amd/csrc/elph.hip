@@ -267,44 +267,49 @@ __host__ __device__ inline int cache_pass_cols(int nw) { return (256 / nw) * nw;
 // real parts in rows 0..15, imaginary parts in rows 16..31.  Applies the U1
 // rotation over the w1 range of those columns and the factor s(k1, Rp), and
 // writes (c0 == 0) or adds (later column passes) the panel entries.  Each
-// output (b1, w2) is owned by the same thread in every pass.
+// output (outer point, b1, w2) is owned by the same thread in every pass (the
+// thread-to-output map does not depend on c0).
 template <typename T, typename OutT>
 __device__ __forceinline__ void cache_panel_epilogue(const CacheParams& p, const T* Cb, int c0,
                                                      int kbase, int rp, int nu, int kend) {
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, nw = p.nw;
   OutT* Bout = static_cast<OutT*>(p.B);
-  for (int kl = 0; kl < 16; ++kl) {
+  const int nkl = min(16, kend - kbase);
+  // one thread per (outer point, b1, w2) of the batch: the band count of the
+  // batch's widest point bounds b1, so all 256 threads share the work
+  int nbm = 0;
+  for (int kl = 0; kl < nkl; ++kl) nbm = max(nbm, p.nb1[kbase + kl]);
+  nbm = __builtin_amdgcn_readfirstlane(nbm);
+  const int per = nbm * nw;
+  // columns of this pass: w1 in [c0/nw, (c0 + pass)/nw), all w2
+  const int w1lo = c0 / nw, w1hi = min(nw, (c0 + cache_pass_cols(nw)) / nw);
+  for (int idx = t; idx < nkl * per; idx += 256) {
+    const int kl = idx / per, rem = idx - kl * per, b1 = rem / nw, w2 = rem - b1 * nw;
     const int kk = kbase + kl;
-    if (kk >= kend) break;
-    const int nb1 = p.nb1[kk];
+    if (b1 >= p.nb1[kk]) continue;
     const double2* U1 = p.U1pool + p.U1off[kk];
     const double2 s = p.phP[size_t(kk) * p.nRp + rp];
-    // columns of this pass: w1 in [c0/nw, (c0 + pass)/nw), all w2
-    const int w1lo = c0 / p.nw, w1hi = min(p.nw, (c0 + cache_pass_cols(p.nw)) / p.nw);
-    for (int o = t; o < nb1 * p.nw; o += 256) {
-      const int b1 = o / p.nw, w2 = o % p.nw;
-      double2 v = make_double2(0., 0.);
-      for (int w1 = w1lo; w1 < w1hi; ++w1) {
-        const int cl = w1 * p.nw + w2 - c0;
-        const double ar = double(Cb[kl * 256 + cl]), ai = double(Cb[(16 + kl) * 256 + cl]);
-        const double2 u = U1[w1 + p.nw * b1];
-        v.x += ar * u.x - ai * u.y;
-        v.y += ar * u.y + ai * u.x;
-      }
-      const double2 c = make_double2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
-      OutT* panel = Bout + size_t(p.panelBase[kk] + b1) * p.panelStride;
-      const size_t col = 2 * (size_t(nu) * p.nw + w2);
-      if (c0 == 0) {
-        panel[size_t(2 * rp) * p.ldb + col] = OutT(c.x);
-        panel[size_t(2 * rp) * p.ldb + col + 1] = OutT(c.y);
-        panel[size_t(2 * rp + 1) * p.ldb + col] = OutT(-c.y);
-        panel[size_t(2 * rp + 1) * p.ldb + col + 1] = OutT(c.x);
-      } else {  // later column passes add their w1 range
-        panel[size_t(2 * rp) * p.ldb + col] += OutT(c.x);
-        panel[size_t(2 * rp) * p.ldb + col + 1] += OutT(c.y);
-        panel[size_t(2 * rp + 1) * p.ldb + col] += OutT(-c.y);
-        panel[size_t(2 * rp + 1) * p.ldb + col + 1] += OutT(c.x);
-      }
+    double2 v = make_double2(0., 0.);
+    for (int w1 = w1lo; w1 < w1hi; ++w1) {
+      const int cl = w1 * nw + w2 - c0;
+      const double ar = double(Cb[kl * 256 + cl]), ai = double(Cb[(16 + kl) * 256 + cl]);
+      const double2 u = U1[w1 + nw * b1];
+      v.x += ar * u.x - ai * u.y;
+      v.y += ar * u.y + ai * u.x;
+    }
+    const double2 c = make_double2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
+    OutT* panel = Bout + size_t(p.panelBase[kk] + b1) * p.panelStride;
+    const size_t col = 2 * (size_t(nu) * nw + w2);
+    if (c0 == 0) {
+      panel[size_t(2 * rp) * p.ldb + col] = OutT(c.x);
+      panel[size_t(2 * rp) * p.ldb + col + 1] = OutT(c.y);
+      panel[size_t(2 * rp + 1) * p.ldb + col] = OutT(-c.y);
+      panel[size_t(2 * rp + 1) * p.ldb + col + 1] = OutT(c.x);
+    } else {  // later column passes add their w1 range
+      panel[size_t(2 * rp) * p.ldb + col] += OutT(c.x);
+      panel[size_t(2 * rp) * p.ldb + col + 1] += OutT(c.y);
+      panel[size_t(2 * rp + 1) * p.ldb + col] += OutT(-c.y);
+      panel[size_t(2 * rp + 1) * p.ldb + col + 1] += OutT(c.x);
     }
   }
 }
