@@ -103,12 +103,13 @@ def roofline(res, form, workload):
         if sp and sp["launches"] > 0:
             # split last stage: both kernels stream HBM.  The x-stage kernel reads
             # the x-stage slabs of the slots it computes and writes the kept rows'
-            # g3 (complex FP32); the epilogue reads those g3 rows back (and the
-            # per-pair reduction bytes are reported beside)
+            # g3 (complex FP32), or g' (complex FP64 per kept band) after reading
+            # U2 when it does the electron rotation; the epilogue reads its output
+            # back (and the per-pair reduction bytes are reported beside)
             n = sp["launches"]
             t_xg, t_ep = sp["xg_ms"] / n * 1e-3, sp["ep_ms"] / n * 1e-3
-            g3 = sp["g3_bytes"] / n
-            xg_bytes = (sp["slab_bytes"] + sp["g3_bytes"]) / n
+            g3 = sp["ep_bytes"] / n
+            xg_bytes = sp["xg_bytes"] / n
             # per computed (k1, k2) pair the reduction reads |g|^2 (8 nb1 nb2 nm bytes), the
             # inner point's occupations (16 nb2) and the phonon data (16 nm); a computed
             # row is one (k1, b1, k2) triple, so pairs = rows / nb1

@@ -457,25 +457,39 @@ int phx_lw_set_panel_chunk(phx_lw* h, int panels);
  * to device memory, (panel chunk) x (y, z lines) x m0 x nm nw complex FP32,
  * and an epilogue kernel reads them; both use far fewer registers and LDS
  * than the fused kernel, so more workgroups hide the memory latency.  Applies
- * to PHX_FP32 with nm, nw <= 16, nm nw <= 192, an x mesh <= 64, at most 24
- * x-groups and no learned screen; elsewhere the fused kernel runs.  The g3
- * values are the same either way and the linewidths agree to FP64 rounding.
+ * to PHX_FP32 with nm, nw <= 16, nm nw <= 192, an x mesh <= 64 and at most 24
+ * x-groups; elsewhere the fused kernel runs.  The g3 values are the same
+ * either way and the linewidths agree to FP64 rounding.
  * on = 1 (default) runs the lean epilogue kernel where it applies (fixed
- * Gaussian smearing, at most 8 bands per inner point), on = 2 the first
- * epilogue kernel always; both give the same linewidths.  on = 3 runs the
- * lean epilogue kernel alone, computing each kept row's x-stage from the
- * x-stage slabs itself (FP32 FMAs instead of the x-stage kernel's MFMAs), so
- * no g3 goes through device memory; the linewidths agree to FP32 rounding.
- * No counterpart in the reference. */
+ * Gaussian smearing, at most 8 bands per inner point; it also takes the
+ * learned screen's slot masks), on = 2 the first epilogue kernel always;
+ * both give the same linewidths.  on = 3 moves the electron rotation
+ * g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2) (FP64, the epilogue's own
+ * arithmetic on the same FP32 g3) into the x-stage kernel, which writes g'
+ * (nb2 x nm complex FP64 per kept row, slot rows of nb2max x nm) instead of
+ * g3, and the lean epilogue starts from g'; where the lean epilogue does not
+ * apply it runs as on = 1.  The linewidths agree with on = 1 to FP64
+ * rounding.  No counterpart in the reference. */
 int phx_lw_set_split_last_stage(phx_lw* h, int on);
+/* Split last stage: run the epilogue kernel of each panel chunk on a second
+ * stream of the handle, concurrently with the next chunk's transform stages
+ * (on = 1, the default), or after them on the handle's stream (on = 0).  The
+ * chunks then alternate between two sets of g3 rows and row masks (twice the
+ * device memory of those); the learned screen's passes always run in turn.
+ * The linewidths are the same either way.  Takes effect for batches prepared
+ * after the call.  No counterpart in the reference. */
+int phx_lw_set_epilogue_overlap(phx_lw* h, int on);
 /* With timing on, the launches of phx_lw_stage_times that ran split:
  * out4 = {x-stage kernel ms, epilogue kernel ms, launches, sum over the rows
  * the epilogue computed of their band counts}.  No counterpart in the reference. */
 int phx_lw_last_stage_split_times(phx_lw* h, double* out4);
-/* With timing on and the last stage split: out2 = {bytes of the x-stage
- * slabs the x-stage kernel read (slots computed x x-groups x nm nw complex
- * FP32), bytes of g3 rows it wrote and the epilogue read (rows x nm nw
- * complex FP32)}.  No counterpart in the reference (a measurement call). */
+/* With timing on and the last stage split: out2 = {bytes the x-stage kernel
+ * moved, bytes of its output the epilogue read}.  The x-stage kernel reads
+ * the x-stage slabs (slots computed x x-groups x nm nw complex FP32) and
+ * writes g3 (kept rows x nm nw complex FP32); with the rotation in the
+ * x-stage (split mode 3) it writes g' instead (kept bands x nm complex FP64)
+ * and reads U2 (kept bands x nw complex FP64).  No counterpart in the
+ * reference (a measurement call). */
 int phx_lw_last_stage_split_bytes(phx_lw* h, double* out2);
 /* ---- Learned q-block screen (separable transform with the energy screen on).
  * No counterpart in the reference, which evaluates every (k1, k2) pair

@@ -178,6 +178,7 @@ EXPORTS = {
     "phx_phel_destroy": (C.c_int, [C.c_void_p]),
     "phx_phel_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_split_last_stage": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_set_epilogue_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_last_stage_split_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_last_stage_split_bytes": (C.c_int, [C.c_void_p, _dp]),
     "phx_phel_kernel_times": (C.c_int, [C.c_void_p, _dp]),
@@ -698,9 +699,15 @@ class ElectronLinewidths:
 
     def set_split_last_stage(self, on=True):
         """Separable last stage as two kernels (default) or the fused one;
-        on=2 runs the split stage with the first epilogue kernel, on=3 the
-        lean epilogue kernel computing the x-stage itself."""
+        on=2 runs the split stage with the first epilogue kernel, on=3 with
+        the electron rotation in the x-stage kernel (g' through device memory
+        instead of g3)."""
         _check(lib().phx_lw_set_split_last_stage(self._h, on if on in (2, 3) else int(bool(on))))
+
+    def set_epilogue_overlap(self, on=True):
+        """Split last stage: each chunk's epilogue kernel on a second stream,
+        concurrent with the next chunk's transform stages (default), or in turn."""
+        _check(lib().phx_lw_set_epilogue_overlap(self._h, int(bool(on))))
 
     def split_times(self):
         """Split last stage with timing on: x-stage ms, epilogue ms, launches,
@@ -710,11 +717,12 @@ class ElectronLinewidths:
         return dict(xg_ms=o[0], ep_ms=o[1], launches=int(o[2]), bands=o[3])
 
     def split_bytes(self):
-        """Split last stage with timing on: bytes of x-stage slabs read by the
-        x-stage kernel, bytes of g3 rows it wrote (and the epilogue read)."""
+        """Split last stage with timing on: bytes the x-stage kernel moved (slabs
+        read, g3 or g' written, U2 read when it rotates), bytes of its output
+        the epilogue read."""
         o = np.zeros(2)
         _check(lib().phx_lw_last_stage_split_bytes(self._h, _p(o)))
-        return dict(slab_bytes=o[0], g3_bytes=o[1])
+        return dict(xg_bytes=o[0], ep_bytes=o[1])
 
     # ---- learned q-block screen
     def screen_collect(self, k1_list=None, sample_frac=0.02, seed=1):

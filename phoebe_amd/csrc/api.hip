@@ -67,8 +67,10 @@ extern template void launch_separable<float>(const SepParams&, const CoupParams&
                                              hipStream_t);
 extern template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t);
 extern template void launch_separable_zy<float>(const SepParams&, int, int, hipStream_t);
-extern template bool launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t);
-extern template bool launch_separable_x<float>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t);
+extern template bool launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t,
+                                                  const EpFork&);
+extern template bool launch_separable_x<float>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t,
+                                                  const EpFork&);
 int coupling_rows_per_tile_f64(int ldb);
 
 
@@ -779,8 +781,16 @@ struct LwHandle {
   int lineMin = 0;
   int panelChunk = 48;     // panels per separable-transform launch (bounds T1 / T2)
   int splitLast = 1;       // last stage as k_sep_xg + epilogue where they apply (2: the first epilogue,
-                           // k_sep_ep; 3: the lean epilogue computing the x-stage itself)
-  DevBuf<float2> G3f;      // their g3 rows of one panel chunk, (slot, jx, col)
+                           // k_sep_ep; 3: the electron rotation in k_sep_xg, g' rows through HBM)
+  DevBuf<float2> G3f;      // their g3 rows of one panel chunk, (slot, jx, col), or g' rows (slot, jx, b2, nu)
+  // the split stage's epilogue of chunk i on streamEp, overlapping chunk i + 1's
+  // transform stages on `stream`: chunks alternate between two sets of g3 rows
+  // and row masks; epEv holds two events per chunk (x-stage done, epilogue done)
+  int epOverlap = 1;
+  hipStream_t streamEp = nullptr;
+  std::vector<hipEvent_t> epEv;
+  DevBuf<float2> G3f2;
+  DevBuf<unsigned long long> keepMask2;
   int linesPerBatch = kLinesPerBatch;  // mesh lines per line batch (1 .. kLinesPerBatch)
   bool reLattice = false;  // every R_e is a lattice vector (required by the line form)
   int nGe = 0;
@@ -825,8 +835,9 @@ struct LwHandle {
   double kernelMs[3] = {0., 0., 0.};
   long long kernelLaunches = 0;
   double couplingFlops = 0.;  // algorithmic FLOPs of the coupling launches timed
-  // last-stage launches of the separable transform: event pairs, summed at the next sync
-  // (start, mid, end) per launch; mid is recorded between the split stage's kernels
+  // last-stage launches of the separable transform: events summed at the next sync,
+  // (start, mid, epilogue start, end) per launch; mid is recorded after the split
+  // stage's x-stage kernel, the last two on the stream its epilogue ran on
   std::vector<hipEvent_t> xev;
   std::vector<char> xevSplit;        // per launch: whether it ran split
   size_t xevUsed = 0;
@@ -840,6 +851,8 @@ struct LwHandle {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : xev) (void)hipEventDestroy(e);
+    for (auto& e : epEv) (void)hipEventDestroy(e);
+    if (streamEp) (void)hipStreamDestroy(streamEp);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -866,15 +879,15 @@ static void lw_harvest_timing(LwHandle* L) {
   L->kernelMs[2] += c;
   L->kernelLaunches += 1;
   L->evPending = false;
-  for (size_t i = 0; i + 3 <= L->xevUsed; i += 3) {
+  for (size_t i = 0; i + 4 <= L->xevUsed; i += 4) {
     float x = 0.f;
-    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 2]));
+    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 3]));
     L->xMs += x;
     L->xLaunches += 1;
-    if (L->xevSplit[i / 3]) {
+    if (L->xevSplit[i / 4]) {
       float a = 0.f, b = 0.f;
       PHX_HIP(hipEventElapsedTime(&a, L->xev[i], L->xev[i + 1]));
-      PHX_HIP(hipEventElapsedTime(&b, L->xev[i + 1], L->xev[i + 2]));
+      PHX_HIP(hipEventElapsedTime(&b, L->xev[i + 2], L->xev[i + 3]));
       L->xgMs += a;
       L->epMs += b;
       L->splitLaunches += 1;
@@ -1612,7 +1625,14 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
       L->screenOf.reserve(size_t(L->numStates));
       L->xminPart.reserve(size_t(chunk) * kXminBlocks);
     }
-    if (lw_split_last(L, p)) L->G3f.reserve(slots * p.mesh[0] * ncol);
+    if (lw_split_last(L, p)) {  // g3 (complex FP32) or g' (complex FP64, nb2max nm per row)
+      const size_t g3 = slots * p.mesh[0] * std::max(ncol, size_t(2) * b.nb2max * nm);
+      L->G3f.reserve(g3);
+      if (L->epOverlap && p.mesh[0] <= 64) {
+        L->G3f2.reserve(g3);
+        L->keepMask2.reserve(slots);
+      }
+    }
     if (L->lsMode && L->mlpTrained) {
       L->lsFeat.reserve(slots * kMlpIn);
       L->lsNscr.reserve(slots);
@@ -1832,7 +1852,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     if (lw_split_last(L, p)) {
       cq.G3 = L->G3f.p;
       cq.epFirst = L->splitLast == 2;
-      cq.epFuse = L->splitLast == 3;
+      cq.epRot = L->splitLast == 3;
     }
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
     const int chunk = std::min(nPanels, L->panelChunk);
@@ -1846,8 +1866,31 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
       cq.T2 = L->T2f.p;
     }
     double* const partial0 = cq.partial;
-    for (int p0 = 0; p0 < nPanels; p0 += chunk) {
+    // the epilogue of each chunk on streamEp (the learned screen's passes stay on s)
+    const bool learned = L->lsCollect || (L->lsMode && L->mlpTrained);
+    // g3 / g' rows of one chunk (what the batch's preparation reserved, unless a
+    // setting changed since: then the fused kernel runs, or the chunks run in turn)
+    const size_t slotsC = size_t(chunk) * nMtiles;
+    const size_t g3Need = slotsC * mx * std::max(size_t(ncol), size_t(2) * b.nb2max * nm);
+    const bool splitOk = lw_split_last(L, p) && L->G3f.n >= g3Need;
+    const bool overlap = L->epOverlap && !learned && splitOk && L->G3f2.n >= g3Need &&
+                         L->keepMask2.n >= slotsC;
+    const int nChunks = (nPanels + chunk - 1) / chunk;
+    if (overlap) {
+      if (!L->streamEp) PHX_HIP(hipStreamCreateWithFlags(&L->streamEp, hipStreamNonBlocking));
+      while (L->epEv.size() < 2 * size_t(nChunks)) {
+        hipEvent_t e;
+        PHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        L->epEv.push_back(e);
+      }
+    }
+    bool epPending[2] = {false, false};  // an epilogue on streamEp reads set 0 / 1
+    int lastEp = -1;                     // chunk of the last epilogue on streamEp
+    for (int p0 = 0, ci = 0; p0 < nPanels; p0 += chunk, ++ci) {
       const int n = std::min(chunk, nPanels - p0);
+      const int set = overlap ? (ci & 1) : 0;
+      cq.G3 = splitOk ? (set ? L->G3f2.p : L->G3f.p) : nullptr;
+      if (p.mesh[0] <= 64) cq.keepMask = set ? L->keepMask2.p : L->keepMask.p;
       sp.nPanels = n;
       cq.nPanels = n;
       cq.panelKK = dPKK + p0;
@@ -1860,11 +1903,16 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
       const bool f64 = L->precision == PHX_FP64;
       if (f64) sp.B = L->Bd.p + size_t(p0) * pstride;
       else sp.B = L->Bf.p + size_t(p0) * pstride;
+      EpFork fork{};
+      if (overlap) {
+        fork.ep = L->streamEp;
+        fork.xgDone = L->epEv[2 * ci];
+      }
       auto xstage = [&](const CoupParams& c0) {
         CoupParams c = c0;
         hipEvent_t mid = nullptr;
         if (L->timing) {
-          while (L->xev.size() < L->xevUsed + 3) {
+          while (L->xev.size() < L->xevUsed + 4) {
             hipEvent_t e;
             PHX_HIP(hipEventCreate(&e));
             L->xev.push_back(e);
@@ -1873,24 +1921,36 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
           c.keptBands = L->xBands.p;
           c.keptSlots = L->xSlots.p;
           mid = L->xev[L->xevUsed + 1];
+          fork.epStart = L->xev[L->xevUsed + 2];
           PHX_HIP(hipEventRecord(L->xev[L->xevUsed], s));
         }
-        const bool split = f64 ? launch_separable_x<double>(sp, c, true, s, mid)
-                               : launch_separable_x<float>(sp, c, true, s, mid);
+        const bool split = f64 ? launch_separable_x<double>(sp, c, true, s, mid, fork)
+                               : launch_separable_x<float>(sp, c, true, s, mid, fork);
+        const bool forked = split && fork.ep;
+        if (forked) {
+          PHX_HIP(hipEventRecord(L->epEv[2 * ci + 1], L->streamEp));
+          epPending[set] = true;
+          lastEp = ci;
+        }
         if (L->timing) {
-          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 2], s));
-          L->xevUsed += 3;
+          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 3], forked ? L->streamEp : s));
+          L->xevUsed += 4;
           L->xevSplit.push_back(split ? 1 : 0);
         }
       };
       if (f64) launch_separable_zy<double>(sp, L->maxLineE, L->maxGrpE, s);
       else launch_separable_zy<float>(sp, L->maxLineE, L->maxGrpE, s);
-      if (!L->lsCollect && !(L->lsMode && L->mlpTrained)) {
+      if (!learned) {
+        // this chunk rewrites set `set`: the epilogue of chunk ci - 2 must have read it
+        if (epPending[set]) PHX_HIP(hipStreamWaitEvent(s, L->epEv[2 * (ci - 2) + 1], 0));
+        epPending[set] = false;
         xstage(cq);
         continue;
       }
       learned_screen_chunk(L, cq, xstage, p0, n, nMtiles, k2b, k2e);
     }
+    // the collect (and the next batch) read what the last epilogues wrote
+    if (lastEp >= 0) PHX_HIP(hipStreamWaitEvent(s, L->epEv[2 * lastEp + 1], 0));
   }
   if (L->timing) {
     PHX_HIP(hipEventRecord(L->ev[2], s));
@@ -3484,6 +3544,14 @@ int phx_lw_set_split_last_stage(phx_lw* h, int on) {
   });
 }
 
+int phx_lw_set_epilogue_overlap(phx_lw* h, int on) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_lw_set_epilogue_overlap: null handle");
+    PHX_REQUIRE(on == 0 || on == 1, "phx_lw_set_epilogue_overlap: 0 or 1");
+    h->epOverlap = on;
+  });
+}
+
 int phx_lw_set_panel_chunk(phx_lw* h, int panels) {
   return guard([&] {
     PHX_REQUIRE(h, "null handle");
@@ -3788,13 +3856,23 @@ int phx_lw_last_stage_split_bytes(phx_lw* h, double* out2) {
     PHX_REQUIRE(h && out2, "null argument");
     PHX_HIP(hipStreamSynchronize(h->stream));
     lw_harvest_timing(h);
-    unsigned long long slots = 0, rows = 0;
+    unsigned long long slots = 0, rows = 0, bands = 0;
     if (h->xSlots.p) PHX_HIP(hipMemcpy(&slots, h->xSlots.p, sizeof(slots), hipMemcpyDeviceToHost));
     if (h->xKept.p) PHX_HIP(hipMemcpy(&rows, h->xKept.p, sizeof(rows), hipMemcpyDeviceToHost));
+    if (h->xBands.p) PHX_HIP(hipMemcpy(&bands, h->xBands.p, sizeof(bands), hipMemcpyDeviceToHost));
     // complex FP32: the x-stage reads nG0 x (nm nw) per slot and writes nm nw per kept row
     const double row = 8. * h->nm * h->nw;
     out2[0] = double(slots) * h->nG0 * row;
     out2[1] = h->splitLaunches > 0 ? double(rows) * row : 0.;
+    if (h->splitLast == 3 && h->splitLaunches > 0) {
+      // rotation in the x-stage: it reads U2 (nw complex FP64 per kept band) and writes
+      // g' (nm complex FP64 per kept band), which the epilogue reads back
+      const double gp = 16. * double(bands) * h->nm;
+      out2[0] += gp + 16. * double(bands) * h->nw;
+      out2[1] = gp;
+    } else {
+      out2[0] += out2[1];
+    }
   });
 }
 

@@ -101,7 +101,7 @@ struct CoupParams {
   void* G3;
   unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
   int epFirst;                            // split epilogue: 1 = k_sep_ep, 0 = k_sep_ep2 where it applies
-  int epFuse;                             // k_sep_ep2 computes the x-stage itself (no k_sep_xg, no g3 in HBM)
+  int epRot;                              // k_sep_xg also does the electron rotation (g' to HBM, not g3)
   unsigned long long* keptSlots;          // k_sep_xg: slots (panel, yz line) it computed (null: off)
 };
 
@@ -685,7 +685,8 @@ struct SepParams {
 };
 
 // Register-blocked short transform shared by k_sep_z and k_sep_y:
-// out(j, c) = sum_e W(e, j) X(e, c) for j < m, c < ncol, X and W in LDS.
+// out(j, c) = sum_e W(e, j) X(e, c) for j < m, c < ncol, X (the data's
+// precision, converted to FP64 on reading) and W in LDS.
 // Thread (wave w, lane l) takes the columns c = l, l + 64, ... and the rows
 // j = w + 4 i, kSepJB of them at once, so one LDS read of X(e, c) serves
 // kSepJB rows and W(e, j) is the same for the whole wave (a broadcast); no
@@ -693,8 +694,8 @@ struct SepParams {
 // loop over the outputs would.
 constexpr int kSepJB = 10;
 template <typename T, typename Store>
-__device__ __forceinline__ void sep_short_transform(const double2* X, const double2* W, int ne, int m, int ncol,
-                                                    Store store) {
+__device__ __forceinline__ void sep_short_transform(const typename Vec2<T>::type* X, const double2* W, int ne,
+                                                    int m, int ncol, Store store) {
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   for (int c = l; c < ncol; c += 64) {
     for (int j0 = w; j0 < m; j0 += 4 * kSepJB) {
@@ -702,7 +703,8 @@ __device__ __forceinline__ void sep_short_transform(const double2* X, const doub
 #pragma unroll
       for (int i = 0; i < kSepJB; ++i) acc[i] = make_double2(0., 0.);
       for (int e = 0; e < ne; ++e) {
-        const double2 x = X[e * ncol + c];
+        const typename Vec2<T>::type xv = X[e * ncol + c];
+        const double2 x = make_double2(double(xv.x), double(xv.y));
         const double2* we = W + e * m;
 #pragma unroll
         for (int i = 0; i < kSepJB; ++i) {
@@ -721,6 +723,48 @@ __device__ __forceinline__ void sep_short_transform(const double2* X, const doub
   }
 }
 
+// Entries of a short transform into LDS: X(e, c) = row(e)[c] for e < ne,
+// c < ncol (the data's precision), and the twiddles W(e, j) = tw[(j f(e)) mod m],
+// j < m.  The rows' offsets and the f(e) are staged first (off, fe: ne ints
+// each); every thread then keeps up to 8 row loads in flight before its LDS
+// stores, so a workgroup waits about one memory latency, not one per entry.
+template <typename T2v, typename Meta>
+__device__ __forceinline__ void sep_stage_entries(const T2v* base, T2v* X, double2* W, int* off, int* fe, int ne,
+                                                  int ncol, int m, const double2* tw, Meta meta) {
+  const int t = threadIdx.x, bs = blockDim.x;
+  for (int e = t; e < ne; e += bs) meta(e, off[e], fe[e]);
+  __syncthreads();
+  const int n = ne * ncol;
+  for (int i0 = t; i0 < n; i0 += 8 * bs) {
+    T2v v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + bs * u;
+      if (i < n) {
+        const int e = i / ncol;
+        v[u] = base[size_t(off[e]) + (i - e * ncol)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + bs * u;
+      if (i < n) X[i] = v[u];
+    }
+  }
+  for (int i = t; i < ne * m; i += bs) {
+    const int e = i / m, j = i - e * m;
+    W[i] = tw[(j * fe[e]) % m];
+  }
+  __syncthreads();
+}
+
+// LDS of k_sep_z / k_sep_y for ne entries: X (ne x ncol in the data's
+// precision), W (ne x m complex FP64), the entries' row offsets and factors.
+inline size_t sep_stage_lds_bytes(int elem, int ne, int ncol, int m) {
+  const size_t x = (size_t(ne) * ncol * 2 * elem + 15) / 16 * 16;
+  return x + size_t(ne) * m * sizeof(double2) + size_t(ne) * 2 * sizeof(int);
+}
+
 // T1(line, jz, c) = sum_{(f2, Rp) in line} w_z^{jz f2} C(Rp, c).  The line's
 // entries and their twiddles w_z^{jz f2} (an entries x mz table) sit in LDS.
 template <typename T>
@@ -730,16 +774,16 @@ __global__ void __launch_bounds__(256) k_sep_z(SepParams p) {
   const int line = blockIdx.x, panel = blockIdx.y;
   const int e0 = p.lineStart[line], ne = p.lineStart[line + 1] - e0;
   const int ncol = p.ncol, mz = p.mz;
-  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
-  double2* W = X + size_t(ne) * ncol;             // (entries, mz)
-  const T* B = static_cast<const T*>(p.B) + size_t(panel) * p.panelStride;
-  for (int e = 0; e < ne; ++e) {
-    const T* row = B + size_t(2 * p.lineRp[e0 + e]) * p.ldb;
-    for (int c = threadIdx.x; c < ncol; c += 256) X[e * ncol + c] = make_double2(double(row[2 * c]), double(row[2 * c + 1]));
-    const int f = p.lineF2[e0 + e];
-    for (int jz = threadIdx.x; jz < mz; jz += 256) W[e * mz + jz] = p.twz[(jz * f) % mz];
-  }
-  __syncthreads();
+  T2v* X = reinterpret_cast<T2v*>(smem);  // (entries, ncol)
+  double2* W = reinterpret_cast<double2*>(smem + (size_t(ne) * ncol * sizeof(T2v) + 15) / 16 * 16);  // (entries, mz)
+  int* off = reinterpret_cast<int*>(W + size_t(ne) * mz);
+  int* fe = off + ne;
+  // C(Rp, c): row 2 Rp of the panel, complex entries (re, im) at 2c, 2c + 1
+  const T2v* B = reinterpret_cast<const T2v*>(static_cast<const T*>(p.B) + size_t(panel) * p.panelStride);
+  sep_stage_entries(B, X, W, off, fe, ne, ncol, mz, p.twz, [&](int e, int& o, int& f) {
+    o = p.lineRp[e0 + e] * p.ldb;  // row 2 Rp: 2 Rp ldb elements of T = Rp ldb complex
+    f = p.lineF2[e0 + e];
+  });
   T2v* out = reinterpret_cast<T2v*>(static_cast<T*>(p.T1) + size_t(panel) * p.t1PanelStride) +
              size_t(line) * mz * ncol;
   sep_short_transform<T>(X, W, ne, mz, ncol, [&](int jz, int c, double2 v) {
@@ -756,16 +800,15 @@ __global__ void __launch_bounds__(256) k_sep_y(SepParams p) {
   const int grp = blockIdx.x, jz = blockIdx.y, panel = blockIdx.z;
   const int e0 = p.grpStart[grp], ne = p.grpStart[grp + 1] - e0;
   const int ncol = p.ncol, my = p.my;
-  double2* X = reinterpret_cast<double2*>(smem);  // (entries, ncol)
-  double2* W = X + size_t(ne) * ncol;             // (entries, my)
-  const T* T1 = static_cast<const T*>(p.T1) + size_t(panel) * p.t1PanelStride;
-  for (int e = 0; e < ne; ++e) {
-    const T* src = T1 + (size_t(p.grpLine[e0 + e]) * p.mz + jz) * ncol * 2;
-    for (int c = threadIdx.x; c < ncol; c += 256) X[e * ncol + c] = make_double2(double(src[2 * c]), double(src[2 * c + 1]));
-    const int f = p.grpF1[e0 + e];
-    for (int jy = threadIdx.x; jy < my; jy += 256) W[e * my + jy] = p.twy[(jy * f) % my];
-  }
-  __syncthreads();
+  T2v* X = reinterpret_cast<T2v*>(smem);  // (entries, ncol)
+  double2* W = reinterpret_cast<double2*>(smem + (size_t(ne) * ncol * sizeof(T2v) + 15) / 16 * 16);  // (entries, my)
+  int* off = reinterpret_cast<int*>(W + size_t(ne) * my);
+  int* fe = off + ne;
+  const T2v* T1 = reinterpret_cast<const T2v*>(static_cast<const T*>(p.T1) + size_t(panel) * p.t1PanelStride);
+  sep_stage_entries(T1, X, W, off, fe, ne, ncol, my, p.twy, [&](int e, int& o, int& f) {
+    o = (p.grpLine[e0 + e] * p.mz + jz) * ncol;
+    f = p.grpF1[e0 + e];
+  });
   T2v* out = reinterpret_cast<T2v*>(static_cast<T*>(p.T2) + size_t(panel) * p.t2PanelStride);
   sep_short_transform<T>(X, W, ne, my, ncol, [&](int jy, int c, double2 v) {
     out[((size_t(grp) * my + jy) * p.mz + jz) * ncol + c] = T2v{T(v.x), T(v.y)};
@@ -1874,8 +1917,14 @@ __global__ void __launch_bounds__(256, 2) k_sep_xm(CoupParams p) {
 // results equal the fused kernel's.
 
 // x-stage: g3(row, (nu, w2)) = sum_g w_x^{jx n0(g)} T2(g; nu, w2) of every
-// row kept by the slot's mask, stored at G3 (slot, jx).
-template <typename T, int kXmNt>
+// row kept by the slot's mask, stored at G3 (slot, jx) as complex T.
+// ROT: the kernel also applies the electron rotation of the epilogue,
+// g'(nu, b2) = sum_w2 conj(U2(w2, b2)) g3(nu, w2) in FP64 with w2 in order
+// (the arithmetic of k_sep_ep2's phase A, on the same FP32 g3 values), and
+// stores g' instead: complex FP64 at G3 (slot, jx, b2, nu), nb2max nm entries
+// per row of which the row's nb2 nm are written.  At cfg2's 1.7 bands per
+// inner point that is about a fifth of the bytes of g3.
+template <typename T, int kXmNt, bool ROT = false>
 __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
@@ -1898,9 +1947,12 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
   const int nm = p.nm, nw = p.nw, ncol = nm * nw;
   const int nG0 = p.nG0, nG0p = nG0 + 1;
   int* jxs = reinterpret_cast<int*>(smem);                                     // 64: kept rows, ascending
-  int* sg0 = jxs + 64;                                                         // nG0
-  T2v* stw = reinterpret_cast<T2v*>(smem + (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16);  // m0
+  int* snb = jxs + 64;                                                         // ROT: 64 band counts
+  int* su2 = snb + 64;                                                         // ROT: 64 U2 offsets
+  int* sg0 = su2 + 64;                                                         // nG0
+  T2v* stw = reinterpret_cast<T2v*>(smem + (size_t(192 + nG0) * sizeof(int) + 15) / 16 * 16);  // m0
   T2v* Wk = stw + p.m0;                                                        // nkeep x nG0p
+  T2v* sG = Wk + size_t(p.m0) * nG0p;  // ROT: one tile's g3, 16 rows x (nm (nw + 1) + 1)
   const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
                  (size_t(jy) * p.m2 + jz) * ncol;
   const size_t gStride = size_t(p.m1) * p.m2 * ncol;
@@ -1923,7 +1975,15 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
     stw[i] = T2v{T(d.x), T(d.y)};
   }
   for (int i = t; i < nG0; i += 256) sg0[i] = p.g0val[i];
-  if (t < p.m0 && ((mask >> t) & 1ull)) jxs[__popcll(mask & ((1ull << t) - 1ull))] = t;
+  if (t < p.m0 && ((mask >> t) & 1ull)) {
+    const int k = __popcll(mask & ((1ull << t) - 1ull));
+    jxs[k] = t;
+    if (ROT) {
+      const int4 mr = p.meshRow[(jz * p.m1 + jy) * p.m0 + t];
+      snb[k] = mr.y;
+      su2[k] = mr.z;
+    }
+  }
   const int nkeep = __popcll(mask);
   __syncthreads();
   for (int i = t; i < nkeep * nG0p; i += 256) {
@@ -1932,6 +1992,15 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
   }
   __syncthreads();
   T2v* out = static_cast<T2v*>(p.G3) + size_t(slot) * p.m0 * ncol;
+  // ROT: g' rows of nb2max nm complex FP64, (b2, nu) within a row
+  const int gs = nw + 1, rs = nm * gs + 1, rowG = p.nb2max * nm;
+  double2* outR = static_cast<double2*>(p.G3) + size_t(slot) * p.m0 * rowG;
+  int sgOff[kXmNt];  // ROT: LDS offset of the lane's column (nu, w2) within a tile row
+#pragma unroll
+  for (int j = 0; j < kXmNt; ++j) {
+    const int c = (wave * kXmNt + j) * 16 + lr;
+    sgOff[j] = (c / nw) * gs + c % nw;
+  }
   for (int m0r = 0; m0r < nkeep; m0r += 16) {
     const int nr = min(16, nkeep - m0r);
     const bool rowOk = lr < nr;
@@ -1957,22 +2026,62 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
         }
       }
     }
+    if constexpr (ROT) {
+      __syncthreads();  // the previous tile's rotation has read sG
 #pragma unroll
-    for (int j = 0; j < kXmNt; ++j) {
-      const int c = (wave * kXmNt + j) * 16 + lr;
-      if (c < ncol) {
+      for (int j = 0; j < kXmNt; ++j) {
+        const int c = (wave * kXmNt + j) * 16 + lr;
+        if (c < ncol) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = Mfma4<T>::row(lg, i);
-          if (row < nr) out[size_t(jxs[m0r + row]) * ncol + c] = T2v{accr[j][i], acci[j][i]};
+          for (int i = 0; i < 4; ++i) {
+            const int row = Mfma4<T>::row(lg, i);
+            if (row < nr) sG[row * rs + sgOff[j]] = T2v{accr[j][i], acci[j][i]};
+          }
+        }
+      }
+      __syncthreads();
+      // thread (row r, mode nu), nm <= 16 so 16 rows x nm <= 256 threads
+      if (t < nr * nm) {
+        const int r = t / nm, nu = t - r * nm, k = m0r + r;
+        const int nb2 = snb[k];
+        const double2* u = p.U2pool + su2[k];
+        const T2v* x = sG + r * rs + nu * gs;
+        double2* o = outR + size_t(jxs[k]) * rowG + nu;
+        for (int b2 = 0; b2 < nb2; ++b2) {
+          double ar = 0., ai = 0.;
+#pragma unroll
+          for (int w2 = 0; w2 < 16; ++w2) {
+            if (w2 < nw) {
+              const double2 c = u[b2 * nw + w2];
+              const double xr = double(x[w2].x), xi = double(x[w2].y);
+              ar += c.x * xr + c.y * xi;
+              ai += c.x * xi - c.y * xr;
+            }
+          }
+          o[b2 * nm] = make_double2(ar, ai);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kXmNt; ++j) {
+        const int c = (wave * kXmNt + j) * 16 + lr;
+        if (c < ncol) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = Mfma4<T>::row(lg, i);
+            if (row < nr) out[size_t(jxs[m0r + row]) * ncol + c] = T2v{accr[j][i], acci[j][i]};
+          }
         }
       }
     }
   }
 }
 
-inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0) {
-  return (size_t(64 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem + 64;
+// rot: the x-stage with the electron rotation (a tile of g3 in LDS)
+inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0, int nm = 0, int nw = 0, bool rot = false) {
+  size_t b = (size_t(192 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem;
+  if (rot) b += size_t(16) * (nm * (nw + 1) + 1) * 2 * elem;
+  return b + 64;
 }
 
 constexpr int kSepEpThreads = 64;  // k_sep_ep workgroup: one wave
@@ -2208,13 +2317,8 @@ struct EpParams {
   const double* fermi;     // (calc, state)
   const double* bose;      // (q, nu, calc)
   const void* E3T;         // (q, nu, nu2) in the transform's precision
-  const void* G3;          // (slot, jx, (nu, w2)) in the transform's precision
-  // FUSE: the x-stage in the epilogue kernel, from the x-stage slabs
-  const void* T2;          // (panel, x-group, jy, jz, (nu, w2)) complex T
-  size_t t2PanelStride;    // T elements per panel
-  int nG0;                 // x-groups
-  const int* g0val;        // folded x coordinate of each group
-  const double2* tw0;      // e^{2 pi i t / m0}, t < m0
+  const void* G3;          // (slot, jx, (nu, w2)) in the transform's precision; with the
+                           // rotation done by the x-stage, (slot, jx, b2, nu) complex FP64
   const double* screenOf;  // per outer state: screen width of the panel its b1 group starts (null: screen)
   unsigned long long* keptRows;
   unsigned long long* keptBands;
@@ -2223,6 +2327,7 @@ struct EpParams {
   int slotZero;                 // slots off: 1 = write zero partials, 0 = leave them
   double invSigma, deltaPref, norm, cutoff, screen;
   int m0, m1, m2, nPanels, nm, nw, nCalc, numStates;
+  int nb2max;              // g' rows of the rotating x-stage hold nb2max nm entries
   int rpw;                 // rows per pass: whole rows of nm lanes, rpw nm <= 64
 };
 
@@ -2287,7 +2392,7 @@ __device__ __forceinline__ void ep2_rot_ph(const T2v (&e3c)[NMX], const double2*
   for (int b2 = 0; b2 < NB; ++b2) own[b2] = gr[b2] * gr[b2] + gi[b2] * gi[b2];
 }
 
-template <typename T, int CM, int NMX, int GM, int NB2, bool FUSE = false>
+template <typename T, int CM, int NMX, int GM, int NB2, bool PRE = false>
 __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   typedef typename Vec2<T>::type T2v;
@@ -2331,7 +2436,6 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   double* se2 = reinterpret_cast<double*>(gp + size_t(R) * gps);
   double* sf2 = se2 + R * NB2;
   double* se1 = sf2 + CM * R * NB2;
-  T2v* sWx = reinterpret_cast<T2v*>(se1 + kGMax);  // FUSE: (kept row, x-group) twiddles w_x^{jx n0(g)}
   if (t < gsize) se1[t] = p.e1pool[is1 + t];
   const int nkeep = __popcll(mask);
   {
@@ -2344,14 +2448,6 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     }
   }
   __syncthreads();
-  if constexpr (FUSE) {
-    for (int i = t; i < nkeep * p.nG0; i += 64) {
-      const int k = i / p.nG0, gg = i % p.nG0;
-      const double2 d = p.tw0[(rows[k].jx * p.g0val[gg]) % p.m0];
-      sWx[i] = T2v{T(d.x), T(d.y)};
-    }
-    __syncthreads();
-  }
   if (t == 0) {
     if (p.keptRows) atomicAdd(p.keptRows, (unsigned long long)nkeep);
     if (p.keptBands) {
@@ -2364,6 +2460,8 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   const bool laneOn = t < R * nm;
   const int r = laneOn ? t / nm : 0, sub = laneOn ? t % nm : 0;
   const T2v* G3 = static_cast<const T2v*>(p.G3) + size_t(slot) * p.m0 * ncol;
+  const int rowG = p.nb2max * nm;  // PRE: g' row length, (b2, nu)
+  const double2* GP = static_cast<const double2*>(p.G3) + size_t(slot) * p.m0 * rowG;
   const T2v* E3T = static_cast<const T2v*>(p.E3T);
   double part[GM][CM];
 #pragma unroll
@@ -2373,6 +2471,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
 
   // registers of one pass, loaded at its start
   T2v g[NWX];         // g3(nu = sub, w2) of the lane's row
+  double2 gv[NB2];    // PRE: g'(nu = sub, b2) of the lane's row
   T2v e3c[NMX];       // e3(nu, nu2 = sub) at the row's q
   double2 u2v[UPL];   // U2 entries of the pass (lane-strided over R x NWX x NB2)
   double e2v = 0., f2v[CM], w3 = 0., n3[CM];
@@ -2381,40 +2480,11 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     const int nr = min(R, nkeep - m0r);
     const bool rowOn = laneOn && r < nr;
     const SepXmRow ri = rows[m0r + (rowOn ? r : 0)];
-    if (rowOn && FUSE) {
-      // x-stage of the lane's (row, mode): g3(nu, w2) = sum_g w_x^{jx n0(g)} T2(g; nu, w2)
-      const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
-                     (size_t(jy) * p.m2 + jz) * ncol + sub * nw;
-      const size_t gStride = size_t(p.m1) * p.m2 * ncol;
-      const T2v* wk = sWx + (m0r + r) * p.nG0;
-#pragma unroll
-      for (int w2 = 0; w2 < NWX; ++w2) g[w2] = T2v{T(0), T(0)};
-      for (int gg = 0; gg < p.nG0; ++gg) {
-        const T2v w = wk[gg];
-        const T2v* xg = X + size_t(gg) * gStride;
-        if (sizeof(T) == 4 && nw == NWX) {
-          const float4* s4 = reinterpret_cast<const float4*>(xg);
-#pragma unroll
-          for (int h = 0; h < NWX / 2; ++h) {
-            const float4 v = s4[h];
-            g[2 * h].x += w.x * v.x - w.y * v.y;
-            g[2 * h].y += w.x * v.y + w.y * v.x;
-            g[2 * h + 1].x += w.x * v.z - w.y * v.w;
-            g[2 * h + 1].y += w.x * v.w + w.y * v.z;
-          }
-        } else {
-#pragma unroll
-          for (int w2 = 0; w2 < NWX; ++w2)
-            if (w2 < nw) {
-              const T2v v = xg[w2];
-              g[w2].x += w.x * v.x - w.y * v.y;
-              g[w2].y += w.x * v.y + w.y * v.x;
-            }
-        }
-      }
-    }
     if (rowOn) {
-      if (FUSE) {
+      if (PRE) {
+#pragma unroll
+        for (int b2 = 0; b2 < NB2; ++b2)
+          gv[b2] = b2 < ri.nb2 ? GP[size_t(ri.jx) * rowG + b2 * nm + sub] : make_double2(0., 0.);
       } else if (sizeof(T) == 4 && nw == NWX) {
         const float4* s4 = reinterpret_cast<const float4*>(G3 + size_t(ri.jx) * ncol + sub * nw);
 #pragma unroll
@@ -2443,7 +2513,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     }
     // U2 of the pass: entry i = (row, b2, w2), w2 fastest, NWX x NB2 per row
 #pragma unroll
-    for (int u = 0; u < UPL; ++u) {
+    for (int u = 0; u < (PRE ? 0 : UPL); ++u) {
       const int i = t + 64 * u, rr = i / (NWX * NB2), o = i % (NWX * NB2), b2 = o / NWX, w2 = o % NWX;
       double2 v = make_double2(0., 0.);
       if (rr < nr) {
@@ -2480,7 +2550,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     nbw = __builtin_amdgcn_readfirstlane(nbw);
     // row data of the pass into LDS
 #pragma unroll
-    for (int u = 0; u < UPL; ++u) {
+    for (int u = 0; u < (PRE ? 0 : UPL); ++u) {
       const int i = t + 64 * u, rr = i / (NWX * NB2);
       if (rr < R) sU2[rr * U2S + i % (NWX * NB2)] = u2v[u];  // R <= RMX
     }
@@ -2490,8 +2560,13 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
       for (int c = 0; c < CM; ++c) sf2[c * R * NB2 + t] = f2v[c];
     }
     __syncthreads();
-    // A: electron rotation, thread nu = sub
-    if (rowOn) {
+    // A: electron rotation, thread nu = sub (PRE: done by the x-stage kernel)
+    if (PRE && rowOn) {
+      double2* out = gp + r * gps + sub * NB2;
+#pragma unroll
+      for (int b2 = 0; b2 < NB2; ++b2) out[b2] = gv[b2];
+    }
+    if (!PRE && rowOn) {
       const double2* u = sU2 + r * U2S;
       double2* out = gp + r * gps + sub * NB2;
       switch (nbw) {
@@ -2590,14 +2665,12 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
   }
 }
 
-// xtw: entries of the fused x-stage's twiddle table (kept rows x x-groups), 0 without it
-inline size_t sep_ep2_lds_bytes(int nm, int nb2, int cm, int rpw, int xtw = 0) {
+inline size_t sep_ep2_lds_bytes(int nm, int nb2, int cm, int rpw) {
   size_t b = (64 * sizeof(SepXmRow) + 15) / 16 * 16;
   b += size_t(rpw) * (16 * nb2 + 1) * sizeof(double2);              // U2, later |g|^2
   b += size_t(rpw) * (nm * nb2 + 1) * sizeof(double2);              // g'
   b += size_t(rpw) * nb2 * sizeof(double) * (1 + cm);               // e2, occupations
   b += size_t(kGMax) * sizeof(double);                              // e1
-  b += size_t(xtw) * sizeof(float2);                                // x-stage twiddles (FP32)
   return std::max(b, size_t(kGMax) * kCMax * sizeof(double)) + 64;
 }
 
@@ -2741,8 +2814,8 @@ inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb
 template <typename T>
 void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStream_t s) {
   if (sp.nPanels <= 0) return;
-  const size_t ldsZ = size_t(maxLineE) * (sp.ncol + sp.mz) * sizeof(double2);
-  const size_t ldsY = size_t(maxGrpE) * (sp.ncol + sp.my) * sizeof(double2);
+  const size_t ldsZ = sep_stage_lds_bytes(sizeof(T), maxLineE, sp.ncol, sp.mz);
+  const size_t ldsY = sep_stage_lds_bytes(sizeof(T), maxGrpE, sp.ncol, sp.my);
   PHX_REQUIRE(ldsZ <= 160 * 1024 && ldsY <= 160 * 1024, "separable transform: LDS budget exceeded");
   PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_z<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(ldsZ, 16))));
@@ -2754,14 +2827,21 @@ void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStre
   PHX_HIP(hipGetLastError());
 }
 
-// The fused x-stage of k_sep_ep2: a whole slot's twiddle table (m0 x x-groups) in LDS.
-inline bool ep2okFuse(const CoupParams& p) { return p.T2 && p.m0 <= 64 && p.nG0 <= 24; }
+// Where the split last stage's epilogue kernel runs: on the launch's stream
+// (ep null), or on the stream ep once the x-stage kernel is done, so that it
+// overlaps the work enqueued on the launch's stream after it.
+struct EpFork {
+  hipStream_t ep = nullptr;
+  hipEvent_t xgDone = nullptr;   // recorded on the launch's stream after the x-stage kernel (ep set)
+  hipEvent_t epStart = nullptr;  // timing: recorded on the epilogue's stream before it (null: off)
+};
 
 // mid (optional): recorded between the x-stage and the epilogue kernel when
-// the last stage runs split.  Returns whether it did.
+// the last stage runs split.  Returns whether it did (then the epilogue ran
+// on fork.ep when that is set).
 template <typename T>
 bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, hipStream_t s,
-                        hipEvent_t mid) {
+                        hipEvent_t mid, const EpFork& fork = EpFork{}) {
   PHX_REQUIRE(p.nm <= kSepNm && p.nb2max <= kSepNb2,
               "separable transform supports nm <= 32 and at most 8 bands per inner point");
   if (sp.nPanels <= 0) return false;
@@ -2799,7 +2879,9 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
       const bool split = p.G3 && !bf3 && !wide && p.keepMask && (ep2ok || (!p.slotOn && !p.gate)) &&
                          p.nm <= 16 && p.nw <= 16 && (2 * p.nG0 + 3) / 4 <= 12;
       if (split) {
-        const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0);
+        // the electron rotation in the x-stage kernel: with the lean epilogue only
+        const bool rot = p.epRot && ep2ok;
+        const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0, p.nm, p.nw, rot);
         // whole rows of nm threads per wave, at most 8, fewer if the LDS
         // budget needs it (4 rows per wave always fit what 16 x 16 threads did)
         CoupParams pe = p;
@@ -2809,18 +2891,26 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
         pe.epRows = nWaves * rpw;
         const size_t ldsB = sep_ep_lds_bytes(p.nw, p.nm, p.nb2max, p.nCalc, pe.epRows);
         PHX_REQUIRE(ldsA <= 160 * 1024 && ldsB <= 160 * 1024, "separable transform: LDS budget exceeded");
-        const bool fuse = p.epFuse && ep2ok && ep2okFuse(p);
-        if (!fuse) {
-          PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_xg<T, 3>),
+        auto xg = [&](auto kern) {
+          PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
-          hipLaunchKernelGGL((k_sep_xg<T, 3>), dim3(grid), dim3(256), ldsA, s, p);
-          PHX_HIP(hipGetLastError());
-        }
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsA, s, p);
+        };
+        if (rot) xg(k_sep_xg<T, 3, true>);
+        else xg(k_sep_xg<T, 3, false>);
+        PHX_HIP(hipGetLastError());
         if (mid) PHX_HIP(hipEventRecord(mid, s));
+        hipStream_t se = s;
+        if (fork.ep) {
+          PHX_HIP(hipEventRecord(fork.xgDone, s));
+          PHX_HIP(hipStreamWaitEvent(fork.ep, fork.xgDone, 0));
+          se = fork.ep;
+        }
+        if (fork.epStart) PHX_HIP(hipEventRecord(fork.epStart, se));
         auto ep = [&](auto kern) {
           PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsB)));
-          hipLaunchKernelGGL(kern, dim3(grid), dim3(kSepEpThreads), ldsB, s, pe);
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(kSepEpThreads), ldsB, se, pe);
         };
         // partial sums held per thread: calculations x b1-group members of
         // the launch, no more than the launch needs
@@ -2831,7 +2921,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           const int nb2 = p.nb2max <= 4 ? 4 : 8;
           const int rpw2 = std::min(64 / p.nm, nb2 <= 4 ? 5 : 4);
           const int cm = one ? 1 : p.nCalc <= 2 ? 2 : kCMax;
-          const size_t lds2 = sep_ep2_lds_bytes(p.nm, nb2, cm, rpw2, fuse ? p.m0 * p.nG0 : 0);
+          const size_t lds2 = sep_ep2_lds_bytes(p.nm, nb2, cm, rpw2);
           PHX_REQUIRE(lds2 <= 160 * 1024, "separable transform: LDS budget exceeded");
           EpParams q{};
           q.keepMask = p.keepMask;
@@ -2871,15 +2961,11 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           q.nCalc = p.nCalc;
           q.numStates = p.numStates;
           q.rpw = rpw2;
-          q.T2 = p.T2;
-          q.t2PanelStride = p.t2PanelStride;
-          q.nG0 = p.nG0;
-          q.g0val = p.g0val;
-          q.tw0 = p.tw0;
+          q.nb2max = p.nb2max;
           auto ep2 = [&](auto kern) {
             PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2)));
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds2, s, q);
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds2, se, q);
           };
           auto ep2f = [&](auto nmx, auto nb2c, auto fz) {
             constexpr int NM = decltype(nmx)::value, NB = decltype(nb2c)::value;
@@ -2895,7 +2981,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
             }
           };
           auto ep2c = [&](auto nmx, auto nb2c) {
-            if (fuse) ep2f(nmx, nb2c, std::true_type{});
+            if (rot) ep2f(nmx, nb2c, std::true_type{});
             else ep2f(nmx, nb2c, std::false_type{});
           };
           using I12 = std::integral_constant<int, 12>;

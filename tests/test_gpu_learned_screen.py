@@ -83,12 +83,13 @@ def test_trained_screen_skips_blocks_within_its_error_bound():
     assert np.array_equal(g1 == 0, g0 == 0)
 
 
-@pytest.mark.parametrize("precision,split", [("fp64", True), ("fp32", True), ("fp32", False)])
+@pytest.mark.parametrize("precision,split", [("fp64", True), ("fp32", True), ("fp32", 3), ("fp32", False)])
 def test_failed_audit_falls_back_to_the_exact_blocks(precision, split):
     """The three passes of a screened batch (computed blocks, audited blocks,
     the gated fallback over the skipped ones) reproduce the exact screened
-    run, in the fused kernel and in the split x-stage + epilogue kernels."""
-    say("failed audit", precision, "split" if split else "fused")
+    run, in the fused kernel and in the split x-stage + epilogue kernels
+    (split mode 3: the electron rotation in the x-stage kernel)."""
+    say("failed audit", precision, f"split {int(split)}" if split else "fused")
     p = _problem()
     gW = p.gw_host()
     exact = _handle(p, gW, precision, split)
@@ -117,7 +118,8 @@ def test_failed_audit_falls_back_to_the_exact_blocks(precision, split):
 
 def test_learned_screen_split_equals_fused():
     """FP32 with the trained network: the split last stage (which takes the
-    screen's slot masks and gate) skips the same blocks and gives the fused
+    screen's slot masks and gate), with the electron rotation in the epilogue
+    or in the x-stage kernel, skips the same blocks and gives the fused
     kernel's linewidths."""
     p = _problem()
     gW = p.gw_host()
@@ -126,7 +128,7 @@ def test_learned_screen_split_equals_fused():
     lw.screen_train(epochs=200, batch=512, lr=2e-3, seed=5)
     params = lw.screen_params()
     out, stats = [], []
-    for split in (True, False):
+    for split in (1, 3, 0):
         h = _handle(p, gW, "fp32", split)
         h.screen_params(params)
         h.set_learned_screen(True, threshold=1e-6, audit_frac=0.05, seed=9)
@@ -134,11 +136,12 @@ def test_learned_screen_split_equals_fused():
         out.append(h.linewidths(average=False))
         st = h.screen_stats()
         stats.append((st["blocks_in_screen"], st["learned_skips"], st["audited"], st["fallbacks"]))
-    say(f"{ns} samples; split vs fused stats {stats}")
-    assert stats[0] == stats[1] and stats[0][1] > 0
-    nz = out[1] != 0
-    assert np.array_equal(out[0] != 0, nz)
-    assert rel_err(out[0][nz], out[1][nz]).max() < 1e-12
+    say(f"{ns} samples; split 1, split 3, fused stats {stats}")
+    assert stats[0] == stats[1] == stats[2] and stats[0][1] > 0
+    nz = out[2] != 0
+    for got in out[:2]:
+        assert np.array_equal(got != 0, nz)
+        assert rel_err(got[nz], out[2][nz]).max() < 1e-12
 
 
 def test_learned_screen_argument_checks():

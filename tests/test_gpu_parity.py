@@ -112,11 +112,13 @@ def test_coupling_squared_vs_oracle(precision, tol):
     ((6, 5, 4), 0.0, 2, 1, 2, 6), ((5, 5, 5), 6.0, 4, 0, 5, 12), ((5, 5, 5), 6.0, 1, 0, 2, 16)])
 def test_split_last_stage_equals_fused(mesh, screen, nt, smearing, natoms, nw):
     """FP32 separable transform: the split last stage (x-stage and epilogue
-    kernels, g3 through device memory) gives the fused kernel's linewidths,
-    with and without the energy screen, for 1 to 4 (T, mu), 6 and 15 modes,
-    6, 12 and 16 Wannier functions (16: whole-slice g3 loads).  The g3 values
-    are the same; the FP64 epilogue arithmetic may fuse multiply-adds
-    differently in the two kernels, so agreement is to FP64 rounding."""
+    kernels, g3 through device memory; mode 3: the electron rotation in the
+    x-stage kernel, g' through device memory) gives the fused kernel's
+    linewidths, with and without the energy screen, for 1 to 4 (T, mu), 6 and
+    15 modes, 6, 12 and 16 Wannier functions (16: whole-slice g3 loads).  The
+    g3 values are the same; the FP64 epilogue arithmetic may fuse
+    multiply-adds differently in the kernels, so agreement is to FP64
+    rounding."""
     say("split vs fused last stage", mesh, screen, nt)
     # a copy: tiny_problem's instances are shared between tests
     p = dataclasses.replace(tiny_problem(natoms=natoms, nw=nw, rg_e=3, rg_p=3, mesh=mesh, seed=23))
@@ -125,22 +127,56 @@ def test_split_last_stage_equals_fused(mesh, screen, nt, smearing, natoms, nw):
     p.smearing = smearing  # 1: adaptive Gaussian
     gW = p.gw_host()
     out = []
-    for split in (True, False):
+    for split in (0, 1, 3):
         lw = P.ElectronLinewidths(p, gW=gW, precision="fp32", transform="separable")
         lw.set_split_last_stage(split)
         if screen:
             lw.set_screen(screen)
         lw.run()
         out.append(lw.linewidths(average=False))
+    nz = out[0] != 0
+    assert np.any(nz)
+    for split, got in zip((1, 3), out[1:]):
+        assert np.array_equal(got != 0, nz)
+        err = np.abs(got[nz] - out[0][nz]) / np.abs(out[0][nz])
+        say("  split mode", split, "max rel diff", err.max())
+        assert err.max() < 1e-12
+
+
+@pytest.mark.parametrize("split", [1, 3])
+def test_epilogue_overlap_gives_identical_linewidths(split):
+    """Split last stage with each chunk's epilogue on a second stream (two
+    alternating sets of g3 rows and row masks) against the chunks in turn:
+    the same kernels on the same data, so the linewidths are bit-identical,
+    in eager runs and in replays of a captured sweep (three panels per chunk,
+    many chunks per batch)."""
+    say("epilogue overlap, split mode", split)
+    p = dataclasses.replace(tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=(6, 6, 6), seed=29))
+    gW = p.gw_host()
+    out = []
+    for overlap in (0, 1):
+        lw = P.ElectronLinewidths(p, gW=gW, precision="fp32", transform="separable")
+        lw.set_split_last_stage(split)
+        lw.set_epilogue_overlap(overlap)
+        lw.set_panel_chunk(3)
+        lw.set_screen(6.0)
+        lw.run()
+        out.append(lw.linewidths(average=False))
+        if overlap:
+            lw.reset()
+            assert lw.plan() >= 1
+            for rep in range(3):
+                lw.plan_run(graph=True)
+                lw.sync()
+                g = lw.linewidths(average=False)
+                say("  graph replay", rep, "equal:", np.array_equal(g, out[0]))
+                assert np.array_equal(g, out[0])
+                lw.reset()
     assert np.any(out[0])
-    nz = out[1] != 0
-    assert np.array_equal(out[0] != 0, nz)
-    err = np.abs(out[0][nz] - out[1][nz]) / np.abs(out[1][nz])
-    say("  max rel diff", err.max())
-    assert err.max() < 1e-12
+    assert np.array_equal(out[0], out[1])
 
 
-@pytest.mark.parametrize("split", [True, False])
+@pytest.mark.parametrize("split", [1, 3, 0])
 def test_fully_degenerate_bands_match_oracle(split):
     """H(k) = eps(k) * identity: the six bands form one degenerate group at
     every k, so the b1 groups have six members (beyond the four-member form of
