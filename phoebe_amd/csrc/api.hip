@@ -797,6 +797,9 @@ struct LwHandle {
   std::vector<int> reF1, reF2, grpF0e;  // n1, n2 mod m of the grouped R_e; n0 mod m0 per group
   std::vector<double> reInvDeg;         // 1/de of the grouped R_e
   DevBuf<int> reOrderD, reGroupEndD;
+  int nRePad = 0;                       // groups padded to multiples of 4 entries
+  DevBuf<int2> rePadD;                  // (gW row, group-order position), (-1, -1) = padding
+  DevBuf<int> reBlkEndD;                // per 4-entry block: group index if it ends its group, else -1
   DevBuf<double2> TgD;
   DevBuf<float2> TgF;
 
@@ -1330,6 +1333,20 @@ static void lw_setup(LwHandle* L, const phx_problem* P, int precision, int k1Bat
       L->nGe = int(L->grpF0e.size());
       h2d(L->reOrderD, order, s);
       h2d(L->reGroupEndD, groupEnd, s);
+      std::vector<int2> pad;
+      std::vector<int> blkEnd;
+      for (int i = 0, start = 0; i < p.nRe; ++i) {
+        pad.push_back(make_int2(order[i], i));
+        if (groupEnd[i] >= 0) {
+          for (int k = i + 1 - start; k % 4; ++k) pad.push_back(make_int2(-1, -1));
+          blkEnd.resize(pad.size() / 4, -1);
+          blkEnd.back() = groupEnd[i];
+          start = i + 1;
+        }
+      }
+      L->nRePad = int(pad.size());
+      h2d(L->rePadD, pad, s);
+      h2d(L->reBlkEndD, blkEnd, s);
       PHX_HIP(hipStreamSynchronize(s));
     }
   }
@@ -1710,6 +1727,9 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     cp.nGe = L->nGe;
     cp.reOrder = L->reOrderD.p;
     cp.reGroupEnd = L->reGroupEndD.p;
+    cp.nRePad = L->nRePad;
+    cp.rePad = L->rePadD.p;
+    cp.reBlkEnd = L->reBlkEndD.p;
     cp.nLinesB = nLinesB;
     cp.nLinesA = line_count_padded(nLinesB);
     cp.lineKK = b.lineKK.p;

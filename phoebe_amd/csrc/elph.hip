@@ -496,6 +496,75 @@ __global__ void __launch_bounds__(256) k_cache_line_a(CacheParams p) {
   }
 }
 
+// Stage a for 16 lines on the matrix cores (FP32 data).  Per R_e group g,
+// Tg(l, g, c) = sum_{R_e in g} ph(R_e, l) gW(R_e, c) is a complex GEMM with
+// M = 16 lines, K = the group's R_e and N = the gW columns.  It runs as real
+// products on v_mfma_f32_16x16x4f32, four R_e at a time (groups padded to a
+// multiple of 4 with zero entries): lane (n, kq) = (L % 16, L / 16) loads
+// gW(R_kq, column n) of each 16-column tile (8 bytes, both components) and
+// ph(R_kq, line n), and two K-steps take the x and the y components,
+//   Re += a x - b y,   Im += b x + a y      (ph = a + i b, gW = x + i y),
+// with A = (a | -b) and (b | a).  D(line 4 kq + i, column n) holds Tg.  A wave
+// owns kLineMT tiles of 16 columns and keeps the next block's loads in flight;
+// the padded order sits in LDS.  8 bytes of gW per element against 128 FLOPs
+// on the matrix cores: bound by HBM.
+constexpr int kLineMT = 8;
+
+__global__ void __launch_bounds__(256, 4) k_cache_line_am(CacheParams p) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  extern __shared__ __align__(16) unsigned char smem[];
+  int2* sPad = reinterpret_cast<int2*>(smem);  // nRePad
+  for (int i = threadIdx.x; i < p.nRePad; i += 256) sPad[i] = p.rePad[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, n = lane & 15, kq = lane >> 4;
+  const size_t colw = (size_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * (16 * kLineMT);
+  if (colw >= p.N) return;  // whole waves
+  const float2* G = static_cast<const float2*>(p.gW);
+  const float2* ph = static_cast<const float2*>(p.phLine);
+  float2* out = static_cast<float2*>(p.Tg);
+  const int nBlk = p.nRePad / 4;
+  auto load = [&](int blk, float2(&v)[kLineMT], float2& f) {
+    const int2 e = blk < nBlk ? sPad[4 * blk + kq] : make_int2(-1, -1);
+#pragma unroll
+    for (int t = 0; t < kLineMT; ++t) {
+      const size_t c = colw + 16 * t + n;
+      v[t] = (e.x >= 0 && c < p.N) ? G[size_t(e.x) * p.N + c] : make_float2(0.f, 0.f);
+    }
+    f = e.y >= 0 ? ph[size_t(e.y) * 16 + n] : make_float2(0.f, 0.f);
+  };
+  f4 re[kLineMT], im[kLineMT];
+#pragma unroll
+  for (int t = 0; t < kLineMT; ++t) re[t] = im[t] = f4{0.f, 0.f, 0.f, 0.f};
+  float2 x[kLineMT], xn[kLineMT], f, fn;
+  load(0, x, f);
+  for (int b = 0; b < nBlk; ++b) {
+    load(b + 1, xn, fn);
+#pragma unroll
+    for (int t = 0; t < kLineMT; ++t) {
+      re[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.x, x[t].x, re[t], 0, 0, 0);
+      re[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(-f.y, x[t].y, re[t], 0, 0, 0);
+      im[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.y, x[t].x, im[t], 0, 0, 0);
+      im[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.x, x[t].y, im[t], 0, 0, 0);
+    }
+    const int g = p.reBlkEnd[b];
+    if (g >= 0) {
+#pragma unroll
+      for (int t = 0; t < kLineMT; ++t) {
+        const size_t c = colw + 16 * t + n;
+        if (c < p.N) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            out[(size_t(4 * kq + i) * p.nGe + g) * p.N + c] = make_float2(re[t][i], im[t][i]);
+        }
+        re[t] = im[t] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kLineMT; ++t) x[t] = xn[t];
+    f = fn;
+  }
+}
+
 // One workgroup per (Rp, nu) block of nw^2 columns, in passes of 256 columns
 // and, per line, of 16 outer points.  With at most kLineGe R_e groups, a
 // thread's partial sums of the line (one per group) are loaded once, all at
@@ -576,6 +645,18 @@ __global__ void __launch_bounds__(256) k_cache_line_b(CacheParams p) {
 
 template <typename T, int NL>
 static void launch_line_a(const CacheParams& p, hipStream_t s) {
+  if constexpr (sizeof(T) == 4 && NL == 16) {
+    if (p.rePad && p.reBlkEnd && p.nRePad % 4 == 0) {
+      const size_t lds = size_t(p.nRePad) * sizeof(int2);
+      PHX_REQUIRE(lds <= 160 * 1024, "cacheElPh line: too many R_e for the matrix-core stage");
+      PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_cache_line_am),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(lds, 16))));
+      hipLaunchKernelGGL(k_cache_line_am, dim3(ceil_div((long long)p.N, 4 * 16 * kLineMT)), dim3(256), lds, s,
+                         p);
+      PHX_HIP(hipGetLastError());
+      return;
+    }
+  }
   if constexpr (sizeof(T) == 4) {
     if (p.N % 2 == 0) {
       hipLaunchKernelGGL((k_cache_line_a<T, NL, 2>), dim3(ceil_div((long long)p.N / 2, 256)), dim3(256),

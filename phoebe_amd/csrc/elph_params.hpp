@@ -39,6 +39,10 @@ struct CacheParams {
   int nGe;                // R_e groups: distinct n0(R_e) mod m0
   const int* reOrder;     // nRe: native index of the i-th R_e in group order
   const int* reGroupEnd;  // nRe: group index if the i-th R_e ends its group, else -1
+  // the same order with every group padded to a multiple of 4 entries (k_cache_line_am)
+  int nRePad;
+  const int2* rePad;      // nRePad: (gW row, position in group order) of each entry, (-1, -1) = padding
+  const int* reBlkEnd;    // nRePad / 4: group index if the 4-entry block ends its group, else -1
   int nLinesB;            // lines of the batch (<= kLinesPerBatch)
   int nLinesA;            // lines stage a computes: line_count_padded(nLinesB), rows of phLine and Tg
   const int* lineKK;      // nLinesB + 1: first outer point of each line (points grouped by line)
