@@ -473,11 +473,11 @@ int phx_lw_set_panel_chunk(phx_lw* h, int panels);
 int phx_lw_set_split_last_stage(phx_lw* h, int on);
 /* Split last stage: run the epilogue kernel of each panel chunk on a second
  * stream of the handle, concurrently with the next chunk's transform stages
- * (on = 1, the default), or after them on the handle's stream (on = 0).  The
- * chunks then alternate between two sets of g3 rows and row masks (twice the
- * device memory of those); the learned screen's passes always run in turn.
- * The linewidths are the same either way.  Takes effect for batches prepared
- * after the call.  No counterpart in the reference. */
+ * (on = 1), or after them on the handle's stream (on = 0, the default).  With
+ * on = 1 the chunks alternate between two sets of g3 rows and row masks
+ * (twice the device memory of those); the learned screen's passes always run
+ * in turn.  The linewidths are the same either way.  Takes effect for batches
+ * prepared after the call.  No counterpart in the reference. */
 int phx_lw_set_epilogue_overlap(phx_lw* h, int on);
 /* With timing on, the launches of phx_lw_stage_times that ran split:
  * out4 = {x-stage kernel ms, epilogue kernel ms, launches, sum over the rows

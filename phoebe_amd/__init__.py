@@ -706,7 +706,8 @@ class ElectronLinewidths:
 
     def set_epilogue_overlap(self, on=True):
         """Split last stage: each chunk's epilogue kernel on a second stream,
-        concurrent with the next chunk's transform stages (default), or in turn."""
+        concurrent with the next chunk's transform stages, or in turn (the
+        handle's default)."""
         _check(lib().phx_lw_set_epilogue_overlap(self._h, int(bool(on))))
 
     def split_times(self):

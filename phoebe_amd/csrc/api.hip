@@ -786,7 +786,7 @@ struct LwHandle {
   // the split stage's epilogue of chunk i on streamEp, overlapping chunk i + 1's
   // transform stages on `stream`: chunks alternate between two sets of g3 rows
   // and row masks; epEv holds two events per chunk (x-stage done, epilogue done)
-  int epOverlap = 1;
+  int epOverlap = 0;
   hipStream_t streamEp = nullptr;
   std::vector<hipEvent_t> epEv;
   DevBuf<float2> G3f2;
@@ -839,8 +839,9 @@ struct LwHandle {
   long long kernelLaunches = 0;
   double couplingFlops = 0.;  // algorithmic FLOPs of the coupling launches timed
   // last-stage launches of the separable transform: events summed at the next sync,
-  // (start, mid, epilogue start, end) per launch; mid is recorded after the split
-  // stage's x-stage kernel, the last two on the stream its epilogue ran on
+  // (start, x-stage start, mid, epilogue start, end) per launch; the split stage's
+  // x-stage kernel runs between the second and mid, the last two are recorded on
+  // the stream its epilogue ran on
   std::vector<hipEvent_t> xev;
   std::vector<char> xevSplit;        // per launch: whether it ran split
   size_t xevUsed = 0;
@@ -882,15 +883,15 @@ static void lw_harvest_timing(LwHandle* L) {
   L->kernelMs[2] += c;
   L->kernelLaunches += 1;
   L->evPending = false;
-  for (size_t i = 0; i + 4 <= L->xevUsed; i += 4) {
+  for (size_t i = 0; i + 5 <= L->xevUsed; i += 5) {
     float x = 0.f;
-    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 3]));
+    PHX_HIP(hipEventElapsedTime(&x, L->xev[i], L->xev[i + 4]));
     L->xMs += x;
     L->xLaunches += 1;
-    if (L->xevSplit[i / 4]) {
+    if (L->xevSplit[i / 5]) {
       float a = 0.f, b = 0.f;
-      PHX_HIP(hipEventElapsedTime(&a, L->xev[i], L->xev[i + 1]));
-      PHX_HIP(hipEventElapsedTime(&b, L->xev[i + 2], L->xev[i + 3]));
+      PHX_HIP(hipEventElapsedTime(&a, L->xev[i + 1], L->xev[i + 2]));
+      PHX_HIP(hipEventElapsedTime(&b, L->xev[i + 3], L->xev[i + 4]));
       L->xgMs += a;
       L->epMs += b;
       L->splitLaunches += 1;
@@ -1932,7 +1933,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
         CoupParams c = c0;
         hipEvent_t mid = nullptr;
         if (L->timing) {
-          while (L->xev.size() < L->xevUsed + 4) {
+          while (L->xev.size() < L->xevUsed + 5) {
             hipEvent_t e;
             PHX_HIP(hipEventCreate(&e));
             L->xev.push_back(e);
@@ -1940,8 +1941,9 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
           c.keptRows = L->xKept.p;
           c.keptBands = L->xBands.p;
           c.keptSlots = L->xSlots.p;
-          mid = L->xev[L->xevUsed + 1];
-          fork.epStart = L->xev[L->xevUsed + 2];
+          fork.xgStart = L->xev[L->xevUsed + 1];
+          mid = L->xev[L->xevUsed + 2];
+          fork.epStart = L->xev[L->xevUsed + 3];
           PHX_HIP(hipEventRecord(L->xev[L->xevUsed], s));
         }
         const bool split = f64 ? launch_separable_x<double>(sp, c, true, s, mid, fork)
@@ -1953,8 +1955,8 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
           lastEp = ci;
         }
         if (L->timing) {
-          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 3], forked ? L->streamEp : s));
-          L->xevUsed += 4;
+          PHX_HIP(hipEventRecord(L->xev[L->xevUsed + 4], forked ? L->streamEp : s));
+          L->xevUsed += 5;
           L->xevSplit.push_back(split ? 1 : 0);
         }
       };

@@ -2834,6 +2834,7 @@ struct EpFork {
   hipStream_t ep = nullptr;
   hipEvent_t xgDone = nullptr;   // recorded on the launch's stream after the x-stage kernel (ep set)
   hipEvent_t epStart = nullptr;  // timing: recorded on the epilogue's stream before it (null: off)
+  hipEvent_t xgStart = nullptr;  // timing: recorded on the launch's stream right before the x-stage kernel
 };
 
 // mid (optional): recorded between the x-stage and the epilogue kernel when
@@ -2896,6 +2897,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsA)));
           hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsA, s, p);
         };
+        if (fork.xgStart) PHX_HIP(hipEventRecord(fork.xgStart, s));
         if (rot) xg(k_sep_xg<T, 3, true>);
         else xg(k_sep_xg<T, 3, false>);
         PHX_HIP(hipGetLastError());
