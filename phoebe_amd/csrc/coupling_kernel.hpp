@@ -1953,6 +1953,11 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
   T2v* stw = reinterpret_cast<T2v*>(smem + (size_t(192 + nG0) * sizeof(int) + 15) / 16 * 16);  // m0
   T2v* Wk = stw + p.m0;                                                        // nkeep x nG0p
   T2v* sG = Wk + size_t(p.m0) * nG0p;  // ROT: one tile's g3, 16 rows x (nm (nw + 1) + 1)
+  // ROT: the tile's U2, 16 rows x (nb2max x 16 + 1) complex FP64 (w2 fastest, zero beyond the
+  // row's bands and nw), 16-byte aligned after sG
+  const int us = p.nb2max * 16 + 1;
+  double2* sUx = reinterpret_cast<double2*>(
+      smem + (size_t(reinterpret_cast<unsigned char*>(sG + 16 * (nm * (nw + 1) + 1)) - smem) + 15) / 16 * 16);
   const T2v* X = static_cast<const T2v*>(p.T2) + size_t(panel) * (p.t2PanelStride / 2) +
                  (size_t(jy) * p.m2 + jz) * ncol;
   const size_t gStride = size_t(p.m1) * p.m2 * ncol;
@@ -2039,12 +2044,19 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
           }
         }
       }
+      // U2 of the tile's rows, each entry read from HBM once (not once per mode)
+      for (int i = t; i < 16 * p.nb2max * 16; i += 256) {
+        const int r = i / (p.nb2max * 16), o = i - r * (p.nb2max * 16), b2 = o >> 4, w2 = o & 15;
+        double2 v = make_double2(0., 0.);
+        if (r < nr && b2 < snb[m0r + r] && w2 < nw) v = p.U2pool[su2[m0r + r] + b2 * nw + w2];
+        sUx[r * us + o] = v;
+      }
       __syncthreads();
       // thread (row r, mode nu), nm <= 16 so 16 rows x nm <= 256 threads
       if (t < nr * nm) {
         const int r = t / nm, nu = t - r * nm, k = m0r + r;
         const int nb2 = snb[k];
-        const double2* u = p.U2pool + su2[k];
+        const double2* u = sUx + r * us;
         const T2v* x = sG + r * rs + nu * gs;
         double2* o = outR + size_t(jxs[k]) * rowG + nu;
         for (int b2 = 0; b2 < nb2; ++b2) {
@@ -2052,7 +2064,7 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
 #pragma unroll
           for (int w2 = 0; w2 < 16; ++w2) {
             if (w2 < nw) {
-              const double2 c = u[b2 * nw + w2];
+              const double2 c = u[b2 * 16 + w2];
               const double xr = double(x[w2].x), xi = double(x[w2].y);
               ar += c.x * xr + c.y * xi;
               ai += c.x * xi - c.y * xr;
@@ -2077,10 +2089,12 @@ __global__ void __launch_bounds__(256) k_sep_xg(CoupParams p) {
   }
 }
 
-// rot: the x-stage with the electron rotation (a tile of g3 in LDS)
-inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0, int nm = 0, int nw = 0, bool rot = false) {
+// rot: the x-stage with the electron rotation (a tile of g3 and its rows' U2 in LDS)
+inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0, int nm = 0, int nw = 0, bool rot = false,
+                               int nb2max = 0) {
   size_t b = (size_t(192 + nG0) * sizeof(int) + 15) / 16 * 16 + size_t(m0) * (nG0 + 2) * 2 * elem;
-  if (rot) b += size_t(16) * (nm * (nw + 1) + 1) * 2 * elem;
+  if (rot) b += (size_t(16) * (nm * (nw + 1) + 1) * 2 * elem + 15) / 16 * 16 +
+                size_t(16) * (nb2max * 16 + 1) * sizeof(double2);
   return b + 64;
 }
 
@@ -2616,6 +2630,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
     if (rowOn && w3c >= p.cutoff) {
       const int mystart = grpc & 0xff, mydeg = grpc >> 8;
       const double* g0 = gsq + (size_t(r) * nm + mystart) * NB2;
+      const double wfac = p.norm / w3c * 3.14159265358979323846;  // once per lane, not per term
 #pragma unroll 1
       for (int b2 = 0; b2 < nb2; ++b2) {
         double vb;
@@ -2639,8 +2654,7 @@ __global__ void __launch_bounds__(64, 3) k_sep_ep2(EpParams p) {
           for (int c = 0; c < CM; ++c) {
             if (c >= p.nCalc) break;
             const double f2 = sf2[(c * R + r) * NB2 + b2];
-            part[gi][c] += cpl * ((f2 + n3c[c]) * d1 + (1. - f2 + n3c[c]) * d2) * p.norm / w3c *
-                           3.14159265358979323846;
+            part[gi][c] += cpl * ((f2 + n3c[c]) * d1 + (1. - f2 + n3c[c]) * d2) * wfac;
           }
         }
       }
@@ -2674,7 +2688,7 @@ inline size_t sep_ep2_lds_bytes(int nm, int nb2, int cm, int rpw) {
   return std::max(b, size_t(kGMax) * kCMax * sizeof(double)) + 64;
 }
 
-constexpr int kXminBlocks = 16;
+constexpr int kXminBlocks = 64;
 
 // Closest approach to resonance of every panel of the chunk, for the energy
 // screen widths: the minimum over the panel's rows (inner points of the
@@ -2882,7 +2896,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
       if (split) {
         // the electron rotation in the x-stage kernel: with the lean epilogue only
         const bool rot = p.epRot && ep2ok;
-        const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0, p.nm, p.nw, rot);
+        const size_t ldsA = sep_xg_lds_bytes(sizeof(T), p.nG0, p.m0, p.nm, p.nw, rot, p.nb2max);
         // whole rows of nm threads per wave, at most 8, fewer if the LDS
         // budget needs it (4 rows per wave always fit what 16 x 16 threads did)
         CoupParams pe = p;
