@@ -63,6 +63,7 @@ def run_form(P, prob, form, make_batches, warmup, k1_batch):
     sst = lw.split_times() if transform == "separable" else None
     if sst is not None:
         sst.update(lw.split_bytes())
+        sst["fused_y"] = lw.last_stage_fused_y()
     gam = lw.linewidths(average=False)
     log(f"[{form}] {len(k1_batches)} run calls in {elapsed:.3f}s, {pairs} pairs, "
         f"{pairs / elapsed:.4g} pairs/s; kernels {kt}")
@@ -116,10 +117,12 @@ def roofline(res, form, workload):
             nb1, nb2, nm = res["nb1_mean"], res["nb2_mean"], res["nm"]
             pair_bytes = 8.0 * nb1 * nb2 * nm + 16.0 * nb2 + 16.0 * nm
             pairs = st["rows"] / nb1 / n
-            xg = {"kernel": "k_sep_xg", "avg_launch_ms": round(t_xg * 1e3, 4), "bytes_per_launch": xg_bytes,
+            # with the y-stage inside the x-stage kernel (k_sep_yx) it reads T1 instead of the slabs
+            xk = "k_sep_yx" if sp.get("fused_y") else "k_sep_xg"
+            xg = {"kernel": xk, "avg_launch_ms": round(t_xg * 1e3, 4), "bytes_per_launch": xg_bytes,
                   "achieved_GBps": round(xg_bytes / t_xg / 1e9, 1),
                   "frac": round(xg_bytes / t_xg / 1e9 / HBM_PEAK_GBS, 4),
-                  "traffic": traffic_per_dispatch("k_sep_xg", workload),
+                  "traffic": traffic_per_dispatch(xk, workload),
                   "mfma_TFLOPs": round(st["flops"] / n / t_xg / 1e12, 3)}
             ep = {"kernel": "k_sep_ep2", "avg_launch_ms": round(t_ep * 1e3, 4), "bytes_per_launch": g3,
                   "achieved_GBps": round(g3 / t_ep / 1e9, 1), "frac": round(g3 / t_ep / 1e9 / HBM_PEAK_GBS, 4),
@@ -134,7 +137,7 @@ def roofline(res, form, workload):
                     "flop_model_note": "the per-pair FLOP model of the direct R_ph transform "
                                        "(8 N_Rph nm nb1 Nw + rotations) does not apply: the separable "
                                        "transform computes the same g3 with about 26x fewer operations",
-                    "k_sep_xg": xg, "k_sep_ep2": ep}
+                    xk: xg, "k_sep_ep2": ep}
         else:
             ms, launches, flops = st["ms"], max(st["launches"], 1), st["flops"]
             kern = "k_sep_xm"

@@ -479,6 +479,17 @@ int phx_lw_set_split_last_stage(phx_lw* h, int on);
  * in turn.  The linewidths are the same either way.  Takes effect for batches
  * prepared after the call.  No counterpart in the reference. */
 int phx_lw_set_epilogue_overlap(phx_lw* h, int on);
+/* Split last stage, PHX_FP32 without the learned screen: run the y-stage of
+ * the separable transform inside the x-stage kernel (on = 1, the default),
+ * so that the x-stage slabs T2 stay in LDS instead of going through device
+ * memory, or as its own kernel (on = 0).  Not with split mode 3.  The g3
+ * values, and so the linewidths, are the same either way.  No counterpart
+ * in the reference. */
+int phx_lw_set_fused_y_stage(phx_lw* h, int on);
+/* With timing on: *on = 1 when every timed split launch ran the y-stage inside
+ * the x-stage kernel (the x-stage times and bytes then cover both stages).
+ * No counterpart in the reference (a measurement call). */
+int phx_lw_last_stage_fused_y(phx_lw* h, int* on);
 /* With timing on, the launches of phx_lw_stage_times that ran split:
  * out4 = {x-stage kernel ms, epilogue kernel ms, launches, sum over the rows
  * the epilogue computed of their band counts}.  No counterpart in the reference. */
@@ -488,7 +499,9 @@ int phx_lw_last_stage_split_times(phx_lw* h, double* out4);
  * the x-stage slabs (slots computed x x-groups x nm nw complex FP32) and
  * writes g3 (kept rows x nm nw complex FP32); with the rotation in the
  * x-stage (split mode 3) it writes g' instead (kept bands x nm complex FP64)
- * and reads U2 (kept bands x nw complex FP64).  No counterpart in the
+ * and reads U2 (kept bands x nw complex FP64); with the y-stage inside it
+ * reads the y-stage input T1 (panels x lines x mz x nm nw complex FP32)
+ * instead of the slabs.  No counterpart in the
  * reference (a measurement call). */
 int phx_lw_last_stage_split_bytes(phx_lw* h, double* out2);
 /* ---- Learned q-block screen (separable transform with the energy screen on).

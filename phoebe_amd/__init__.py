@@ -179,6 +179,8 @@ EXPORTS = {
     "phx_phel_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_split_last_stage": (C.c_int, [C.c_void_p, C.c_int]),
     "phx_lw_set_epilogue_overlap": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_set_fused_y_stage": (C.c_int, [C.c_void_p, C.c_int]),
+    "phx_lw_last_stage_fused_y": (C.c_int, [C.c_void_p, _ip]),
     "phx_lw_last_stage_split_times": (C.c_int, [C.c_void_p, _dp]),
     "phx_lw_last_stage_split_bytes": (C.c_int, [C.c_void_p, _dp]),
     "phx_phel_kernel_times": (C.c_int, [C.c_void_p, _dp]),
@@ -709,6 +711,16 @@ class ElectronLinewidths:
         concurrent with the next chunk's transform stages, or in turn (the
         handle's default)."""
         _check(lib().phx_lw_set_epilogue_overlap(self._h, int(bool(on))))
+
+    def set_fused_y_stage(self, on=True):
+        """FP32 split stage: the y-stage inside the x-stage kernel (default) or its own kernel."""
+        _check(lib().phx_lw_set_fused_y_stage(self._h, int(bool(on))))
+
+    def last_stage_fused_y(self):
+        """With timing on: whether every timed split launch ran the y-stage inside the x-stage kernel."""
+        v = C.c_int(0)
+        _check(lib().phx_lw_last_stage_fused_y(self._h, C.byref(v)))
+        return bool(v.value)
 
     def split_times(self):
         """Split last stage with timing on: x-stage ms, epilogue ms, launches,

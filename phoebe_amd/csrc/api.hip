@@ -65,8 +65,8 @@ extern template void launch_separable<double>(const SepParams&, const CoupParams
                                               hipStream_t);
 extern template void launch_separable<float>(const SepParams&, const CoupParams&, bool, int, int,
                                              hipStream_t);
-extern template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t);
-extern template void launch_separable_zy<float>(const SepParams&, int, int, hipStream_t);
+extern template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t, bool);
+extern template void launch_separable_zy<float>(const SepParams&, int, int, hipStream_t, bool);
 extern template bool launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t,
                                                   const EpFork&);
 extern template bool launch_separable_x<float>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t,
@@ -787,6 +787,10 @@ struct LwHandle {
   // transform stages on `stream`: chunks alternate between two sets of g3 rows
   // and row masks; epEv holds two events per chunk (x-stage done, epilogue done)
   int epOverlap = 0;
+  // FP32 split stage: the y-stage inside the x-stage kernel (k_sep_yx), no T2 in HBM
+  int fuseYX = 1;
+  double yxT1Bytes = 0.;   // timed launches of k_sep_yx: T1 bytes they read
+  long long yxLaunches = 0;
   hipStream_t streamEp = nullptr;
   std::vector<hipEvent_t> epEv;
   DevBuf<float2> G3f2;
@@ -1896,6 +1900,10 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
     const bool splitOk = lw_split_last(L, p) && L->G3f.n >= g3Need;
     const bool overlap = L->epOverlap && !learned && splitOk && L->G3f2.n >= g3Need &&
                          L->keepMask2.n >= slotsC;
+    // the y-stage inside the x-stage kernel: FP32 split stage without slot masks
+    const bool yx = L->fuseYX && splitOk && !learned && L->precision == PHX_FP32 && L->splitLast != 3 &&
+                    L->nG0 <= 24 && sep_yx_lds_bytes(L->nLines, L->nG0, mx) <= 160 * 1024;
+    cq.fuseYX = yx;
     const int nChunks = (nPanels + chunk - 1) / chunk;
     if (overlap) {
       if (!L->streamEp) PHX_HIP(hipStreamCreateWithFlags(&L->streamEp, hipStreamNonBlocking));
@@ -1949,6 +1957,10 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
         const bool split = f64 ? launch_separable_x<double>(sp, c, true, s, mid, fork)
                                : launch_separable_x<float>(sp, c, true, s, mid, fork);
         const bool forked = split && fork.ep;
+        if (L->timing && split && c.fuseYX) {
+          L->yxT1Bytes += double(sp.nPanels) * mz * L->nLines * ncol * 8.;
+          L->yxLaunches += 1;
+        }
         if (forked) {
           PHX_HIP(hipEventRecord(L->epEv[2 * ci + 1], L->streamEp));
           epPending[set] = true;
@@ -1960,8 +1972,8 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
           L->xevSplit.push_back(split ? 1 : 0);
         }
       };
-      if (f64) launch_separable_zy<double>(sp, L->maxLineE, L->maxGrpE, s);
-      else launch_separable_zy<float>(sp, L->maxLineE, L->maxGrpE, s);
+      if (f64) launch_separable_zy<double>(sp, L->maxLineE, L->maxGrpE, s, true);
+      else launch_separable_zy<float>(sp, L->maxLineE, L->maxGrpE, s, !yx);
       if (!learned) {
         // this chunk rewrites set `set`: the epilogue of chunk ci - 2 must have read it
         if (epPending[set]) PHX_HIP(hipStreamWaitEvent(s, L->epEv[2 * (ci - 2) + 1], 0));
@@ -3574,6 +3586,23 @@ int phx_lw_set_epilogue_overlap(phx_lw* h, int on) {
   });
 }
 
+int phx_lw_set_fused_y_stage(phx_lw* h, int on) {
+  return guard([&] {
+    PHX_REQUIRE(h, "phx_lw_set_fused_y_stage: null handle");
+    PHX_REQUIRE(on == 0 || on == 1, "phx_lw_set_fused_y_stage: 0 or 1");
+    h->fuseYX = on;
+  });
+}
+
+int phx_lw_last_stage_fused_y(phx_lw* h, int* on) {
+  return guard([&] {
+    PHX_REQUIRE(h && on, "null argument");
+    PHX_HIP(hipStreamSynchronize(h->stream));
+    lw_harvest_timing(h);
+    *on = h->yxLaunches > 0 && h->yxLaunches == h->splitLaunches ? 1 : 0;
+  });
+}
+
 int phx_lw_set_panel_chunk(phx_lw* h, int panels) {
   return guard([&] {
     PHX_REQUIRE(h, "null handle");
@@ -3819,6 +3848,8 @@ int phx_lw_set_timing(phx_lw* h, int on) {
     h->couplingFlops = 0.;
     h->xMs = h->xgMs = h->epMs = 0.;
     h->xLaunches = h->splitLaunches = 0;
+    h->yxT1Bytes = 0.;
+    h->yxLaunches = 0;
     h->xevUsed = 0;
     h->xevSplit.clear();
     if (!h->xKept.p) h->xKept.alloc(1);
@@ -3882,9 +3913,10 @@ int phx_lw_last_stage_split_bytes(phx_lw* h, double* out2) {
     if (h->xSlots.p) PHX_HIP(hipMemcpy(&slots, h->xSlots.p, sizeof(slots), hipMemcpyDeviceToHost));
     if (h->xKept.p) PHX_HIP(hipMemcpy(&rows, h->xKept.p, sizeof(rows), hipMemcpyDeviceToHost));
     if (h->xBands.p) PHX_HIP(hipMemcpy(&bands, h->xBands.p, sizeof(bands), hipMemcpyDeviceToHost));
-    // complex FP32: the x-stage reads nG0 x (nm nw) per slot and writes nm nw per kept row
+    // complex FP32: the x-stage reads nG0 x (nm nw) per slot and writes nm nw per kept row;
+    // with the y-stage inside (k_sep_yx) it reads T1 instead of the slabs (slots not counted)
     const double row = 8. * h->nm * h->nw;
-    out2[0] = double(slots) * h->nG0 * row;
+    out2[0] = double(slots) * h->nG0 * row + h->yxT1Bytes;
     out2[1] = h->splitLaunches > 0 ? double(rows) * row : 0.;
     if (h->splitLast == 3 && h->splitLaunches > 0) {
       // rotation in the x-stage: it reads U2 (nw complex FP64 per kept band) and writes

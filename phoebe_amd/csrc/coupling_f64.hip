@@ -4,7 +4,7 @@
 namespace phx {
 template void launch_coupling<double>(const CoupParams&, bool, hipStream_t);
 template void launch_separable<double>(const SepParams&, const CoupParams&, bool, int, int, hipStream_t);
-template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t);
+template void launch_separable_zy<double>(const SepParams&, int, int, hipStream_t, bool);
 template bool launch_separable_x<double>(const SepParams&, const CoupParams&, bool, hipStream_t, hipEvent_t,
                                                   const EpFork&);
 int coupling_rows_per_tile_f64(int ldb) { return coupling_rows_per_tile(ldb); }

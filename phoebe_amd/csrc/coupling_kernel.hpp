@@ -102,6 +102,7 @@ struct CoupParams {
   unsigned long long* keptBands;          // k_sep_ep: sum of nb2 over the rows it computed (null: off)
   int epFirst;                            // split epilogue: 1 = k_sep_ep, 0 = k_sep_ep2 where it applies
   int epRot;                              // k_sep_xg also does the electron rotation (g' to HBM, not g3)
+  int fuseYX;                             // the y-stage runs inside the x-stage kernel (k_sep_yx; no T2 in HBM)
   unsigned long long* keptSlots;          // k_sep_xg: slots (panel, yz line) it computed (null: off)
 };
 
@@ -2098,6 +2099,160 @@ inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0, int nm = 0, int nw = 0
   return b + 64;
 }
 
+// The y-stage and the x-stage in one kernel, so that the x-stage slabs T2
+// never go through HBM.  Workgroup (block of 16 columns, jz, panel): the T1
+// entries of every line at (jz, the block's columns) are staged in LDS once;
+// then, per chunk of kYxJC y coordinates, the y-stage sums
+// T2(g, jy, c) = sum_{lines of group g} w_y^{jy f1} T1(line, jz, c) into LDS
+// (FP64, the group's lines in ascending order, as k_sep_y does) and wave w
+// runs the x-stage of slots (jy0 + w + 4i, jz) on v_mfma_f32_16x16x4f32 as
+// k_sep_xg does (A = the twiddles of the slot's kept rows, B = T2 read from
+// LDS), writing the kept rows' g3 for the block's columns (waves take the
+// chunk's slots in turn).  The same values
+// in the same order as k_sep_y + k_sep_xg, so the same g3; HBM sees T1 once
+// and the g3 rows, not T2 twice.
+constexpr int kYxJC = 4;  // y coordinates per chunk: one slot per wave (8: 47 KB of LDS, slower)
+
+inline size_t sep_yx_lds_bytes(int nLines, int nG0, int m0) {
+  size_t b = (size_t(nLines) * 16 * sizeof(float2) + 15) / 16 * 16;  // T1 of the block
+  b += size_t(nLines) * kYxJC * sizeof(double2);                     // y twiddles of a chunk
+  b += size_t(nG0) * kYxJC * 16 * sizeof(float2);                    // T2 of a chunk
+  b += size_t(m0) * (nG0 + 1) * sizeof(float2);                      // x twiddles of every row
+  b += size_t(2 * nLines + nG0 + 1 + 4 * 64) * sizeof(int);          // entries, groups, kept rows
+  return b + 64;
+}
+
+static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename Mfma4<float>::acc_t acc_t;
+  const int cb = blockIdx.x, jz = blockIdx.y, panel = blockIdx.z;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
+  const int ncol = sp.ncol, my = sp.my, mz = sp.mz, m0 = p.m0, nG0 = sp.nG0, nL = sp.nLines;
+  const int nG0p = nG0 + 1;
+  float2* sT1 = reinterpret_cast<float2*>(smem);  // (entry in group order, 16)
+  double2* sWy = reinterpret_cast<double2*>(smem + (size_t(nL) * 16 * sizeof(float2) + 15) / 16 * 16);  // (entry, jj)
+  float2* sT2 = reinterpret_cast<float2*>(sWy + size_t(nL) * kYxJC);  // (g, jj, 16)
+  float2* sWx = sT2 + size_t(nG0) * kYxJC * 16;                        // (jx, g), g < nG0p
+  int* sOff = reinterpret_cast<int*>(sWx + size_t(m0) * nG0p);         // T1 row of each entry
+  int* sF1 = sOff + nL;                                                // f1 of each entry
+  int* sGs = sF1 + nL;                                                 // group starts
+  int* sJx = sGs + nG0 + 1;                                            // kept rows, 64 per wave
+  const float2* T1 = reinterpret_cast<const float2*>(static_cast<const float*>(sp.T1) + size_t(panel) * sp.t1PanelStride);
+  for (int e = t; e < nL; e += 256) {
+    sOff[e] = (sp.grpLine[e] * mz + jz) * ncol;
+    sF1[e] = sp.grpF1[e];
+  }
+  for (int i = t; i <= nG0; i += 256) sGs[i] = sp.grpStart[i];
+  for (int i = t; i < m0 * nG0p; i += 256) {
+    const int jx = i / nG0p, g = i - jx * nG0p;
+    float2 w = make_float2(0.f, 0.f);
+    if (g < nG0) {
+      const double2 d = p.tw0[(jx * p.g0val[g]) % m0];
+      w = make_float2(float(d.x), float(d.y));
+    }
+    sWx[i] = w;
+  }
+  __syncthreads();
+  // T1 of the block: up to 8 loads per thread in flight before their LDS stores
+  for (int i0 = t; i0 < nL * 16; i0 += 8 * 256) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + 256 * u, col = cb * 16 + (i & 15);
+      v[u] = make_float2(0.f, 0.f);
+      if (i < nL * 16 && col < ncol) v[u] = T1[size_t(sOff[i >> 4]) + col];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + 256 * u;
+      if (i < nL * 16) sT1[i] = v[u];
+    }
+  }
+  const int nLinesYZ = p.m1 * p.m2;
+  const int KS = (2 * nG0 + 3) / 4;  // <= 12, checked by the launcher
+  const int c = cb * 16 + lr;
+  for (int jy0 = 0; jy0 < my; jy0 += kYxJC) {
+    const int nj = min(kYxJC, my - jy0);
+    for (int i = t; i < nL * kYxJC; i += 256) {
+      const int e = i / kYxJC, jj = i - e * kYxJC;
+      sWy[i] = jj < nj ? sp.twy[((jy0 + jj) * sF1[e]) % my] : make_double2(0., 0.);
+    }
+    __syncthreads();  // also: T1 staged (first chunk)
+    // thread (g, column): all kYxJC y coordinates of the chunk from one read of each T1 entry
+    for (int i = t; i < nG0 * 16; i += 256) {
+      const int cc = i & 15, g = i >> 4;
+      double2 acc[kYxJC];
+#pragma unroll
+      for (int jj = 0; jj < kYxJC; ++jj) acc[jj] = make_double2(0., 0.);
+      for (int e = sGs[g]; e < sGs[g + 1]; ++e) {
+        const float2 xv = sT1[e * 16 + cc];
+        const double2 x = make_double2(double(xv.x), double(xv.y));
+#pragma unroll
+        for (int jj = 0; jj < kYxJC; ++jj) {
+          const double2 ww = sWy[e * kYxJC + jj];
+          acc[jj].x += ww.x * x.x - ww.y * x.y;
+          acc[jj].y += ww.x * x.y + ww.y * x.x;
+        }
+      }
+#pragma unroll
+      for (int jj = 0; jj < kYxJC; ++jj) sT2[(g * kYxJC + jj) * 16 + cc] = make_float2(float(acc[jj].x), float(acc[jj].y));
+    }
+    __syncthreads();
+    for (int jw = wave; jw < nj; jw += 4) {
+      const int slot = panel * nLinesYZ + jz * p.m1 + jy0 + jw;
+      const unsigned long long mask = p.keepMask[slot];
+      if (mask != 0ull) {
+        int* jxs = sJx + wave * 64;
+        wave_sync();  // the wave's previous slot has read its list
+        if (lane < m0 && ((mask >> lane) & 1ull)) jxs[__popcll(mask & ((1ull << lane) - 1ull))] = lane;
+        wave_sync();
+        const int nkeep = __popcll(mask);
+        float bq[12];
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+          const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+          float b = 0.f;
+          if (q < KS && g < nG0) {
+            const float2 v = sT2[(g * kYxJC + jw) * 16 + lr];
+            b = kc ? v.y : v.x;
+          }
+          bq[q] = b;
+        }
+        float2* out = static_cast<float2*>(p.G3) + size_t(slot) * m0 * ncol + c;
+        for (int m0r = 0; m0r < nkeep; m0r += 16) {
+          const int nr = min(16, nkeep - m0r);
+          const bool rowOk = lr < nr;
+          const float2* wrow = sWx + (rowOk ? jxs[m0r + lr] : 0) * nG0p;
+          acc_t accr, acci;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) accr[i] = acci[i] = 0.f;
+#pragma unroll
+          for (int q = 0; q < 12; ++q) {
+            if (q < KS) {
+              const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
+              const bool gOk = g < nG0;
+              const float2 w = wrow[gOk ? g : nG0];
+              const bool on = rowOk && gOk;
+              const float a1 = on ? (kc ? -w.y : w.x) : 0.f;
+              const float a2 = on ? (kc ? w.x : w.y) : 0.f;
+              accr = Mfma4<float>::mac(a1, bq[q], accr);
+              acci = Mfma4<float>::mac(a2, bq[q], acci);
+            }
+          }
+          if (c < ncol) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int row = Mfma4<float>::row(lg, i);
+              if (row < nr) out[size_t(jxs[m0r + row]) * ncol] = make_float2(accr[i], acci[i]);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next chunk rewrites the twiddles, T2 and the kept-row lists
+  }
+}
+
 constexpr int kSepEpThreads = 64;  // k_sep_ep workgroup: one wave
 
 // Row data of an rpp-row pass of k_sep_ep into LDS: U2 (row r at sU2 + r u2s,
@@ -2826,7 +2981,7 @@ inline size_t sep_xm_lds_bytes(int elem, int nG0, int m0, int nw, int nm, int nb
 
 // maxLineE / maxGrpE: largest number of entries of a line / group (LDS sizing).
 template <typename T>
-void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStream_t s) {
+void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStream_t s, bool yStage = true) {
   if (sp.nPanels <= 0) return;
   const size_t ldsZ = sep_stage_lds_bytes(sizeof(T), maxLineE, sp.ncol, sp.mz);
   const size_t ldsY = sep_stage_lds_bytes(sizeof(T), maxGrpE, sp.ncol, sp.my);
@@ -2837,6 +2992,7 @@ void launch_separable_zy(const SepParams& sp, int maxLineE, int maxGrpE, hipStre
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(ldsY, 16))));
   hipLaunchKernelGGL(k_sep_z<T>, dim3(sp.nLines, sp.nPanels), dim3(256), ldsZ, s, sp);
   PHX_HIP(hipGetLastError());
+  if (!yStage) return;  // the x-stage kernel runs the y-stage itself (k_sep_yx)
   hipLaunchKernelGGL(k_sep_y<T>, dim3(sp.nG0, sp.mz, sp.nPanels), dim3(256), ldsY, s, sp);
   PHX_HIP(hipGetLastError());
 }
@@ -2859,6 +3015,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
                         hipEvent_t mid, const EpFork& fork = EpFork{}) {
   PHX_REQUIRE(p.nm <= kSepNm && p.nb2max <= kSepNb2,
               "separable transform supports nm <= 32 and at most 8 bands per inner point");
+  PHX_REQUIRE(!p.fuseYX || (sizeof(T) == 4 && reduce), "separable transform: fused y/x-stage is FP32 only");
   if (sp.nPanels <= 0) return false;
   const int tiles = sp.nPanels * sp.my * sp.mz;
   const int grid = ceil_div(tiles, 8) * 8;
@@ -2893,6 +3050,7 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
       const bool ep2ok = !p.epFirst && !p.adaptive && p.nb2max <= 8;
       const bool split = p.G3 && !bf3 && !wide && p.keepMask && (ep2ok || (!p.slotOn && !p.gate)) &&
                          p.nm <= 16 && p.nw <= 16 && (2 * p.nG0 + 3) / 4 <= 12;
+      PHX_REQUIRE(split || !p.fuseYX, "separable transform: fused y/x-stage outside the split last stage");
       if (split) {
         // the electron rotation in the x-stage kernel: with the lean epilogue only
         const bool rot = p.epRot && ep2ok;
@@ -2912,8 +3070,21 @@ bool launch_separable_x(const SepParams& sp, const CoupParams& p, bool reduce, h
           hipLaunchKernelGGL(kern, dim3(grid), dim3(256), ldsA, s, p);
         };
         if (fork.xgStart) PHX_HIP(hipEventRecord(fork.xgStart, s));
-        if (rot) xg(k_sep_xg<T, 3, true>);
-        else xg(k_sep_xg<T, 3, false>);
+        if (p.fuseYX) {
+          if constexpr (sizeof(T) == 4) {
+            PHX_REQUIRE(!rot && !p.slotOn && !p.gate, "separable transform: fused y/x-stage without slot masks");
+            const size_t ldsYX = sep_yx_lds_bytes(sp.nLines, sp.nG0, p.m0);
+            PHX_REQUIRE(ldsYX <= 160 * 1024, "separable transform: LDS budget exceeded");
+            PHX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sep_yx),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsYX)));
+            hipLaunchKernelGGL(k_sep_yx, dim3(ceil_div(sp.ncol, 16), sp.mz, sp.nPanels), dim3(256), ldsYX, s, sp,
+                               p);
+          }
+        } else if (rot) {
+          xg(k_sep_xg<T, 3, true>);
+        } else {
+          xg(k_sep_xg<T, 3, false>);
+        }
         PHX_HIP(hipGetLastError());
         if (mid) PHX_HIP(hipEventRecord(mid, s));
         hipStream_t se = s;

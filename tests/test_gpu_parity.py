@@ -176,6 +176,39 @@ def test_epilogue_overlap_gives_identical_linewidths(split):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("mesh,nw,nt", [((6, 6, 6), 6, 1), ((5, 7, 6), 16, 2), ((8, 5, 7), 12, 3)])
+def test_fused_y_stage_gives_identical_linewidths(mesh, nw, nt):
+    """The y-stage inside the x-stage kernel (T2 in LDS) against the y-stage
+    kernel: the same sums in the same order, so bit-identical linewidths, on
+    uneven meshes with 6, 16 and 12 Wannier functions (36, 96 and 72 panel
+    columns: blocks of 16 with a ragged last block at 36 and 72) and 1 to 3
+    (T, mu); the fused run also against the oracle at the 1e-6 gate."""
+    say("fused y-stage", mesh, nw, nt)
+    p = dataclasses.replace(tiny_problem(natoms=2, nw=nw, rg_e=3, rg_p=3, mesh=mesh, seed=37))
+    p.temperatures = p.temperatures[:1].repeat(nt) * np.arange(1, nt + 1)
+    p.chem_pots = p.chem_pots[:1].repeat(nt)
+    gW = p.gw_host()
+    out = []
+    for fused in (0, 1):
+        lw = P.ElectronLinewidths(p, gW=gW, precision="fp32", transform="separable")
+        lw.set_fused_y_stage(fused)
+        lw.set_screen(6.0)
+        lw.set_panel_chunk(5)
+        lw.set_timing(True)
+        lw.run()
+        lw.sync()
+        assert lw.last_stage_fused_y() == bool(fused)
+        out.append(lw.linewidths(average=False))
+        if fused:
+            avg = lw.linewidths()
+    assert np.any(out[0])
+    assert np.array_equal(out[0], out[1])
+    ref = O.Oracle(p, gW.reshape(-1)).linewidths()
+    err = rel_err(avg, ref)
+    say("  max rel err vs oracle", err.max())
+    assert err.max() < 1e-6
+
+
 @pytest.mark.parametrize("split", [1, 3, 0])
 def test_fully_degenerate_bands_match_oracle(split):
     """H(k) = eps(k) * identity: the six bands form one degenerate group at
