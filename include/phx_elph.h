@@ -448,9 +448,12 @@ int phx_lw_autotune(phx_lw* h, int nk1, const int* k1_list, double* out4);
  * to rounding.  No counterpart in the reference, whose cacheElPh
  * (interaction_elph.cpp:229-448) sums over R_e once per outer point. */
 int phx_lw_set_line_batching(phx_lw* h, int min_points);
-/* Panels ((k1, b1) pairs) per launch of the separable transform (default 48).
- * Its two stage buffers hold one chunk: (lines * mz + groups * my * mz) *
- * nm * nw complex values per panel.  No counterpart in the reference. */
+/* Panels ((k1, b1) pairs) per launch of the separable transform; 0 (the
+ * default) = as many as keep one chunk's g3 rows of the split last stage
+ * within 16 GiB, between 16 and 128.  The stage buffers hold one chunk:
+ * (lines * mz + groups * my * mz) * nm * nw complex values per panel for T1 /
+ * T2, and mesh points * max(nm nw, 2 nb2max nm) complex FP32 for g3.  No
+ * counterpart in the reference. */
 int phx_lw_set_panel_chunk(phx_lw* h, int panels);
 /* Last stage of the separable transform as two kernels (on, the default) or
  * one fused kernel (off).  Split: an x-stage kernel writes the kept rows' g3

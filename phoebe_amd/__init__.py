@@ -696,7 +696,7 @@ class ElectronLinewidths:
         _check(lib().phx_lw_set_line_batching(self._h, int(min_points)))
 
     def set_panel_chunk(self, panels):
-        """Panels per separable-transform launch (bounds its stage buffers)."""
+        """Panels per separable-transform launch (bounds its stage buffers); 0 = automatic."""
         _check(lib().phx_lw_set_panel_chunk(self._h, int(panels)))
 
     def set_split_last_stage(self, on=True):

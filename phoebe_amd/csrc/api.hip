@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <mutex>
 #include <numeric>
 #include <random>
@@ -779,7 +780,7 @@ struct LwHandle {
   // cacheElPh on mesh lines: outer points of a run that share (j1, j2) are
   // batched together when the line holds at least lineMin of them (0 = off)
   int lineMin = 0;
-  int panelChunk = 48;     // panels per separable-transform launch (bounds T1 / T2)
+  int panelChunk = 0;      // panels per separable-transform launch (bounds T1 / T2 / g3); 0 = automatic
   int splitLast = 1;       // last stage as k_sep_xg + epilogue where they apply (2: the first epilogue,
                            // k_sep_ep; 3: the electron rotation in k_sep_xg, g' rows through HBM)
   DevBuf<float2> G3f;      // their g3 rows of one panel chunk, (slot, jx, col), or g' rows (slot, jx, b2, nu)
@@ -873,6 +874,20 @@ struct LwHandle {
 static bool lw_split_last(const LwHandle* L, const phx_problem& p) {
   return L->splitLast && L->transform == PHX_TRANSFORM_SEPARABLE && L->precision == PHX_FP32 && p.nm <= 16 &&
          p.nw <= 16 && p.nm * p.nw <= 192 && p.mesh[0] <= 64;
+}
+
+// Panels per separable-transform launch: the handle's setting, or (0) as many
+// as keep one chunk's g3 rows of the split last stage within kChunkG3Bytes,
+// between 16 and 128: fewer, larger launches have shorter tails (cfg2: 128
+// panels, 12.6 GB of g3; a 60^3 mesh with 18 modes: 34).
+constexpr double kChunkG3Bytes = 16. * (1ull << 30);
+static int lw_panel_chunk(const LwHandle* L, const phx_problem& p, int nPanels, int nb2max) {
+  if (L->panelChunk > 0) return std::min(nPanels, L->panelChunk);
+  const double ncol = double(p.nm) * p.nw;
+  const double perPanel =
+      double(p.mesh[1]) * p.mesh[2] * p.mesh[0] * std::max(ncol, 2. * nb2max * p.nm) * sizeof(float2);
+  const int c = int(std::min(128., std::max(16., std::floor(kChunkG3Bytes / perPanel))));
+  return std::max(1, std::min(nPanels, c));
 }
 
 // Fold the events of the last timed batch into kernelMs (stream must be idle).
@@ -1631,7 +1646,9 @@ static void lw_prepare(LwHandle* L, const int* k1s, int nkk, int k2b, int k2e, b
   // scratch of the separable transform: one panel chunk of stage buffers, row masks,
   // and the learned screen's per-block arrays
   if (L->transform == PHX_TRANSFORM_SEPARABLE && M > 0) {
-    const int chunk = std::min(nPanels, L->panelChunk);
+    // sized for the handle's full chunk, not this batch's panel count, so that a
+    // later, larger batch does not reallocate them
+    const int chunk = lw_panel_chunk(L, p, std::numeric_limits<int>::max(), b.nb2max);
     const size_t ncol = size_t(nm) * nw;
     const size_t t1 = size_t(L->nLines) * p.mesh[2] * ncol * 2, t2 = size_t(L->nG0) * p.mesh[1] * p.mesh[2] * ncol * 2;
     if (L->precision == PHX_FP64) {
@@ -1880,7 +1897,7 @@ static long long lw_launch(LwHandle* L, const BatchMeta& b) {
       cq.epRot = L->splitLast == 3;
     }
     // panels in chunks: the stage buffers T1 / T2 hold one chunk
-    const int chunk = std::min(nPanels, L->panelChunk);
+    const int chunk = lw_panel_chunk(L, p, nPanels, b.nb2max);
     if (L->precision == PHX_FP64) {
       sp.T1 = L->T1d.p;
       sp.T2 = L->T2d.p;
@@ -3606,7 +3623,7 @@ int phx_lw_last_stage_fused_y(phx_lw* h, int* on) {
 int phx_lw_set_panel_chunk(phx_lw* h, int panels) {
   return guard([&] {
     PHX_REQUIRE(h, "null handle");
-    PHX_REQUIRE(panels >= 1, "phx_lw_set_panel_chunk: panels must be >= 1");
+    PHX_REQUIRE(panels >= 0, "phx_lw_set_panel_chunk: panels must be >= 0 (0: automatic)");
     h->panelChunk = panels;
   });
 }

@@ -501,7 +501,8 @@ def test_line_batching_argument_validation():
     with pytest.raises(P.PhxError):
         s.set_line_batching(-1)
     with pytest.raises(P.PhxError):
-        s.set_panel_chunk(0)
+        s.set_panel_chunk(-1)
+    s.set_panel_chunk(0)  # automatic
     s.set_line_batching(0)
 
 
