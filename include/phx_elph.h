@@ -450,7 +450,7 @@ int phx_lw_autotune(phx_lw* h, int nk1, const int* k1_list, double* out4);
 int phx_lw_set_line_batching(phx_lw* h, int min_points);
 /* Panels ((k1, b1) pairs) per launch of the separable transform; 0 (the
  * default) = as many as keep one chunk's g3 rows of the split last stage
- * within 16 GiB, between 16 and 128.  The stage buffers hold one chunk:
+ * within 24 GiB, between 16 and 128.  The stage buffers hold one chunk:
  * (lines * mz + groups * my * mz) * nm * nw complex values per panel for T1 /
  * T2, and mesh points * max(nm nw, 2 nb2max nm) complex FP32 for g3.  No
  * counterpart in the reference. */

@@ -879,8 +879,8 @@ static bool lw_split_last(const LwHandle* L, const phx_problem& p) {
 // Panels per separable-transform launch: the handle's setting, or (0) as many
 // as keep one chunk's g3 rows of the split last stage within kChunkG3Bytes,
 // between 16 and 128: fewer, larger launches have shorter tails (cfg2: 128
-// panels, 12.6 GB of g3; a 60^3 mesh with 18 modes: 34).
-constexpr double kChunkG3Bytes = 16. * (1ull << 30);
+// panels, 12.6 GB of g3; a 60^3 mesh with 18 modes: 51).
+constexpr double kChunkG3Bytes = 24. * (1ull << 30);
 static int lw_panel_chunk(const LwHandle* L, const phx_problem& p, int nPanels, int nb2max) {
   if (L->panelChunk > 0) return std::min(nPanels, L->panelChunk);
   const double ncol = double(p.nm) * p.nw;

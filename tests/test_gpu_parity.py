@@ -176,6 +176,27 @@ def test_epilogue_overlap_gives_identical_linewidths(split):
     assert np.array_equal(out[0], out[1])
 
 
+def test_panel_chunk_does_not_change_linewidths():
+    """Panels per separable launch (automatic, 1, 3, 7, 128): every chunking
+    runs the same kernels on the same panels, so the linewidths are
+    bit-identical; the automatic chunk takes every panel of these small
+    batches in one launch."""
+    say("panel chunks")
+    p = dataclasses.replace(tiny_problem(natoms=2, nw=6, rg_e=3, rg_p=3, mesh=(6, 5, 6), seed=41))
+    gW = p.gw_host()
+    out = []
+    for chunk in (0, 1, 3, 7, 128):
+        lw = P.ElectronLinewidths(p, gW=gW, precision="fp32", transform="separable")
+        lw.set_screen(6.0)
+        lw.set_line_batching(4)
+        lw.set_panel_chunk(chunk)
+        lw.run()
+        out.append(lw.linewidths(average=False))
+    assert np.any(out[0])
+    for g in out[1:]:
+        assert np.array_equal(g, out[0])
+
+
 @pytest.mark.parametrize("mesh,nw,nt", [((6, 6, 6), 6, 1), ((5, 7, 6), 16, 2), ((8, 5, 7), 12, 3)])
 def test_fused_y_stage_gives_identical_linewidths(mesh, nw, nt):
     """The y-stage inside the x-stage kernel (T2 in LDS) against the y-stage
