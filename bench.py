@@ -584,7 +584,8 @@ def main():
                    "chosen_precision": FORMS[args.chosen][0],
                    "chosen_transform": FORMS[args.chosen][1],
                    "chosen_screen_sigma": FORMS[args.chosen][2],
-                   "chosen_line_batching": FORMS[args.chosen][3]},
+                   "chosen_line_min_points": FORMS[args.chosen][3],
+                   "lines_per_cache_stream": 16, "panel_chunk": "automatic (128 at cfg2)"},
         "plain_fp64": None if r64 is None else {
             "value": v64, "ms_per_step": r64["seconds"] * 1e3 / args.steps, "roofline": rf64},
         "fp64_separable": sep64,
