@@ -471,8 +471,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"], help="phel: coupling precision")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k1 sharding "
-              "across GPUs is described in DESIGN.md and not run here.", flush=True)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); sharding the outer "
+              "points k1 across GPUs is not run here.", flush=True)
         sys.exit(2)
 
     if args.config == "cfg1":

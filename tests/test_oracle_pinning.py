@@ -57,7 +57,9 @@ def test_mesh_index_matches_integer_folding():
 
 # ------------------------------------------------------------------ coupling
 def _brute_force_g(p, gW, U1, k1, U2, e3, q):
-    """Direct 5-fold sum of the coupling restated in SURVEY/Appendix A form."""
+    """Direct 5-fold sum of the coupling from its definition: the R_e and R_ph
+    Fourier sums, the U1 / U2 rotations and the phonon eigenvector, as one
+    loop nest (interaction_elph.cpp:369-402, 565-646)."""
     pe = np.exp(1j * p.Re @ k1) / p.degE
     pp = np.exp(1j * p.Rp @ q) / p.degP
     # g(b1, b2, nu2) = sum e3(nu,nu2) conj(U2(w2,b2)) pp(Rp) U1(w1,b1) pe(Re) gW(Re,Rp,nu,w1,w2)
