@@ -2112,12 +2112,13 @@ inline size_t sep_xg_lds_bytes(int elem, int nG0, int m0, int nm = 0, int nw = 0
 // in the same order as k_sep_y + k_sep_xg, so the same g3; HBM sees T1 once
 // and the g3 rows, not T2 twice.
 constexpr int kYxJC = 4;  // y coordinates per chunk: one slot per wave (8: 47 KB of LDS, slower)
+constexpr int kYxRow = 17;  // LDS row of 16 columns, padded: rows of different groups in different banks
 
 inline size_t sep_yx_lds_bytes(int nLines, int nG0, int m0) {
-  size_t b = (size_t(nLines) * 16 * sizeof(float2) + 15) / 16 * 16;  // T1 of the block
-  b += size_t(nLines) * kYxJC * sizeof(double2);                     // y twiddles of a chunk
-  b += size_t(nG0) * kYxJC * 16 * sizeof(float2);                    // T2 of a chunk
-  b += size_t(m0) * (nG0 + 1) * sizeof(float2);                      // x twiddles of every row
+  size_t b = (size_t(nLines) * kYxRow * sizeof(float2) + 15) / 16 * 16;  // T1 of the block
+  b += size_t(nLines) * kYxJC * sizeof(double2);                         // y twiddles of a chunk
+  b += size_t(nG0) * kYxJC * kYxRow * sizeof(float2);                    // T2 of a chunk
+  b += size_t(m0) * (nG0 + 2) * sizeof(float2);                          // x twiddles of every row
   b += size_t(2 * nLines + nG0 + 1 + 4 * 64) * sizeof(int);          // entries, groups, kept rows
   return b + 64;
 }
@@ -2128,12 +2129,12 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
   const int cb = blockIdx.x, jz = blockIdx.y, panel = blockIdx.z;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15, lg = lane >> 4;
   const int ncol = sp.ncol, my = sp.my, mz = sp.mz, m0 = p.m0, nG0 = sp.nG0, nL = sp.nLines;
-  const int nG0p = nG0 + 1;
-  float2* sT1 = reinterpret_cast<float2*>(smem);  // (entry in group order, 16)
-  double2* sWy = reinterpret_cast<double2*>(smem + (size_t(nL) * 16 * sizeof(float2) + 15) / 16 * 16);  // (entry, jj)
-  float2* sT2 = reinterpret_cast<float2*>(sWy + size_t(nL) * kYxJC);  // (g, jj, 16)
-  float2* sWx = sT2 + size_t(nG0) * kYxJC * 16;                        // (jx, g), g < nG0p
-  int* sOff = reinterpret_cast<int*>(sWx + size_t(m0) * nG0p);         // T1 row of each entry
+  const int nG0p = nG0 + 1, wxs = nG0 + 2;  // wxs: padded row of the x twiddles
+  float2* sT1 = reinterpret_cast<float2*>(smem);  // (entry in group order, kYxRow)
+  double2* sWy = reinterpret_cast<double2*>(smem + (size_t(nL) * kYxRow * sizeof(float2) + 15) / 16 * 16);  // (entry, jj)
+  float2* sT2 = reinterpret_cast<float2*>(sWy + size_t(nL) * kYxJC);  // (g, jj, kYxRow)
+  float2* sWx = sT2 + size_t(nG0) * kYxJC * kYxRow;                    // (jx, g), g < nG0p, row wxs
+  int* sOff = reinterpret_cast<int*>(sWx + size_t(m0) * wxs);          // T1 row of each entry
   int* sF1 = sOff + nL;                                                // f1 of each entry
   int* sGs = sF1 + nL;                                                 // group starts
   int* sJx = sGs + nG0 + 1;                                            // kept rows, 64 per wave
@@ -2150,7 +2151,7 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
       const double2 d = p.tw0[(jx * p.g0val[g]) % m0];
       w = make_float2(float(d.x), float(d.y));
     }
-    sWx[i] = w;
+    sWx[jx * wxs + g] = w;
   }
   __syncthreads();
   // T1 of the block: up to 8 loads per thread in flight before their LDS stores
@@ -2165,7 +2166,7 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int i = i0 + 256 * u;
-      if (i < nL * 16) sT1[i] = v[u];
+      if (i < nL * 16) sT1[(i >> 4) * kYxRow + (i & 15)] = v[u];
     }
   }
   const int nLinesYZ = p.m1 * p.m2;
@@ -2185,7 +2186,7 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
 #pragma unroll
       for (int jj = 0; jj < kYxJC; ++jj) acc[jj] = make_double2(0., 0.);
       for (int e = sGs[g]; e < sGs[g + 1]; ++e) {
-        const float2 xv = sT1[e * 16 + cc];
+        const float2 xv = sT1[e * kYxRow + cc];
         const double2 x = make_double2(double(xv.x), double(xv.y));
 #pragma unroll
         for (int jj = 0; jj < kYxJC; ++jj) {
@@ -2195,7 +2196,8 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
         }
       }
 #pragma unroll
-      for (int jj = 0; jj < kYxJC; ++jj) sT2[(g * kYxJC + jj) * 16 + cc] = make_float2(float(acc[jj].x), float(acc[jj].y));
+      for (int jj = 0; jj < kYxJC; ++jj)
+        sT2[(g * kYxJC + jj) * kYxRow + cc] = make_float2(float(acc[jj].x), float(acc[jj].y));
     }
     __syncthreads();
     for (int jw = wave; jw < nj; jw += 4) {
@@ -2213,7 +2215,7 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
           const int k = 4 * q + lg, g = k >> 1, kc = k & 1;
           float b = 0.f;
           if (q < KS && g < nG0) {
-            const float2 v = sT2[(g * kYxJC + jw) * 16 + lr];
+            const float2 v = sT2[(g * kYxJC + jw) * kYxRow + lr];
             b = kc ? v.y : v.x;
           }
           bq[q] = b;
@@ -2222,7 +2224,7 @@ static __global__ void __launch_bounds__(256) k_sep_yx(SepParams sp, CoupParams 
         for (int m0r = 0; m0r < nkeep; m0r += 16) {
           const int nr = min(16, nkeep - m0r);
           const bool rowOk = lr < nr;
-          const float2* wrow = sWx + (rowOk ? jxs[m0r + lr] : 0) * nG0p;
+          const float2* wrow = sWx + (rowOk ? jxs[m0r + lr] : 0) * wxs;
           acc_t accr, acci;
 #pragma unroll
           for (int i = 0; i < 4; ++i) accr[i] = acci[i] = 0.f;
